@@ -1,0 +1,186 @@
+/*
+ * moons_amd.h -- C ABI of the MI355X-native MOONS hot path (libmoons_amd.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  Every
+ * entry point returns MOONS_OK (0) or a non-zero status; moons_last_error()
+ * returns the message of the last failure on the calling thread.  The
+ * reference stops the program on error (`stop 'Error: ...'`,
+ * code/handwritten/ops/ops_del.f90:267-288) -- the Fortran shim
+ * (moons_amd/fortran/moons_amd_bindings.f90) turns a non-zero status into
+ * `error stop` with this message, so callers see the reference behaviour.
+ *
+ * Data layout (device): one allocation per scalar field, Fortran column-major
+ * with one ghost layer, exactly the layout of the reference's
+ * grid_field%f(:,:,:) (code/generated_code/grid_field/grid_field.f90:39-43), so
+ * host<->device transfers of a Fortran allocatable array are one memcpy.
+ *
+ * Threading: one host thread per library; the library owns one HIP stream
+ * (default priority) and every call is stream-ordered on it.  Not re-entrant,
+ * as the reference (single host thread + OpenMP inside each routine).
+ * One block per mesh (the reference's m%s > 1 multi-block meshes are rejected).
+ */
+#ifndef MOONS_AMD_H
+#define MOONS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define MOONS_OK 0
+#define MOONS_ERR_ARG 1
+#define MOONS_ERR_HIP 2
+#define MOONS_ERR_STATE 3
+#define MOONS_ERR_NAN 4
+
+/* --------------------------------------------------------- data location
+ * 3-bit mask, bit d set when the field is node-centred along direction d
+ * (code/generated_code/fields/data_location.f90:38-51; the shape along d is
+ * sn = N+3 when node-centred, sc = N+2 when cell-centred,
+ * code/handwritten/grid/grid_extend.f90:220-234). */
+#define MOONS_DL_CC 0
+#define MOONS_DL_FACE_X 1
+#define MOONS_DL_FACE_Y 2
+#define MOONS_DL_FACE_Z 4
+#define MOONS_DL_EDGE_X 6
+#define MOONS_DL_EDGE_Y 5
+#define MOONS_DL_EDGE_Z 3
+#define MOONS_DL_NODE 7
+
+/* ------------------------------------------------------------- BC types
+ * Face order xmin,xmax,ymin,ymax,zmin,zmax (code/handwritten/BCs/
+ * face_edge_corner_indexing.f90:75-85,326-339). */
+#define MOONS_BC_NONE 0
+#define MOONS_BC_DIRICHLET 1
+#define MOONS_BC_NEUMANN 2
+#define MOONS_BC_PERIODIC 3
+#define MOONS_BC_SYMMETRIC 4
+#define MOONS_BC_ANTISYMMETRIC 5
+
+/* ----------------------------------------------------------- operators */
+#define MOONS_OP_LAP_UNIFORM_SF 1 /* code/handwritten/solvers/matrix_free_operators.f90:35-62 */
+#define MOONS_PREC_LAP_SF 1       /* code/handwritten/solvers/preconditioners/preconditioners.f90:55-71 */
+
+typedef struct moons_mesh_s *moons_mesh_t;
+typedef struct moons_field_s *moons_field_t;
+typedef struct moons_pcg_s *moons_pcg_t;
+
+/* Exit information of one solve, the reference's ISP and norms after
+ * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
+ * iterations = ISP%iter_per_call, res_norm0 = res_norm0%L2 (sqrt of the sum of
+ * r0^2 over all cells), res_norm = last checked res_norm%L2, exit_loop =
+ * ISP%exit_loop(1:3) as bits 0 (tol_rel), 1 (tol_abs), 2 (iter_max). */
+typedef struct {
+  int iterations;
+  int exit_loop;
+  double res_norm0;
+  double res_norm;
+  double ms; /* device time of the solve (HIP events on the library stream) */
+} moons_pcg_info;
+
+/* ------------------------------------------------------------ library */
+/* No reference counterpart (the reference has no device): selects the GPU and
+ * creates the library stream.  Must precede every other call. */
+int moons_init(int device);
+int moons_finalize(void);
+const char *moons_last_error(void);
+int moons_synchronize(void);
+/* The library's hipStream_t, as void*, for callers that time or order work on it. */
+void *moons_stream(void);
+
+/* ------------------------------------------------------------ 1D grids */
+/* uniform_func + add_ghost_nodes: code/handwritten/grid/coordinate_distribution_funcs.f90:34-61,
+ * code/handwritten/grid/coordinates_extend.f90:529-538.  hn has N+3 entries. */
+int moons_nodes_uniform(double hmin, double hmax, int N, double *hn);
+/* robertsBoth_func (Tannehill transformation 2, alpha = 1/2) + ghosts:
+ * code/handwritten/grid/coordinate_distribution_funcs.f90:192-248,325-331,
+ * code/handwritten/grid/grid_init.f90:60-68.  hn has N+3 entries. */
+int moons_nodes_roberts_both(double hmin, double hmax, int N, double beta, double *hn);
+
+/* ---------------------------------------------------------------- mesh */
+/* init_c per direction (code/handwritten/grid/coordinates_extend.f90:78-173):
+ * node coordinates including the two ghost nodes, sn = N+3 >= 5 entries. */
+int moons_mesh_create(const double *hn_x, int sn_x, const double *hn_y, int sn_y,
+                      const double *hn_z, int sn_z, moons_mesh_t *out);
+int moons_mesh_destroy(moons_mesh_t m);
+/* Copies one 1D coordinate array (which: 0 hn, 1 hc, 2 dhn, 3 dhc) of direction
+ * dir (0..2) into out; returns its length through *n. */
+int moons_mesh_coords(moons_mesh_t m, int dir, int which, double *out, int *n);
+
+/* --------------------------------------------------------------- fields */
+/* A scalar field (SF, code/generated_code/fields/SF.f90:43-51) of one block. */
+int moons_field_create(moons_mesh_t m, int dl, moons_field_t *out);
+int moons_field_destroy(moons_field_t f);
+int moons_field_shape(moons_field_t f, int s[3]);
+/* BCs of the field (code/handwritten/boundary_conditions): per-face type and
+ * uniform value; all_neumann is the reference's SF%all_Neumann flag, which the
+ * caller sets (code/handwritten/solvers/momentum/init_P_BCs.f90:26). */
+int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6], int all_neumann);
+/* Host <-> device copies of the whole ghost-padded array (n = element count). */
+int moons_field_upload(moons_field_t f, const double *host, size_t n);
+int moons_field_download(moons_field_t f, double *host, size_t n);
+int moons_field_fill(moons_field_t f, double val);
+int moons_field_copy(moons_field_t dst, moons_field_t src);
+/* Device address of the field's array, for callers that share memory with
+ * the library on its stream (e.g. torch tensors over the same buffers). */
+int moons_field_device_ptr(moons_field_t f, void **dptr);
+
+/* --------------------------------------------------------------- BCs */
+/* apply_BCs / apply_BCs_implicit (code/handwritten/BCs/apply/apply_BCs.f90:42-64,
+ * code/handwritten/BCs/direct/apply_BCs_faces_raw.f90:29-283,
+ * code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90:27-257).
+ * Faces are applied in order 1..6. */
+int moons_apply_bcs(moons_field_t u);
+int moons_apply_bcs_implicit(moons_field_t u);
+
+/* ------------------------------------------------------ discrete operators */
+/* del%assign/add/subtract (code/handwritten/ops/ops_del.f90:236-261):
+ * gen 1 assign, 2 add, 3 subtract; n = 1,2; dir = 0..2; pad = 0,1. */
+int moons_del(moons_field_t dfdh, moons_field_t f, int n, int dir, int pad, int gen);
+/* grad_SF (code/handwritten/ops/ops_discrete.f90:173-182) */
+int moons_grad(moons_field_t gx, moons_field_t gy, moons_field_t gz, moons_field_t u);
+/* div_SF (code/handwritten/ops/ops_discrete.f90:158-171) */
+int moons_div(moons_field_t divU, moons_field_t u, moons_field_t v, moons_field_t w);
+/* curl_SF component dir = 0..2 (code/handwritten/ops/ops_discrete.f90:184-202) */
+int moons_curl(moons_field_t curlU, moons_field_t u, moons_field_t v, moons_field_t w, int dir);
+/* Lap_uniform_SF (code/handwritten/solvers/matrix_free_operators.f90:51-62):
+ * applies the implicit BCs to x (its ghosts change) then Ax = div(grad(x)),
+ * one fused kernel for cell-centred x. */
+int moons_lap_uniform_sf(moons_field_t Ax, moons_field_t x);
+/* volume_SF (code/handwritten/fields/SF_extend.f90:571-581) */
+int moons_volume(moons_field_t vol);
+
+/* ------------------------------------------------------------------ PCG */
+/* init_PCG_SF (code/handwritten/PCG/PCG_solver_extend.f90:51-107): binds the
+ * operator and preconditioner, computes vol and Minv; x carries the BCs used
+ * by the solver (its copy `p` in the reference). */
+int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons_pcg_t *out);
+int moons_pcg_destroy(moons_pcg_t pcg);
+/* init(ISP, iter_max, tol_rel, tol_abs, n_skip_check_res, ...)
+ * (code/handwritten/var_set/iter_solver_params_extend.f90:44-58). */
+int moons_pcg_set_isp(moons_pcg_t pcg, int iter_max, double tol_rel, double tol_abs,
+                      int n_skip_check_res);
+/* solve_PCG_SF (code/handwritten/PCG/PCG_solver_extend.f90:182-193): x in/out,
+ * b the right-hand side, both device fields of the solver's location.  The
+ * whole loop runs on the device; info may be NULL. */
+int moons_pcg_sf_solve(moons_pcg_t pcg, moons_field_t x, moons_field_t b, moons_pcg_info *info);
+/* clean_div_PCG (code/handwritten/solvers/clean_divergence.f90:37-59):
+ *   temp_CC = scale*div(Xstar); solve(PCG, phi, temp_CC);
+ *   X = Xstar - grad(phi)/scale; apply_BCs(X).
+ * X, Xstar, temp_F are the three face components. */
+int moons_clean_div(moons_pcg_t pcg, moons_field_t X[3], moons_field_t Xstar[3], moons_field_t phi,
+                    double scale, moons_field_t temp_F[3], moons_field_t temp_CC,
+                    moons_pcg_info *info);
+
+/* Times one application of each kernel of the solver's iteration (matvec +
+ * (p,Ap); x/r update + (r,r),(z,r); p update) with HIP events on the library
+ * stream, averaged over reps runs on the solver's current vectors.  ms[3]
+ * receives the per-kernel averages.  No reference counterpart (profiling). */
+int moons_pcg_time_kernels(moons_pcg_t pcg, int reps, double ms[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""ctypes binding of libmoons_amd.so (the C ABI declared in include/moons_amd.h).
+
+The library is built in-tree by moons_amd.build; loading fails loudly when it
+is missing -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmoons_amd.so")
+
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+
+
+class PcgInfo(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("exit_loop", C.c_int),
+        ("res_norm0", C.c_double),
+        ("res_norm", C.c_double),
+        ("ms", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/moons_amd.h
+SIGNATURES = {
+    "moons_init": (C.c_int, [C.c_int]),
+    "moons_finalize": (C.c_int, []),
+    "moons_last_error": (C.c_char_p, []),
+    "moons_synchronize": (C.c_int, []),
+    "moons_stream": (C.c_void_p, []),
+    "moons_nodes_uniform": (C.c_int, [C.c_double, C.c_double, C.c_int, c_double_p]),
+    "moons_nodes_roberts_both": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_double, c_double_p]),
+    "moons_mesh_create": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "moons_mesh_coords": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_int_p]),
+    "moons_field_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_field_destroy": (C.c_int, [C.c_void_p]),
+    "moons_field_shape": (C.c_int, [C.c_void_p, c_int_p]),
+    "moons_field_set_bcs": (C.c_int, [C.c_void_p, c_int_p, c_double_p, C.c_int]),
+    "moons_field_upload": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
+    "moons_field_download": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
+    "moons_field_fill": (C.c_int, [C.c_void_p, C.c_double]),
+    "moons_field_copy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "moons_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_apply_bcs": (C.c_int, [C.c_void_p]),
+    "moons_apply_bcs_implicit": (C.c_int, [C.c_void_p]),
+    "moons_del": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "moons_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_div": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_curl": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "moons_lap_uniform_sf": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "moons_volume": (C.c_int, [C.c_void_p]),
+    "moons_pcg_sf_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_pcg_destroy": (C.c_int, [C.c_void_p]),
+    "moons_pcg_set_isp": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "moons_pcg_sf_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgInfo)]),
+    "moons_clean_div": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p, C.c_double,
+                                  C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(PcgInfo)]),
+    "moons_pcg_time_kernels": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+}
+
+_lib = None
+
+
+class MoonsError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load libmoons_amd.so (built in-tree); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MoonsError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP library has no fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = load().moons_last_error().decode(errors="replace")
+        raise MoonsError(f"libmoons_amd status {status}: {msg}")

@@ -1,0 +1,241 @@
+"""Python host mirror of the MOONS SF / mesh / PCG call surface over libmoons_amd.
+
+Names follow the reference (code/handwritten): `SF` (scalar field,
+fields/SF_extend.f90), `mesh` built from per-direction node coordinates
+(grid/coordinates_extend.f90:78-173), `PCG_solver_SF` with init/solve
+(PCG/PCG_solver_extend.f90:51-206), `clean_div` (solvers/clean_divergence.f90:37-59),
+and the discrete operators grad/div/curl/lap (ops/ops_discrete.f90).  All
+compute runs in the HIP library on the GPU; this module only holds handles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import PcgInfo, check, load
+
+DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
+BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    """Select the GPU and create the library stream (once per process)."""
+    global _initialised
+    check(load().moons_init(int(device)))
+    _initialised = True
+
+
+def synchronize() -> None:
+    check(load().moons_synchronize())
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def nodes_uniform(hmin: float, hmax: float, N: int) -> np.ndarray:
+    """uniform_func + ghost nodes (N+3 node coordinates)."""
+    hn = np.zeros(N + 3)
+    check(load().moons_nodes_uniform(hmin, hmax, N, _dp(hn)))
+    return hn
+
+
+def nodes_roberts_both(hmin: float, hmax: float, N: int, beta: float) -> np.ndarray:
+    """robertsBoth_func (Tannehill transformation 2, alpha=1/2) + ghost nodes."""
+    hn = np.zeros(N + 3)
+    check(load().moons_nodes_roberts_both(hmin, hmax, N, beta, _dp(hn)))
+    return hn
+
+
+class mesh:
+    """Single-block mesh from three node-coordinate arrays (ghosts included)."""
+
+    def __init__(self, hn_x, hn_y, hn_z):
+        if not _initialised:
+            init(0)
+        self.hn = [np.ascontiguousarray(h, dtype=np.float64) for h in (hn_x, hn_y, hn_z)]
+        h = C.c_void_p()
+        check(load().moons_mesh_create(_dp(self.hn[0]), len(self.hn[0]), _dp(self.hn[1]), len(self.hn[1]),
+                                       _dp(self.hn[2]), len(self.hn[2]), C.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def cube_uniform(cls, N: int, hmin: float = 0.0, hmax: float = 1.0) -> "mesh":
+        hn = nodes_uniform(hmin, hmax, N)
+        return cls(hn, hn, hn)
+
+    @property
+    def sn(self):
+        return [len(h) for h in self.hn]
+
+    def shape(self, dl: int):
+        return tuple(self.sn[d] if (dl >> d) & 1 else self.sn[d] - 1 for d in range(3))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_mesh_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class SF:
+    """Device-resident scalar field of one block (reference SF, single BF)."""
+
+    def __init__(self, m: mesh, dl="CC"):
+        self.m = m
+        self.dl = DL[dl] if isinstance(dl, str) else int(dl)
+        h = C.c_void_p()
+        check(load().moons_field_create(m.handle, self.dl, C.byref(h)))
+        self.handle = h
+        s = (C.c_int * 3)()
+        check(load().moons_field_shape(h, s))
+        self.shape = (s[0], s[1], s[2])
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape))
+
+    def set_BCs(self, types, vals=None, all_Neumann: bool = False) -> "SF":
+        t = (C.c_int * 6)(*[BC[x] if isinstance(x, str) else int(x) for x in types])
+        v = (C.c_double * 6)(*(vals if vals is not None else [0.0] * 6))
+        check(load().moons_field_set_bcs(self.handle, t, v, 1 if all_Neumann else 0))
+        return self
+
+    def upload(self, a: np.ndarray) -> "SF":
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != field shape {self.shape}")
+        flat = a.ravel(order="F")
+        check(load().moons_field_upload(self.handle, _dp(flat), flat.size))
+        return self
+
+    def download(self) -> np.ndarray:
+        flat = np.empty(self.size, dtype=np.float64)
+        check(load().moons_field_download(self.handle, _dp(flat), flat.size))
+        return flat.reshape(self.shape, order="F")
+
+    def fill(self, v: float) -> "SF":
+        check(load().moons_field_fill(self.handle, float(v)))
+        return self
+
+    def assign(self, other: "SF") -> "SF":
+        check(load().moons_field_copy(self.handle, other.handle))
+        return self
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_field_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ operators
+def apply_BCs(u: SF) -> None:
+    check(load().moons_apply_bcs(u.handle))
+
+
+def apply_BCs_implicit(u: SF) -> None:
+    check(load().moons_apply_bcs_implicit(u.handle))
+
+
+def del_op(dfdh: SF, f: SF, n: int, dir: int, pad: int, gen: int) -> None:
+    """del%assign (gen=1) / add (2) / subtract (3), dir 0..2 (ops_del.f90)."""
+    check(load().moons_del(dfdh.handle, f.handle, n, dir, pad, gen))
+
+
+def grad(gx: SF, gy: SF, gz: SF, u: SF) -> None:
+    check(load().moons_grad(gx.handle, gy.handle, gz.handle, u.handle))
+
+
+def div(divU: SF, u: SF, v: SF, w: SF) -> None:
+    check(load().moons_div(divU.handle, u.handle, v.handle, w.handle))
+
+
+def curl(curlU: SF, u: SF, v: SF, w: SF, dir: int) -> None:
+    check(load().moons_curl(curlU.handle, u.handle, v.handle, w.handle, dir))
+
+
+def Lap_uniform_SF(Ax: SF, x: SF) -> None:
+    check(load().moons_lap_uniform_sf(Ax.handle, x.handle))
+
+
+def volume(vol: SF) -> None:
+    check(load().moons_volume(vol.handle))
+
+
+# ------------------------------------------------------------------ solvers
+@dataclass
+class iter_solver_params:
+    """ISP: exit criteria of an iterative solve (var_set/iter_solver_params_extend.f90)."""
+
+    iter_max: int = 1000
+    tol_rel: float = 1e-6
+    tol_abs: float = 1e-13
+    n_skip_check_res: int = 1
+
+
+@dataclass
+class pcg_info:
+    iterations: int
+    exit_loop: int
+    res_norm0: float
+    res_norm: float
+    ms: float
+
+    @property
+    def rel(self) -> float:
+        return self.res_norm / self.res_norm0 if self.res_norm0 else 0.0
+
+
+class PCG_solver_SF:
+    """Jacobi-PCG for the cell-centred Laplacian (op Lap_uniform_SF, prec_Lap_SF)."""
+
+    OP_LAP_UNIFORM_SF = 1
+    PREC_LAP_SF = 1
+
+    def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None):
+        h = C.c_void_p()
+        check(load().moons_pcg_sf_create(m.handle, self.OP_LAP_UNIFORM_SF, self.PREC_LAP_SF, x.handle, C.byref(h)))
+        self.handle = h
+        self.m = m
+        self.set_ISP(ISP or iter_solver_params())
+
+    def set_ISP(self, ISP: iter_solver_params) -> None:
+        self.ISP = ISP
+        check(load().moons_pcg_set_isp(self.handle, ISP.iter_max, ISP.tol_rel, ISP.tol_abs, ISP.n_skip_check_res))
+
+    def solve(self, x: SF, b: SF) -> pcg_info:
+        info = PcgInfo()
+        check(load().moons_pcg_sf_solve(self.handle, x.handle, b.handle, C.byref(info)))
+        return pcg_info(info.iterations, info.exit_loop, info.res_norm0, info.res_norm, info.ms)
+
+    def time_kernels(self, reps: int = 20):
+        ms = (C.c_double * 3)()
+        check(load().moons_pcg_time_kernels(self.handle, reps, ms))
+        return [ms[0], ms[1], ms[2]]
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_pcg_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def clean_div(PCG: PCG_solver_SF, X, Xstar, phi: SF, scale: float, temp_F, temp_CC: SF) -> pcg_info:
+    """X = Xstar - grad(phi)/scale with lap(phi) = scale*div(Xstar) (clean_divergence.f90:37-59)."""
+    arr = C.c_void_p * 3
+    info = PcgInfo()
+    check(load().moons_clean_div(PCG.handle, arr(*[u.handle for u in X]), arr(*[u.handle for u in Xstar]), phi.handle,
+                                 float(scale), arr(*[u.handle for u in temp_F]), temp_CC.handle, C.byref(info)))
+    return pcg_info(info.iterations, info.exit_loop, info.res_norm0, info.res_norm, info.ms)

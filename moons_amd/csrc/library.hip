@@ -1,0 +1,713 @@
+// library.hip -- context, mesh, fields, BCs and the general staggered
+// operators of libmoons_amd.  gfx950 only.
+#include "common.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace moons {
+
+static thread_local std::string g_err;
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+hipStream_t stream() { return g_stream; }
+
+// ------------------------------------------------------------------ 1D grids
+static void add_ghost_nodes(const std::vector<double> &h, double *out) {
+  size_t np = h.size();
+  double dh0 = h[1] - h[0];
+  double dhe = h[np - 1] - h[np - 2];
+  out[0] = h[0] - dh0;
+  for (size_t i = 0; i < np; ++i) out[i + 1] = h[i];
+  out[np + 1] = h[np - 1] + dhe;
+}
+
+// ------------------------------------------------------------ coordinates
+// init_c + init_stencils_c: code/handwritten/grid/coordinates_extend.f90:78-173,506-527.
+// Stencil formulas: code/handwritten/grid/derivative_stencils.f90:31-208,
+// code/handwritten/grid/interpolation_stencils.f90:12-35.
+void coords_init(HostCoords &c, const double *hn, int sn) {
+  c.sn = sn;
+  c.sc = sn - 1;
+  const int sc = c.sc;
+  c.hn.assign(hn, hn + sn);
+  c.dhn.resize(sn - 1);
+  for (int i = 0; i < sn - 1; ++i) c.dhn[i] = hn[i + 1] - hn[i];
+  c.hc.resize(sc);
+  for (int i = 0; i < sc; ++i) c.hc[i] = (hn[i + 1] + hn[i]) / 2.0;
+  c.dhc.resize(sc - 1);
+  for (int i = 0; i < sc - 1; ++i) c.dhc[i] = c.hc[i + 1] - c.hc[i];
+  c.theta_D.resize(sc - 1);
+  c.theta_U.resize(sc - 1);
+  for (int i = 0; i < sc - 1; ++i) {
+    c.theta_D[i] = (c.hn[i + 1] - c.hc[i]) / (c.hc[i + 1] - c.hc[i]);
+    c.theta_U[i] = 1.0 - c.theta_D[i];
+  }
+  c.cc2n_D.resize(sc - 1);
+  c.cc2n_U.resize(sc - 1);
+  for (int i = 0; i < sc - 1; ++i) {
+    c.cc2n_U[i] = 1.0 / c.dhc[i];
+    c.cc2n_D[i] = -c.cc2n_U[i];
+  }
+  c.n2cc_D.resize(sn - 1);
+  c.n2cc_U.resize(sn - 1);
+  for (int i = 0; i < sn - 1; ++i) {
+    c.n2cc_U[i] = 1.0 / c.dhn[i];
+    c.n2cc_D[i] = -c.n2cc_U[i];
+  }
+  // Collocated 3-point stencils, one-sided/half-cell rows at the walls.
+  // Written with 1-based helpers to follow the reference's index algebra.
+  for (int n = 0; n < 2; ++n) {
+    c.colCC_L[n].assign(sc - 2, 0.0); c.colCC_D[n].assign(sc - 2, 0.0); c.colCC_U[n].assign(sc - 2, 0.0);
+    c.colN_L[n].assign(sn - 2, 0.0); c.colN_D[n].assign(sn - 2, 0.0); c.colN_U[n].assign(sn - 2, 0.0);
+  }
+  {
+    // nodes (dh = dhn, s = sn)
+    const double *dh = c.dhn.data();
+    auto DH = [&](int i) { return dh[i - 1]; };
+    int s = sn;
+    double *L1 = c.colN_L[0].data(), *D1 = c.colN_D[0].data(), *U1 = c.colN_U[0].data();
+    double *L2 = c.colN_L[1].data(), *D2 = c.colN_D[1].data(), *U2 = c.colN_U[1].data();
+    if (s > 3) {
+      int i = 2;
+      L1[0] = -(DH(i + 1) + 2.0 * DH(i)) / (DH(i) * (DH(i + 1) + DH(i)));
+      D1[0] = (DH(i + 1) + DH(i)) / (DH(i + 1) * DH(i));
+      U1[0] = -DH(i) / (DH(i + 1) * (DH(i + 1) + DH(i)));
+      L2[0] = 2.0 / (DH(i) * (DH(i + 1) + DH(i)));
+      D2[0] = -2.0 / (DH(i + 1) * DH(i));
+      U2[0] = 2.0 / (DH(i + 1) * (DH(i + 1) + DH(i)));
+      for (i = 3; i <= s - 2; ++i) {
+        L1[i - 2] = -(DH(i) / (DH(i - 1) * (DH(i - 1) + DH(i))));
+        D1[i - 2] = ((-DH(i - 1) + DH(i)) / (DH(i - 1) * DH(i)));
+        U1[i - 2] = (DH(i - 1) / (DH(i) * (DH(i - 1) + DH(i))));
+        L2[i - 2] = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i)));
+        D2[i - 2] = -2.0 / (DH(i - 1) * DH(i));
+        U2[i - 2] = 2.0 / (DH(i) * (DH(i - 1) + DH(i)));
+      }
+      i = s - 1;
+      L1[s - 3] = DH(i - 1) / (DH(i - 2) * (DH(i - 1) + DH(i - 2)));
+      D1[s - 3] = -(DH(i - 1) + DH(i - 2)) / (DH(i - 1) * DH(i - 2));
+      U1[s - 3] = (2.0 * DH(i - 1) + DH(i - 2)) / (DH(i - 1) * (DH(i - 1) + DH(i - 2)));
+      L2[s - 3] = 2.0 / (DH(i - 2) * (DH(i - 1) + DH(i - 2)));
+      D2[s - 3] = -2.0 / (DH(i - 1) * DH(i - 2));
+      U2[s - 3] = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i - 2)));
+    }
+  }
+  {
+    // cell centres (dh = dhc, s = sc); wall rows use half spacings
+    const double *dh = c.dhc.data();
+    auto DH = [&](int i) { return dh[i - 1]; };
+    int s = sc;
+    double *L1 = c.colCC_L[0].data(), *D1 = c.colCC_D[0].data(), *U1 = c.colCC_U[0].data();
+    double *L2 = c.colCC_L[1].data(), *D2 = c.colCC_D[1].data(), *U2 = c.colCC_U[1].data();
+    if (s > 2) {
+      int i = 2;
+      double h = 0.5 * DH(i - 1);
+      L1[0] = (-(DH(i)) / (h * (h + DH(i))));
+      D1[0] = ((-h + DH(i)) / (h * DH(i)));
+      U1[0] = (h / (DH(i) * (h + DH(i))));
+      L2[0] = 2.0 / (h * (h + DH(i)));
+      D2[0] = -2.0 / (h * DH(i));
+      U2[0] = 2.0 / (DH(i) * (h + DH(i)));
+      for (i = 3; i <= s - 2; ++i) {
+        L1[i - 2] = (-DH(i) / (DH(i - 1) * (DH(i - 1) + DH(i))));
+        D1[i - 2] = ((-DH(i - 1) + DH(i)) / (DH(i - 1) * DH(i)));
+        U1[i - 2] = (DH(i - 1) / (DH(i) * (DH(i - 1) + DH(i))));
+        L2[i - 2] = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i)));
+        D2[i - 2] = -(2.0 / (DH(i - 1) * DH(i)));
+        U2[i - 2] = 2.0 / (DH(i) * (DH(i - 1) + DH(i)));
+      }
+      i = s - 1;
+      double g = 0.5 * DH(i);
+      L1[s - 3] = (-0.5 * DH(i) / (DH(i - 1) * (DH(i - 1) + g)));
+      D1[s - 3] = ((-DH(i - 1) + g) / (DH(i - 1) * g));
+      U1[s - 3] = (DH(i - 1) / (g * (DH(i - 1) + g)));
+      L2[s - 3] = 2.0 / (DH(i - 1) * (DH(i - 1) + g));
+      D2[s - 3] = -2.0 / (DH(i - 1) * g);
+      U2[s - 3] = 2.0 / (g * (DH(i - 1) + g));
+    }
+  }
+  c.lapA.assign(sc, 0.0);
+  c.lapB.assign(sc, 0.0);
+  for (int i = 1; i < sc - 1; ++i) {
+    c.lapA[i] = c.n2cc_D[i] * c.cc2n_U[i - 1];
+    c.lapB[i] = c.n2cc_U[i] * c.cc2n_D[i];
+  }
+}
+
+BCDev make_bcdev(const moons_field_s *f) {
+  BCDev b;
+  for (int face = 0; face < 6; ++face) {
+    b.type[face] = f->has_bcs ? f->bct[face] : MOONS_BC_NONE;
+    b.val[face] = f->bval[face];
+    int d = face / 2;
+    const HostCoords &c = f->m->hc[d];
+    double dh = (face % 2) ? c.dhn[c.sn - 2] : c.dhn[0];
+    double nhat = (face % 2) ? 1.0 : -1.0;
+    b.dh_nhat[face] = dh * nhat;
+  }
+  return b;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__device__ __forceinline__ size_t idx3(const int s[3], int i, int j, int k) {
+  return (size_t)i + (size_t)s[0] * ((size_t)j + (size_t)s[1] * (size_t)k);
+}
+
+struct Shape3 {
+  int s[3];
+};
+
+// One face of apply_BCs / apply_BCs_implicit over the full plane (tangential
+// ghosts included), raw rules of code/handwritten/BCs/direct/apply_BCs_faces_raw.f90
+// and code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90.
+__global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int face, int type, int nalong,
+                                                       double bval, double dh_nhat, int implicit_) {
+  const int d = face / 2, is_max = face % 2;
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na * nb) return;
+  const int ja = t % na, jb = t / na;
+  const int s = S.s[d];
+  int G, B, I, Io;
+  if (!nalong) {
+    G = is_max ? s - 1 : 0; B = -1; I = is_max ? s - 2 : 1; Io = is_max ? 1 : s - 2;
+  } else {
+    G = is_max ? s - 1 : 0; B = is_max ? s - 2 : 1; I = is_max ? s - 3 : 2; Io = is_max ? 2 : s - 3;
+  }
+  int ii[3];
+  ii[a] = ja; ii[b] = jb;
+  ii[d] = G; const size_t ig = idx3(S.s, ii[0], ii[1], ii[2]);
+  ii[d] = I; const size_t iin = idx3(S.s, ii[0], ii[1], ii[2]);
+  switch (type) {
+    case MOONS_BC_DIRICHLET:
+      if (!nalong) {
+        u[ig] = implicit_ ? -u[iin] : 2.0 * bval - u[iin];
+      } else {
+        ii[d] = B; const size_t ib = idx3(S.s, ii[0], ii[1], ii[2]);
+        double ub = implicit_ ? 0.0 : bval;
+        u[ib] = ub;
+        u[ig] = implicit_ ? -u[iin] : 2.0 * ub - u[iin];
+      }
+      break;
+    case MOONS_BC_NEUMANN:
+      if (implicit_) u[ig] = u[iin];
+      else if (!nalong) u[ig] = u[iin] + bval * dh_nhat;
+      else u[ig] = u[iin] + 2.0 * bval * dh_nhat;
+      break;
+    case MOONS_BC_PERIODIC: {
+      int Gp = G, Iop = Io;
+      if (nalong && is_max && s > 4) { Gp = B; Iop = 1; }
+      ii[d] = Gp; const size_t igp = idx3(S.s, ii[0], ii[1], ii[2]);
+      ii[d] = Iop; const size_t io = idx3(S.s, ii[0], ii[1], ii[2]);
+      u[igp] = u[io];
+    } break;
+    case MOONS_BC_SYMMETRIC: u[ig] = u[iin]; break;
+    case MOONS_BC_ANTISYMMETRIC: u[ig] = -u[iin]; break;
+    default: break;
+  }
+}
+
+// Assigns val to one whole plane idx along direction d.
+__global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d, int idx, double val) {
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na * nb) return;
+  int ii[3];
+  ii[a] = t % na; ii[b] = t / na; ii[d] = idx;
+  u[idx3(S.s, ii[0], ii[1], ii[2])] = val;
+}
+
+// del along dir for every output element of the padded range, one thread per
+// output element.  kind: 0 stag CC->N (gt=1), 1 stag N->CC (gt=0), 2 col CC, 3 col N.
+// Follows code/handwritten/stencils/stencils.f90 element by element, including
+// the end entries the staggered kernels zero before accumulating.
+__global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const double *__restrict__ f, Shape3 So,
+                                             Shape3 Sf, int dir, int kind, const double *__restrict__ L,
+                                             const double *__restrict__ D, const double *__restrict__ U,
+                                             int gen, int pad) {
+  const int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  const int nd = So.s[dir];
+  const int nlo = Sf.s[lo] - 2 * pad, nhi = Sf.s[hi] - 2 * pad;
+  const long long total = (long long)nd * nlo * nhi;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total || nlo <= 0 || nhi <= 0) return;
+  const int o = (int)(t % nd);
+  const long long rest = t / nd;
+  const int plo = pad + (int)(rest % nlo), phi = pad + (int)(rest / nlo);
+  int io[3];
+  io[dir] = o; io[lo] = plo; io[hi] = phi;
+  const size_t oo = idx3(So.s, io[0], io[1], io[2]);
+  const size_t fs = (dir == 0) ? 1 : (dir == 1 ? (size_t)Sf.s[0] : (size_t)Sf.s[0] * Sf.s[1]);
+  int iff[3];
+  iff[dir] = 0; iff[lo] = plo; iff[hi] = phi;
+  const double *fp = f + idx3(Sf.s, iff[0], iff[1], iff[2]);
+  const int s = Sf.s[dir];
+  double cur = out[oo];
+  double res;
+  if (kind == 0 || kind == 1) {
+    const int gt = (kind == 0) ? 1 : 0;
+    const bool zero_end = (o == 0 || o == nd - 1);
+    const int i = o - gt;
+    const bool written = (i >= 0 && i <= s - 2);
+    if (zero_end) cur = 0.0;
+    if (!written) {
+      res = cur;
+    } else {
+      const double x0 = fp[(size_t)i * fs] * D[i];
+      const double x1 = fp[(size_t)(i + 1) * fs] * U[i];
+      if (gen == 1) res = x0 + x1;
+      else if (gen == 2) res = cur + x0 + x1;
+      else res = cur - (x0 + x1);
+    }
+  } else {
+    // collocated, pad1 = pad2 = 0
+#define FF(q) fp[(size_t)(q) * fs]
+    if (o == 0 || o == s - 1) {
+      res = 0.0;
+    } else if (o == 1) {
+      const double t1 = 0.5 * (FF(0) + FF(1)) * L[0] * 1.0;
+      const double t2 = FF(0) * L[0] * 0.0;
+      if (kind == 2) {
+        if (gen == 1) res = t1 + t2 + FF(1) * D[0] + FF(2) * U[0];
+        else if (gen == 2) res = cur + t1 + t2 + FF(1) * D[0] + FF(2) * U[0];
+        else res = cur - (t1 + t2 + FF(1) * D[0] + FF(2) * U[0]);
+      } else {
+        if (gen == 1) res = FF(1) * L[0] + FF(2) * D[0] + FF(3) * U[0];
+        else if (gen == 2) res = cur + FF(1) * L[0] + FF(2) * D[0] + FF(3) * U[0];
+        else res = cur - (FF(1) * L[0] + FF(2) * D[0] + FF(3) * U[0]);
+      }
+    } else if (o == s - 2) {
+      if (kind == 2) {
+        const double t3 = 0.5 * (FF(s - 2) + FF(s - 1)) * U[s - 3] * 1.0;
+        const double t4 = FF(s - 1) * U[s - 3] * 0.0;
+        if (gen == 1) res = FF(s - 3) * L[s - 3] + FF(s - 2) * D[s - 3] + t3 + t4;
+        else if (gen == 2) res = cur + FF(s - 3) * L[s - 3] + FF(s - 2) * D[s - 3] + t3 + t4;
+        else res = cur - (FF(s - 3) * L[s - 3] + FF(s - 2) * D[s - 3] + t3 + t4);
+      } else {
+        if (gen == 1) res = FF(s - 4) * L[s - 3] + FF(s - 3) * D[s - 3] + FF(s - 2) * U[s - 3];
+        else if (gen == 2) res = cur + FF(s - 4) * L[s - 3] + FF(s - 3) * D[s - 3] + FF(s - 2) * U[s - 3];
+        else res = cur - (FF(s - 4) * L[s - 3] + FF(s - 3) * D[s - 3] + FF(s - 2) * U[s - 3]);
+      }
+    } else {
+      const int i = o;
+      const double x0 = FF(i - 1) * L[i - 1];
+      if (gen == 1) res = x0 + FF(i) * D[i - 1] + FF(i + 1) * U[i - 1];
+      else if (gen == 2) res = cur + x0 + FF(i) * D[i - 1] + FF(i + 1) * U[i - 1];
+      else res = cur - (x0 + FF(i) * D[i - 1] + FF(i + 1) * U[i - 1]);
+    }
+#undef FF
+  }
+  out[oo] = res;
+}
+
+__global__ void __launch_bounds__(256) k_scale(double *u, size_t n, double s) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) u[t] = u[t] * s;
+}
+__global__ void __launch_bounds__(256) k_sub(double *out, const double *a, const double *b, size_t n) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = a[t] - b[t];
+}
+__global__ void __launch_bounds__(256) k_fill(double *u, size_t n, double v) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) u[t] = v;
+}
+
+// volume_DL_GF: code/handwritten/grid_field/GF_distributions.f90:61-88
+__global__ void __launch_bounds__(256) k_volume(double *vol, Shape3 S, int e0, int e1, int e2,
+                                                const double *dh0, const double *dh1, const double *dh2) {
+  size_t n = (size_t)S.s[0] * S.s[1] * S.s[2];
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int i = (int)(t % S.s[0]);
+  int j = (int)((t / S.s[0]) % S.s[1]);
+  int k = (int)(t / ((size_t)S.s[0] * S.s[1]));
+  double v = 0.0;
+  if (i >= 1 && i < S.s[0] - 1 && j >= 1 && j < S.s[1] - 1 && k >= 1 && k < S.s[2] - 1)
+    v = dh0[i - e0] * dh1[j - e1] * dh2[k - e2];
+  vol[t] = v;
+}
+
+// --------------------------------------------------------------- launchers
+static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+int launch_apply_bcs(moons_field_s *u, bool implicit_) {
+  if (!u->has_bcs) return MOONS_OK;
+  Shape3 S{{u->s[0], u->s[1], u->s[2]}};
+  BCDev b = make_bcdev(u);
+  for (int face = 0; face < 6; ++face) {
+    if (b.type[face] == MOONS_BC_NONE) continue;
+    int d = face / 2;
+    int a = (d + 1) % 3, bb = (d + 2) % 3;
+    size_t np = (size_t)u->s[a] * u->s[bb];
+    hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
+                       (u->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0);
+    MOONS_LAUNCH_CHECK();
+  }
+  return MOONS_OK;
+}
+
+int launch_assign_ghost_xperiodic(moons_field_s *u, double val) {
+  Shape3 S{{u->s[0], u->s[1], u->s[2]}};
+  for (int face = 0; face < 6; ++face) {
+    if (u->has_bcs && u->bct[face] == MOONS_BC_PERIODIC) continue;
+    int d = face / 2;
+    int a = (d + 1) % 3, bb = (d + 2) % 3;
+    size_t np = (size_t)u->s[a] * u->s[bb];
+    hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, d,
+                       (face % 2) ? u->s[d] - 1 : 0, val);
+    MOONS_LAUNCH_CHECK();
+  }
+  return MOONS_OK;
+}
+
+int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int pad, int gen) {
+  if (dir < 0 || dir > 2 || (n != 1 && n != 2) || gen < 1 || gen > 3 || pad < 0 || pad > 1)
+    return fail(MOONS_ERR_ARG, "del: bad n/dir/pad/gen");
+  if (out->m != f->m) return fail(MOONS_ERR_ARG, "del: fields on different meshes");
+  int nin = (f->dl >> dir) & 1, nout = (out->dl >> dir) & 1;
+  if ((f->dl & ~(1 << dir)) != (out->dl & ~(1 << dir)))
+    return fail(MOONS_ERR_ARG, "Error: diffType undetermined in ops_del (locations differ off dir)");
+  const DirCoefDev &c = f->m->dev.c[dir];
+  int kind;
+  const double *L = nullptr, *D, *U;
+  if (!nin && nout) { kind = 0; D = c.cc2n_D; U = c.cc2n_U; }
+  else if (nin && !nout) { kind = 1; D = c.n2cc_D; U = c.n2cc_U; }
+  else if (!nin) { kind = 2; L = c.colCC_L[n - 1]; D = c.colCC_D[n - 1]; U = c.colCC_U[n - 1]; }
+  else { kind = 3; L = c.colN_L[n - 1]; D = c.colN_D[n - 1]; U = c.colN_U[n - 1]; }
+  Shape3 So{{out->s[0], out->s[1], out->s[2]}}, Sf{{f->s[0], f->s[1], f->s[2]}};
+  int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  size_t total = (size_t)out->s[dir] * (size_t)(f->s[a] - 2 * pad) * (size_t)(f->s[b] - 2 * pad);
+  hipLaunchKernelGGL(k_del, dim3(nblk(total)), dim3(256), 0, g_stream, out->d, f->d, So, Sf, dir, kind, L, D, U,
+                     gen, pad);
+  MOONS_LAUNCH_CHECK();
+  if (gen == 1 && pad > 0) {
+    // diff(): assign with padding zeroes the ghost planes of the other directions
+    for (int q : {a, b}) {
+      int c2 = (q + 1) % 3, c3 = (q + 2) % 3;
+      size_t np = (size_t)out->s[c2] * out->s[c3];
+      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, 0, 0.0);
+      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, out->s[q] - 1, 0.0);
+      MOONS_LAUNCH_CHECK();
+    }
+  }
+  return MOONS_OK;
+}
+
+int launch_scale(moons_field_s *u, double s) {
+  hipLaunchKernelGGL(k_scale, dim3(nblk(u->n)), dim3(256), 0, g_stream, u->d, u->n, s);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int launch_sub(moons_field_s *out, const moons_field_s *a, const moons_field_s *b) {
+  hipLaunchKernelGGL(k_sub, dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, a->d, b->d, out->n);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+} // namespace moons
+
+using namespace moons;
+
+// =================================================================== C ABI
+extern "C" {
+
+int moons_init(int device) {
+  int n = 0;
+  MOONS_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(MOONS_ERR_ARG, "moons_init: no such device");
+  MOONS_HIP(hipSetDevice(device));
+  if (g_stream && g_device == device) return MOONS_OK;
+  if (g_stream) MOONS_HIP(hipStreamDestroy(g_stream));
+  MOONS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  return MOONS_OK;
+}
+
+int moons_finalize(void) {
+  if (g_stream) {
+    MOONS_HIP(hipStreamSynchronize(g_stream));
+    MOONS_HIP(hipStreamDestroy(g_stream));
+  }
+  g_stream = nullptr;
+  g_device = -1;
+  return MOONS_OK;
+}
+
+const char *moons_last_error(void) { return g_err.c_str(); }
+
+int moons_synchronize(void) {
+  if (!g_stream) return fail(MOONS_ERR_STATE, "moons_init has not been called");
+  MOONS_HIP(hipStreamSynchronize(g_stream));
+  return MOONS_OK;
+}
+
+void *moons_stream(void) { return (void *)g_stream; }
+
+int moons_nodes_uniform(double hmin, double hmax, int N, double *hn) {
+  if (N < 2 || !hn || !(hmax > hmin)) return fail(MOONS_ERR_ARG, "moons_nodes_uniform: need N >= 2, hmax > hmin");
+  std::vector<double> h(N + 1);
+  double dh = (hmax - hmin) / (double)N;
+  for (int i = 0; i <= N; ++i) h[i] = hmin + (double)i * dh;
+  add_ghost_nodes(h, hn);
+  return MOONS_OK;
+}
+
+int moons_nodes_roberts_both(double hmin, double hmax, int N, double beta, double *hn) {
+  if (N < 2 || !hn || !(hmax > hmin) || !(beta > 1.0))
+    return fail(MOONS_ERR_ARG, "moons_nodes_roberts_both: need N >= 2, hmax > hmin, beta > 1");
+  const double one = 1.0, two = 2.0, a = 0.5, b = beta;
+  const double g = (b + one) / (b - one);
+  const double dh = (hmax - hmin) / (double)N;
+  std::vector<double> h(N + 1);
+  for (int i = 0; i <= N; ++i) {
+    double hb = hmin + (double)i * dh;
+    hb = (hb - hmin) / (hmax - hmin);
+    double e = std::pow(g, (hb - a) / (one - a));
+    h[i] = ((b + two * a) * e - b + two * a) / ((two * a + one) * (one + e));
+  }
+  for (int i = 0; i <= N; ++i) h[i] = hmin + (hmax - hmin) * h[i];
+  add_ghost_nodes(h, hn);
+  return MOONS_OK;
+}
+
+int moons_mesh_create(const double *hx, int sx, const double *hy, int sy, const double *hz, int sz,
+                      moons_mesh_t *out) {
+  if (!g_stream) return fail(MOONS_ERR_STATE, "moons_init has not been called");
+  if (!out || !hx || !hy || !hz) return fail(MOONS_ERR_ARG, "moons_mesh_create: null argument");
+  if (sx < 5 || sy < 5 || sz < 5) return fail(MOONS_ERR_ARG, "moons_mesh_create: need sn >= 5 (N >= 2) per direction");
+  const double *h[3] = {hx, hy, hz};
+  const int sn[3] = {sx, sy, sz};
+  for (int d = 0; d < 3; ++d)
+    for (int i = 0; i + 1 < sn[d]; ++i)
+      if (!(h[d][i + 1] > h[d][i])) return fail(MOONS_ERR_ARG, "moons_mesh_create: node coordinates must increase");
+  moons_mesh_s *m = new moons_mesh_s();
+  size_t total = 0;
+  for (int d = 0; d < 3; ++d) {
+    coords_init(m->hc[d], h[d], sn[d]);
+    const HostCoords &c = m->hc[d];
+    total += c.hn.size() + c.hc.size() + c.dhn.size() + c.dhc.size() + c.cc2n_D.size() * 2 + c.n2cc_D.size() * 2 +
+             c.theta_D.size() * 2 + 3 * (c.colCC_L[0].size() * 2 + c.colN_L[0].size() * 2) + 2 * c.lapA.size();
+  }
+  if (hipMalloc(&m->dbuf, total * sizeof(double)) != hipSuccess) {
+    delete m;
+    return fail(MOONS_ERR_HIP, "moons_mesh_create: hipMalloc failed");
+  }
+  std::vector<double> host(total);
+  size_t off = 0;
+  auto put = [&](const std::vector<double> &v) -> const double * {
+    const double *p = m->dbuf + off;
+    std::memcpy(host.data() + off, v.data(), v.size() * sizeof(double));
+    off += v.size();
+    return p;
+  };
+  for (int d = 0; d < 3; ++d) {
+    HostCoords &c = m->hc[d];
+    DirCoefDev &dc = m->dev.c[d];
+    m->dev.sn[d] = c.sn;
+    m->dev.sc[d] = c.sc;
+    dc.hn = put(c.hn); dc.hc = put(c.hc); dc.dhn = put(c.dhn); dc.dhc = put(c.dhc);
+    dc.cc2n_D = put(c.cc2n_D); dc.cc2n_U = put(c.cc2n_U);
+    dc.n2cc_D = put(c.n2cc_D); dc.n2cc_U = put(c.n2cc_U);
+    dc.theta_D = put(c.theta_D); dc.theta_U = put(c.theta_U);
+    for (int n = 0; n < 2; ++n) {
+      dc.colCC_L[n] = put(c.colCC_L[n]); dc.colCC_D[n] = put(c.colCC_D[n]); dc.colCC_U[n] = put(c.colCC_U[n]);
+      dc.colN_L[n] = put(c.colN_L[n]); dc.colN_D[n] = put(c.colN_D[n]); dc.colN_U[n] = put(c.colN_U[n]);
+    }
+    dc.lapA = put(c.lapA); dc.lapB = put(c.lapB);
+  }
+  if (hipMemcpy(m->dbuf, host.data(), total * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(m->dbuf);
+    delete m;
+    return fail(MOONS_ERR_HIP, "moons_mesh_create: hipMemcpy failed");
+  }
+  *out = m;
+  return MOONS_OK;
+}
+
+int moons_mesh_destroy(moons_mesh_t m) {
+  if (!m) return MOONS_OK;
+  if (g_stream) (void)hipStreamSynchronize(g_stream);
+  if (m->dbuf) MOONS_HIP(hipFree(m->dbuf));
+  delete m;
+  return MOONS_OK;
+}
+
+int moons_mesh_coords(moons_mesh_t m, int dir, int which, double *out, int *n) {
+  if (!m || dir < 0 || dir > 2 || which < 0 || which > 3) return fail(MOONS_ERR_ARG, "moons_mesh_coords: bad argument");
+  const HostCoords &c = m->hc[dir];
+  const std::vector<double> *v = which == 0 ? &c.hn : which == 1 ? &c.hc : which == 2 ? &c.dhn : &c.dhc;
+  if (n) *n = (int)v->size();
+  if (out) std::memcpy(out, v->data(), v->size() * sizeof(double));
+  return MOONS_OK;
+}
+
+int moons_field_create(moons_mesh_t m, int dl, moons_field_t *out) {
+  if (!m || !out || dl < 0 || dl > 7) return fail(MOONS_ERR_ARG, "moons_field_create: bad argument");
+  moons_field_s *f = new moons_field_s();
+  f->m = m;
+  f->dl = dl;
+  for (int d = 0; d < 3; ++d) f->s[d] = ((dl >> d) & 1) ? m->hc[d].sn : m->hc[d].sc;
+  f->n = (size_t)f->s[0] * f->s[1] * f->s[2];
+  if (hipMalloc(&f->d, f->n * sizeof(double)) != hipSuccess) {
+    delete f;
+    return fail(MOONS_ERR_HIP, "moons_field_create: hipMalloc failed");
+  }
+  if (hipMemsetAsync(f->d, 0, f->n * sizeof(double), g_stream) != hipSuccess) {
+    (void)hipFree(f->d);
+    delete f;
+    return fail(MOONS_ERR_HIP, "moons_field_create: hipMemsetAsync failed");
+  }
+  *out = f;
+  return MOONS_OK;
+}
+
+int moons_field_destroy(moons_field_t f) {
+  if (!f) return MOONS_OK;
+  if (g_stream) (void)hipStreamSynchronize(g_stream);
+  if (f->d) MOONS_HIP(hipFree(f->d));
+  delete f;
+  return MOONS_OK;
+}
+
+int moons_field_shape(moons_field_t f, int s[3]) {
+  if (!f || !s) return fail(MOONS_ERR_ARG, "moons_field_shape: null argument");
+  for (int d = 0; d < 3; ++d) s[d] = f->s[d];
+  return MOONS_OK;
+}
+
+int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6], int all_neumann) {
+  if (!f || !type) return fail(MOONS_ERR_ARG, "moons_field_set_bcs: null argument");
+  for (int face = 0; face < 6; ++face) {
+    if (type[face] < MOONS_BC_NONE || type[face] > MOONS_BC_ANTISYMMETRIC)
+      return fail(MOONS_ERR_ARG, "moons_field_set_bcs: unknown BC type");
+  }
+  for (int d = 0; d < 3; ++d) {
+    bool p0 = type[2 * d] == MOONS_BC_PERIODIC, p1 = type[2 * d + 1] == MOONS_BC_PERIODIC;
+    if (p0 != p1) return fail(MOONS_ERR_ARG, "moons_field_set_bcs: periodic BCs must pair opposite faces");
+  }
+  f->has_bcs = true;
+  f->all_neumann = all_neumann != 0;
+  for (int face = 0; face < 6; ++face) {
+    f->bct[face] = type[face];
+    f->bval[face] = val ? val[face] : 0.0;
+  }
+  return MOONS_OK;
+}
+
+int moons_field_upload(moons_field_t f, const double *host, size_t n) {
+  if (!f || !host || n != f->n) return fail(MOONS_ERR_ARG, "moons_field_upload: size mismatch");
+  MOONS_HIP(hipMemcpyAsync(f->d, host, n * sizeof(double), hipMemcpyHostToDevice, g_stream));
+  MOONS_HIP(hipStreamSynchronize(g_stream));
+  return MOONS_OK;
+}
+
+int moons_field_download(moons_field_t f, double *host, size_t n) {
+  if (!f || !host || n != f->n) return fail(MOONS_ERR_ARG, "moons_field_download: size mismatch");
+  MOONS_HIP(hipMemcpyAsync(host, f->d, n * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  MOONS_HIP(hipStreamSynchronize(g_stream));
+  return MOONS_OK;
+}
+
+int moons_field_fill(moons_field_t f, double val) {
+  if (!f) return fail(MOONS_ERR_ARG, "moons_field_fill: null argument");
+  hipLaunchKernelGGL(k_fill, dim3(nblk(f->n)), dim3(256), 0, g_stream, f->d, f->n, val);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int moons_field_copy(moons_field_t dst, moons_field_t src) {
+  if (!dst || !src || dst->n != src->n || dst->dl != src->dl) return fail(MOONS_ERR_ARG, "moons_field_copy: shape mismatch");
+  MOONS_HIP(hipMemcpyAsync(dst->d, src->d, dst->n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  return MOONS_OK;
+}
+
+int moons_field_device_ptr(moons_field_t f, void **dptr) {
+  if (!f || !dptr) return fail(MOONS_ERR_ARG, "moons_field_device_ptr: null argument");
+  *dptr = f->d;
+  return MOONS_OK;
+}
+
+int moons_apply_bcs(moons_field_t u) {
+  if (!u) return fail(MOONS_ERR_ARG, "moons_apply_bcs: null argument");
+  return launch_apply_bcs(u, false);
+}
+
+int moons_apply_bcs_implicit(moons_field_t u) {
+  if (!u) return fail(MOONS_ERR_ARG, "moons_apply_bcs_implicit: null argument");
+  return launch_apply_bcs(u, true);
+}
+
+int moons_del(moons_field_t dfdh, moons_field_t f, int n, int dir, int pad, int gen) {
+  if (!dfdh || !f) return fail(MOONS_ERR_ARG, "moons_del: null argument");
+  return launch_del(dfdh, f, n, dir, pad, gen);
+}
+
+int moons_grad(moons_field_t gx, moons_field_t gy, moons_field_t gz, moons_field_t u) {
+  moons_field_t g[3] = {gx, gy, gz};
+  if (!u) return fail(MOONS_ERR_ARG, "moons_grad: null argument");
+  for (int d = 0; d < 3; ++d) {
+    if (!g[d]) continue;
+    if (g[d]->dl != (u->dl ^ (1 << d))) return fail(MOONS_ERR_ARG, "moons_grad: output location mismatch");
+    int st = launch_del(g[d], u, 1, d, 0, 1);
+    if (st) return st;
+  }
+  return MOONS_OK;
+}
+
+int moons_div(moons_field_t divU, moons_field_t u, moons_field_t v, moons_field_t w) {
+  if (!divU || !u || !v || !w) return fail(MOONS_ERR_ARG, "moons_div: null argument");
+  if (u->dl != (divU->dl ^ 1) || v->dl != (divU->dl ^ 2) || w->dl != (divU->dl ^ 4))
+    return fail(MOONS_ERR_ARG, "moons_div: component locations do not match the output");
+  int st;
+  if ((st = launch_del(divU, u, 1, 0, 0, 1))) return st;
+  if ((st = launch_del(divU, v, 1, 1, 0, 2))) return st;
+  if ((st = launch_del(divU, w, 1, 2, 0, 2))) return st;
+  return launch_assign_ghost_xperiodic(divU, 0.0);
+}
+
+int moons_curl(moons_field_t curlU, moons_field_t u, moons_field_t v, moons_field_t w, int dir) {
+  if (!curlU || !u || !v || !w || dir < 0 || dir > 2) return fail(MOONS_ERR_ARG, "moons_curl: bad argument");
+  int st;
+  switch (dir) {
+    case 0:
+      if ((st = launch_del(curlU, w, 1, 1, 0, 1))) return st;
+      return launch_del(curlU, v, 1, 2, 0, 3);
+    case 1:
+      if ((st = launch_del(curlU, u, 1, 2, 0, 1))) return st;
+      return launch_del(curlU, w, 1, 0, 0, 3);
+    default:
+      if ((st = launch_del(curlU, v, 1, 0, 0, 1))) return st;
+      return launch_del(curlU, u, 1, 1, 0, 3);
+  }
+}
+
+int moons_volume(moons_field_t vol) {
+  if (!vol) return fail(MOONS_ERR_ARG, "moons_volume: null argument");
+  Shape3 S{{vol->s[0], vol->s[1], vol->s[2]}};
+  const MeshDev &M = vol->m->dev;
+  int e[3];
+  const double *dh[3];
+  for (int d = 0; d < 3; ++d) {
+    e[d] = (vol->dl >> d) & 1;
+    dh[d] = e[d] ? M.c[d].dhc : M.c[d].dhn;
+  }
+  hipLaunchKernelGGL(k_volume, dim3(nblk(vol->n)), dim3(256), 0, g_stream, vol->d, S, e[0], e[1], e[2], dh[0], dh[1],
+                     dh[2]);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,796 @@
+/*
+ * moons_oracle.c -- CPU FP64 restatement of the MOONS per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never part of it.
+ *
+ * Every function below cites the reference file:line it restates (paths are
+ * relative to the reference root, code/handwritten abbreviated as hw/).  The
+ * arithmetic follows the reference's expression order term by term, so that
+ * element-wise results of a GPU build compiled without FP contraction agree
+ * bit for bit; only global reductions are summed in a different order.
+ *
+ * Parity pins (what checks each part; see tests/test_oracle_*.py):
+ *   grids (uniform, Roberts)       closed forms + compiled reference
+ *                                   coordinate_distribution_funcs.f90 (oracle/_ref)
+ *   stencil coefficients           closed forms (+-1/dh, theta = 1/2 on uniform grids,
+ *                                   exactness on quadratics) + compiled reference
+ *                                   derivative_stencils.f90 / interpolation_stencils.f90
+ *   pencil kernels (del)           compiled reference stencils.f90 (oracle/_ref)
+ *   BC ghost rules                 compiled reference apply_BCs_faces_raw*.f90 (oracle/_ref)
+ *   grad/div/curl/lap_centered     manufactured polynomials, div(curl)=0, curl(grad)=0,
+ *                                   symmetry of vol*A, dense assembly by unit vectors
+ *   Lap diagonal / Jacobi Minv     dense-matrix diagonal away from wall rows
+ *   PCG (PPE)                      dense numpy solve of the assembled system on
+ *                                   4^3-8^3; residual-drop to 1e-12; textbook PCG
+ *                                   recurrence on the assembled matrix
+ */
+#include "moons_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(s, i, j, k) ((size_t)(i) + (size_t)(s)[0] * ((size_t)(j) + (size_t)(s)[1] * (size_t)(k)))
+
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+/* ------------------------------------------------------------------------- */
+/* 1D grids                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Ghost nodes mirror the first/last spacing: hw/grid/coordinates_extend.f90:529-538 */
+static void add_ghost_nodes(const double *hn_phys, int np, double *out) {
+  double dh0 = hn_phys[1] - hn_phys[0];
+  double dhe = hn_phys[np - 1] - hn_phys[np - 2];
+  out[0] = hn_phys[0] - dh0;
+  memcpy(out + 1, hn_phys, sizeof(double) * (size_t)np);
+  out[np + 1] = hn_phys[np - 1] + dhe;
+}
+
+/* hw/grid/coordinate_distribution_funcs.f90:34-61 (uniform_func) then ghost nodes */
+int orc_uniform_nodes(double hmin, double hmax, int N, double *hn_out) {
+  if (N < 1) return -1;
+  double *h = dalloc((size_t)N + 1);
+  double dh = (hmax - hmin) / (double)N;
+  for (int i = 0; i <= N; ++i) h[i] = hmin + (double)i * dh;
+  add_ghost_nodes(h, N + 1, hn_out);
+  free(h);
+  return 0;
+}
+
+/* Tannehill transformation 2: hw/grid/coordinate_distribution_funcs.f90:192-248 */
+static void transformation2(double hmin, double hmax, int N, double alpha, double beta, double *hn) {
+  const double one = 1.0, two = 2.0;
+  double a = alpha, b = beta;
+  double g = (b + one) / (b - one);
+  double dh = (hmax - hmin) / (double)N;
+  for (int i = 0; i <= N; ++i) {
+    double hb = hmin + (double)i * dh;
+    hb = (hb - hmin) / (hmax - hmin);
+    double e = pow(g, (hb - a) / (one - a));
+    double v = ((b + two * a) * e - b + two * a) / ((two * a + one) * (one + e));
+    hn[i] = v;
+  }
+  for (int i = 0; i <= N; ++i) hn[i] = hmin + (hmax - hmin) * hn[i];
+}
+
+/* Tannehill transformation 1: hw/grid/coordinate_distribution_funcs.f90:143-177 */
+static void transformation1(double hmin, double hmax, int N, double beta, double *hn) {
+  double a = 1.0, b = beta;
+  double g = (b + 1.0) / (b - 1.0);
+  double dh = (hmax - hmin) / (double)N;
+  for (int i = 0; i <= N; ++i) {
+    double hb = hmin + (double)i * dh;
+    hb = (hb - hmin) / (hmax - hmin);
+    double e = pow(g, a - hb);
+    hn[i] = ((b + a) - (b - a) * e) / (e + a);
+  }
+  for (int i = 0; i <= N; ++i) hn[i] = hmin + (hmax - hmin) * hn[i];
+}
+
+/* robertsBoth_func (alpha = 1/2): hw/grid/coordinate_distribution_funcs.f90:325-331;
+   grid_Roberts_B adds ghost nodes: hw/grid/grid_init.f90:60-68 */
+int orc_roberts_both_nodes(double hmin, double hmax, int N, double beta, double *hn_out) {
+  if (N < 1 || !(beta > 1.0)) return -1;
+  double *h = dalloc((size_t)N + 1);
+  transformation2(hmin, hmax, N, 0.5, beta, h);
+  add_ghost_nodes(h, N + 1, hn_out);
+  free(h);
+  return 0;
+}
+int orc_roberts_right_nodes(double hmin, double hmax, int N, double beta, double *hn_out) {
+  if (N < 1 || !(beta > 1.0)) return -1;
+  double *h = dalloc((size_t)N + 1);
+  transformation2(hmin, hmax, N, 0.0, beta, h);
+  add_ghost_nodes(h, N + 1, hn_out);
+  free(h);
+  return 0;
+}
+int orc_roberts_left_nodes(double hmin, double hmax, int N, double beta, double *hn_out) {
+  if (N < 1 || !(beta > 1.0)) return -1;
+  double *h = dalloc((size_t)N + 1);
+  transformation1(hmin, hmax, N, beta, h);
+  add_ghost_nodes(h, N + 1, hn_out);
+  free(h);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* coordinates: hw/grid/coordinates_extend.f90:78-173 (init_c), :506-527      */
+/* ------------------------------------------------------------------------- */
+
+/* collocated first derivative on nodes: hw/grid/derivative_stencils.f90:79-107 */
+static void col_node_1(const double *dh, int s, double *L, double *D, double *U) {
+#define DH(i) dh[(i)-1]
+#define SL(k) L[(k)-1]
+#define SD(k) D[(k)-1]
+#define SU(k) U[(k)-1]
+  int i;
+  if (s > 3) {
+    i = 2;
+    SL(1) = -(DH(i + 1) + 2.0 * DH(i)) / (DH(i) * (DH(i + 1) + DH(i)));
+    SD(1) = (DH(i + 1) + DH(i)) / (DH(i + 1) * DH(i));
+    SU(1) = -DH(i) / (DH(i + 1) * (DH(i + 1) + DH(i)));
+    for (i = 3; i <= s - 2; ++i) {
+      SL(i - 1) = -(DH(i) / (DH(i - 1) * (DH(i - 1) + DH(i))));
+      SD(i - 1) = ((-DH(i - 1) + DH(i)) / (DH(i - 1) * DH(i)));
+      SU(i - 1) = (DH(i - 1) / (DH(i) * (DH(i - 1) + DH(i))));
+    }
+    i = s - 1;
+    SL(s - 2) = DH(i - 1) / (DH(i - 2) * (DH(i - 1) + DH(i - 2)));
+    SD(s - 2) = -(DH(i - 1) + DH(i - 2)) / (DH(i - 1) * DH(i - 2));
+    SU(s - 2) = (2.0 * DH(i - 1) + DH(i - 2)) / (DH(i - 1) * (DH(i - 1) + DH(i - 2)));
+  }
+}
+
+/* collocated second derivative on nodes: hw/grid/derivative_stencils.f90:109-137 */
+static void col_node_2(const double *dh, int s, double *L, double *D, double *U) {
+  int i;
+  if (s > 3) {
+    i = 2;
+    SL(1) = 2.0 / (DH(i) * (DH(i + 1) + DH(i)));
+    SD(1) = -2.0 / (DH(i + 1) * DH(i));
+    SU(1) = 2.0 / (DH(i + 1) * (DH(i + 1) + DH(i)));
+    for (i = 3; i <= s - 2; ++i) {
+      SL(i - 1) = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i)));
+      SD(i - 1) = -2.0 / (DH(i - 1) * DH(i));
+      SU(i - 1) = 2.0 / (DH(i) * (DH(i - 1) + DH(i)));
+    }
+    i = s - 1;
+    SL(s - 2) = 2.0 / (DH(i - 2) * (DH(i - 1) + DH(i - 2)));
+    SD(s - 2) = -2.0 / (DH(i - 1) * DH(i - 2));
+    SU(s - 2) = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i - 2)));
+  }
+}
+
+/* collocated first derivative on cell centres: hw/grid/derivative_stencils.f90:139-170 */
+static void col_cc_1(const double *dh, int s, double *L, double *D, double *U) {
+  int i;
+  if (s > 2) {
+    i = 2;
+    SL(1) = (-(DH(i)) / ((0.5 * DH(i - 1)) * ((0.5 * DH(i - 1)) + DH(i))));
+    SD(1) = ((-(0.5 * DH(i - 1)) + DH(i)) / ((0.5 * DH(i - 1)) * DH(i)));
+    SU(1) = ((0.5 * DH(i - 1)) / (DH(i) * ((0.5 * DH(i - 1)) + DH(i))));
+    for (i = 3; i <= s - 2; ++i) {
+      SL(i - 1) = (-DH(i) / (DH(i - 1) * (DH(i - 1) + DH(i))));
+      SD(i - 1) = ((-DH(i - 1) + DH(i)) / (DH(i - 1) * DH(i)));
+      SU(i - 1) = (DH(i - 1) / (DH(i) * (DH(i - 1) + DH(i))));
+    }
+    i = s - 1;
+    SL(s - 2) = (-0.5 * DH(i) / (DH(i - 1) * (DH(i - 1) + (0.5 * DH(i)))));
+    SD(s - 2) = ((-DH(i - 1) + (0.5 * DH(i))) / (DH(i - 1) * (0.5 * DH(i))));
+    SU(s - 2) = (DH(i - 1) / ((0.5 * DH(i)) * (DH(i - 1) + (0.5 * DH(i)))));
+  }
+}
+
+/* collocated second derivative on cell centres: hw/grid/derivative_stencils.f90:172-208 */
+static void col_cc_2(const double *dh, int s, double *L, double *D, double *U) {
+  int i;
+  if (s > 2) {
+    i = 2;
+    SL(1) = 2.0 / ((0.5 * DH(i - 1)) * ((0.5 * DH(i - 1)) + DH(i)));
+    SD(1) = -2.0 / ((0.5 * DH(i - 1)) * DH(i));
+    SU(1) = 2.0 / (DH(i) * ((0.5 * DH(i - 1)) + DH(i)));
+    for (i = 3; i <= s - 2; ++i) {
+      SL(i - 1) = 2.0 / (DH(i - 1) * (DH(i - 1) + DH(i)));
+      SD(i - 1) = -(2.0 / (DH(i - 1) * DH(i)));
+      SU(i - 1) = 2.0 / (DH(i) * (DH(i - 1) + DH(i)));
+    }
+    i = s - 1;
+    SL(s - 2) = 2.0 / (DH(i - 1) * (DH(i - 1) + (0.5 * DH(i))));
+    SD(s - 2) = -2.0 / (DH(i - 1) * (0.5 * DH(i)));
+    SU(s - 2) = 2.0 / ((0.5 * DH(i)) * (DH(i - 1) + (0.5 * DH(i))));
+  }
+#undef DH
+#undef SL
+#undef SD
+#undef SU
+}
+
+static void coords_init(orc_coords *c, const double *hn, int sn) {
+  memset(c, 0, sizeof(*c));
+  c->sn = sn;
+  c->sc = sn - 1;
+  int sc = c->sc;
+  c->hn = dalloc((size_t)sn);
+  c->hc = dalloc((size_t)sc);
+  c->dhn = dalloc((size_t)sn - 1);
+  c->dhc = dalloc((size_t)sc - 1);
+  memcpy(c->hn, hn, sizeof(double) * (size_t)sn);
+  for (int i = 0; i < sn - 1; ++i) c->dhn[i] = hn[i + 1] - hn[i];
+  for (int i = 0; i < sc; ++i) c->hc[i] = (hn[i + 1] + hn[i]) / 2.0;
+  for (int i = 0; i < sc - 1; ++i) c->dhc[i] = c->hc[i + 1] - c->hc[i];
+  /* interpolation_stencil: hw/grid/interpolation_stencils.f90:12-35 */
+  c->theta_D = dalloc((size_t)sc - 1);
+  c->theta_U = dalloc((size_t)sc - 1);
+  for (int i = 0; i < sc - 1; ++i) {
+    c->theta_D[i] = (c->hn[i + 1] - c->hc[i]) / (c->hc[i + 1] - c->hc[i]);
+    c->theta_U[i] = 1.0 - c->theta_D[i];
+  }
+  /* staggered_CC2N / staggered_N2CC: hw/grid/derivative_stencils.f90:31-73 */
+  c->cc2n_D = dalloc((size_t)sc - 1);
+  c->cc2n_U = dalloc((size_t)sc - 1);
+  for (int i = 0; i < sc - 1; ++i) {
+    c->cc2n_D[i] = -(1.0 / c->dhc[i]);
+    c->cc2n_U[i] = 1.0 / c->dhc[i];
+  }
+  c->n2cc_D = dalloc((size_t)sn - 1);
+  c->n2cc_U = dalloc((size_t)sn - 1);
+  for (int i = 0; i < sn - 1; ++i) {
+    c->n2cc_D[i] = -(1.0 / c->dhn[i]);
+    c->n2cc_U[i] = 1.0 / c->dhn[i];
+  }
+  for (int n = 0; n < 2; ++n) {
+    c->colCC_L[n] = dalloc((size_t)sc - 2);
+    c->colCC_D[n] = dalloc((size_t)sc - 2);
+    c->colCC_U[n] = dalloc((size_t)sc - 2);
+    c->colN_L[n] = dalloc((size_t)sn - 2);
+    c->colN_D[n] = dalloc((size_t)sn - 2);
+    c->colN_U[n] = dalloc((size_t)sn - 2);
+  }
+  col_cc_1(c->dhc, sc, c->colCC_L[0], c->colCC_D[0], c->colCC_U[0]);
+  col_cc_2(c->dhc, sc, c->colCC_L[1], c->colCC_D[1], c->colCC_U[1]);
+  col_node_1(c->dhn, sn, c->colN_L[0], c->colN_D[0], c->colN_U[0]);
+  col_node_2(c->dhn, sn, c->colN_L[1], c->colN_D[1], c->colN_U[1]);
+}
+
+static void coords_free(orc_coords *c) {
+  double **ps[] = {&c->hn, &c->hc, &c->dhn, &c->dhc, &c->cc2n_D, &c->cc2n_U, &c->n2cc_D,
+                   &c->n2cc_U, &c->theta_D, &c->theta_U};
+  for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(*ps[i]);
+  for (int n = 0; n < 2; ++n) {
+    free(c->colCC_L[n]); free(c->colCC_D[n]); free(c->colCC_U[n]);
+    free(c->colN_L[n]); free(c->colN_D[n]); free(c->colN_U[n]);
+  }
+}
+
+orc_grid *orc_grid_create(const double *hx, int sx, const double *hy, int sy, const double *hz, int sz) {
+  if (sx < 4 || sy < 4 || sz < 4) return NULL;
+  orc_grid *g = (orc_grid *)calloc(1, sizeof(orc_grid));
+  coords_init(&g->c[0], hx, sx);
+  coords_init(&g->c[1], hy, sy);
+  coords_init(&g->c[2], hz, sz);
+  return g;
+}
+
+void orc_grid_destroy(orc_grid *g) {
+  if (!g) return;
+  for (int d = 0; d < 3; ++d) coords_free(&g->c[d]);
+  free(g);
+}
+
+void orc_field_shape(const orc_grid *g, int dl, int s[3]) {
+  for (int d = 0; d < 3; ++d) s[d] = ((dl >> d) & 1) ? g->c[d].sn : g->c[d].sc;
+}
+
+size_t orc_field_size(const orc_grid *g, int dl) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  return (size_t)s[0] * (size_t)s[1] * (size_t)s[2];
+}
+
+int orc_coord_array(const orc_grid *g, int dir, int which, double *out) {
+  const orc_coords *c = &g->c[dir];
+  const double *src = NULL;
+  int n = 0;
+  switch (which) {
+    case 0: src = c->hn; n = c->sn; break;
+    case 1: src = c->hc; n = c->sc; break;
+    case 2: src = c->dhn; n = c->sn - 1; break;
+    case 3: src = c->dhc; n = c->sc - 1; break;
+    case 4: src = c->cc2n_D; n = c->sc - 1; break;
+    case 5: src = c->cc2n_U; n = c->sc - 1; break;
+    case 6: src = c->n2cc_D; n = c->sn - 1; break;
+    case 7: src = c->n2cc_U; n = c->sn - 1; break;
+    case 8: src = c->theta_D; n = c->sc - 1; break;
+    case 9: src = c->theta_U; n = c->sc - 1; break;
+    default:
+      if (which >= 10 && which < 16) {
+        int nn = (which - 10) / 3, t = (which - 10) % 3;
+        src = t == 0 ? c->colCC_L[nn] : t == 1 ? c->colCC_D[nn] : c->colCC_U[nn];
+        n = c->sc - 2;
+      } else if (which >= 16 && which < 22) {
+        int nn = (which - 16) / 3, t = (which - 16) % 3;
+        src = t == 0 ? c->colN_L[nn] : t == 1 ? c->colN_D[nn] : c->colN_U[nn];
+        n = c->sn - 2;
+      } else {
+        return -1;
+      }
+  }
+  if (out) memcpy(out, src, sizeof(double) * (size_t)n);
+  return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* pencil kernels: hw/stencils/stencils.f90                                  */
+/* ------------------------------------------------------------------------- */
+
+/* stag_assign/add/subtract (stencils.f90:47-59,108-120,169-181).  gen 1/2/3. */
+static void pencil_stag(double *out, size_t os, int sd, const double *f, size_t fs, int s,
+                        const double *D, const double *U, int gt, int gen) {
+  out[0] = 0.0;
+  out[(size_t)(sd - 1) * os] = 0.0;
+  for (int i = 0; i < s - 1; ++i) {
+    double a = f[(size_t)i * fs], b = f[(size_t)(i + 1) * fs];
+    double *o = &out[(size_t)(i + gt) * os];
+    if (gen == 1) *o = a * D[i] + b * U[i];
+    else if (gen == 2) *o = *o + a * D[i] + b * U[i];
+    else *o = *o - (a * D[i] + b * U[i]);
+  }
+}
+
+/* col_CC_assign/add/subtract with pad1 = pad2 = 0 (stencils.f90:60-84,121-145,182-206) */
+static void pencil_col_cc(double *out, size_t os, const double *f, size_t fs, int s,
+                          const double *L, const double *D, const double *U, int gen) {
+#define F(i) f[(size_t)(i) * fs]
+#define O(i) out[(size_t)(i) * os]
+  for (int i = 2; i <= s - 3; ++i) {
+    double t = F(i - 1) * L[i - 1];
+    if (gen == 1) O(i) = t + F(i) * D[i - 1] + F(i + 1) * U[i - 1];
+    else if (gen == 2) O(i) = O(i) + t + F(i) * D[i - 1] + F(i + 1) * U[i - 1];
+    else O(i) = O(i) - (t + F(i) * D[i - 1] + F(i + 1) * U[i - 1]);
+  }
+  {
+    /* near-wall rows: 0.5*(f(1)+f(2))*L(1)*(1-pad1) + f(1)*L(1)*pad1 + f(2)*D(1) + f(3)*U(1) */
+    double t1 = 0.5 * (F(0) + F(1)) * L[0] * 1.0;
+    double t2 = F(0) * L[0] * 0.0;
+    double v = t1 + t2 + F(1) * D[0] + F(2) * U[0];
+    double t3 = 0.5 * (F(s - 2) + F(s - 1)) * U[s - 3] * 1.0;
+    double t4 = F(s - 1) * U[s - 3] * 0.0;
+    double w = F(s - 3) * L[s - 3] + F(s - 2) * D[s - 3] + t3 + t4;
+    if (gen == 1) { O(1) = v; O(s - 2) = w; }
+    else if (gen == 2) {
+      O(1) = O(1) + t1 + t2 + F(1) * D[0] + F(2) * U[0];
+      O(s - 2) = O(s - 2) + F(s - 3) * L[s - 3] + F(s - 2) * D[s - 3] + t3 + t4;
+    } else { O(1) = O(1) - v; O(s - 2) = O(s - 2) - w; }
+  }
+  O(0) = 0.0;
+  O(s - 1) = 0.0;
+}
+
+/* col_N_assign/add/subtract with pad1 = pad2 = 0 (stencils.f90:85-106,146-167,207-228) */
+static void pencil_col_n(double *out, size_t os, const double *f, size_t fs, int s,
+                         const double *L, const double *D, const double *U, int gen) {
+  for (int i = 2; i <= s - 3; ++i) {
+    double t = F(i - 1) * L[i - 1];
+    if (gen == 1) O(i) = t + F(i) * D[i - 1] + F(i + 1) * U[i - 1];
+    else if (gen == 2) O(i) = O(i) + t + F(i) * D[i - 1] + F(i + 1) * U[i - 1];
+    else O(i) = O(i) - (t + F(i) * D[i - 1] + F(i + 1) * U[i - 1]);
+  }
+  double v = F(1) * L[0] + F(2) * D[0] + F(3) * U[0];
+  double w = F(s - 4) * L[s - 3] + F(s - 3) * D[s - 3] + F(s - 2) * U[s - 3];
+  if (gen == 1) { O(1) = v; O(s - 2) = w; }
+  else if (gen == 2) {
+    O(1) = O(1) + F(1) * L[0] + F(2) * D[0] + F(3) * U[0];
+    O(s - 2) = O(s - 2) + F(s - 4) * L[s - 3] + F(s - 3) * D[s - 3] + F(s - 2) * U[s - 3];
+  } else { O(1) = O(1) - v; O(s - 2) = O(s - 2) - w; }
+  O(0) = 0.0;
+  O(s - 1) = 0.0;
+#undef F
+#undef O
+}
+
+static void zero_plane(double *u, const int s[3], int d, int idx, double val) {
+  int a = (d + 1) % 3, b = (d + 2) % 3;
+  int ii[3];
+  for (int jb = 0; jb < s[b]; ++jb)
+    for (int ja = 0; ja < s[a]; ++ja) {
+      ii[d] = idx; ii[a] = ja; ii[b] = jb;
+      u[IDX(s, ii[0], ii[1], ii[2])] = val;
+    }
+}
+
+/* diff / diff_tree_search / diff_stag / diff_col: hw/ops/ops_del.f90:59-230 */
+void orc_del(double *dfdh, int dl_out, const double *f, int dl_in, const orc_grid *g, int n,
+             int dir, int pad, int gen) {
+  int sf[3], so[3];
+  orc_field_shape(g, dl_in, sf);
+  orc_field_shape(g, dl_out, so);
+  const orc_coords *c = &g->c[dir];
+  int nin = (dl_in >> dir) & 1, nout = (dl_out >> dir) & 1;
+  size_t strf[3] = {1, (size_t)sf[0], (size_t)sf[0] * sf[1]};
+  size_t stro[3] = {1, (size_t)so[0], (size_t)so[0] * so[1]};
+  int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  int lo = (a < b) ? a : b, hi = (a < b) ? b : a; /* reference loops the outer dims in k,j / k,i / j,i */
+  for (int q = pad; q < sf[hi] - pad; ++q)
+    for (int p = pad; p < sf[lo] - pad; ++p) {
+      size_t offf = (size_t)p * strf[lo] + (size_t)q * strf[hi];
+      size_t offo = (size_t)p * stro[lo] + (size_t)q * stro[hi];
+      if (!nin && nout)
+        pencil_stag(dfdh + offo, stro[dir], so[dir], f + offf, strf[dir], sf[dir], c->cc2n_D, c->cc2n_U, 1, gen);
+      else if (nin && !nout)
+        pencil_stag(dfdh + offo, stro[dir], so[dir], f + offf, strf[dir], sf[dir], c->n2cc_D, c->n2cc_U, 0, gen);
+      else if (!nin && !nout)
+        pencil_col_cc(dfdh + offo, stro[dir], f + offf, strf[dir], sf[dir], c->colCC_L[n - 1], c->colCC_D[n - 1], c->colCC_U[n - 1], gen);
+      else
+        pencil_col_n(dfdh + offo, stro[dir], f + offf, strf[dir], sf[dir], c->colN_L[n - 1], c->colN_D[n - 1], c->colN_U[n - 1], gen);
+    }
+  if (gen == 1 && pad > 0) {
+    zero_plane(dfdh, so, a, 0, 0.0); zero_plane(dfdh, so, a, so[a] - 1, 0.0);
+    zero_plane(dfdh, so, b, 0, 0.0); zero_plane(dfdh, so, b, so[b] - 1, 0.0);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* BCs                                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* Ghost/boundary/first-interior plane indices per face, hw/sub_domain/face_SD_extend.f90:37-90 */
+static void face_planes(int nalong, int s, int is_max, int *G, int *B, int *I, int *Iopp) {
+  if (!nalong) {
+    *G = is_max ? s - 1 : 0; *B = -1; *I = is_max ? s - 2 : 1; *Iopp = is_max ? 1 : s - 2;
+  } else {
+    *G = is_max ? s - 1 : 0; *B = is_max ? s - 2 : 1; *I = is_max ? s - 3 : 2; *Iopp = is_max ? 2 : s - 3;
+  }
+}
+
+/* apply_BCs_faces (hw/BCs/apply/apply_BCs.f90:42-64) over raw kernels
+   hw/BCs/direct/apply_BCs_faces_raw.f90:29-283 and
+   hw/BCs/implicit/apply_BCs_faces_raw_implicit.f90:27-257.  Faces are applied
+   in order 1..6 (the reference runs them in an OpenMP loop; only edge/corner
+   ghosts, shared by two faces, depend on that order). */
+void orc_apply_bcs(double *u, int dl, const orc_grid *g, const orc_bcs *bc, int implicit_) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  for (int face = 0; face < 6; ++face) {
+    int t = bc->type[face];
+    if (t == ORC_BC_NONE) continue;
+    int d = face / 2, is_max = face % 2;
+    int nal = (dl >> d) & 1;
+    int G, B, I, Io;
+    face_planes(nal, s[d], is_max, &G, &B, &I, &Io);
+    double nhat = is_max ? 1.0 : -1.0;
+    const orc_coords *c = &g->c[d];
+    double dh = is_max ? c->dhn[c->sn - 2] : c->dhn[0];
+    double bv = implicit_ ? 0.0 : bc->val[face];
+    int Gp = G, Iop = Io;
+    if (nal && t == ORC_BC_PERIODIC && is_max && s[d] > 4) { Gp = B; Iop = 1; }
+    int a = (d + 1) % 3, bb = (d + 2) % 3;
+    int ii[3];
+    for (int jb = 0; jb < s[bb]; ++jb)
+      for (int ja = 0; ja < s[a]; ++ja) {
+        ii[a] = ja; ii[bb] = jb;
+        ii[d] = G; size_t ig = IDX(s, ii[0], ii[1], ii[2]);
+        ii[d] = I; size_t iin = IDX(s, ii[0], ii[1], ii[2]);
+        switch (t) {
+          case ORC_BC_DIRICHLET:
+            if (!nal) {
+              u[ig] = implicit_ ? -u[iin] : 2.0 * bv - u[iin];
+            } else {
+              ii[d] = B; size_t ib = IDX(s, ii[0], ii[1], ii[2]);
+              u[ib] = bv;
+              u[ig] = implicit_ ? -u[iin] : 2.0 * u[ib] - u[iin];
+            }
+            break;
+          case ORC_BC_NEUMANN:
+            if (implicit_) u[ig] = u[iin];
+            else if (!nal) u[ig] = u[iin] + bv * (dh * nhat);
+            else u[ig] = u[iin] + 2.0 * bv * (dh * nhat);
+            break;
+          case ORC_BC_PERIODIC: {
+            ii[d] = Gp; size_t igp = IDX(s, ii[0], ii[1], ii[2]);
+            ii[d] = Iop; size_t io = IDX(s, ii[0], ii[1], ii[2]);
+            u[igp] = u[io];
+          } break;
+          case ORC_BC_SYMMETRIC: u[ig] = u[iin]; break;
+          case ORC_BC_ANTISYMMETRIC: u[ig] = -u[iin]; break;
+          default: break;
+        }
+      }
+  }
+}
+
+/* assign_ghost_XPeriodic: hw/block_field/block_field_extend.f90:233-271.
+   bc == NULL means the field carries no BCs: all six ghost planes are assigned. */
+void orc_assign_ghost_xperiodic(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  for (int face = 0; face < 6; ++face) {
+    if (bc && bc->type[face] == ORC_BC_PERIODIC) continue;
+    int d = face / 2;
+    zero_plane(u, s, d, face % 2 ? s[d] - 1 : 0, val);
+  }
+}
+
+static void assign_ghost_n_xperiodic(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  if (!bc) return;
+  for (int face = 0; face < 6; ++face) {
+    if (bc->type[face] == ORC_BC_PERIODIC) continue;
+    int d = face / 2;
+    if (!((dl >> d) & 1)) continue;
+    zero_plane(u, s, d, face % 2 ? s[d] - 1 : 0, val);
+  }
+}
+
+/* set_assign_wall_Dirichlet: hw/block_field/block_field_extend.f90:273-308 */
+void orc_assign_wall_dirichlet(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  for (int face = 0; face < 6; ++face) {
+    int d = face / 2;
+    if (!((dl >> d) & 1)) continue;
+    if (bc && bc->type[face] != ORC_BC_DIRICHLET) continue;
+    zero_plane(u, s, d, face % 2 ? s[d] - 2 : 1, val);
+  }
+}
+
+/* set_multiply_wall_Neumann: hw/block_field/block_field_extend.f90:348-377 */
+void orc_multiply_wall_neumann(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  for (int face = 0; face < 6; ++face) {
+    int d = face / 2;
+    if (!((dl >> d) & 1)) continue;
+    if (bc && bc->type[face] != ORC_BC_NEUMANN) continue;
+    int idx = face % 2 ? s[d] - 2 : 1;
+    int a = (d + 1) % 3, b = (d + 2) % 3, ii[3];
+    for (int jb = 0; jb < s[b]; ++jb)
+      for (int ja = 0; ja < s[a]; ++ja) {
+        ii[d] = idx; ii[a] = ja; ii[b] = jb;
+        size_t q = IDX(s, ii[0], ii[1], ii[2]);
+        u[q] = u[q] * val;
+      }
+  }
+}
+
+/* set_assign_wall_Periodic_single: hw/block_field/block_field_extend.f90:310-346 */
+static void assign_wall_periodic_single(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  if (!bc) return;
+  for (int d = 0; d < 3; ++d) {
+    if (!((dl >> d) & 1)) continue;
+    if (bc->type[2 * d] != ORC_BC_PERIODIC || bc->type[2 * d + 1] != ORC_BC_PERIODIC) continue;
+    if (!(s[d] > 4)) continue;
+    zero_plane(u, s, d, s[d] - 2, val);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* discrete operators: hw/ops/ops_discrete.f90                               */
+/* ------------------------------------------------------------------------- */
+
+/* grad_SF: ops_discrete.f90:173-182 (pad 0) */
+void orc_grad(double *gx, double *gy, double *gz, const double *u, int dl_u, const orc_grid *g) {
+  double *out[3] = {gx, gy, gz};
+  for (int d = 0; d < 3; ++d)
+    if (out[d]) orc_del(out[d], dl_u ^ (1 << d), u, dl_u, g, 1, d, 0, 1);
+}
+
+/* div_SF: ops_discrete.f90:158-171.  dl_u is the location of u (x comp.);
+   components are u (N along x), v, w.  bc_out: BCs of divU (NULL = none). */
+void orc_div(double *divU, int dl_out, const double *u, const double *v, const double *w, int dl_u,
+             const orc_grid *g, const orc_bcs *bc_out) {
+  (void)dl_u;
+  orc_del(divU, dl_out, u, dl_out ^ 1, g, 1, 0, 0, 1);
+  orc_del(divU, dl_out, v, dl_out ^ 2, g, 1, 1, 0, 2);
+  orc_del(divU, dl_out, w, dl_out ^ 4, g, 1, 2, 0, 2);
+  orc_assign_ghost_xperiodic(divU, dl_out, g, bc_out, 0.0);
+}
+
+/* curl_SF: ops_discrete.f90:184-202.  Component dir of curl(u,v,w); the
+   inputs' locations follow from the output location by toggling. */
+void orc_curl_component(double *curlU, int dl_out, const double *u, const double *v, const double *w,
+                        int dl_in, const orc_grid *g, int dir) {
+  (void)dl_in;
+  switch (dir) {
+    case 0:
+      orc_del(curlU, dl_out, w, dl_out ^ 2, g, 1, 1, 0, 1);
+      orc_del(curlU, dl_out, v, dl_out ^ 4, g, 1, 2, 0, 3);
+      break;
+    case 1:
+      orc_del(curlU, dl_out, u, dl_out ^ 4, g, 1, 2, 0, 1);
+      orc_del(curlU, dl_out, w, dl_out ^ 1, g, 1, 0, 0, 3);
+      break;
+    default:
+      orc_del(curlU, dl_out, v, dl_out ^ 1, g, 1, 0, 0, 1);
+      orc_del(curlU, dl_out, u, dl_out ^ 2, g, 1, 1, 0, 3);
+      break;
+  }
+}
+
+/* lap_centered_SF_new: ops_discrete.f90:124-132 (grad then div) */
+void orc_lap_centered_sf(double *lapU, const double *U, int dl, const orc_grid *g, const orc_bcs *bc_out) {
+  double *t[3];
+  for (int d = 0; d < 3; ++d) t[d] = dalloc(orc_field_size(g, dl ^ (1 << d)));
+  orc_grad(t[0], t[1], t[2], U, dl, g);
+  orc_div(lapU, dl, t[0], t[1], t[2], dl ^ 1, g, bc_out);
+  for (int d = 0; d < 3; ++d) free(t[d]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* auxiliaries                                                               */
+/* ------------------------------------------------------------------------- */
+
+/* volume_DL_GF: hw/grid_field/GF_distributions.f90:61-88 */
+void orc_volume(double *vol, int dl, const orc_grid *g) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  memset(vol, 0, sizeof(double) * orc_field_size(g, dl));
+  const double *dh[3];
+  int e[3];
+  for (int d = 0; d < 3; ++d) {
+    e[d] = (dl >> d) & 1;
+    dh[d] = e[d] ? g->c[d].dhc : g->c[d].dhn;
+  }
+  for (int k = 1; k < s[2] - 1; ++k)
+    for (int j = 1; j < s[1] - 1; ++j)
+      for (int i = 1; i < s[0] - 1; ++i)
+        vol[IDX(s, i, j, k)] = dh[0][i - e[0]] * dh[1][j - e[1]] * dh[2][k - e[2]];
+}
+
+/* laplacian_diagonal: hw/grid_field/GF_diagonals.f90:15-35 */
+void orc_lap_diagonal(double *diag, int dl, const orc_grid *g) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  memset(diag, 0, sizeof(double) * orc_field_size(g, dl));
+  int p[3];
+  for (int d = 0; d < 3; ++d) p[d] = (dl >> d) & 1;
+  const orc_coords *cx = &g->c[0], *cy = &g->c[1], *cz = &g->c[2];
+  for (int k = 1; k < s[2] - 1; ++k)
+    for (int j = 1; j < s[1] - 1; ++j)
+      for (int i = 1; i < s[0] - 1; ++i)
+        diag[IDX(s, i, j, k)] = cx->n2cc_D[i] * cx->cc2n_U[i - 1] + cx->n2cc_U[i - p[0]] * cx->cc2n_D[i - p[0]] +
+                                cy->n2cc_D[j] * cy->cc2n_U[j - 1] + cy->n2cc_U[j - p[1]] * cy->cc2n_D[j - p[1]] +
+                                cz->n2cc_D[k] * cz->cc2n_U[k - 1] + cz->n2cc_U[k - p[2]] * cz->cc2n_D[k - p[2]];
+}
+
+/* sum_GF_pad: hw/grid_field/GF_aux.f90:140-158 (sequential order k, j, i) */
+double orc_sum_pad(const double *u, const int s[3], int pad) {
+  double m = 0.0;
+  for (int k = pad; k < s[2] - pad; ++k)
+    for (int j = pad; j < s[1] - pad; ++j)
+      for (int i = pad; i < s[0] - pad; ++i) m += u[IDX(s, i, j, k)];
+  return m;
+}
+
+double orc_sum_all(const double *u, size_t n) {
+  double m = 0.0;
+  for (size_t q = 0; q < n; ++q) m += u[q];
+  return m;
+}
+
+/* dot_product_SF: hw/ops/ops_aux.f90:209-217 */
+static double dot_sf(const double *A, const double *B, int dl, const orc_grid *g, const orc_bcs *bcx, double *tmp) {
+  int s[3];
+  orc_field_shape(g, dl, s);
+  size_t n = orc_field_size(g, dl);
+  for (size_t q = 0; q < n; ++q) tmp[q] = A[q] * B[q];
+  orc_assign_wall_dirichlet(tmp, dl, g, bcx, 0.0);
+  return orc_sum_pad(tmp, s, 1);
+}
+
+/* Lap_uniform_SF_explicit (matrix_free_operators.f90:35-50): lap_centered;
+   the output carries no BCs so all its ghost planes are zeroed by div_SF. */
+static void lap_explicit(double *Ax, const double *x, int dl, const orc_grid *g) {
+  orc_lap_centered_sf(Ax, x, dl, g, NULL);
+}
+
+/* ------------------------------------------------------------------------- */
+/* PCG: hw/solvers/PCG/PCG_solver_algorithm.f90:37-107, PCG_aux.f90:30-123    */
+/* ------------------------------------------------------------------------- */
+int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann, double *x,
+                   const double *b, int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+                   double *res_hist) {
+  if (n_skip < 1) return -1;
+  int s[3];
+  orc_field_shape(g, dl, s);
+  size_t n = orc_field_size(g, dl);
+  double *r = dalloc(n), *z = dalloc(n), *p = dalloc(n), *Ax = dalloc(n), *tmp = dalloc(n);
+  double *vol = dalloc(n), *Minv = dalloc(n);
+  orc_volume(vol, dl, g);
+  double volsum = orc_sum_all(vol, n); /* volume_SF: SF_extend.f90:571-581 */
+  /* prec_Lap_SF: preconditioners.f90:55-71 */
+  orc_lap_diagonal(Minv, dl, g);
+  for (size_t q = 0; q < n; ++q) Minv[q] = Minv[q] * vol[q];
+  for (size_t q = 0; q < n; ++q) Minv[q] = 1.0 / Minv[q];
+  orc_assign_ghost_xperiodic(Minv, dl, g, NULL, 0.0);
+
+  /* modify_RHS_SF: PCG_aux.f90:30-54 */
+  memcpy(r, b, sizeof(double) * n);
+  if (all_neumann) { /* subtract_phys_mean_vol_SF: ops_aux.f90:146-167 */
+    for (size_t q = 0; q < n; ++q) tmp[q] = r[q] * vol[q];
+    double mean = orc_sum_pad(tmp, s, 1) / volsum;
+    for (size_t q = 0; q < n; ++q) r[q] = r[q] - mean;
+    orc_assign_ghost_xperiodic(r, dl, g, NULL, 0.0);
+  }
+  /* compute_Ax_BC / compute_x_BC: PCG_aux.f90:81-123 */
+  memset(tmp, 0, sizeof(double) * n);
+  orc_apply_bcs(tmp, dl, g, bc, 0);
+  assign_ghost_n_xperiodic(tmp, dl, g, bc, 0.0);
+  lap_explicit(Ax, tmp, dl, g);
+  orc_assign_ghost_xperiodic(Ax, dl, g, bc, 0.0);
+  for (size_t q = 0; q < n; ++q) r[q] = r[q] - Ax[q];
+  /* operator(Ax,x): Lap_uniform_SF applies the implicit BCs to x first */
+  orc_apply_bcs(x, dl, g, bc, 1);
+  lap_explicit(Ax, x, dl, g);
+  for (size_t q = 0; q < n; ++q) r[q] = r[q] - Ax[q];
+  for (size_t q = 0; q < n; ++q) r[q] = r[q] * vol[q];
+  if (dl != ORC_DL_CC) {
+    assign_wall_periodic_single(r, dl, g, bc, 0.0);
+    orc_assign_wall_dirichlet(r, dl, g, bc, 0.0);
+    orc_multiply_wall_neumann(r, dl, g, bc, 0.5);
+  }
+
+  /* norms compute: hw/PCG/norms_extend.f90:131-139, GF_norms.f90:24-41 (all cells) */
+  double res0 = 0.0;
+  for (size_t q = 0; q < n; ++q) res0 += r[q] * r[q];
+  res0 = sqrt(res0);
+  for (size_t q = 0; q < n; ++q) z[q] = Minv[q] * r[q];
+  memcpy(p, z, sizeof(double) * n);
+  double rhok = dot_sf(r, z, dl, g, bc, tmp);
+  double res = sqrt(fabs(rhok));
+  int nh = 0;
+  if (res_hist) res_hist[nh++] = res;
+  int iter = 0;
+  /* update_exit_loop_ISP: hw/var_set/iter_solver_params_extend.f90:86-93 */
+  int e1 = res / res0 < tol_rel, e2 = res < tol_abs, e3 = iter >= iter_max;
+  (void)e1; (void)e3;
+  if (!e2) {
+    for (int it = 1; it <= iter_max; ++it) {
+      orc_apply_bcs(p, dl, g, bc, 1);
+      lap_explicit(Ax, p, dl, g);
+      orc_multiply_wall_neumann(Ax, dl, g, bc, 0.5);
+      orc_assign_wall_dirichlet(Ax, dl, g, bc, 0.0);
+      assign_wall_periodic_single(Ax, dl, g, bc, 0.0);
+      for (size_t q = 0; q < n; ++q) Ax[q] = Ax[q] * vol[q];
+      double alpha = rhok / dot_sf(p, Ax, dl, g, bc, tmp);
+      for (size_t q = 0; q < n; ++q) x[q] = x[q] + p[q] * alpha;
+      orc_apply_bcs(x, dl, g, bc, 0);
+      iter++;
+      for (size_t q = 0; q < n; ++q) r[q] = r[q] + Ax[q] * (-alpha);
+      if (iter % n_skip == 0) {
+        res = sqrt(fabs(dot_sf(r, r, dl, g, bc, tmp)));
+        if (res_hist) res_hist[nh++] = res;
+        e1 = res / res0 < tol_rel; e2 = res < tol_abs; e3 = iter >= iter_max;
+        if (e1 || e2 || e3) break;
+      }
+      for (size_t q = 0; q < n; ++q) z[q] = Minv[q] * r[q];
+      double rhokp1 = dot_sf(z, r, dl, g, bc, tmp);
+      double beta = rhokp1 / rhok;
+      for (size_t q = 0; q < n; ++q) p[q] = p[q] * beta + z[q];
+      rhok = rhokp1;
+    }
+  } else {
+    orc_apply_bcs(x, dl, g, bc, 0);
+    iter++;
+  }
+  if (out) { out[0] = (double)iter; out[1] = res0; out[2] = res; out[3] = (double)nh; }
+  free(r); free(z); free(p); free(Ax); free(tmp); free(vol); free(Minv);
+  return 0;
+}
+
+/* SplitMix64 (Steele, Lea & Flood, OOPSLA 2014), counter form: value q uses
+   state seed + (q+1)*golden, identical to the sequential generator. */
+void orc_splitmix_fill(double *out, size_t n, unsigned long long seed, double amplitude) {
+  for (size_t q = 0; q < n; ++q) {
+    unsigned long long zz = seed + (unsigned long long)(q + 1) * 0x9E3779B97F4A7C15ULL;
+    zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBULL;
+    zz = zz ^ (zz >> 31);
+    double u = (double)(zz >> 11) * (1.0 / 9007199254740992.0); /* [0,1) */
+    out[q] = amplitude * (2.0 * u - 1.0);
+  }
+}

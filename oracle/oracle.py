@@ -1,0 +1,251 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and of the compiled
+reference subset (_ref/libmoons_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py -- as the checker, never as the measured path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmoons_ref.so")
+
+DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
+BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
+
+_dp = C.POINTER(C.c_double)
+
+
+class BCs(C.Structure):
+    _fields_ = [("type", C.c_int * 6), ("val", C.c_double * 6)]
+
+    @classmethod
+    def make(cls, types, vals=None):
+        b = cls()
+        for f in range(6):
+            t = types[f]
+            b.type[f] = BC[t] if isinstance(t, str) else int(t)
+            b.val[f] = 0.0 if vals is None else float(vals[f])
+        return b
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_grid_create.restype = C.c_void_p
+        L.orc_grid_create.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int]
+        L.orc_grid_destroy.argtypes = [C.c_void_p]
+        L.orc_field_size.restype = C.c_size_t
+        L.orc_field_size.argtypes = [C.c_void_p, C.c_int]
+        L.orc_field_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.orc_coord_array.restype = C.c_int
+        L.orc_coord_array.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        for nm in ("orc_uniform_nodes",):
+            getattr(L, nm).argtypes = [C.c_double, C.c_double, C.c_int, _dp]
+        for nm in ("orc_roberts_both_nodes", "orc_roberts_left_nodes", "orc_roberts_right_nodes"):
+            getattr(L, nm).argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, _dp]
+        L.orc_del.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_grad.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_void_p]
+        L.orc_div.argtypes = [_dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p, C.POINTER(BCs)]
+        L.orc_curl_component.argtypes = [_dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p, C.c_int]
+        L.orc_lap_centered_sf.argtypes = [_dp, _dp, C.c_int, C.c_void_p, C.POINTER(BCs)]
+        L.orc_apply_bcs.argtypes = [_dp, C.c_int, C.c_void_p, C.POINTER(BCs), C.c_int]
+        L.orc_volume.argtypes = [_dp, C.c_int, C.c_void_p]
+        L.orc_lap_diagonal.argtypes = [_dp, C.c_int, C.c_void_p]
+        L.orc_pcg_lap_sf.restype = C.c_int
+        L.orc_pcg_lap_sf.argtypes = [C.c_void_p, C.c_int, C.POINTER(BCs), C.c_int, _dp, _dp, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, _dp, _dp]
+        L.orc_splitmix_fill.argtypes = [_dp, C.c_size_t, C.c_ulonglong, C.c_double]
+        _lib = L
+    return _lib
+
+
+def p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def uniform_nodes(hmin, hmax, N):
+    hn = np.zeros(N + 3)
+    lib().orc_uniform_nodes(hmin, hmax, N, p(hn))
+    return hn
+
+
+def roberts_both_nodes(hmin, hmax, N, beta):
+    hn = np.zeros(N + 3)
+    lib().orc_roberts_both_nodes(hmin, hmax, N, beta, p(hn))
+    return hn
+
+
+def splitmix(n, seed, amplitude=1.0):
+    out = np.zeros(n)
+    lib().orc_splitmix_fill(p(out), n, seed, amplitude)
+    return out
+
+
+class Grid:
+    """Oracle grid from three node arrays (ghosts included)."""
+
+    def __init__(self, hx, hy, hz):
+        self.hn = [np.ascontiguousarray(h, dtype=np.float64) for h in (hx, hy, hz)]
+        self.h = lib().orc_grid_create(p(self.hn[0]), len(self.hn[0]), p(self.hn[1]), len(self.hn[1]),
+                                       p(self.hn[2]), len(self.hn[2]))
+        if not self.h:
+            raise ValueError("bad grid")
+
+    def __del__(self):
+        try:
+            lib().orc_grid_destroy(self.h)
+        except Exception:
+            pass
+
+    def shape(self, dl):
+        dl = DL[dl] if isinstance(dl, str) else dl
+        s = (C.c_int * 3)()
+        lib().orc_field_shape(self.h, dl, s)
+        return (s[0], s[1], s[2])
+
+    def zeros(self, dl):
+        return np.zeros(self.shape(dl), order="F")
+
+    def coord(self, d, which):
+        n = lib().orc_coord_array(self.h, d, which, None)
+        out = np.zeros(n)
+        lib().orc_coord_array(self.h, d, which, p(out))
+        return out
+
+
+def _F(a):
+    assert a.flags.f_contiguous and a.dtype == np.float64
+    return p(a)
+
+
+def del_op(g: Grid, out, dl_out, f, dl_in, n, d, pad, gen):
+    lib().orc_del(_F(out), dl_out, _F(f), dl_in, g.h, n, d, pad, gen)
+
+
+def grad(g: Grid, u, dl_u):
+    outs = [g.zeros(dl_u ^ (1 << d)) for d in range(3)]
+    lib().orc_grad(_F(outs[0]), _F(outs[1]), _F(outs[2]), _F(u), dl_u, g.h)
+    return outs
+
+
+def div(g: Grid, u, v, w, dl_out, bcs=None):
+    out = g.zeros(dl_out)
+    lib().orc_div(_F(out), dl_out, _F(u), _F(v), _F(w), dl_out ^ 1, g.h, C.byref(bcs) if bcs else None)
+    return out
+
+
+def curl(g: Grid, u, v, w, dl_out, d):
+    out = g.zeros(dl_out)
+    lib().orc_curl_component(_F(out), dl_out, _F(u), _F(v), _F(w), 0, g.h, d)
+    return out
+
+
+def lap_centered(g: Grid, U, dl, bcs=None):
+    out = g.zeros(dl)
+    lib().orc_lap_centered_sf(_F(out), _F(U), dl, g.h, C.byref(bcs) if bcs else None)
+    return out
+
+
+def apply_bcs(g: Grid, u, dl, bcs: BCs, implicit=False):
+    lib().orc_apply_bcs(_F(u), dl, g.h, C.byref(bcs), 1 if implicit else 0)
+
+
+def volume(g: Grid, dl):
+    out = g.zeros(dl)
+    lib().orc_volume(_F(out), dl, g.h)
+    return out
+
+
+def lap_diagonal(g: Grid, dl):
+    out = g.zeros(dl)
+    lib().orc_lap_diagonal(_F(out), dl, g.h)
+    return out
+
+
+def pcg_lap(g: Grid, dl, bcs: BCs, all_neumann, x, b, iter_max, tol_rel, tol_abs, n_skip=1):
+    """Returns (iterations, res0, res, res_history); x is updated in place."""
+    out = np.zeros(4)
+    hist = np.zeros(iter_max + 2)
+    st = lib().orc_pcg_lap_sf(g.h, dl, C.byref(bcs), 1 if all_neumann else 0, _F(x), _F(b), iter_max, tol_rel,
+                              tol_abs, n_skip, p(out), p(hist))
+    if st != 0:
+        raise RuntimeError("oracle pcg failed")
+    return int(out[0]), out[1], out[2], hist[: int(out[3])].copy()
+
+
+# ------------------------------------------------------------- compiled reference
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own routines (flang-built from its sources), lazily bound."""
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_SO, mode=os.RTLD_LAZY)
+        _ip = C.POINTER(C.c_int)
+        R.ref_grid.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _dp]
+        R.ref_stencil.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip]
+        R.ref_interp_stencil.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, _dp, _ip]
+        R.ref_pencil.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
+        R.ref_bc_plane.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        _ref = R
+    return _ref
+
+
+def ref_grid(kind, hmin, hmax, N, beta=0.0):
+    hn = np.zeros(N + 1)
+    ref().ref_grid(kind, hmin, hmax, N, beta, p(hn))
+    return hn
+
+
+def ref_stencil(kind, dh, s):
+    dh = np.ascontiguousarray(dh, dtype=np.float64)
+    L, D, U = np.zeros(s), np.zeros(s), np.zeros(s)
+    nL, nD, nU = C.c_int(), C.c_int(), C.c_int()
+    ref().ref_stencil(kind, p(dh), s, p(L), p(D), p(U), C.byref(nL), C.byref(nD), C.byref(nU))
+    return L[: nL.value], D[: nD.value], U[: nU.value]
+
+
+def ref_interp(hc, hn):
+    hc = np.ascontiguousarray(hc, dtype=np.float64)
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    D, U = np.zeros(len(hc)), np.zeros(len(hc))
+    n = C.c_int()
+    ref().ref_interp_stencil(p(hc), len(hc), p(hn), len(hn), p(D), p(U), C.byref(n))
+    return D[: n.value], U[: n.value]
+
+
+def ref_pencil(kind, dfdh, f, L, D, U, gt):
+    dfdh = np.ascontiguousarray(dfdh, dtype=np.float64).copy()
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nc = len(D)
+    Lp = np.ascontiguousarray(L if len(L) == nc else np.zeros(nc), dtype=np.float64)
+    ref().ref_pencil(kind, p(dfdh), len(dfdh), p(f), len(f), p(Lp), p(np.ascontiguousarray(D)),
+                     p(np.ascontiguousarray(U)), nc, gt)
+    return dfdh
+
+
+def ref_bc_plane(kind, ug, ub, ui, bvals, dh_nhat):
+    ug = np.array(ug, dtype=np.float64, order="F")
+    ub = np.array(ub, dtype=np.float64, order="F")
+    ui = np.asfortranarray(ui, dtype=np.float64)
+    bvals = np.asfortranarray(bvals, dtype=np.float64)
+    x, y = ug.shape
+    ref().ref_bc_plane(kind, p(ug), p(ub), p(ui), p(bvals), dh_nhat, x, y)
+    return ug, ub
