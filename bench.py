@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="4 PCG iterations, then 50 timed launches of each iteration kernel "
+                         "(no early-exited launches; for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -114,6 +117,16 @@ def main():
     M.init(0)
     m, x, b, types, hs, bh = make_inputs(M)
     P = M.PCG_solver_SF(m, x, M.iter_solver_params(20000, TOL, 1e-300, 1))
+    if args.kernels_only:
+        P.set_ISP(M.iter_solver_params(4, 0.0, 0.0, 1))
+        x.fill(0.0)
+        P.solve(x, b)
+        kms = P.time_kernels(50)
+        names = ["pcg_matvec", "pcg_update", "pcg_pupdate"]
+        per = {"pcg_matvec": 16, "pcg_update": 48, "pcg_pupdate": 24}
+        print(json.dumps({n: {"us": round(1e3 * t, 2), "GBs": round(per[n] * N ** 3 / (t * 1e-3) / 1e9, 1)}
+                          for n, t in zip(names, kms)}), flush=True)
+        return
 
     def step():
         x.fill(0.0)

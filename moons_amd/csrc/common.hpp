@@ -89,34 +89,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// Publishes this block's partial and returns true in every thread of the
-// last block to arrive (ticket reset by that block).  Standard
-// fence-then-ticket pattern; no block ever waits on another.
-__device__ __forceinline__ bool arrive_last(unsigned int *ticket, unsigned int nblocks, bool *sh_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    bool last = (t == nblocks - 1);
-    if (last) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *sh_flag = last;
-  }
-  __syncthreads();
-  return *sh_flag;
-}
-
-__device__ __forceinline__ double load_agent(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of nb partials in a fixed order independent of which block runs it.
-template <int NT>
-__device__ __forceinline__ double sum_partials(const double *part, int nb, double *sh) {
-  double v = 0.0;
-  for (int q = threadIdx.x; q < nb; q += NT) v += load_agent(part + q);
-  return block_sum<NT>(v, sh);
-}
-
 // ------------------------------------------------------ XCD-aware tiling
 // Workgroups are dispatched round-robin over the 8 XCDs (b -> XCD b%8);
 // remap so that each XCD processes one contiguous range of logical tiles

@@ -7,16 +7,22 @@
 // and modify_RHS_SF (code/handwritten/solvers/PCG/PCG_aux.f90:30-54).
 //
 // One iteration is three HBM-bound kernels over the interior cells:
-//   k_pcg_matvec   q = vol*lap(p), (p,q)                  read p, write q      16 B/cell
-//   k_pcg_update   x += a p, r -= a q, (r,r), (Minv r, r)  read x,p,r,q write x,r 48 B/cell
-//   k_pcg_pupdate  p = b p + Minv r (+ face ghosts of p)   read r,p write p      24 B/cell
+//   k_pcg_matvec   q = vol*lap(p), partial (p,q)              read p, write q        16 B/cell
+//   k_pcg_update   x += a p, r -= a q, partials (r,r),(Minv r,r) read x,p,r,q write x,r 48 B/cell
+//   k_pcg_pupdate  exit test; p = b p + Minv r (+ ghosts of p)  read r,p write p        24 B/cell
 // Minv = 1/(vol*diag) is rebuilt in registers from 1D arrays (no 3D array).
-// Scalars (rho, alpha, beta, norms, iteration count, exit flags) live in a
-// device control block; each reduction is finished by the last block to arrive,
-// in a fixed order, and every kernel returns at once when the loop has exited.
-// The host only launches chunks of iterations and reads the exit flag between
+//
+// Scalars live in a device control block.  A kernel producing a dot product
+// writes one partial per block; the next kernel finishes that sum in its
+// prologue: every block sums the same partials in the same fixed order, so
+// all blocks derive identical alpha, beta and exit decisions with no atomics
+// and no cross-block fences (the kernel boundary makes the partials visible).
+// rho is double-buffered by iteration parity, since block 0 publishes the new
+// value while the other blocks of that kernel still read the old one.  Every
+// kernel returns at once after the exit decision; the host launches chunks of
+// iterations (passing the iteration number) and reads the exit flag between
 // chunks.
-#include "common.hpp"
+#include "cc_stencil.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -25,149 +31,41 @@
 namespace moons {
 
 struct PcgCtrl {
-  unsigned int ticket[4];
   int iter, done, exit_loop, iter_max, n_skip, pad_;
   double tol_rel, tol_abs;
-  double rho, alpha, beta, res0, res, mean, volsum, pq, rr, rz;
+  double rho[2]; // rho of the last iteration, indexed by its parity
+  double alpha, beta, res0, res, mean, volsum, pq, rr, rz;
 };
-
-struct Coef1 {
-  const double *cD, *cU; // staggered CC->N (sc-1)
-  const double *nD, *nU; // staggered N->CC (sn-1)
-  const double *dh;      // dhn (cell widths, CC dual)
-  const double *lA, *lB; // Laplacian diagonal terms
-};
-
-struct CCGeom {
-  int s0, s1, s2, n0, n1, n2;
-  int ntx, nty, ntz, kz, nblocks;
-  Coef1 c[3];
-  int bct[6];
-  double bval[6];
-  double dh_nhat[6];
-};
-
-// 7-point cell-centred Laplacian in the reference's operation order:
-// gx(node) = f(i-1)*D + f(i)*U, div = gx(i)*D' + gx(i+1)*U', then y and z
-// terms added left to right (code/handwritten/stencils/stencils.f90:47-59,108-120).
-__device__ __forceinline__ double lap7(double c, double xm, double xp, double ym, double yp, double zm, double zp,
-                                       const Coef1 &X, int i, const Coef1 &Y, int j, const Coef1 &Z, int k) {
-  const double gxm = xm * X.cD[i - 1] + c * X.cU[i - 1];
-  const double gxp = c * X.cD[i] + xp * X.cU[i];
-  double v = gxm * X.nD[i] + gxp * X.nU[i];
-  const double gym = ym * Y.cD[j - 1] + c * Y.cU[j - 1];
-  const double gyp = c * Y.cD[j] + yp * Y.cU[j];
-  v = v + gym * Y.nD[j] + gyp * Y.nU[j];
-  const double gzm = zm * Z.cD[k - 1] + c * Z.cU[k - 1];
-  const double gzp = c * Z.cD[k] + zp * Z.cU[k];
-  v = v + gzm * Z.nD[k] + gzp * Z.nU[k];
-  return v;
-}
-
-// Face-ghost value of a cell-centred field (never an edge or corner ghost).
-// implicit_: homogeneous rules; else the BC value rule.  vI is the first
-// interior value behind the ghost, vIo the first interior value behind the
-// opposite face (periodic), vmem the value stored in the ghost itself.
-__device__ __forceinline__ double ghost_cc(int type, bool implicit_, double bval, double dh_nhat, double vI,
-                                           double vIo, double vmem) {
-  switch (type) {
-    case MOONS_BC_DIRICHLET: return implicit_ ? -vI : 2.0 * bval - vI;
-    case MOONS_BC_NEUMANN: return implicit_ ? vI : vI + bval * dh_nhat;
-    case MOONS_BC_PERIODIC: return vIo;
-    case MOONS_BC_SYMMETRIC: return vI;
-    case MOONS_BC_ANTISYMMETRIC: return -vI;
-    default: return vmem;
-  }
-}
-
-struct TileIdx {
-  int i, j, k0, k1;
-  bool active;
-};
-
-__device__ __forceinline__ TileIdx tile_of(const CCGeom &G) {
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int ti = L % G.ntx;
-  const int tj = (L / G.ntx) % G.nty;
-  const int tk = L / (G.ntx * G.nty);
-  TileIdx t;
-  t.i = 1 + ti * kTX + (threadIdx.x % kTX);
-  t.j = 1 + tj * kTY + (threadIdx.x / kTX);
-  t.k0 = 1 + tk * G.kz;
-  t.k1 = min(t.k0 + G.kz, G.n2 + 1);
-  t.active = (t.i <= G.n0) && (t.j <= G.n1);
-  return t;
-}
-
-__device__ __forceinline__ double minv_at(const CCGeom &G, double dxy, double axy, int k) {
-  const double diag = axy + G.c[2].lA[k] + G.c[2].lB[k];
-  const double vol = dxy * G.c[2].dh[k];
-  return 1.0 / (diag * vol);
-}
-
-// Writes the face ghosts of p next to interior cell (i,j,k) with the implicit
-// BC rules (apply_BCs_implicit applied to p before each matvec,
-// code/handwritten/solvers/matrix_free_operators.f90:60).
-__device__ __forceinline__ void fill_p_ghosts(double *p, const CCGeom &G, int i, int j, int k, double v) {
-  const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
-  const size_t base = (size_t)j * sx + (size_t)k * sxy;
-  auto put = [&](int face, size_t at, size_t at_opp, double val) {
-    const int t = G.bct[face];
-    if (t == MOONS_BC_NONE) return;
-    if (t == MOONS_BC_PERIODIC) p[at_opp] = val;
-    else if (t == MOONS_BC_DIRICHLET || t == MOONS_BC_ANTISYMMETRIC) p[at] = -val;
-    else p[at] = val;
-  };
-  // For periodic faces the thread at the far interior plane feeds the near
-  // ghost: ghost(min) = interior(n), ghost(max) = interior(1).
-  if (i == 1) {
-    put(0, base + 0, base + (size_t)(G.n0 + 1), v);
-  }
-  if (i == G.n0) {
-    put(1, base + (size_t)(G.n0 + 1), base + 0, v);
-  }
-  const size_t bi = (size_t)i + (size_t)k * sxy;
-  if (j == 1) put(2, bi, bi + (size_t)(G.n1 + 1) * sx, v);
-  if (j == G.n1) put(3, bi + (size_t)(G.n1 + 1) * sx, bi, v);
-  const size_t bj = (size_t)i + (size_t)j * sx;
-  if (k == 1) put(4, bj, bj + (size_t)(G.n2 + 1) * sxy, v);
-  if (k == G.n2) put(5, bj + (size_t)(G.n2 + 1) * sxy, bj, v);
-}
 
 // ------------------------------------------------------------------ kernels
 
-// Sum of b*vol over interior cells (subtract_phys_mean_vol_SF,
-// code/handwritten/ops/ops_aux.f90:146-167); last block sets mean.
-__global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, CCGeom G, PcgCtrl *ctrl,
-                                                 double *part) {
+// Partial sums of b*vol over interior cells (subtract_phys_mean_vol_SF,
+// code/handwritten/ops/ops_aux.f90:146-167).
+__global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, CCGeom G, double *partM) {
   __shared__ double sh[kNT / kWave];
-  __shared__ bool last;
   const TileIdx t = tile_of(G);
   double acc = 0.0;
   if (t.active) {
     const double dxy = G.c[0].dh[t.i] * G.c[1].dh[t.j];
     const size_t sxy = (size_t)G.s0 * G.s1;
     const double *bp = b + t.i + (size_t)t.j * G.s0;
-    for (int k = t.k0; k < t.k1; ++k) acc += bp[k * sxy] * (dxy * G.c[2].dh[k]);
+    for (int k = t.k0; k < t.k1; ++k) acc += bp[(size_t)k * sxy] * (dxy * G.c[2].dh[k]);
   }
-  double s = block_sum<kNT>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (arrive_last(&ctrl->ticket[0], gridDim.x, &last)) {
-    double tot = sum_partials<kNT>(part, gridDim.x, sh);
-    if (threadIdx.x == 0) ctrl->mean = tot / ctrl->volsum;
-  }
+  const double s = block_sum<kNT>(acc, sh);
+  if (threadIdx.x == 0) partM[blockIdx.x] = s;
 }
 
-// modify_RHS_SF + PCG set-up (PCG_aux.f90:30-54; PCG_solver_algorithm.f90:57-68):
-// r = vol*((b - mean) - A x_BC - A x), z = Minv r, p = z, res0 = |r|, rho = (r,z).
+// modify_RHS_SF + PCG set-up (PCG_aux.f90:30-54; PCG_solver_algorithm.f90:57-66):
+// r = vol*((b - mean) - A x_BC - A x), z = Minv r, p = z; partials of (r,r), (r,z).
 __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, const double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ p, CCGeom G,
-                                                PcgCtrl *ctrl, double *part, int all_neumann) {
+                                                const PcgCtrl *ctrl, const double *partM, double *partB,
+                                                int all_neumann) {
   __shared__ double sh[kNT / kWave];
-  __shared__ bool last;
+  double mean = 0.0;
+  if (all_neumann) mean = prologue_sum<kNT>(partM, gridDim.x, 1, 0, sh) / ctrl->volsum;
   const TileIdx t = tile_of(G);
   double a_rr = 0.0, a_rz = 0.0;
-  const double mean = all_neumann ? ctrl->mean : 0.0;
   if (t.active) {
     const int i = t.i, j = t.j;
     const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
@@ -176,7 +74,8 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
     for (int k = t.k0; k < t.k1; ++k) {
       const size_t o = (size_t)i + j * sx + k * sxy;
       const double c = x[o];
-      // neighbours of x with the implicit ghost rules, and of x_BC (zero interior)
+      // neighbours of x with the implicit ghost rules, and of x_BC (zero interior,
+      // ghosts from the BC values: compute_x_BC, PCG_aux.f90:109-115)
       double nb[6], nbc[6];
       const size_t off[6] = {o - 1, o + 1, o - sx, o + sx, o - sxy, o + sxy};
       const bool ghost[6] = {i == 1, i == G.n0, j == 1, j == G.n1, k == 1, k == G.n2};
@@ -202,179 +101,218 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
       rv = rv - axbc;
       rv = rv - ax;
       rv = rv * (dxy * G.c[2].dh[k]);
-      const double z = minv_at(G, dxy, axy, k) * rv;
+      const double z = minv_at<double>(G, dxy, axy, k) * rv;
       r[o] = rv;
       p[o] = z;
-      fill_p_ghosts(p, G, i, j, k, z);
+      fill_ghosts_implicit(p, G, i, j, k, z);
       a_rr += rv * rv;
       a_rz += rv * z;
     }
   }
-  double srr = block_sum<kNT>(a_rr, sh);
-  double srz = block_sum<kNT>(a_rz, sh);
+  const double srr = block_sum<kNT>(a_rr, sh);
+  const double srz = block_sum<kNT>(a_rz, sh);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = srr;
-    part[2 * blockIdx.x + 1] = srz;
+    partB[2 * blockIdx.x] = srr;
+    partB[2 * blockIdx.x + 1] = srz;
   }
-  if (arrive_last(&ctrl->ticket[1], gridDim.x, &last)) {
-    double v0 = 0.0, v1 = 0.0;
-    for (int q = threadIdx.x; q < (int)gridDim.x; q += kNT) {
-      v0 += load_agent(part + 2 * q);
-      v1 += load_agent(part + 2 * q + 1);
-    }
-    double rr = block_sum<kNT>(v0, sh);
-    double rz = block_sum<kNT>(v1, sh);
-    if (threadIdx.x == 0) {
-      const double res0 = sqrt(rr);
-      const double res = sqrt(fabs(rz));
-      ctrl->res0 = res0;
-      ctrl->rho = rz;
-      ctrl->res = res;
-      ctrl->iter = 0;
-      const bool e1 = res / res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = 0 >= ctrl->iter_max;
-      ctrl->exit_loop = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
-      if (e2) {
-        ctrl->done = 1;
-        ctrl->iter = 1; // the reference's else-branch counts one iteration
-      } else {
-        ctrl->done = 0;
-      }
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<PcgCtrl *>(ctrl)->mean = mean;
+}
+
+// Set-up exit test (PCG_solver_algorithm.f90:60-68), one block:
+// res0 = |r0|, rho = (r,z), res = sqrt|rho|; exit_loop(2) skips the loop.
+__global__ void __launch_bounds__(kNT) k_pcg_init(PcgCtrl *ctrl, const double *partB, int nb) {
+  __shared__ double sh[kNT / kWave];
+  const double rr = prologue_sum<kNT>(partB, nb, 2, 0, sh);
+  const double rz = prologue_sum<kNT>(partB, nb, 2, 1, sh);
+  if (threadIdx.x == 0) {
+    const double res0 = sqrt(rr);
+    const double res = sqrt(fabs(rz));
+    ctrl->res0 = res0;
+    ctrl->rho[0] = rz;
+    ctrl->res = res;
+    ctrl->iter = 0;
+    const bool e1 = res / res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = 0 >= ctrl->iter_max;
+    ctrl->exit_loop = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
+    ctrl->done = e2 ? 1 : 0;
+    if (e2) ctrl->iter = 1; // the reference's else-branch counts one iteration
   }
 }
 
-// q = vol*lap(p) over the interior (p's face ghosts already hold the implicit
-// BC values) and (p,q); last block sets alpha = rho/(p,q).
+// q = vol*lap(p) over the interior (p's face ghosts hold the implicit BC
+// values) and partial (p,q).
+template <int KZ>
 __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p, double *__restrict__ q, CCGeom G,
-                                                   PcgCtrl *ctrl, double *part) {
+                                                   const PcgCtrl *ctrl, double *partA) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
-  __shared__ bool last;
   const TileIdx t = tile_of(G);
   double acc = 0.0;
   if (t.active) {
     const int i = t.i, j = t.j;
     const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
     const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
-    const double *pc = p + i + (size_t)j * sx;
-    double pm = pc[(size_t)(t.k0 - 1) * sxy];
-    double p0 = pc[(size_t)t.k0 * sxy];
-    for (int k = t.k0; k < t.k1; ++k) {
-      const size_t ok = (size_t)k * sxy;
-      const double pp = pc[ok + sxy];
-      const double v = lap7(p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, G.c[0], i, G.c[1], j, G.c[2], k);
-      const double qv = v * (dxy * G.c[2].dh[k]);
-      q[i + (size_t)j * sx + ok] = qv;
-      acc += p0 * qv;
-      pm = p0;
-      p0 = pp;
+    const double *pc = p + i + (size_t)j * sx + (size_t)t.k0 * sxy;
+    double *qc = q + i + (size_t)j * sx + (size_t)t.k0 * sxy;
+    const int nk = t.k1 - t.k0;
+    // all planes' loads first (the column and its four in-plane neighbours)
+    // col[u+1] = plane k0+u for u = -1 .. nk (both z neighbours included)
+    double col[KZ + 2], xm[KZ], xp[KZ], ym[KZ], yp[KZ];
+    col[0] = pc[-(ptrdiff_t)sxy];
+#pragma unroll
+    for (int u = 0; u <= KZ; ++u)
+      if (u <= nk) col[u + 1] = pc[(size_t)u * sxy];
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const size_t o = (size_t)u * sxy;
+        xm[u] = pc[o - 1];
+        xp[u] = pc[o + 1];
+        ym[u] = pc[o - sx];
+        yp[u] = pc[o + sx];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const int k = t.k0 + u;
+        const double v = lap7(col[u + 1], xm[u], xp[u], ym[u], yp[u], col[u], col[u + 2], G.c[0], i, G.c[1], j, G.c[2], k);
+        const double qv = v * (dxy * G.c[2].dh[k]);
+        qc[(size_t)u * sxy] = qv;
+        acc += col[u + 1] * qv;
+      }
     }
   }
-  double s = block_sum<kNT>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (arrive_last(&ctrl->ticket[2], gridDim.x, &last)) {
-    double tot = sum_partials<kNT>(part, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ctrl->pq = tot;
-      ctrl->alpha = ctrl->rho / tot;
-    }
-  }
+  const double s = block_sum<kNT>(acc, sh);
+  if (threadIdx.x == 0) partA[blockIdx.x] = s;
 }
 
-// x = x + p*alpha; r = r + q*(-alpha); (r,r); z = Minv r; (z,r).
-// Last block: iteration count, residual check every n_skip, beta.
+// alpha = rho/(p,q); x = x + p*alpha; r = r + q*(-alpha); partials (r,r), (Minv r, r).
+template <int KZ>
 __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, const double *__restrict__ p,
                                                    double *__restrict__ r, const double *__restrict__ q, CCGeom G,
-                                                   PcgCtrl *ctrl, double *part) {
+                                                   PcgCtrl *ctrl, const double *partA, double *partB, int it) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
-  __shared__ bool last;
-  const TileIdx t = tile_of(G);
-  const double alpha = ctrl->alpha;
+  const double pq = prologue_sum<kNT>(partA, gridDim.x, 1, 0, sh);
+  const double alpha = ctrl->rho[(it - 1) & 1] / pq;
   const double malpha = -alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctrl->pq = pq;
+    ctrl->alpha = alpha;
+  }
+  const TileIdx t = tile_of(G);
   double a_rr = 0.0, a_rz = 0.0;
   if (t.active) {
     const int i = t.i, j = t.j;
     const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
     const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
     const double axy = G.c[0].lA[i] + G.c[0].lB[i] + G.c[1].lA[j] + G.c[1].lB[j];
-    const size_t base = (size_t)i + (size_t)j * sx;
-    for (int k = t.k0; k < t.k1; ++k) {
-      const size_t o = base + (size_t)k * sxy;
-      const double pv = p[o];
-      x[o] = x[o] + pv * alpha;
-      const double rv = r[o] + q[o] * malpha;
-      r[o] = rv;
-      const double z = minv_at(G, dxy, axy, k) * rv;
-      a_rr += rv * rv;
-      a_rz += z * rv;
+    const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
+    const int nk = t.k1 - t.k0;
+    double xv[KZ], pv[KZ], rv[KZ], qv[KZ];
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const size_t o = base + (size_t)u * sxy;
+        xv[u] = x[o];
+        pv[u] = p[o];
+        rv[u] = r[o];
+        qv[u] = q[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const size_t o = base + (size_t)u * sxy;
+        x[o] = xv[u] + pv[u] * alpha;
+        const double rn = rv[u] + qv[u] * malpha;
+        r[o] = rn;
+        const double z = minv_at<double>(G, dxy, axy, t.k0 + u) * rn;
+        a_rr += rn * rn;
+        a_rz += z * rn;
+      }
     }
   }
-  double srr = block_sum<kNT>(a_rr, sh);
-  double srz = block_sum<kNT>(a_rz, sh);
+  const double srr = block_sum<kNT>(a_rr, sh);
+  const double srz = block_sum<kNT>(a_rz, sh);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = srr;
-    part[2 * blockIdx.x + 1] = srz;
-  }
-  if (arrive_last(&ctrl->ticket[3], gridDim.x, &last)) {
-    double v0 = 0.0, v1 = 0.0;
-    for (int qq = threadIdx.x; qq < (int)gridDim.x; qq += kNT) {
-      v0 += load_agent(part + 2 * qq);
-      v1 += load_agent(part + 2 * qq + 1);
-    }
-    double rr = block_sum<kNT>(v0, sh);
-    double rz = block_sum<kNT>(v1, sh);
-    if (threadIdx.x == 0) {
-      const int it = ctrl->iter + 1;
-      ctrl->iter = it;
-      ctrl->rr = rr;
-      ctrl->rz = rz;
-      bool stop = false;
-      if (it % ctrl->n_skip == 0) {
-        const double res = sqrt(fabs(rr));
-        ctrl->res = res;
-        const bool e1 = res / ctrl->res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = it >= ctrl->iter_max;
-        ctrl->exit_loop = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
-        stop = e1 || e2 || e3;
-      }
-      if (!stop) {
-        ctrl->beta = rz / ctrl->rho;
-        ctrl->rho = rz;
-        if (it >= ctrl->iter_max) {
-          ctrl->exit_loop |= 4;
-          stop = true;
-        }
-      }
-      if (stop) ctrl->done = 1;
-    }
+    partB[2 * blockIdx.x] = srr;
+    partB[2 * blockIdx.x + 1] = srz;
   }
 }
 
-// p = p*beta + Minv r, then the face ghosts of p.  Skipped after the exit
-// decision (the loop has ended and p is not an output).
+// Exit test of iteration `it` (check_res every n_skip, update_exit_loop,
+// hw/var_set/iter_solver_params_extend.f90:86-113), then p = p*beta + Minv r
+// and the face ghosts of p.  Block 0 publishes the new state.
+template <int KZ>
 __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, const double *__restrict__ r, CCGeom G,
-                                                    const PcgCtrl *ctrl) {
+                                                    PcgCtrl *ctrl, const double *partB, int it) {
   if (ctrl->done) return;
+  __shared__ double sh[kNT / kWave];
+  const double rr = prologue_sum<kNT>(partB, gridDim.x, 2, 0, sh);
+  const double rz = prologue_sum<kNT>(partB, gridDim.x, 2, 1, sh);
+  const double rho_old = ctrl->rho[(it - 1) & 1];
+  bool stop = false, checked = false;
+  int flags = 0;
+  double res = 0.0;
+  if (it % ctrl->n_skip == 0) {
+    checked = true;
+    res = sqrt(fabs(rr));
+    const bool e1 = res / ctrl->res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = it >= ctrl->iter_max;
+    flags = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
+    stop = e1 || e2 || e3;
+  }
+  const double beta = rz / rho_old;
+  if (!stop && it >= ctrl->iter_max) {
+    flags |= 4;
+    stop = true;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctrl->iter = it;
+    ctrl->rr = rr;
+    ctrl->rz = rz;
+    if (checked) {
+      ctrl->res = res;
+      ctrl->exit_loop = flags;
+    } else if (stop) {
+      ctrl->exit_loop |= flags;
+    }
+    ctrl->beta = beta;
+    ctrl->rho[it & 1] = rz;
+    if (stop) ctrl->done = 1;
+  }
+  if (stop) return;
   const TileIdx t = tile_of(G);
   if (!t.active) return;
-  const double beta = ctrl->beta;
   const int i = t.i, j = t.j;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
   const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
   const double axy = G.c[0].lA[i] + G.c[0].lB[i] + G.c[1].lA[j] + G.c[1].lB[j];
-  const size_t base = (size_t)i + (size_t)j * sx;
+  const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
   const bool edge_ij = (i == 1) || (i == G.n0) || (j == 1) || (j == G.n1);
-  for (int k = t.k0; k < t.k1; ++k) {
-    const size_t o = base + (size_t)k * sxy;
-    const double z = minv_at(G, dxy, axy, k) * r[o];
-    const double pv = p[o] * beta + z;
-    p[o] = pv;
-    if (edge_ij || k == 1 || k == G.n2) fill_p_ghosts(p, G, i, j, k, pv);
+  const int nk = t.k1 - t.k0;
+  double rv[KZ], pv[KZ];
+#pragma unroll
+  for (int u = 0; u < KZ; ++u) {
+    if (u < nk) {
+      rv[u] = r[base + (size_t)u * sxy];
+      pv[u] = p[base + (size_t)u * sxy];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < KZ; ++u) {
+    if (u < nk) {
+      const int k = t.k0 + u;
+      const double z = minv_at<double>(G, dxy, axy, k) * rv[u];
+      const double pn = pv[u] * beta + z;
+      p[base + (size_t)u * sxy] = pn;
+      if (edge_ij || k == 1 || k == G.n2) fill_ghosts_implicit(p, G, i, j, k, pn);
+    }
   }
 }
 
 // Fused Lap_uniform_SF for a cell-centred x whose ghosts already hold the
-// implicit BC values: Ax = lap(x) in the interior, zero ghosts.
+// implicit BC values: Ax = lap(x) in the interior.
 __global__ void __launch_bounds__(kNT) k_lap_cc(const double *__restrict__ x, double *__restrict__ Ax, CCGeom G) {
   const TileIdx t = tile_of(G);
   if (!t.active) return;
@@ -414,7 +352,7 @@ CCGeom make_geom(const moons_field_s *x, int kz) {
   return G;
 }
 
-static int pick_kz(int n2) {
+int pick_kz(int n2) {
   // z cells marched per thread: enough blocks to fill 256 CUs several times over
   if (n2 >= 128) return 8;
   if (n2 >= 32) return 4;
@@ -434,15 +372,15 @@ struct moons_pcg_s {
   double volsum = 0.0;
   CCGeom G;
   double *r = nullptr, *p = nullptr, *q = nullptr; // device work vectors
-  double *part = nullptr;
+  double *partA = nullptr, *partB = nullptr, *partM = nullptr;
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-static int pcg_upload_ctrl(moons_pcg_s *S, bool reset_scalars) {
+static int pcg_upload_ctrl(moons_pcg_s *S) {
   PcgCtrl &h = *S->host_ctrl;
-  if (reset_scalars) std::memset(&h, 0, sizeof(PcgCtrl));
+  std::memset(&h, 0, sizeof(PcgCtrl));
   h.iter_max = S->iter_max;
   h.n_skip = S->n_skip;
   h.tol_rel = S->tol_rel;
@@ -452,6 +390,28 @@ static int pcg_upload_ctrl(moons_pcg_s *S, bool reset_scalars) {
   MOONS_HIP(hipStreamSynchronize(stream()));
   return MOONS_OK;
 }
+
+template <int KZ>
+static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
+  const dim3 grid(S->G.nblocks), block(kNT);
+  if (which & 1) hipLaunchKernelGGL(k_pcg_matvec<KZ>, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA);
+  if (which & 2)
+    hipLaunchKernelGGL(k_pcg_update<KZ>, grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl, S->partA, S->partB, it);
+  if (which & 4) hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
+}
+
+// which: bit 0 matvec, bit 1 update, bit 2 p-update
+static int pcg_launch(moons_pcg_s *S, double *x, int it, int which) {
+  switch (S->G.kz) {
+    case 8: pcg_launch_kz<8>(S, x, it, which); break;
+    case 4: pcg_launch_kz<4>(S, x, it, which); break;
+    default: pcg_launch_kz<2>(S, x, it, which); break;
+  }
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+static int pcg_launch_iteration(moons_pcg_s *S, double *x, int it) { return pcg_launch(S, x, it, 7); }
 
 extern "C" {
 
@@ -470,8 +430,8 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
   S->op = op;
   S->prec = prec;
   S->G = make_geom(x, pick_kz(x->s[2] - 2));
-  // volume sum (vol%vol), sequential over interior cells
   {
+    // vol%vol (volume_SF, SF_extend.f90:571-581), sequential over interior cells
     const HostCoords *c = m->hc;
     double vs = 0.0;
     for (int k = 1; k < x->s[2] - 1; ++k)
@@ -482,7 +442,9 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
   const size_t bytes = x->n * sizeof(double);
   const size_t nb = (size_t)S->G.nblocks;
   bool ok = hipMalloc(&S->r, bytes) == hipSuccess && hipMalloc(&S->p, bytes) == hipSuccess &&
-            hipMalloc(&S->q, bytes) == hipSuccess && hipMalloc(&S->part, 2 * nb * sizeof(double)) == hipSuccess &&
+            hipMalloc(&S->q, bytes) == hipSuccess && hipMalloc(&S->partA, nb * sizeof(double)) == hipSuccess &&
+            hipMalloc(&S->partB, 2 * nb * sizeof(double)) == hipSuccess &&
+            hipMalloc(&S->partM, nb * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->ctrl, sizeof(PcgCtrl)) == hipSuccess &&
             hipHostMalloc(&S->host_ctrl, sizeof(PcgCtrl), hipHostMallocDefault) == hipSuccess &&
             hipEventCreate(&S->ev0) == hipSuccess && hipEventCreate(&S->ev1) == hipSuccess;
@@ -493,7 +455,7 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
   (void)hipMemsetAsync(S->r, 0, bytes, stream());
   (void)hipMemsetAsync(S->p, 0, bytes, stream());
   (void)hipMemsetAsync(S->q, 0, bytes, stream());
-  int st = pcg_upload_ctrl(S, true);
+  int st = pcg_upload_ctrl(S);
   if (st) {
     moons_pcg_destroy(S);
     return st;
@@ -505,10 +467,9 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
 int moons_pcg_destroy(moons_pcg_t S) {
   if (!S) return MOONS_OK;
   if (stream()) (void)hipStreamSynchronize(stream());
-  if (S->r) (void)hipFree(S->r);
-  if (S->p) (void)hipFree(S->p);
-  if (S->q) (void)hipFree(S->q);
-  if (S->part) (void)hipFree(S->part);
+  double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
   if (S->ctrl) (void)hipFree(S->ctrl);
   if (S->host_ctrl) (void)hipHostFree(S->host_ctrl);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
@@ -527,26 +488,20 @@ int moons_pcg_set_isp(moons_pcg_t S, int iter_max, double tol_rel, double tol_ab
   return MOONS_OK;
 }
 
-static int pcg_launch_iteration(moons_pcg_s *S, moons_field_s *x) {
-  const dim3 grid(S->G.nblocks), block(kNT);
-  hipLaunchKernelGGL(k_pcg_matvec, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->part);
-  hipLaunchKernelGGL(k_pcg_update, grid, block, 0, stream(), x->d, S->p, S->r, S->q, S->G, S->ctrl, S->part);
-  hipLaunchKernelGGL(k_pcg_pupdate, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl);
-  MOONS_LAUNCH_CHECK();
-  return MOONS_OK;
-}
+
 
 int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pcg_info *info) {
   if (!S || !x || !b) return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: null argument");
   if (x->m != S->m || b->m != S->m || x->dl != S->xbc.dl || b->dl != S->xbc.dl)
     return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: x and b must match the solver's mesh and location");
-  int st = pcg_upload_ctrl(S, true);
+  int st = pcg_upload_ctrl(S);
   if (st) return st;
   const dim3 grid(S->G.nblocks), block(kNT);
   MOONS_HIP(hipEventRecord(S->ev0, stream()));
   const int alln = S->xbc.all_neumann ? 1 : 0;
-  if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->ctrl, S->part);
-  hipLaunchKernelGGL(k_pcg_rhs, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->part, alln);
+  if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->partM);
+  hipLaunchKernelGGL(k_pcg_rhs, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM, S->partB, alln);
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->G.nblocks);
   MOONS_LAUNCH_CHECK();
   // Chunks of iterations; every kernel returns at once after the exit decision.
   int launched = 0;
@@ -557,7 +512,7 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
     int chunk = std::max(4, std::min(64, launched / 4));
     chunk = std::min(chunk, S->iter_max - launched);
     for (int c = 0; c < chunk; ++c) {
-      st = pcg_launch_iteration(S, x);
+      st = pcg_launch_iteration(S, x->d, launched + c + 1);
       if (st) return st;
     }
     launched += chunk;
@@ -628,9 +583,8 @@ int moons_clean_div(moons_pcg_t S, moons_field_t X[3], moons_field_t Xstar[3], m
 
 int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   if (!S || !ms || reps < 1) return fail(MOONS_ERR_ARG, "moons_pcg_time_kernels: bad argument");
-  // Runs each kernel on the solver's current vectors with the loop forced
-  // open (done = 0) and x updated into scratch q... the update kernel writes
-  // x; use p's storage pattern on a scratch copy so the solver state is kept.
+  // Runs plain CG iterations on the solver's current p and r (which change)
+  // with the exit test disabled, updating a scratch x; events bracket each kernel.
   double *xs = nullptr;
   const size_t bytes = (size_t)S->G.s0 * S->G.s1 * S->G.s2 * sizeof(double);
   MOONS_HIP(hipMalloc(&xs, bytes));
@@ -639,32 +593,28 @@ int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   std::memset(&h, 0, sizeof(PcgCtrl));
   h.iter_max = 1 << 30;
   h.n_skip = 1 << 30;
-  h.tol_rel = 0.0;
-  h.tol_abs = 0.0;
-  h.rho = 1.0;
-  h.alpha = 0.0;
-  h.beta = 1.0;
+  h.rho[0] = h.rho[1] = 1.0;
   h.res0 = 1.0;
   MOONS_HIP(hipMemcpyAsync(S->ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, stream()));
-  const dim3 grid(S->G.nblocks), block(kNT);
   hipEvent_t e[4];
   for (int i = 0; i < 4; ++i) MOONS_HIP(hipEventCreate(&e[i]));
   double acc[3] = {0, 0, 0};
   for (int rep = 0; rep < reps + 1; ++rep) {
+    const int it = rep + 1;
+    int st;
     MOONS_HIP(hipEventRecord(e[0], stream()));
-    hipLaunchKernelGGL(k_pcg_matvec, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->part);
+    if ((st = pcg_launch(S, xs, it, 1))) return st;
     MOONS_HIP(hipEventRecord(e[1], stream()));
-    hipLaunchKernelGGL(k_pcg_update, grid, block, 0, stream(), xs, S->p, S->r, S->q, S->G, S->ctrl, S->part);
+    if ((st = pcg_launch(S, xs, it, 2))) return st;
     MOONS_HIP(hipEventRecord(e[2], stream()));
-    hipLaunchKernelGGL(k_pcg_pupdate, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl);
+    if ((st = pcg_launch(S, xs, it, 4))) return st;
     MOONS_HIP(hipEventRecord(e[3], stream()));
-    MOONS_LAUNCH_CHECK();
     MOONS_HIP(hipEventSynchronize(e[3]));
     if (rep == 0) continue; // warm-up
     for (int kk = 0; kk < 3; ++kk) {
-      float t = 0.f;
-      MOONS_HIP(hipEventElapsedTime(&t, e[kk], e[kk + 1]));
-      acc[kk] += t;
+      float tms = 0.f;
+      MOONS_HIP(hipEventElapsedTime(&tms, e[kk], e[kk + 1]));
+      acc[kk] += tms;
     }
   }
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
