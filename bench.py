@@ -128,22 +128,34 @@ def main():
                           for n, t in zip(names, kms)}), flush=True)
         return
 
-    def step():
-        x.fill(0.0)
-        return P.solve(x, b)
+    def timed(solver, xf):
+        def step():
+            xf.fill(0.0)
+            return solver.solve(xf, b)
 
-    for _ in range(args.warmup):
-        step()
-    M.synchronize()
-    infos = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        infos.append(step())
-    M.synchronize()
-    t1 = time.perf_counter()
-    ms_step = 1e3 * (t1 - t0) / args.steps
-    iters = infos[-1].iterations
-    rel = infos[-1].res_norm / infos[-1].res_norm0
+        for _ in range(args.warmup):
+            step()
+        M.synchronize()
+        infos = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            infos.append(step())
+        M.synchronize()
+        t1 = time.perf_counter()
+        return 1e3 * (t1 - t0) / args.steps, infos[-1]
+
+    ms_step, info = timed(P, x)
+    iters = info.iterations
+    rel = info.res_norm / info.res_norm0
+    # mixed precision: FP32 multigrid V-cycle inside the FP64 flexible PCG
+    xm = M.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
+    Pm = M.PCG_solver_SF(m, xm, M.iter_solver_params(2000, TOL, 1e-300, 1), prec="MG_FP32")
+    mg_ms, mg_info = timed(Pm, xm)
+    mg_levels, mg_coarsest = Pm.mg_info()
+    # agreement of the two solutions (free constant removed, interior cells)
+    xa, xb = x.download()[1:-1, 1:-1, 1:-1], xm.download()[1:-1, 1:-1, 1:-1]
+    xa, xb = xa - xa.mean(), xb - xb.mean()
+    mg_vs_fp64 = float(np.linalg.norm(xb - xa) / np.linalg.norm(xa))
 
     # per-kernel durations (HIP events on the library stream)
     kms = P.time_kernels(50)
@@ -173,14 +185,16 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(hs, bh, iters)
 
+    best = min(ms_step, mg_ms)
+    method = "mixed_mg_fp32_in_fp64_fpcg" if mg_ms <= ms_step else "fp64_jacobi_pcg"
     out = {
         "metric": METRIC,
-        "value": round(ms_step, 4),
+        "value": round(best, 4),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(best, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -188,10 +202,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ppe_128_duct", "cells": "128^3", "mesh": "x periodic uniform; y,z Roberts beta=1.05",
                    "bcs": "homogeneous Neumann walls, all_Neumann", "rhs": "(1/dt) div(U*), U* = 1e-3 SplitMix64 face noise",
-                   "tol_rel": TOL, "method": "fp64_jacobi_pcg", "iterations": iters,
-                   "final_rel_residual": rel},
+                   "tol_rel": TOL, "method": method,
+                   "iterations": mg_info.iterations if method.startswith("mixed") else iters},
         "plain_fp64_ms": round(ms_step, 4),
         "plain_fp64_iterations": iters,
+        "plain_fp64_final_rel_residual": rel,
+        "mixed_mg_ms": round(mg_ms, 4),
+        "mixed_mg_iterations": mg_info.iterations,
+        "mixed_mg_final_rel_residual": mg_info.res_norm / mg_info.res_norm0,
+        "mixed_mg_levels": mg_levels,
+        "mixed_mg_coarsest_cells": list(mg_coarsest),
+        "mixed_vs_fp64_solution_rel_l2": mg_vs_fp64,
+        "speedup_mixed_vs_plain": round(ms_step / mg_ms, 3),
         "pcie_inclusive_ms": round(pcie_ms, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -62,6 +62,10 @@ extern "C" {
 /* ----------------------------------------------------------- operators */
 #define MOONS_OP_LAP_UNIFORM_SF 1 /* code/handwritten/solvers/matrix_free_operators.f90:35-62 */
 #define MOONS_PREC_LAP_SF 1       /* code/handwritten/solvers/preconditioners/preconditioners.f90:55-71 */
+/* FP32 geometric multigrid V-cycle inside an FP64 flexible PCG.  No compiled
+ * counterpart in the reference (its V-cycle is dead code,
+ * code_not_compiled/9.6.2017/MG/MG.f90:241-396); same solve/exit interface. */
+#define MOONS_PREC_MG_FP32 2
 
 typedef struct moons_mesh_s *moons_mesh_t;
 typedef struct moons_field_s *moons_field_t;
@@ -173,6 +177,20 @@ int moons_pcg_sf_solve(moons_pcg_t pcg, moons_field_t x, moons_field_t b, moons_
 int moons_clean_div(moons_pcg_t pcg, moons_field_t X[3], moons_field_t Xstar[3], moons_field_t phi,
                     double scale, moons_field_t temp_F[3], moons_field_t temp_CC,
                     moons_pcg_info *info);
+
+/* Multigrid parameters of a MOONS_PREC_MG_FP32 solver (rebuilds the level
+ * hierarchy): at most max_levels levels, nu1/nu2 red-black Gauss-Seidel sweeps
+ * before/after the coarse correction, coarse_sweeps sweeps on the coarsest
+ * level.  No reference counterpart (see MOONS_PREC_MG_FP32). */
+int moons_pcg_set_mg(moons_pcg_t pcg, int max_levels, int nu1, int nu2, int coarse_sweeps);
+/* Number of multigrid levels and the coarsest level's cells per direction. */
+int moons_pcg_mg_info(moons_pcg_t pcg, int *levels, int n_coarsest[3]);
+/* Applies the solver's preconditioner to a volume-weighted residual r:
+ * z = Minv r (prec_Lap_SF) or z = V(r/vol) (one FP32 V-cycle).  The
+ * preconditioner interface of the reference is preconditioner_SF
+ * (code/handwritten/abstract_interfaces/preconditioner_interfaces.f90:13-23),
+ * which builds Minv; here the application itself is exposed. */
+int moons_pcg_apply_prec(moons_pcg_t pcg, moons_field_t r, moons_field_t z);
 
 /* Times one application of each kernel of the solver's iteration (matvec +
  * (p,Ap); x/r update + (r,r),(z,r); p update) with HIP events on the library

@@ -197,17 +197,34 @@ class pcg_info:
 
 
 class PCG_solver_SF:
-    """Jacobi-PCG for the cell-centred Laplacian (op Lap_uniform_SF, prec_Lap_SF)."""
+    """PCG for the cell-centred Laplacian (op Lap_uniform_SF).
+
+    prec="Lap_SF": the reference's Jacobi preconditioner (prec_Lap_SF), FP64.
+    prec="MG_FP32": one FP32 multigrid V-cycle inside an FP64 flexible PCG.
+    """
 
     OP_LAP_UNIFORM_SF = 1
-    PREC_LAP_SF = 1
+    PRECS = {"Lap_SF": 1, "MG_FP32": 2}
 
-    def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None):
+    def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None, prec: str = "Lap_SF"):
         h = C.c_void_p()
-        check(load().moons_pcg_sf_create(m.handle, self.OP_LAP_UNIFORM_SF, self.PREC_LAP_SF, x.handle, C.byref(h)))
+        check(load().moons_pcg_sf_create(m.handle, self.OP_LAP_UNIFORM_SF, self.PRECS[prec], x.handle, C.byref(h)))
         self.handle = h
         self.m = m
+        self.prec = prec
         self.set_ISP(ISP or iter_solver_params())
+
+    def set_MG(self, max_levels: int = 16, nu1: int = 2, nu2: int = 2, coarse_sweeps: int = 40) -> None:
+        check(load().moons_pcg_set_mg(self.handle, max_levels, nu1, nu2, coarse_sweeps))
+
+    def mg_info(self):
+        lv = C.c_int()
+        n = (C.c_int * 3)()
+        check(load().moons_pcg_mg_info(self.handle, C.byref(lv), n))
+        return lv.value, (n[0], n[1], n[2])
+
+    def apply_prec(self, r: SF, z: SF) -> None:
+        check(load().moons_pcg_apply_prec(self.handle, r.handle, z.handle))
 
     def set_ISP(self, ISP: iter_solver_params) -> None:
         self.ISP = ISP

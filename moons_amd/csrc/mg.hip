@@ -1,0 +1,389 @@
+// mg.hip -- FP32 geometric multigrid V-cycle for the cell-centred Laplacian.
+//
+// The reference has no compiled multigrid (its V-cycle exists only in
+// code_not_compiled/9.6.2017/MG/MG.f90:241-396: smooth, residual, restrict,
+// recurse, prolongate, correct, smooth).  This V-cycle follows those steps
+// with operators written from the finite-volume mathematics:
+//   operator    L x = lap(x) with the homogeneous (implicit) BCs folded into
+//               per-direction weights; L = A/vol for the PCG's A = vol*lap
+//   smoother    red-black Gauss-Seidel, red then black before the coarse
+//               correction and black then red after it (a symmetric cycle)
+//   restriction volume-weighted sum of the 8 children's residuals
+//   prolongation trilinear in the true cell-centre coordinates (stretched
+//               grids), coarse ghosts from the BC rule
+//   coarsest    one workgroup, a fixed number of red-black sweeps
+// Everything is FP32 (4 B/cell per array): this is the preconditioner of an
+// FP64 flexible PCG whose residual, dot products and BCs stay FP64.
+#include "mg.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cc_stencil.hpp"
+
+namespace moons {
+
+struct LevelDev {
+  int n0, n1, n2, s0, s1;
+  int per0, per1, per2;
+  float *x, *f;
+  const float *wm[3], *wp[3], *dg[3];
+  const float *rf[3];          // restriction fractions into the next coarser level
+  const int *pI[3], *pI2[3];   // prolongation sources in the next coarser level
+  const float *pa[3], *pb[3];  // and their weights
+};
+
+struct MGHier {
+  std::vector<LevelDev> L;
+  std::vector<float *> arrays; // device allocations to free
+  int nu1 = 2, nu2 = 2, coarse_sweeps = 40;
+};
+
+__device__ __forceinline__ size_t lidx(const LevelDev &L, int i, int j, int k) {
+  return (size_t)i + (size_t)L.s0 * ((size_t)j + (size_t)L.s1 * (size_t)k);
+}
+
+// Gauss-Seidel update of one cell: x = (f - sum_nb w x_nb) / diag.
+__device__ __forceinline__ float gs_cell(const LevelDev &L, const float *__restrict__ x, int i, int j, int k,
+                                         bool zero_guess) {
+  const float d = L.dg[0][i] + L.dg[1][j] + L.dg[2][k];
+  const float fv = L.f[lidx(L, i, j, k)];
+  if (zero_guess) return fv / d;
+  const int im = (i == 1) ? (L.per0 ? L.n0 : 1) : i - 1;
+  const int ip = (i == L.n0) ? (L.per0 ? 1 : L.n0) : i + 1;
+  const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1;
+  const int jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
+  const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1;
+  const int kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
+  float s = L.wm[0][i] * x[lidx(L, im, j, k)] + L.wp[0][i] * x[lidx(L, ip, j, k)];
+  s += L.wm[1][j] * x[lidx(L, i, jm, k)] + L.wp[1][j] * x[lidx(L, i, jp, k)];
+  s += L.wm[2][k] * x[lidx(L, i, j, km)] + L.wp[2][k] * x[lidx(L, i, j, kp)];
+  return (fv - s) / d;
+}
+
+// residual f - L x of one cell
+__device__ __forceinline__ float res_cell(const LevelDev &L, int i, int j, int k) {
+  const float *x = L.x;
+  const int im = (i == 1) ? (L.per0 ? L.n0 : 1) : i - 1;
+  const int ip = (i == L.n0) ? (L.per0 ? 1 : L.n0) : i + 1;
+  const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1;
+  const int jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
+  const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1;
+  const int kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
+  const float d = L.dg[0][i] + L.dg[1][j] + L.dg[2][k];
+  float s = L.wm[0][i] * x[lidx(L, im, j, k)] + L.wp[0][i] * x[lidx(L, ip, j, k)];
+  s += L.wm[1][j] * x[lidx(L, i, jm, k)] + L.wp[1][j] * x[lidx(L, i, jp, k)];
+  s += L.wm[2][k] * x[lidx(L, i, j, km)] + L.wp[2][k] * x[lidx(L, i, j, kp)];
+  s += d * x[lidx(L, i, j, k)];
+  return L.f[lidx(L, i, j, k)] - s;
+}
+
+// Cells of one colour, (i+j+k) % 2 == color, one thread each.
+__device__ __forceinline__ bool color_cell(const LevelDev &L, long long t, int color, int &i, int &j, int &k) {
+  const int h0 = (L.n0 + 1) / 2;
+  const long long per_k = (long long)h0 * L.n1;
+  if (t >= per_k * L.n2) return false;
+  k = 1 + (int)(t / per_k);
+  const long long r = t % per_k;
+  j = 1 + (int)(r / h0);
+  const int ipair = (int)(r % h0);
+  i = 2 * ipair + 1 + ((color + j + k + 1) & 1);
+  return i <= L.n0;
+}
+
+__global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int zero_guess, const int *done) {
+  if (done && *done) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const long long t = (long long)b * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!color_cell(L, t, color, i, j, k)) return;
+  L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, zero_guess != 0);
+}
+
+// Coarse f = sum over the 8 children of (child vol / coarse vol) * residual.
+__global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, const int *done) {
+  if (done && *done) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const long long t = (long long)b * blockDim.x + threadIdx.x;
+  const long long nc = (long long)C.n0 * C.n1 * C.n2;
+  if (t >= nc) return;
+  const int I = 1 + (int)(t % C.n0);
+  const int J = 1 + (int)((t / C.n0) % C.n1);
+  const int K = 1 + (int)(t / ((long long)C.n0 * C.n1));
+  float acc = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
+    acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
+  }
+  C.f[lidx(C, I, J, K)] = acc;
+}
+
+// x_fine += trilinear interpolation of x_coarse.
+__global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, const int *done) {
+  if (done && *done) return;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const long long t = (long long)b * blockDim.x + threadIdx.x;
+  const long long nf = (long long)F.n0 * F.n1 * F.n2;
+  if (t >= nf) return;
+  const int i = 1 + (int)(t % F.n0);
+  const int j = 1 + (int)((t / F.n0) % F.n1);
+  const int k = 1 + (int)(t / ((long long)F.n0 * F.n1));
+  const int I = F.pI[0][i], I2 = F.pI2[0][i];
+  const int J = F.pI[1][j], J2 = F.pI2[1][j];
+  const int K = F.pI[2][k], K2 = F.pI2[2][k];
+  const float ax = F.pa[0][i], bx = F.pb[0][i];
+  const float ay = F.pa[1][j], by = F.pb[1][j];
+  const float az = F.pa[2][k], bz = F.pb[2][k];
+  const float *c = C.x;
+  const float e = ax * (ay * (az * c[lidx(C, I, J, K)] + bz * c[lidx(C, I, J, K2)]) +
+                        by * (az * c[lidx(C, I, J2, K)] + bz * c[lidx(C, I, J2, K2)])) +
+                  bx * (ay * (az * c[lidx(C, I2, J, K)] + bz * c[lidx(C, I2, J, K2)]) +
+                        by * (az * c[lidx(C, I2, J2, K)] + bz * c[lidx(C, I2, J2, K2)]));
+  F.x[lidx(F, i, j, k)] += e;
+}
+
+// Coarsest level: one workgroup, `sweeps` red-black sweeps from x = 0.
+__global__ void __launch_bounds__(1024) k_mg_coarse(LevelDev L, int sweeps, const int *done) {
+  if (done && *done) return;
+  const int h0 = (L.n0 + 1) / 2;
+  const long long ncol = (long long)h0 * L.n1 * L.n2;
+  for (int s = 0; s < sweeps; ++s) {
+    for (int color = 0; color < 2; ++color) {
+      for (long long t = threadIdx.x; t < ncol; t += blockDim.x) {
+        int i, j, k;
+        if (color_cell(L, t, color, i, j, k)) L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, s == 0 && color == 0);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------- host
+namespace {
+
+struct HostLevel {
+  int n[3];
+  std::vector<double> hn[3]; // nodes incl. ghosts
+};
+
+std::vector<double> coarsen_nodes(const std::vector<double> &hn) {
+  // physical nodes hn[1..sn-2]; keep every other one, then mirror ghosts
+  std::vector<double> phys(hn.begin() + 1, hn.end() - 1);
+  std::vector<double> c;
+  for (size_t m = 0; m < phys.size(); m += 2) c.push_back(phys[m]);
+  std::vector<double> out(c.size() + 2);
+  out[0] = c[0] - (c[1] - c[0]);
+  for (size_t m = 0; m < c.size(); ++m) out[m + 1] = c[m];
+  out[c.size() + 1] = c.back() + (c.back() - c[c.size() - 2]);
+  return out;
+}
+
+bool dirichlet_like(int t) { return t == MOONS_BC_DIRICHLET || t == MOONS_BC_ANTISYMMETRIC; }
+
+} // namespace
+
+MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int coarse_sweeps, std::string *err) {
+  if (x->dl != MOONS_DL_CC) {
+    if (err) *err = "multigrid needs a cell-centred field";
+    return nullptr;
+  }
+  int bct[6];
+  for (int f = 0; f < 6; ++f) bct[f] = x->has_bcs ? x->bct[f] : MOONS_BC_NEUMANN;
+  std::vector<HostLevel> H;
+  {
+    HostLevel l0;
+    for (int d = 0; d < 3; ++d) {
+      l0.hn[d] = x->m->hc[d].hn;
+      l0.n[d] = x->s[d] - 2;
+    }
+    H.push_back(l0);
+  }
+  while ((int)H.size() < max_levels) {
+    const HostLevel &f = H.back();
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) ok = ok && (f.n[d] % 2 == 0) && (f.n[d] / 2 >= 2);
+    if (!ok) break;
+    HostLevel c;
+    for (int d = 0; d < 3; ++d) {
+      c.hn[d] = coarsen_nodes(f.hn[d]);
+      c.n[d] = f.n[d] / 2;
+    }
+    H.push_back(c);
+  }
+  MGHier *h = new MGHier();
+  h->nu1 = nu1;
+  h->nu2 = nu2;
+  h->coarse_sweeps = coarse_sweeps;
+  const int nl = (int)H.size();
+  h->L.resize(nl);
+  auto dalloc = [&](size_t n) -> float * {
+    float *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) return nullptr;
+    (void)hipMemsetAsync(p, 0, n * sizeof(float), stream());
+    h->arrays.push_back(p);
+    return p;
+  };
+  for (int l = 0; l < nl; ++l) {
+    const HostLevel &hl = H[l];
+    LevelDev &L = h->L[l];
+    L.n0 = hl.n[0]; L.n1 = hl.n[1]; L.n2 = hl.n[2];
+    L.s0 = L.n0 + 2; L.s1 = L.n1 + 2;
+    L.per0 = bct[0] == MOONS_BC_PERIODIC;
+    L.per1 = bct[2] == MOONS_BC_PERIODIC;
+    L.per2 = bct[4] == MOONS_BC_PERIODIC;
+    const size_t ncell = (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+    L.x = dalloc(ncell);
+    L.f = dalloc(ncell);
+    if (!L.x || !L.f) {
+      if (err) *err = "multigrid: hipMalloc failed";
+      mg_destroy(h);
+      return nullptr;
+    }
+    for (int d = 0; d < 3; ++d) {
+      const int n = hl.n[d], s = n + 2;
+      const std::vector<double> &hn = hl.hn[d];
+      std::vector<double> dhn(s), hc(s), dhc(s - 1);
+      for (int i = 0; i < s; ++i) {
+        dhn[i] = hn[i + 1] - hn[i];
+        hc[i] = 0.5 * (hn[i + 1] + hn[i]);
+      }
+      for (int i = 0; i < s - 1; ++i) dhc[i] = hc[i + 1] - hc[i];
+      std::vector<float> wm(s, 0.f), wp(s, 0.f), dg(s, 0.f), rf(s, 0.f), pa(s, 0.f), pb(s, 0.f);
+      std::vector<int> pI(s, 1), pI2(s, 1);
+      const int bmin = bct[2 * d], bmax = bct[2 * d + 1];
+      for (int i = 1; i <= n; ++i) {
+        double a = 1.0 / (dhc[i - 1] * dhn[i]);
+        double c = 1.0 / (dhc[i] * dhn[i]);
+        double diag = -(a + c);
+        if (i == 1 && bmin != MOONS_BC_PERIODIC) {
+          diag += dirichlet_like(bmin) ? -a : a; // ghost = -x or +x folded in
+          a = 0.0;
+        }
+        if (i == n && bmax != MOONS_BC_PERIODIC) {
+          diag += dirichlet_like(bmax) ? -c : c;
+          c = 0.0;
+        }
+        wm[i] = (float)a;
+        wp[i] = (float)c;
+        dg[i] = (float)diag;
+      }
+      if (l + 1 < nl) {
+        const std::vector<double> &hnc = H[l + 1].hn[d];
+        for (int i = 1; i <= n; ++i) rf[i] = (float)(dhn[i] / (hnc[(i + 1) / 2 + 1] - hnc[(i + 1) / 2]));
+      }
+      // prolongation weights for this level from its coarser level l+1
+      if (l + 1 < nl) {
+        const int nc = H[l + 1].n[d];
+        const std::vector<double> &hnc = H[l + 1].hn[d];
+        std::vector<double> hcc(nc + 2);
+        for (int I = 0; I < nc + 2; ++I) hcc[I] = 0.5 * (hnc[I + 1] + hnc[I]);
+        for (int i = 1; i <= n; ++i) {
+          const int I = (i + 1) / 2;
+          const bool left = (i % 2) == 1;
+          int I2 = left ? I - 1 : I + 1;
+          double a = (hc[i] - hcc[I2]) / (hcc[I] - hcc[I2]);
+          double bw = 1.0 - a;
+          if (I2 == 0 || I2 == nc + 1) {
+            const int bt = left ? bmin : bmax;
+            if (bt == MOONS_BC_PERIODIC) I2 = (I2 == 0) ? nc : 1;
+            else {
+              I2 = I;
+              if (dirichlet_like(bt)) bw = -bw;
+            }
+          }
+          pI[i] = I;
+          pI2[i] = I2;
+          pa[i] = (float)a;
+          pb[i] = (float)bw;
+        }
+      }
+      float *wmd = dalloc(s), *wpd = dalloc(s), *dgd = dalloc(s), *rfd = dalloc(s), *pad = dalloc(s), *pbd = dalloc(s);
+      int *pId = nullptr, *pI2d = nullptr;
+      float *tmp1 = dalloc(s), *tmp2 = dalloc(s);
+      pId = reinterpret_cast<int *>(tmp1);
+      pI2d = reinterpret_cast<int *>(tmp2);
+      if (!wmd || !wpd || !dgd || !rfd || !pad || !pbd || !tmp1 || !tmp2) {
+        if (err) *err = "multigrid: hipMalloc failed";
+        mg_destroy(h);
+        return nullptr;
+      }
+      (void)hipMemcpyAsync(wmd, wm.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(wpd, wp.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(dgd, dg.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(rfd, rf.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(pad, pa.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(pbd, pb.data(), s * sizeof(float), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(pId, pI.data(), s * sizeof(int), hipMemcpyHostToDevice, stream());
+      (void)hipMemcpyAsync(pI2d, pI2.data(), s * sizeof(int), hipMemcpyHostToDevice, stream());
+      (void)hipStreamSynchronize(stream());
+      L.wm[d] = wmd; L.wp[d] = wpd; L.dg[d] = dgd; L.rf[d] = rfd;
+      L.pa[d] = pad; L.pb[d] = pbd; L.pI[d] = pId; L.pI2[d] = pI2d;
+    }
+  }
+  return h;
+}
+
+void mg_destroy(MGHier *h) {
+  if (!h) return;
+  if (stream()) (void)hipStreamSynchronize(stream());
+  for (float *p : h->arrays) (void)hipFree(p);
+  delete h;
+}
+
+int mg_levels(const MGHier *h) { return (int)h->L.size(); }
+
+int mg_level_cells(const MGHier *h, int l, int n[3]) {
+  if (l < 0 || l >= (int)h->L.size()) return -1;
+  n[0] = h->L[l].n0; n[1] = h->L[l].n1; n[2] = h->L[l].n2;
+  return 0;
+}
+
+float *mg_f0(MGHier *h) { return h->L[0].f; }
+float *mg_x0(MGHier *h) { return h->L[0].x; }
+
+void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps) {
+  h->nu1 = nu1;
+  h->nu2 = nu2;
+  h->coarse_sweeps = coarse_sweeps;
+}
+
+static unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+static void smooth(const LevelDev &L, int color, bool zero_guess, const int *done) {
+  const long long nt = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
+  hipLaunchKernelGGL(k_mg_smooth, dim3(blocks_for(nt)), dim3(256), 0, stream(), L, color, zero_guess ? 1 : 0, done);
+}
+
+int mg_vcycle(MGHier *h, const int *done) {
+  const int nl = (int)h->L.size();
+  // descend
+  for (int l = 0; l < nl - 1; ++l) {
+    const LevelDev &L = h->L[l];
+    for (int s = 0; s < h->nu1; ++s) {
+      smooth(L, 0, s == 0, done);
+      smooth(L, 1, false, done);
+    }
+    const LevelDev &C = h->L[l + 1];
+    const long long nc = (long long)C.n0 * C.n1 * C.n2;
+    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, stream(), L, C, done);
+  }
+  hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
+  // ascend
+  for (int l = nl - 2; l >= 0; --l) {
+    const LevelDev &L = h->L[l];
+    const LevelDev &C = h->L[l + 1];
+    const long long nf = (long long)L.n0 * L.n1 * L.n2;
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(nf)), dim3(256), 0, stream(), L, C, done);
+    for (int s = 0; s < h->nu2; ++s) {
+      smooth(L, 1, false, done);
+      smooth(L, 0, false, done);
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));
+  return MOONS_OK;
+}
+
+} // namespace moons

@@ -1,0 +1,30 @@
+// mg.hpp -- FP32 geometric multigrid V-cycle for the cell-centred Laplacian,
+// used as the preconditioner of the mixed-precision flexible PCG.
+#pragma once
+
+#include <string>
+
+#include "common.hpp"
+
+namespace moons {
+
+struct MGHier;
+
+// Builds the level hierarchy for a cell-centred field's mesh and BCs:
+// coarse nodes are every other node of the finer level (the reference's
+// coordinate restriction, code/handwritten/grid/coordinates_extend.f90:336-355),
+// coarsening stops at an odd or < 4 cell count or after max_levels levels.
+MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int coarse_sweeps, std::string *err);
+void mg_destroy(MGHier *h);
+int mg_levels(const MGHier *h);
+int mg_level_cells(const MGHier *h, int l, int n[3]);
+// Fine-level right-hand side f0 = r/vol and output x0, FP32, with the padded
+// shape of the cell-centred field.
+float *mg_f0(MGHier *h);
+float *mg_x0(MGHier *h);
+// One V-cycle x0 = V f0 on the library stream; every kernel returns at once
+// when *done (device) is non-zero.
+int mg_vcycle(MGHier *h, const int *done);
+void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
+
+} // namespace moons

@@ -23,9 +23,11 @@
 // iterations (passing the iteration number) and reads the exit flag between
 // chunks.
 #include "cc_stencil.hpp"
+#include "mg.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace moons {
@@ -57,10 +59,11 @@ __global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, 
 
 // modify_RHS_SF + PCG set-up (PCG_aux.f90:30-54; PCG_solver_algorithm.f90:57-66):
 // r = vol*((b - mean) - A x_BC - A x), z = Minv r, p = z; partials of (r,r), (r,z).
+template <bool MG>
 __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, const double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ p, CCGeom G,
                                                 const PcgCtrl *ctrl, const double *partM, double *partB,
-                                                int all_neumann) {
+                                                int all_neumann, float *__restrict__ f0) {
   __shared__ double sh[kNT / kWave];
   double mean = 0.0;
   if (all_neumann) mean = prologue_sum<kNT>(partM, gridDim.x, 1, 0, sh) / ctrl->volsum;
@@ -100,13 +103,18 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
       if (all_neumann) rv = rv - mean;
       rv = rv - axbc;
       rv = rv - ax;
-      rv = rv * (dxy * G.c[2].dh[k]);
-      const double z = minv_at<double>(G, dxy, axy, k) * rv;
+      const double vol = dxy * G.c[2].dh[k];
+      rv = rv * vol;
       r[o] = rv;
-      p[o] = z;
-      fill_ghosts_implicit(p, G, i, j, k, z);
       a_rr += rv * rv;
-      a_rz += rv * z;
+      if (MG) {
+        f0[o] = (float)(rv / vol);
+      } else {
+        const double z = minv_at<double>(G, dxy, axy, k) * rv;
+        p[o] = z;
+        fill_ghosts_implicit(p, G, i, j, k, z);
+        a_rz += rv * z;
+      }
     }
   }
   const double srr = block_sum<kNT>(a_rr, sh);
@@ -151,35 +159,18 @@ __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p
     const int i = t.i, j = t.j;
     const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
     const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
-    const double *pc = p + i + (size_t)j * sx + (size_t)t.k0 * sxy;
-    double *qc = q + i + (size_t)j * sx + (size_t)t.k0 * sxy;
-    const int nk = t.k1 - t.k0;
-    // all planes' loads first (the column and its four in-plane neighbours)
-    // col[u+1] = plane k0+u for u = -1 .. nk (both z neighbours included)
-    double col[KZ + 2], xm[KZ], xp[KZ], ym[KZ], yp[KZ];
-    col[0] = pc[-(ptrdiff_t)sxy];
-#pragma unroll
-    for (int u = 0; u <= KZ; ++u)
-      if (u <= nk) col[u + 1] = pc[(size_t)u * sxy];
-#pragma unroll
-    for (int u = 0; u < KZ; ++u) {
-      if (u < nk) {
-        const size_t o = (size_t)u * sxy;
-        xm[u] = pc[o - 1];
-        xp[u] = pc[o + 1];
-        ym[u] = pc[o - sx];
-        yp[u] = pc[o + sx];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < KZ; ++u) {
-      if (u < nk) {
-        const int k = t.k0 + u;
-        const double v = lap7(col[u + 1], xm[u], xp[u], ym[u], yp[u], col[u], col[u + 2], G.c[0], i, G.c[1], j, G.c[2], k);
-        const double qv = v * (dxy * G.c[2].dh[k]);
-        qc[(size_t)u * sxy] = qv;
-        acc += col[u + 1] * qv;
-      }
+    const double *pc = p + i + (size_t)j * sx;
+    double pm = pc[(size_t)(t.k0 - 1) * sxy];
+    double p0 = pc[(size_t)t.k0 * sxy];
+    for (int k = t.k0; k < t.k1; ++k) {
+      const size_t ok = (size_t)k * sxy;
+      const double pp = pc[ok + sxy];
+      const double v = lap7(p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, G.c[0], i, G.c[1], j, G.c[2], k);
+      const double qv = v * (dxy * G.c[2].dh[k]);
+      q[i + (size_t)j * sx + ok] = qv;
+      acc += p0 * qv;
+      pm = p0;
+      p0 = pp;
     }
   }
   const double s = block_sum<kNT>(acc, sh);
@@ -187,12 +178,30 @@ __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p
 }
 
 // alpha = rho/(p,q); x = x + p*alpha; r = r + q*(-alpha); partials (r,r), (Minv r, r).
-template <int KZ>
+template <int KZ, bool MG>
 __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, const double *__restrict__ p,
                                                    double *__restrict__ r, const double *__restrict__ q, CCGeom G,
-                                                   PcgCtrl *ctrl, const double *partA, double *partB, int it) {
+                                                   PcgCtrl *ctrl, const double *partA, double *partB, int it,
+                                                   float *__restrict__ f0) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
+  const TileIdx t = tile_of(G);
+  const int i = t.i, j = t.j;
+  const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
+  const int nk = t.active ? t.k1 - t.k0 : 0;
+  const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
+  // stream loads first, so they are in flight during the prologue reduction
+  double xv[KZ], pv[KZ], rv[KZ], qv[KZ];
+#pragma unroll
+  for (int u = 0; u < KZ; ++u) {
+    if (u < nk) {
+      const size_t o = base + (size_t)u * sxy;
+      xv[u] = x[o];
+      pv[u] = p[o];
+      rv[u] = r[o];
+      qv[u] = q[o];
+    }
+  }
   const double pq = prologue_sum<kNT>(partA, gridDim.x, 1, 0, sh);
   const double alpha = ctrl->rho[(it - 1) & 1] / pq;
   const double malpha = -alpha;
@@ -200,26 +209,10 @@ __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, cons
     ctrl->pq = pq;
     ctrl->alpha = alpha;
   }
-  const TileIdx t = tile_of(G);
   double a_rr = 0.0, a_rz = 0.0;
   if (t.active) {
-    const int i = t.i, j = t.j;
-    const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
     const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
     const double axy = G.c[0].lA[i] + G.c[0].lB[i] + G.c[1].lA[j] + G.c[1].lB[j];
-    const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
-    const int nk = t.k1 - t.k0;
-    double xv[KZ], pv[KZ], rv[KZ], qv[KZ];
-#pragma unroll
-    for (int u = 0; u < KZ; ++u) {
-      if (u < nk) {
-        const size_t o = base + (size_t)u * sxy;
-        xv[u] = x[o];
-        pv[u] = p[o];
-        rv[u] = r[o];
-        qv[u] = q[o];
-      }
-    }
 #pragma unroll
     for (int u = 0; u < KZ; ++u) {
       if (u < nk) {
@@ -227,9 +220,13 @@ __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, cons
         x[o] = xv[u] + pv[u] * alpha;
         const double rn = rv[u] + qv[u] * malpha;
         r[o] = rn;
-        const double z = minv_at<double>(G, dxy, axy, t.k0 + u) * rn;
         a_rr += rn * rn;
-        a_rz += z * rn;
+        if (MG) {
+          f0[o] = (float)(rn / (dxy * G.c[2].dh[t.k0 + u]));
+        } else {
+          const double z = minv_at<double>(G, dxy, axy, t.k0 + u) * rn;
+          a_rz += z * rn;
+        }
       }
     }
   }
@@ -241,6 +238,42 @@ __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, cons
   }
 }
 
+// Exit decision after the x/r update of iteration `it` (check_res every
+// n_skip, update_exit_loop: hw/var_set/iter_solver_params_extend.f90:86-113).
+struct ExitDecision {
+  bool stop, checked;
+  int flags;
+  double res;
+};
+
+__device__ __forceinline__ ExitDecision exit_decision(const PcgCtrl *ctrl, int it, double rr) {
+  ExitDecision e{false, false, 0, 0.0};
+  if (it % ctrl->n_skip == 0) {
+    e.checked = true;
+    e.res = sqrt(fabs(rr));
+    const bool e1 = e.res / ctrl->res0 < ctrl->tol_rel, e2 = e.res < ctrl->tol_abs, e3 = it >= ctrl->iter_max;
+    e.flags = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
+    e.stop = e1 || e2 || e3;
+  }
+  if (!e.stop && it >= ctrl->iter_max) {
+    e.flags |= 4;
+    e.stop = true;
+  }
+  return e;
+}
+
+__device__ __forceinline__ void publish_exit(PcgCtrl *ctrl, int it, double rr, const ExitDecision &e) {
+  ctrl->iter = it;
+  ctrl->rr = rr;
+  if (e.checked) {
+    ctrl->res = e.res;
+    ctrl->exit_loop = e.flags;
+  } else if (e.stop) {
+    ctrl->exit_loop |= e.flags;
+  }
+  if (e.stop) ctrl->done = 1;
+}
+
 // Exit test of iteration `it` (check_res every n_skip, update_exit_loop,
 // hw/var_set/iter_solver_params_extend.f90:86-113), then p = p*beta + Minv r
 // and the face ghosts of p.  Block 0 publishes the new state.
@@ -249,48 +282,11 @@ __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, con
                                                     PcgCtrl *ctrl, const double *partB, int it) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
-  const double rr = prologue_sum<kNT>(partB, gridDim.x, 2, 0, sh);
-  const double rz = prologue_sum<kNT>(partB, gridDim.x, 2, 1, sh);
-  const double rho_old = ctrl->rho[(it - 1) & 1];
-  bool stop = false, checked = false;
-  int flags = 0;
-  double res = 0.0;
-  if (it % ctrl->n_skip == 0) {
-    checked = true;
-    res = sqrt(fabs(rr));
-    const bool e1 = res / ctrl->res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = it >= ctrl->iter_max;
-    flags = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
-    stop = e1 || e2 || e3;
-  }
-  const double beta = rz / rho_old;
-  if (!stop && it >= ctrl->iter_max) {
-    flags |= 4;
-    stop = true;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctrl->iter = it;
-    ctrl->rr = rr;
-    ctrl->rz = rz;
-    if (checked) {
-      ctrl->res = res;
-      ctrl->exit_loop = flags;
-    } else if (stop) {
-      ctrl->exit_loop |= flags;
-    }
-    ctrl->beta = beta;
-    ctrl->rho[it & 1] = rz;
-    if (stop) ctrl->done = 1;
-  }
-  if (stop) return;
   const TileIdx t = tile_of(G);
-  if (!t.active) return;
   const int i = t.i, j = t.j;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
-  const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
-  const double axy = G.c[0].lA[i] + G.c[0].lB[i] + G.c[1].lA[j] + G.c[1].lB[j];
+  const int nk = t.active ? t.k1 - t.k0 : 0;
   const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
-  const bool edge_ij = (i == 1) || (i == G.n0) || (j == 1) || (j == G.n1);
-  const int nk = t.k1 - t.k0;
   double rv[KZ], pv[KZ];
 #pragma unroll
   for (int u = 0; u < KZ; ++u) {
@@ -299,6 +295,21 @@ __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, con
       pv[u] = p[base + (size_t)u * sxy];
     }
   }
+  const double rr = prologue_sum<kNT>(partB, gridDim.x, 2, 0, sh);
+  const double rz = prologue_sum<kNT>(partB, gridDim.x, 2, 1, sh);
+  const double rho_old = ctrl->rho[(it - 1) & 1];
+  const ExitDecision e = exit_decision(ctrl, it, rr);
+  const double beta = rz / rho_old;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    publish_exit(ctrl, it, rr, e);
+    ctrl->rz = rz;
+    ctrl->beta = beta;
+    ctrl->rho[it & 1] = rz;
+  }
+  if (e.stop || !t.active) return;
+  const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
+  const double axy = G.c[0].lA[i] + G.c[0].lB[i] + G.c[1].lA[j] + G.c[1].lB[j];
+  const bool edge_ij = (i == 1) || (i == G.n0) || (j == 1) || (j == G.n1);
 #pragma unroll
   for (int u = 0; u < KZ; ++u) {
     if (u < nk) {
@@ -308,6 +319,138 @@ __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, con
       p[base + (size_t)u * sxy] = pn;
       if (edge_ij || k == 1 || k == G.n2) fill_ghosts_implicit(p, G, i, j, k, pn);
     }
+  }
+}
+
+// ----------------------------------------------------- multigrid-mode kernels
+// The FP64 flexible PCG around the FP32 V-cycle: z = V(r/vol), rho = (z,r),
+// beta = (z_new, r_new - r_old)/rho_old = -alpha (z_new, q)/rho_old
+// (Polak-Ribiere form, exact for a fixed preconditioner and robust to the
+// FP32 V-cycle's rounding).
+
+// Exit test after the x/r update (one block; rr partials of k_pcg_update).
+__global__ void __launch_bounds__(kNT) k_pcg_check(PcgCtrl *ctrl, const double *partB, int nb, int it) {
+  if (ctrl->done) return;
+  __shared__ double sh[kNT / kWave];
+  const double rr = prologue_sum<kNT>(partB, nb, 2, 0, sh);
+  const ExitDecision e = exit_decision(ctrl, it, rr);
+  if (threadIdx.x == 0) publish_exit(ctrl, it, rr, e);
+}
+
+// Partials of (z,r) and (z,q) with z the V-cycle output.
+template <int KZ>
+__global__ void __launch_bounds__(kNT) k_mg_zdots(const float *__restrict__ z32, const double *__restrict__ r,
+                                                 const double *__restrict__ q, CCGeom G, const PcgCtrl *ctrl,
+                                                 double *partC, int use_q) {
+  if (ctrl->done) return;
+  __shared__ double sh[kNT / kWave];
+  const TileIdx t = tile_of(G);
+  double a_zr = 0.0, a_zq = 0.0;
+  if (t.active) {
+    const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
+    const size_t base = (size_t)t.i + (size_t)t.j * sx + (size_t)t.k0 * sxy;
+    const int nk = t.k1 - t.k0;
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const size_t o = base + (size_t)u * sxy;
+        const double z = (double)z32[o];
+        a_zr += z * r[o];
+        if (use_q) a_zq += z * q[o];
+      }
+    }
+  }
+  const double s0 = block_sum<kNT>(a_zr, sh);
+  const double s1 = block_sum<kNT>(a_zq, sh);
+  if (threadIdx.x == 0) {
+    partC[2 * blockIdx.x] = s0;
+    partC[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// Set-up exit test with z = V(r0): res0 = |r0|, rho0 = (z,r0), res = sqrt|rho0|.
+__global__ void __launch_bounds__(kNT) k_mg_init(PcgCtrl *ctrl, const double *partB, const double *partC, int nb) {
+  __shared__ double sh[kNT / kWave];
+  const double rr = prologue_sum<kNT>(partB, nb, 2, 0, sh);
+  const double zr = prologue_sum<kNT>(partC, nb, 2, 0, sh);
+  if (threadIdx.x == 0) {
+    const double res0 = sqrt(rr);
+    const double res = sqrt(fabs(zr));
+    ctrl->res0 = res0;
+    ctrl->rho[0] = zr;
+    ctrl->res = res;
+    ctrl->iter = 0;
+    const bool e1 = res / res0 < ctrl->tol_rel, e2 = res < ctrl->tol_abs, e3 = 0 >= ctrl->iter_max;
+    ctrl->exit_loop = (e1 ? 1 : 0) | (e2 ? 2 : 0) | (e3 ? 4 : 0);
+    ctrl->done = e2 ? 1 : 0;
+    if (e2) ctrl->iter = 1;
+  }
+}
+
+// p = p*beta + z (it >= 1) or p = z (it == 0), then the face ghosts of p.
+template <int KZ>
+__global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, const float *__restrict__ z32, CCGeom G,
+                                                   PcgCtrl *ctrl, const double *partC, int it) {
+  if (ctrl->done) return;
+  __shared__ double sh[kNT / kWave];
+  const TileIdx t = tile_of(G);
+  const int i = t.i, j = t.j;
+  const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
+  const int nk = t.active ? t.k1 - t.k0 : 0;
+  const size_t base = (size_t)i + (size_t)j * sx + (size_t)t.k0 * sxy;
+  float zv[KZ];
+  double pv[KZ];
+#pragma unroll
+  for (int u = 0; u < KZ; ++u) {
+    if (u < nk) {
+      zv[u] = z32[base + (size_t)u * sxy];
+      pv[u] = it > 0 ? p[base + (size_t)u * sxy] : 0.0;
+    }
+  }
+  double beta = 0.0;
+  if (it > 0) {
+    const double zr = prologue_sum<kNT>(partC, gridDim.x, 2, 0, sh);
+    const double zq = prologue_sum<kNT>(partC, gridDim.x, 2, 1, sh);
+    const double rho_old = ctrl->rho[(it - 1) & 1];
+    beta = -ctrl->alpha * zq / rho_old;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctrl->rz = zr;
+      ctrl->beta = beta;
+      ctrl->rho[it & 1] = zr;
+    }
+  }
+  if (!t.active) return;
+  const bool edge_ij = (i == 1) || (i == G.n0) || (j == 1) || (j == G.n1);
+#pragma unroll
+  for (int u = 0; u < KZ; ++u) {
+    if (u < nk) {
+      const int k = t.k0 + u;
+      const double z = (double)zv[u];
+      const double pn = it > 0 ? pv[u] * beta + z : z;
+      p[base + (size_t)u * sxy] = pn;
+      if (edge_ij || k == 1 || k == G.n2) fill_ghosts_implicit(p, G, i, j, k, pn);
+    }
+  }
+}
+
+// f0 = r/vol (FP32) for one preconditioner application; z = x0 back to FP64.
+__global__ void __launch_bounds__(kNT) k_mg_r2f(const double *__restrict__ r, float *__restrict__ f0, CCGeom G) {
+  const TileIdx t = tile_of(G);
+  if (!t.active) return;
+  const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
+  const double dxy = G.c[0].dh[t.i] * G.c[1].dh[t.j];
+  for (int k = t.k0; k < t.k1; ++k) {
+    const size_t o = (size_t)t.i + (size_t)t.j * sx + (size_t)k * sxy;
+    f0[o] = (float)(r[o] / (dxy * G.c[2].dh[k]));
+  }
+}
+__global__ void __launch_bounds__(kNT) k_mg_x2z(const float *__restrict__ x0, double *__restrict__ z, CCGeom G) {
+  const TileIdx t = tile_of(G);
+  if (!t.active) return;
+  const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
+  for (int k = t.k0; k < t.k1; ++k) {
+    const size_t o = (size_t)t.i + (size_t)t.j * sx + (size_t)k * sxy;
+    z[o] = (double)x0[o];
   }
 }
 
@@ -353,7 +496,12 @@ CCGeom make_geom(const moons_field_s *x, int kz) {
 }
 
 int pick_kz(int n2) {
-  // z cells marched per thread: enough blocks to fill 256 CUs several times over
+  // z cells marched per thread: enough blocks to fill 256 CUs several times
+  // over.  MOONS_KZ (2, 4, 8 or 16) overrides the choice, for tuning sweeps.
+  if (const char *e = getenv("MOONS_KZ")) {
+    int v = atoi(e);
+    if (v == 2 || v == 4 || v == 8 || v == 16) return v;
+  }
   if (n2 >= 128) return 8;
   if (n2 >= 32) return 4;
   return 2;
@@ -372,7 +520,9 @@ struct moons_pcg_s {
   double volsum = 0.0;
   CCGeom G;
   double *r = nullptr, *p = nullptr, *q = nullptr; // device work vectors
-  double *partA = nullptr, *partB = nullptr, *partM = nullptr;
+  double *partA = nullptr, *partB = nullptr, *partM = nullptr, *partC = nullptr;
+  MGHier *mg = nullptr; // MOONS_PREC_MG_FP32 only
+  int mg_max_levels = 16, mg_nu1 = 2, mg_nu2 = 2, mg_coarse = 40;
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -394,21 +544,45 @@ static int pcg_upload_ctrl(moons_pcg_s *S) {
 template <int KZ>
 static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
   const dim3 grid(S->G.nblocks), block(kNT);
+  const bool mg = S->mg != nullptr;
+  float *f0 = mg ? mg_f0(S->mg) : nullptr;
   if (which & 1) hipLaunchKernelGGL(k_pcg_matvec<KZ>, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA);
-  if (which & 2)
-    hipLaunchKernelGGL(k_pcg_update<KZ>, grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl, S->partA, S->partB, it);
+  if (which & 2) {
+    if (mg)
+      hipLaunchKernelGGL((k_pcg_update<KZ, true>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
+                         S->partA, S->partB, it, f0);
+    else
+      hipLaunchKernelGGL((k_pcg_update<KZ, false>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
+                         S->partA, S->partB, it, f0);
+  }
   if (which & 4) hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
+  if (which & 8) hipLaunchKernelGGL(k_mg_zdots<KZ>, grid, block, 0, stream(), mg_x0(S->mg), S->r, S->q, S->G, S->ctrl,
+                                    S->partC, it > 0 ? 1 : 0);
+  if (which & 16) hipLaunchKernelGGL(k_mg_pupdate<KZ>, grid, block, 0, stream(), S->p, mg_x0(S->mg), S->G, S->ctrl,
+                                     S->partC, it);
 }
 
-// which: bit 0 matvec, bit 1 update, bit 2 p-update
+// which: bit 0 matvec, 1 update, 2 p-update (Jacobi), 3 (z,r),(z,q) dots, 4 p-update (multigrid)
 static int pcg_launch(moons_pcg_s *S, double *x, int it, int which) {
   switch (S->G.kz) {
+    case 16: pcg_launch_kz<16>(S, x, it, which); break;
     case 8: pcg_launch_kz<8>(S, x, it, which); break;
     case 4: pcg_launch_kz<4>(S, x, it, which); break;
     default: pcg_launch_kz<2>(S, x, it, which); break;
   }
   MOONS_LAUNCH_CHECK();
   return MOONS_OK;
+}
+
+// One multigrid-preconditioned iteration: matvec, update (+ f0 = r/vol),
+// exit test, V-cycle, (z,r),(z,q), p update.
+static int pcg_launch_iteration_mg(moons_pcg_s *S, double *x, int it) {
+  int st = pcg_launch(S, x, it, 1 | 2);
+  if (st) return st;
+  hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kNT), 0, stream(), S->ctrl, S->partB, S->G.nblocks, it);
+  MOONS_LAUNCH_CHECK();
+  if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+  return pcg_launch(S, x, it, 8 | 16);
 }
 
 static int pcg_launch_iteration(moons_pcg_s *S, double *x, int it) { return pcg_launch(S, x, it, 7); }
@@ -418,8 +592,8 @@ extern "C" {
 int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons_pcg_t *out) {
   if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
   if (!m || !x || !out) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: null argument");
-  if (op != MOONS_OP_LAP_UNIFORM_SF || prec != MOONS_PREC_LAP_SF)
-    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: only Lap_uniform_SF with prec_Lap_SF is available");
+  if (op != MOONS_OP_LAP_UNIFORM_SF || (prec != MOONS_PREC_LAP_SF && prec != MOONS_PREC_MG_FP32))
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF with prec_Lap_SF or the FP32 multigrid only");
   if (x->dl != MOONS_DL_CC)
     return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF solver needs a cell-centred field");
   if (x->m != m) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: field is on another mesh");
@@ -445,6 +619,7 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
             hipMalloc(&S->q, bytes) == hipSuccess && hipMalloc(&S->partA, nb * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->partB, 2 * nb * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->partM, nb * sizeof(double)) == hipSuccess &&
+            hipMalloc(&S->partC, 2 * nb * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->ctrl, sizeof(PcgCtrl)) == hipSuccess &&
             hipHostMalloc(&S->host_ctrl, sizeof(PcgCtrl), hipHostMallocDefault) == hipSuccess &&
             hipEventCreate(&S->ev0) == hipSuccess && hipEventCreate(&S->ev1) == hipSuccess;
@@ -455,6 +630,14 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
   (void)hipMemsetAsync(S->r, 0, bytes, stream());
   (void)hipMemsetAsync(S->p, 0, bytes, stream());
   (void)hipMemsetAsync(S->q, 0, bytes, stream());
+  if (prec == MOONS_PREC_MG_FP32) {
+    std::string err;
+    S->mg = mg_create(x, S->mg_max_levels, S->mg_nu1, S->mg_nu2, S->mg_coarse, &err);
+    if (!S->mg) {
+      moons_pcg_destroy(S);
+      return fail(MOONS_ERR_HIP, "moons_pcg_sf_create: " + err);
+    }
+  }
   int st = pcg_upload_ctrl(S);
   if (st) {
     moons_pcg_destroy(S);
@@ -467,7 +650,8 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
 int moons_pcg_destroy(moons_pcg_t S) {
   if (!S) return MOONS_OK;
   if (stream()) (void)hipStreamSynchronize(stream());
-  double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM};
+  if (S->mg) mg_destroy(S->mg);
+  double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM, S->partC};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (S->ctrl) (void)hipFree(S->ctrl);
@@ -499,20 +683,34 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
   const dim3 grid(S->G.nblocks), block(kNT);
   MOONS_HIP(hipEventRecord(S->ev0, stream()));
   const int alln = S->xbc.all_neumann ? 1 : 0;
+  const bool mg = S->mg != nullptr;
   if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->partM);
-  hipLaunchKernelGGL(k_pcg_rhs, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM, S->partB, alln);
-  hipLaunchKernelGGL(k_pcg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->G.nblocks);
-  MOONS_LAUNCH_CHECK();
+  if (!mg) {
+    hipLaunchKernelGGL(k_pcg_rhs<false>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
+                       S->partB, alln, (float *)nullptr);
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->G.nblocks);
+    MOONS_LAUNCH_CHECK();
+  } else {
+    // z0 = V(r0/vol), rho0 = (z0,r0), p = z0
+    hipLaunchKernelGGL(k_pcg_rhs<true>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
+                       S->partB, alln, mg_f0(S->mg));
+    MOONS_LAUNCH_CHECK();
+    if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+    if ((st = pcg_launch(S, x->d, 0, 8))) return st;
+    hipLaunchKernelGGL(k_mg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->partC, S->G.nblocks);
+    MOONS_LAUNCH_CHECK();
+    if ((st = pcg_launch(S, x->d, 0, 16))) return st;
+  }
   // Chunks of iterations; every kernel returns at once after the exit decision.
   int launched = 0;
   for (;;) {
     MOONS_HIP(hipMemcpyAsync(S->host_ctrl, S->ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, stream()));
     MOONS_HIP(hipStreamSynchronize(stream()));
     if (S->host_ctrl->done || launched >= S->iter_max) break;
-    int chunk = std::max(4, std::min(64, launched / 4));
+    int chunk = mg ? std::max(2, std::min(8, launched / 2)) : std::max(4, std::min(64, launched / 4));
     chunk = std::min(chunk, S->iter_max - launched);
     for (int c = 0; c < chunk; ++c) {
-      st = pcg_launch_iteration(S, x->d, launched + c + 1);
+      st = mg ? pcg_launch_iteration_mg(S, x->d, launched + c + 1) : pcg_launch_iteration(S, x->d, launched + c + 1);
       if (st) return st;
     }
     launched += chunk;
@@ -579,6 +777,49 @@ int moons_clean_div(moons_pcg_t S, moons_field_t X[3], moons_field_t Xstar[3], m
   for (int d = 0; d < 3; ++d)
     if ((st = launch_apply_bcs(X[d], false))) return st;
   return MOONS_OK;
+}
+
+int moons_pcg_set_mg(moons_pcg_t S, int max_levels, int nu1, int nu2, int coarse_sweeps) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_set_mg: null solver");
+  if (!S->mg) return fail(MOONS_ERR_STATE, "moons_pcg_set_mg: solver was not created with MOONS_PREC_MG_FP32");
+  if (max_levels < 1 || nu1 < 1 || nu2 < 0 || coarse_sweeps < 1)
+    return fail(MOONS_ERR_ARG, "moons_pcg_set_mg: need max_levels >= 1, nu1 >= 1, nu2 >= 0, coarse_sweeps >= 1");
+  mg_destroy(S->mg);
+  S->mg_max_levels = max_levels;
+  S->mg_nu1 = nu1;
+  S->mg_nu2 = nu2;
+  S->mg_coarse = coarse_sweeps;
+  std::string err;
+  S->mg = mg_create(&S->xbc, max_levels, nu1, nu2, coarse_sweeps, &err);
+  if (!S->mg) return fail(MOONS_ERR_HIP, "moons_pcg_set_mg: " + err);
+  return MOONS_OK;
+}
+
+int moons_pcg_mg_info(moons_pcg_t S, int *levels, int n_coarsest[3]) {
+  if (!S || !S->mg) return fail(MOONS_ERR_STATE, "moons_pcg_mg_info: not a multigrid solver");
+  const int nl = mg_levels(S->mg);
+  if (levels) *levels = nl;
+  if (n_coarsest) mg_level_cells(S->mg, nl - 1, n_coarsest);
+  return MOONS_OK;
+}
+
+int moons_pcg_apply_prec(moons_pcg_t S, moons_field_t r, moons_field_t z) {
+  if (!S || !r || !z) return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: null argument");
+  if (r->dl != S->xbc.dl || z->dl != S->xbc.dl || r->m != S->m || z->m != S->m)
+    return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: fields must match the solver");
+  const dim3 grid(S->G.nblocks), block(kNT);
+  if (S->mg) {
+    int st = pcg_upload_ctrl(S); // done = 0
+    if (st) return st;
+    hipLaunchKernelGGL(k_mg_r2f, grid, block, 0, stream(), r->d, mg_f0(S->mg), S->G);
+    MOONS_LAUNCH_CHECK();
+    if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+    MOONS_HIP(hipMemsetAsync(z->d, 0, z->n * sizeof(double), stream()));
+    hipLaunchKernelGGL(k_mg_x2z, grid, block, 0, stream(), mg_x0(S->mg), z->d, S->G);
+    MOONS_LAUNCH_CHECK();
+    return MOONS_OK;
+  }
+  return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: the Jacobi preconditioner is applied inside the PCG kernels only");
 }
 
 int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {

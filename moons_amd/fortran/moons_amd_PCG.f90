@@ -1,0 +1,199 @@
+! moons_amd_PCG.f90 -- Fortran host surface of the GPU pressure-Poisson path.
+!
+! Mirrors the reference's call surface so a MOONS host swaps in the GPU path
+! at the same call sites:
+!   init(PCG, ...)          init_PCG_SF   (code/handwritten/PCG/PCG_solver_extend.f90:51-107)
+!   solve(PCG, x, b, ...)   solve_PCG_SF  (code/handwritten/PCG/PCG_solver_extend.f90:182-193)
+!   clean_div(PCG, ...)     clean_div_PCG (code/handwritten/solvers/clean_divergence.f90:37-59)
+! Fields are the host's ghost-padded, column-major real(8) arrays (the layout
+! of grid_field%f, code/generated_code/grid_field/grid_field.f90:39-43); they
+! are copied to the device at solve entry and back at exit (the PCIe-inclusive
+! boundary).  Device-resident callers use the C ABI handles directly.
+module moons_amd_PCG_mod
+  use iso_c_binding
+  use moons_amd_bindings
+  implicit none
+  private
+
+  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu
+  public :: init, solve, delete, clean_div
+  public :: init_CC, init_Face, set_BCs
+
+  type mesh_gpu
+    type(c_ptr) :: h = c_null_ptr
+    integer :: sn(3) = 0
+  end type
+
+  type SF_gpu
+    type(c_ptr) :: h = c_null_ptr
+    integer :: s(3) = 0
+    integer(c_int) :: dl = 0
+  end type
+
+  type PCG_solver_SF_gpu
+    type(c_ptr) :: h = c_null_ptr
+    integer :: iter_max = 1000, n_skip_check_res = 1
+    real(c_double) :: tol_rel = 1.0d-6, tol_abs = 1.0d-13
+    type(moons_pcg_info) :: info
+    type(SF_gpu) :: x, b ! device mirrors of the solve's x and b
+  end type
+
+  interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu; end interface
+  interface solve;  module procedure solve_PCG_SF_gpu; end interface
+  interface delete; module procedure delete_mesh_gpu; module procedure delete_SF_gpu;
+                    module procedure delete_PCG_SF_gpu; end interface
+
+contains
+
+  subroutine init_mesh_gpu(m, hn_x, hn_y, hn_z)
+    type(mesh_gpu), intent(inout) :: m
+    real(c_double), intent(in) :: hn_x(:), hn_y(:), hn_z(:)
+    call moons_check(moons_init(0_c_int), 'moons_init')
+    call moons_check(moons_mesh_create(hn_x, size(hn_x), hn_y, size(hn_y), hn_z, size(hn_z), m%h), 'mesh')
+    m%sn = [size(hn_x), size(hn_y), size(hn_z)]
+  end subroutine
+
+  subroutine init_field(f, m, dl)
+    type(SF_gpu), intent(inout) :: f
+    type(mesh_gpu), intent(in) :: m
+    integer(c_int), intent(in) :: dl
+    integer :: d
+    call moons_check(moons_field_create(m%h, dl, f%h), 'field')
+    f%dl = dl
+    do d = 1, 3
+      if (btest(dl, d - 1)) then
+        f%s(d) = m%sn(d)
+      else
+        f%s(d) = m%sn(d) - 1
+      endif
+    enddo
+  end subroutine
+
+  subroutine init_CC(f, m)
+    type(SF_gpu), intent(inout) :: f
+    type(mesh_gpu), intent(in) :: m
+    call init_field(f, m, MOONS_DL_CC)
+  end subroutine
+
+  subroutine init_Face(f, m, dir)
+    type(SF_gpu), intent(inout) :: f
+    type(mesh_gpu), intent(in) :: m
+    integer, intent(in) :: dir
+    call init_field(f, m, int(ishft(1, dir - 1), c_int))
+  end subroutine
+
+  ! BC type per face (MOONS_BC_*), uniform value per face, all_Neumann flag.
+  subroutine set_BCs(f, types, vals, all_Neumann)
+    type(SF_gpu), intent(inout) :: f
+    integer(c_int), intent(in) :: types(6)
+    real(c_double), intent(in) :: vals(6)
+    logical, intent(in) :: all_Neumann
+    integer(c_int) :: an
+    an = 0
+    if (all_Neumann) an = 1
+    call moons_check(moons_field_set_bcs(f%h, types, vals, an), 'set_BCs')
+  end subroutine
+
+  ! init_PCG_SF: operator Lap_uniform_SF, prec prec_Lap_SF (prec = MOONS_PREC_LAP_SF)
+  ! or the FP32 multigrid (MOONS_PREC_MG_FP32); x carries the BCs.
+  subroutine init_PCG_SF_gpu(PCG, m, x_types, x_vals, all_Neumann, iter_max, tol_rel, tol_abs, &
+                             n_skip_check_res, prec)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    type(mesh_gpu), intent(in) :: m
+    integer(c_int), intent(in) :: x_types(6)
+    real(c_double), intent(in) :: x_vals(6)
+    logical, intent(in) :: all_Neumann
+    integer, intent(in) :: iter_max, n_skip_check_res
+    real(c_double), intent(in) :: tol_rel, tol_abs
+    integer(c_int), intent(in) :: prec
+    call init_CC(PCG%x, m)
+    call init_CC(PCG%b, m)
+    call set_BCs(PCG%x, x_types, x_vals, all_Neumann)
+    call moons_check(moons_pcg_sf_create(m%h, MOONS_OP_LAP_UNIFORM_SF, prec, PCG%x%h, PCG%h), 'init_PCG_SF')
+    PCG%iter_max = iter_max
+    PCG%tol_rel = tol_rel
+    PCG%tol_abs = tol_abs
+    PCG%n_skip_check_res = n_skip_check_res
+    call moons_check(moons_pcg_set_isp(PCG%h, int(iter_max, c_int), tol_rel, tol_abs, &
+                                       int(n_skip_check_res, c_int)), 'ISP')
+  end subroutine
+
+  ! solve_PCG_SF on host arrays: x (in/out, ghosts included) and b.
+  subroutine solve_PCG_SF_gpu(PCG, x, b, compute_norms)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    real(c_double), intent(inout), contiguous, target :: x(:, :, :)
+    real(c_double), intent(in), contiguous, target :: b(:, :, :)
+    logical, intent(in) :: compute_norms
+    integer(c_size_t) :: n
+    n = int(size(x), c_size_t)
+    if (any(shape(x) .ne. PCG%x%s) .or. any(shape(b) .ne. PCG%x%s)) then
+      error stop 'Error: x and b must have the solver field shape in solve_PCG_SF_gpu'
+    endif
+    call moons_check(moons_field_upload(PCG%x%h, x, n), 'upload x')
+    call moons_check(moons_field_upload(PCG%b%h, b, n), 'upload b')
+    call moons_check(moons_pcg_sf_solve(PCG%h, PCG%x%h, PCG%b%h, PCG%info), 'solve_PCG_SF')
+    call moons_check(moons_field_download(PCG%x%h, x, n), 'download x')
+    if (compute_norms) then
+      write(*, '(a,i8,a,es12.4)') ' PCG iterations = ', PCG%info%iterations, &
+        '   rel. residual = ', PCG%info%res_norm / PCG%info%res_norm0
+    endif
+  end subroutine
+
+  ! clean_div_PCG on host face arrays: X = Xstar - grad(phi)/scale with
+  ! lap(phi) = scale*div(Xstar); X carries its BCs (types, vals).
+  subroutine clean_div(PCG, m, X1, X2, X3, Xs1, Xs2, Xs3, X_types, X_vals, phi, scale)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    type(mesh_gpu), intent(in) :: m
+    real(c_double), intent(inout), contiguous :: X1(:, :, :), X2(:, :, :), X3(:, :, :)
+    real(c_double), intent(in), contiguous :: Xs1(:, :, :), Xs2(:, :, :), Xs3(:, :, :)
+    integer(c_int), intent(in) :: X_types(6)
+    real(c_double), intent(in) :: X_vals(6)
+    real(c_double), intent(inout), contiguous :: phi(:, :, :)
+    real(c_double), intent(in) :: scale
+    type(SF_gpu) :: X(3), Xs(3), T(3), tcc
+    type(c_ptr) :: hX(3), hXs(3), hT(3)
+    integer :: d
+    do d = 1, 3
+      call init_Face(X(d), m, d)
+      call init_Face(Xs(d), m, d)
+      call init_Face(T(d), m, d)
+      call set_BCs(X(d), X_types, X_vals, .false.)
+      hX(d) = X(d)%h; hXs(d) = Xs(d)%h; hT(d) = T(d)%h
+    enddo
+    call init_CC(tcc, m)
+    call moons_check(moons_field_upload(Xs(1)%h, Xs1, int(size(Xs1), c_size_t)), 'upload')
+    call moons_check(moons_field_upload(Xs(2)%h, Xs2, int(size(Xs2), c_size_t)), 'upload')
+    call moons_check(moons_field_upload(Xs(3)%h, Xs3, int(size(Xs3), c_size_t)), 'upload')
+    call moons_check(moons_field_upload(PCG%x%h, phi, int(size(phi), c_size_t)), 'upload')
+    call moons_check(moons_clean_div(PCG%h, hX, hXs, PCG%x%h, scale, hT, tcc%h, PCG%info), 'clean_div')
+    call moons_check(moons_field_download(X(1)%h, X1, int(size(X1), c_size_t)), 'download')
+    call moons_check(moons_field_download(X(2)%h, X2, int(size(X2), c_size_t)), 'download')
+    call moons_check(moons_field_download(X(3)%h, X3, int(size(X3), c_size_t)), 'download')
+    call moons_check(moons_field_download(PCG%x%h, phi, int(size(phi), c_size_t)), 'download')
+    do d = 1, 3
+      call delete(X(d)); call delete(Xs(d)); call delete(T(d))
+    enddo
+    call delete(tcc)
+  end subroutine
+
+  subroutine delete_mesh_gpu(m)
+    type(mesh_gpu), intent(inout) :: m
+    if (c_associated(m%h)) call moons_check(moons_mesh_destroy(m%h), 'mesh destroy')
+    m%h = c_null_ptr
+  end subroutine
+
+  subroutine delete_SF_gpu(f)
+    type(SF_gpu), intent(inout) :: f
+    if (c_associated(f%h)) call moons_check(moons_field_destroy(f%h), 'field destroy')
+    f%h = c_null_ptr
+  end subroutine
+
+  subroutine delete_PCG_SF_gpu(PCG)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    if (c_associated(PCG%h)) call moons_check(moons_pcg_destroy(PCG%h), 'pcg destroy')
+    PCG%h = c_null_ptr
+    call delete(PCG%x)
+    call delete(PCG%b)
+  end subroutine
+
+end module

@@ -1,0 +1,177 @@
+! moons_amd_bindings.f90 -- iso_c_binding interfaces of libmoons_amd
+! (include/moons_amd.h), one per C entry point the Fortran host uses, and
+! the status check that turns a non-zero status into `error stop` with the
+! library's message, as the reference stops on errors
+! (code/handwritten/ops/ops_del.f90:267-288).
+module moons_amd_bindings
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: MOONS_DL_CC = 0
+  integer(c_int), parameter, public :: MOONS_DL_FACE_X = 1, MOONS_DL_FACE_Y = 2, MOONS_DL_FACE_Z = 4
+  integer(c_int), parameter, public :: MOONS_BC_NONE = 0, MOONS_BC_DIRICHLET = 1, MOONS_BC_NEUMANN = 2
+  integer(c_int), parameter, public :: MOONS_BC_PERIODIC = 3
+  integer(c_int), parameter, public :: MOONS_OP_LAP_UNIFORM_SF = 1
+  integer(c_int), parameter, public :: MOONS_PREC_LAP_SF = 1, MOONS_PREC_MG_FP32 = 2
+
+  type, bind(C), public :: moons_pcg_info
+    integer(c_int) :: iterations
+    integer(c_int) :: exit_loop
+    real(c_double) :: res_norm0
+    real(c_double) :: res_norm
+    real(c_double) :: ms
+  end type
+
+  public :: moons_check
+  public :: moons_init, moons_finalize, moons_synchronize
+  public :: moons_nodes_uniform, moons_nodes_roberts_both
+  public :: moons_mesh_create, moons_mesh_destroy
+  public :: moons_field_create, moons_field_destroy, moons_field_set_bcs
+  public :: moons_field_upload, moons_field_download, moons_field_fill
+  public :: moons_apply_bcs, moons_grad, moons_div
+  public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
+  public :: moons_clean_div
+
+  interface
+    integer(c_int) function moons_init(device) bind(C, name="moons_init")
+      import c_int
+      integer(c_int), value :: device
+    end function
+    integer(c_int) function moons_finalize() bind(C, name="moons_finalize")
+      import c_int
+    end function
+    integer(c_int) function moons_synchronize() bind(C, name="moons_synchronize")
+      import c_int
+    end function
+    type(c_ptr) function moons_last_error() bind(C, name="moons_last_error")
+      import c_ptr
+    end function
+    integer(c_int) function moons_nodes_uniform(hmin, hmax, N, hn) bind(C, name="moons_nodes_uniform")
+      import c_int, c_double
+      real(c_double), value :: hmin, hmax
+      integer(c_int), value :: N
+      real(c_double), intent(out) :: hn(*)
+    end function
+    integer(c_int) function moons_nodes_roberts_both(hmin, hmax, N, beta, hn) bind(C, name="moons_nodes_roberts_both")
+      import c_int, c_double
+      real(c_double), value :: hmin, hmax, beta
+      integer(c_int), value :: N
+      real(c_double), intent(out) :: hn(*)
+    end function
+    integer(c_int) function moons_mesh_create(hx, sx, hy, sy, hz, sz, m) bind(C, name="moons_mesh_create")
+      import c_int, c_double, c_ptr
+      real(c_double), intent(in) :: hx(*), hy(*), hz(*)
+      integer(c_int), value :: sx, sy, sz
+      type(c_ptr), intent(out) :: m
+    end function
+    integer(c_int) function moons_mesh_destroy(m) bind(C, name="moons_mesh_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function moons_field_create(m, dl, f) bind(C, name="moons_field_create")
+      import c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: dl
+      type(c_ptr), intent(out) :: f
+    end function
+    integer(c_int) function moons_field_destroy(f) bind(C, name="moons_field_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: f
+    end function
+    integer(c_int) function moons_field_set_bcs(f, types, vals, all_neumann) bind(C, name="moons_field_set_bcs")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      integer(c_int), intent(in) :: types(6)
+      real(c_double), intent(in) :: vals(6)
+      integer(c_int), value :: all_neumann
+    end function
+    integer(c_int) function moons_field_upload(f, host, n) bind(C, name="moons_field_upload")
+      import c_int, c_double, c_ptr, c_size_t
+      type(c_ptr), value :: f
+      real(c_double), intent(in) :: host(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function moons_field_download(f, host, n) bind(C, name="moons_field_download")
+      import c_int, c_double, c_ptr, c_size_t
+      type(c_ptr), value :: f
+      real(c_double), intent(out) :: host(*)
+      integer(c_size_t), value :: n
+    end function
+    integer(c_int) function moons_field_fill(f, v) bind(C, name="moons_field_fill")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      real(c_double), value :: v
+    end function
+    integer(c_int) function moons_apply_bcs(f) bind(C, name="moons_apply_bcs")
+      import c_int, c_ptr
+      type(c_ptr), value :: f
+    end function
+    integer(c_int) function moons_grad(gx, gy, gz, u) bind(C, name="moons_grad")
+      import c_int, c_ptr
+      type(c_ptr), value :: gx, gy, gz, u
+    end function
+    integer(c_int) function moons_div(d, u, v, w) bind(C, name="moons_div")
+      import c_int, c_ptr
+      type(c_ptr), value :: d, u, v, w
+    end function
+    integer(c_int) function moons_pcg_sf_create(m, op, prec, x, pcg) bind(C, name="moons_pcg_sf_create")
+      import c_int, c_ptr
+      type(c_ptr), value :: m, x
+      integer(c_int), value :: op, prec
+      type(c_ptr), intent(out) :: pcg
+    end function
+    integer(c_int) function moons_pcg_destroy(pcg) bind(C, name="moons_pcg_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: pcg
+    end function
+    integer(c_int) function moons_pcg_set_isp(pcg, iter_max, tol_rel, tol_abs, n_skip) bind(C, name="moons_pcg_set_isp")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: pcg
+      integer(c_int), value :: iter_max, n_skip
+      real(c_double), value :: tol_rel, tol_abs
+    end function
+    integer(c_int) function moons_pcg_sf_solve(pcg, x, b, info) bind(C, name="moons_pcg_sf_solve")
+      import c_int, c_ptr, moons_pcg_info
+      type(c_ptr), value :: pcg, x, b
+      type(moons_pcg_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_clean_div(pcg, X, Xstar, phi, scale, temp_F, temp_CC, info) &
+        bind(C, name="moons_clean_div")
+      import c_int, c_double, c_ptr, moons_pcg_info
+      type(c_ptr), value :: pcg, phi, temp_CC
+      type(c_ptr), intent(in) :: X(3), Xstar(3), temp_F(3)
+      real(c_double), value :: scale
+      type(moons_pcg_info), intent(out) :: info
+    end function
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! Non-zero status: print the library's message and stop, the reference's
+  ! convention for errors.
+  subroutine moons_check(status, where)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: where
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: msg(:)
+    integer(c_size_t) :: n
+    character(len=:), allocatable :: text
+    integer :: i
+    if (status .eq. 0) return
+    p = moons_last_error()
+    n = c_strlen(p)
+    call c_f_pointer(p, msg, [n])
+    allocate(character(len=n) :: text)
+    do i = 1, int(n)
+      text(i:i) = msg(i)
+    enddo
+    write(*, '(a)') 'Error: '//where//': '//text
+    error stop 'libmoons_amd call failed'
+  end subroutine
+
+end module

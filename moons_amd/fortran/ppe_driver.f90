@@ -1,0 +1,57 @@
+! ppe_driver.f90 -- Fortran host driving the GPU pressure-Poisson solve
+! through moons_amd_PCG_mod, as a MOONS host would at clean_div/solve.
+!
+! usage: ppe_driver N beta prec iter_max tol b.bin x.bin
+!   mesh: x uniform on [0,1], y and z Roberts beta (uniform when beta <= 1)
+!   BCs:  homogeneous Neumann on all faces, all_Neumann
+!   b.bin: the right-hand side, real(8), ghost-padded (N+2)^3, column-major
+!   x.bin: the solution written back (x0 = 0)
+! prints: iterations and the relative residual.
+program ppe_driver
+  use iso_c_binding
+  use moons_amd_bindings
+  use moons_amd_PCG_mod
+  implicit none
+  integer :: N, iter_max, prec_i, u
+  real(c_double) :: beta, tol
+  character(len=512) :: arg, bpath, xpath
+  real(c_double), allocatable :: hx(:), hy(:), x(:, :, :), b(:, :, :)
+  type(mesh_gpu) :: m
+  type(PCG_solver_SF_gpu) :: PCG
+  integer(c_int) :: types(6)
+  real(c_double) :: vals(6)
+
+  call get_command_argument(1, arg); read(arg, *) N
+  call get_command_argument(2, arg); read(arg, *) beta
+  call get_command_argument(3, arg); read(arg, *) prec_i
+  call get_command_argument(4, arg); read(arg, *) iter_max
+  call get_command_argument(5, arg); read(arg, *) tol
+  call get_command_argument(6, bpath)
+  call get_command_argument(7, xpath)
+
+  allocate(hx(N + 3), hy(N + 3))
+  call moons_check(moons_nodes_uniform(0.0_c_double, 1.0_c_double, int(N, c_int), hx), 'nodes')
+  if (beta .gt. 1.0_c_double) then
+    call moons_check(moons_nodes_roberts_both(0.0_c_double, 1.0_c_double, int(N, c_int), beta, hy), 'nodes')
+  else
+    hy = hx
+  endif
+  call init(m, hx, hy, hy)
+
+  allocate(x(N + 2, N + 2, N + 2), b(N + 2, N + 2, N + 2))
+  open(newunit=u, file=trim(bpath), access='stream', form='unformatted', status='old')
+  read(u) b
+  close(u)
+  x = 0.0_c_double
+
+  types = MOONS_BC_NEUMANN
+  vals = 0.0_c_double
+  call init(PCG, m, types, vals, .true., iter_max, tol, 1.0d-300, 1, int(prec_i, c_int))
+  call solve(PCG, x, b, .true.)
+
+  open(newunit=u, file=trim(xpath), access='stream', form='unformatted', status='replace')
+  write(u) x
+  close(u)
+  call delete(PCG)
+  call delete(m)
+end program

@@ -1,0 +1,203 @@
+"""FP64 NumPy restatement of the multigrid V-cycle of moons_amd/csrc/mg.hip.
+
+TEST INFRASTRUCTURE ONLY.  The reference compiles no multigrid (its V-cycle
+is dead code: code_not_compiled/9.6.2017/MG/MG.f90:241-396 -- smooth,
+residual, restrict, recurse, prolongate, correct, smooth), so this file
+restates the build's own V-cycle step for step, from the same mathematics:
+node-coarsened levels (code/handwritten/grid/coordinates_extend.f90:336-355),
+the Laplacian with homogeneous BCs folded into per-direction weights,
+red-black Gauss-Seidel, volume-weighted restriction of residuals, trilinear
+prolongation in true coordinates.  Pinned by: the level operator equals the
+oracle's vol-weighted Laplacian divided by vol (assembled densely), the V-cycle
+is a convergent stationary iteration, and MG-preconditioned CG reaches the
+dense solution.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PERIODIC, DIRICHLET, NEUMANN, ANTISYM = 3, 1, 2, 5
+
+
+def _dirichlet_like(t):
+    return t in (DIRICHLET, ANTISYM)
+
+
+def coarsen_nodes(hn):
+    phys = hn[1:-1][::2]
+    return np.concatenate([[phys[0] - (phys[1] - phys[0])], phys, [phys[-1] + (phys[-1] - phys[-2])]])
+
+
+class Level:
+    def __init__(self, hn3, bct):
+        self.hn = hn3
+        self.n = [len(h) - 3 for h in hn3]
+        self.per = [bct[2 * d] == PERIODIC for d in range(3)]
+        self.wm, self.wp, self.dg, self.hc, self.dhn = [], [], [], [], []
+        for d in range(3):
+            hn = hn3[d]
+            n = self.n[d]
+            s = n + 2
+            dhn = hn[1:] - hn[:-1]
+            hc = 0.5 * (hn[1:] + hn[:-1])
+            dhc = hc[1:] - hc[:-1]
+            wm = np.zeros(s)
+            wp = np.zeros(s)
+            dg = np.zeros(s)
+            bmin, bmax = bct[2 * d], bct[2 * d + 1]
+            for i in range(1, n + 1):
+                a = 1.0 / (dhc[i - 1] * dhn[i])
+                c = 1.0 / (dhc[i] * dhn[i])
+                diag = -(a + c)
+                if i == 1 and bmin != PERIODIC:
+                    diag += -a if _dirichlet_like(bmin) else a
+                    a = 0.0
+                if i == n and bmax != PERIODIC:
+                    diag += -c if _dirichlet_like(bmax) else c
+                    c = 0.0
+                wm[i], wp[i], dg[i] = a, c, diag
+            self.wm.append(wm)
+            self.wp.append(wp)
+            self.dg.append(dg)
+            self.hc.append(hc)
+            self.dhn.append(dhn)
+
+    def shape(self):
+        return tuple(n + 2 for n in self.n)
+
+    def _nb(self, x, d, sign):
+        """x shifted to the -/+ neighbour along d on interior cells, with wrap for periodic."""
+        n = self.n[d]
+        xi = np.moveaxis(x, d, 0)
+        idx = np.arange(1, n + 1) + sign
+        if self.per[d]:
+            idx = np.where(idx == 0, n, idx)
+            idx = np.where(idx == n + 1, 1, idx)
+        else:
+            idx = np.clip(idx, 1, n)
+        other = [slice(1, m + 1) for k, m in enumerate(self.n) if k != d]
+        v = xi[(idx,) + tuple(other)]
+        return np.moveaxis(v, 0, d)
+
+    def _w(self, arr, d):
+        n = self.n[d]
+        shp = [1, 1, 1]
+        shp[d] = n
+        return arr[1:n + 1].reshape(shp)
+
+    def offdiag(self, x):
+        s = 0.0
+        for d in range(3):
+            s = s + self._w(self.wm[d], d) * self._nb(x, d, -1) + self._w(self.wp[d], d) * self._nb(x, d, +1)
+        return s
+
+    def diag(self):
+        return self._w(self.dg[0], 0) + self._w(self.dg[1], 1) + self._w(self.dg[2], 2)
+
+    def interior(self, a):
+        return a[1:-1, 1:-1, 1:-1]
+
+    def color_mask(self, c):
+        i, j, k = np.meshgrid(*[np.arange(1, n + 1) for n in self.n], indexing="ij")
+        return ((i + j + k) % 2) == c
+
+    def smooth(self, x, f, color, zero_guess):
+        m = self.color_mask(color)
+        fi = self.interior(f)
+        if zero_guess:
+            new = fi / self.diag()
+        else:
+            new = (fi - self.offdiag(x)) / self.diag()
+        xi = self.interior(x)
+        xi[m] = new[m]
+
+    def residual(self, x, f):
+        return self.interior(f) - (self.offdiag(x) + self.diag() * self.interior(x))
+
+
+def build(hn3, bct, max_levels=16):
+    levels = [Level([np.asarray(h, float) for h in hn3], bct)]
+    while len(levels) < max_levels:
+        L = levels[-1]
+        if not all(n % 2 == 0 and n // 2 >= 2 for n in L.n):
+            break
+        levels.append(Level([coarsen_nodes(h) for h in L.hn], bct))
+    return levels
+
+
+def restrict(F: Level, C: Level, res):
+    """coarse f = sum of children (child vol / coarse vol) * residual."""
+    rf = []
+    for d in range(3):
+        dc = C.dhn[d][1:C.n[d] + 1]
+        df = F.dhn[d][1:F.n[d] + 1]
+        rf.append(df / np.repeat(dc, 2))
+    w = rf[0][:, None, None] * rf[1][None, :, None] * rf[2][None, None, :]
+    v = w * res
+    n0, n1, n2 = C.n
+    return v.reshape(n0, 2, n1, 2, n2, 2).sum(axis=(1, 3, 5))
+
+
+def prolong(F: Level, C: Level, xc, bct):
+    """trilinear interpolation of the coarse correction to fine interior cells."""
+    idx, wa, wb, idx2 = [], [], [], []
+    for d in range(3):
+        n, nc = F.n[d], C.n[d]
+        I = (np.arange(1, n + 1) + 1) // 2
+        left = (np.arange(1, n + 1) % 2) == 1
+        I2 = np.where(left, I - 1, I + 1)
+        hcf = F.hc[d][1:n + 1]
+        hcc = C.hc[d]
+        a = (hcf - hcc[I2]) / (hcc[I] - hcc[I2])
+        b = 1.0 - a
+        bmin, bmax = bct[2 * d], bct[2 * d + 1]
+        for q in range(n):
+            if I2[q] in (0, nc + 1):
+                bt = bmin if left[q] else bmax
+                if bt == PERIODIC:
+                    I2[q] = nc if I2[q] == 0 else 1
+                else:
+                    I2[q] = I[q]
+                    if _dirichlet_like(bt):
+                        b[q] = -b[q]
+        idx.append(I)
+        idx2.append(I2)
+        wa.append(a)
+        wb.append(b)
+    e = 0.0
+    for cx, wx in ((idx[0], wa[0]), (idx2[0], wb[0])):
+        for cy, wy in ((idx[1], wa[1]), (idx2[1], wb[1])):
+            for cz, wz in ((idx[2], wa[2]), (idx2[2], wb[2])):
+                e = e + wx[:, None, None] * wy[None, :, None] * wz[None, None, :] * xc[np.ix_(cx, cy, cz)]
+    return e
+
+
+def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=40):
+    """x0 = V f0 (padded arrays; f0 = r/vol on the fine level)."""
+    xs = [np.zeros(L.shape()) for L in levels]
+    fs = [None] * len(levels)
+    fs[0] = f0
+
+    def cyc(l):
+        L = levels[l]
+        x, f = xs[l], fs[l]
+        if l == len(levels) - 1:
+            for s in range(coarse_sweeps):
+                L.smooth(x, f, 0, s == 0)
+                L.smooth(x, f, 1, False)
+            return
+        for s in range(nu1):
+            L.smooth(x, f, 0, s == 0)
+            L.smooth(x, f, 1, False)
+        C = levels[l + 1]
+        fc = np.zeros(C.shape())
+        fc[1:-1, 1:-1, 1:-1] = restrict(L, C, L.residual(x, f))
+        fs[l + 1] = fc
+        cyc(l + 1)
+        L.interior(x)[...] += prolong(L, C, xs[l + 1], bct)
+        for s in range(nu2):
+            L.smooth(x, f, 1, False)
+            L.smooth(x, f, 0, False)
+
+    cyc(0)
+    return xs[0]

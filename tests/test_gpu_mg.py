@@ -1,0 +1,100 @@
+"""GPU FP32 multigrid: one V-cycle against the FP64 NumPy restatement on the
+same residual (FP32 tolerance), and the mixed-precision flexible PCG against
+the FP64 oracle's Jacobi-PCG solution (after removing the free constant of the
+singular Neumann problem) at the 1e-6 bound stated for mixed-precision fields."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mg_oracle as MGO
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {1: "Dirichlet", 2: "Neumann", 3: "Periodic"}
+CASES = {
+    "cube16_neumann": (16, None, [2] * 6),
+    "duct24_roberts": (24, 1.05, [3, 3, 2, 2, 2, 2]),
+    "dirichlet_mix20": (20, 1.2, [1, 2, 1, 1, 2, 1]),
+}
+
+
+def meshes(gpu, N, beta):
+    hx = O.uniform_nodes(0.0, 1.0, N)
+    hy = hx if beta is None else O.roberts_both_nodes(0.0, 1.0, N, beta)
+    return [hx, hy, hy], O.Grid(hx, hy, hy), gpu.mesh(hx, hy, hy)
+
+
+def mg_solver(gpu, m, x, iters=200, tol=1e-8):
+    P = gpu.PCG_solver_SF(m, x, gpu.iter_solver_params(iters, tol, 1e-300, 1), prec="MG_FP32")
+    return P
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_vcycle_matches_restatement(gpu, case):
+    N, beta, bct = CASES[case]
+    hs, g, m = meshes(gpu, N, beta)
+    x = gpu.SF(m, "CC").set_BCs([NAMES[t] for t in bct], None, all_Neumann=(1 not in bct))
+    P = mg_solver(gpu, m, x)
+    rng = np.random.default_rng(3)
+    r = np.zeros(g.shape(0), order="F")
+    r[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(n - 2 for n in g.shape(0)))
+    R = gpu.SF(m, "CC").upload(r)
+    Z = gpu.SF(m, "CC")
+    P.apply_prec(R, Z)
+    z = Z.download()
+    levels = MGO.build(hs, bct)
+    vol = O.volume(g, 0)
+    f = np.zeros(g.shape(0))
+    f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
+    zr = MGO.vcycle(levels, f, bct)
+    zi, zri = z[1:-1, 1:-1, 1:-1], zr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(zi - zri) / np.linalg.norm(zri)
+    assert err < 2e-5, err
+    assert P.mg_info()[0] == len(levels)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_mixed_pcg_matches_fp64_oracle(gpu, case):
+    N, beta, bct = CASES[case]
+    hs, g, m = meshes(gpu, N, beta)
+    alln = 1 not in bct
+    types = [NAMES[t] for t in bct]
+    b = np.asfortranarray(O.splitmix(int(np.prod(g.shape(0))), 21).reshape(g.shape(0), order="F"))
+    xr = np.zeros(g.shape(0), order="F")
+    O.pcg_lap(g, 0, O.BCs.make(types), alln, xr, b, 5000, 1e-13, 1e-300)
+    X = gpu.SF(m, "CC").set_BCs(types, None, all_Neumann=alln)
+    P = mg_solver(gpu, m, X, tol=1e-10)
+    info = P.solve(X, gpu.SF(m, "CC").upload(b))
+    assert info.iterations < 40, info.iterations
+    assert info.res_norm / info.res_norm0 < 1e-10
+    x = X.download()
+    vol = O.volume(g, 0)
+    xi, xri = x[1:-1, 1:-1, 1:-1], xr[1:-1, 1:-1, 1:-1]
+    if alln:
+        vi = vol[1:-1, 1:-1, 1:-1]
+        xi = xi - (xi * vi).sum() / vi.sum()
+        xri = xri - (xri * vi).sum() / vi.sum()
+    err = np.linalg.norm(xi - xri) / np.linalg.norm(xri)
+    assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("beta", [None, 1.05])
+def test_mixed_pcg_128_reaches_1e8_true_residual(gpu, beta):
+    hs, g, m = meshes(gpu, 128, beta)
+    types = ["Periodic", "Periodic"] + ["Neumann"] * 4
+    b = np.asfortranarray(O.splitmix(int(np.prod(g.shape(0))), 23).reshape(g.shape(0), order="F"))
+    X = gpu.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
+    P = mg_solver(gpu, m, X, tol=1e-8)
+    info = P.solve(X, gpu.SF(m, "CC").upload(b))
+    assert info.res_norm / info.res_norm0 < 1e-8
+    assert info.iterations < 60
+    x = X.download()
+    xi = x.copy(order="F")
+    O.apply_bcs(g, xi, 0, O.BCs.make(types), True)
+    vol = O.volume(g, 0)
+    bi, vi = b[1:-1, 1:-1, 1:-1], vol[1:-1, 1:-1, 1:-1]
+    mean = (bi * vi).sum() / vi.sum()
+    r = (bi - mean - O.lap_centered(g, xi, 0)[1:-1, 1:-1, 1:-1]) * vi
+    assert np.linalg.norm(r) / info.res_norm0 < 2e-8
