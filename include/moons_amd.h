@@ -67,9 +67,18 @@ extern "C" {
  * code_not_compiled/9.6.2017/MG/MG.f90:241-396); same solve/exit interface. */
 #define MOONS_PREC_MG_FP32 2
 
+/* Face-centred vector operators of the implicit diffusion solves:
+ *   mom_diffusion: Ax = x + c lap(x)             (matrix_free_operators.f90:176-202)
+ *   ind_diffusion: Ax = x + c curl(k curl(x))    (matrix_free_operators.f90:120-146)
+ * with c = coeff_implicit_time_split (negative for momentum, where the
+ * reference passes -theta dt/Re) and k = sigmaInv on the edges. */
+#define MOONS_OP_MOM_DIFFUSION 1
+#define MOONS_OP_IND_DIFFUSION 2
+
 typedef struct moons_mesh_s *moons_mesh_t;
 typedef struct moons_field_s *moons_field_t;
 typedef struct moons_pcg_s *moons_pcg_t;
+typedef struct moons_pcgvf_s *moons_pcgvf_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
@@ -153,6 +162,19 @@ int moons_curl(moons_field_t curlU, moons_field_t u, moons_field_t v, moons_fiel
  * applies the implicit BCs to x (its ghosts change) then Ax = div(grad(x)),
  * one fused kernel for cell-centred x. */
 int moons_lap_uniform_sf(moons_field_t Ax, moons_field_t x);
+/* interpO2_SF along dir (code/handwritten/ops/ops_interp.f90:190-221): C->N
+ * with theta weights and extrapolated end nodes
+ * (code/handwritten/grid_field/GF_interp/GF_interp.f90:19-47, GF_extrap.f90:17-46)
+ * or N->C half sums (GF_interp.f90:49-74), whichever the locations imply. */
+int moons_interp(moons_field_t out, moons_field_t in, int dir);
+/* edgeCrossFace_E (code/handwritten/ops/ops_advect.f90:202-218): E = U_E x
+ * face2Edge_no_diag(B) on edges.  U_E: Uy@x, Uz@x, Ux@y, Uz@y, Ux@z, Uy@z;
+ * tmp: six edge work fields Bx@y, Bx@z, By@x, By@z, Bz@x, Bz@y. */
+int moons_edge_cross_face_E(moons_field_t E[3], moons_field_t U_E[6], moons_field_t B[3], moons_field_t tmp[6]);
+/* advect_B (code/handwritten/ops/ops_advect.f90:186-200): the constrained-
+ * transport source adv = curl(U_E x B) on faces; div(adv) = 0 discretely. */
+int moons_advect_B(moons_field_t adv[3], moons_field_t U_E[6], moons_field_t B[3], moons_field_t temp_E_TF[6],
+                   moons_field_t temp_E[3]);
 /* volume_SF (code/handwritten/fields/SF_extend.f90:571-581) */
 int moons_volume(moons_field_t vol);
 
@@ -197,6 +219,26 @@ int moons_pcg_apply_prec(moons_pcg_t pcg, moons_field_t r, moons_field_t z);
  * stream, averaged over reps runs on the solver's current vectors.  ms[3]
  * receives the per-kernel averages.  No reference counterpart (profiling). */
 int moons_pcg_time_kernels(moons_pcg_t pcg, int reps, double ms[3]);
+
+/* --------------------------------------------- vector (VF) Jacobi-PCG */
+/* init_PCG_VF (code/handwritten/PCG/PCG_solver_extend.f90:117-180) for
+ * op = MOONS_OP_MOM_DIFFUSION with prec_mom_VF, or MOONS_OP_IND_DIFFUSION
+ * with prec_ind_VF (code/handwritten/solvers/preconditioners/preconditioners.f90:104-129).
+ * x: the three face components (their BCs are the solver's); k: sigmaInv on
+ * the x,y,z edges (copied; NULL for momentum).  The coefficient starts at 0:
+ * set it with moons_pcg_vf_update_mfp before the first solve. */
+int moons_pcg_vf_create(moons_mesh_t m, int op, moons_field_t x[3], moons_field_t k[3], moons_pcgvf_t *out);
+int moons_pcg_vf_destroy(moons_pcgvf_t pcg);
+/* ISP of the VF solver (iter_max, tol_rel, tol_abs, n_skip_check_res). */
+int moons_pcg_vf_set_isp(moons_pcgvf_t pcg, int iter_max, double tol_rel, double tol_abs,
+                         int n_skip_check_res);
+/* update_MFP_PCG_VF (code/handwritten/PCG/PCG_solver_extend.f90:281-291):
+ * sets the operator coefficient; rebuilds Minv when update_preconditioner. */
+int moons_pcg_vf_update_mfp(moons_pcgvf_t pcg, double coeff_implicit_time_split, int update_preconditioner);
+/* solve_PCG_VF (code/handwritten/PCG/PCG_solver_extend.f90:195-206 ->
+ * code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:109-179): x in/out,
+ * b the right-hand side, three face components each. */
+int moons_pcg_vf_solve(moons_pcgvf_t pcg, moons_field_t x[3], moons_field_t b[3], moons_pcg_info *info);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ from .core import (  # noqa: F401
     BC,
     DL,
     PCG_solver_SF,
+    PCG_solver_VF,
     SF,
     Lap_uniform_SF,
     apply_BCs,
@@ -24,4 +25,7 @@ from .core import (  # noqa: F401
     nodes_uniform,
     synchronize,
     volume,
+    interp,
+    edgeCrossFace_E,
+    advect_B,
 )

@@ -54,6 +54,9 @@ SIGNATURES = {
     "moons_curl": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "moons_lap_uniform_sf": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_volume": (C.c_int, [C.c_void_p]),
+    "moons_interp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "moons_edge_cross_face_E": (C.c_int, [C.POINTER(C.c_void_p)] * 4),
+    "moons_advect_B": (C.c_int, [C.POINTER(C.c_void_p)] * 5),
     "moons_pcg_sf_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "moons_pcg_destroy": (C.c_int, [C.c_void_p]),
     "moons_pcg_set_isp": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
@@ -64,6 +67,12 @@ SIGNATURES = {
     "moons_pcg_set_mg": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "moons_pcg_mg_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "moons_pcg_apply_prec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_pcg_vf_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p)]),
+    "moons_pcg_vf_destroy": (C.c_int, [C.c_void_p]),
+    "moons_pcg_vf_set_isp": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "moons_pcg_vf_update_mfp": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "moons_pcg_vf_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(PcgInfo)]),
 }
 
 _lib = None

@@ -168,6 +168,25 @@ def Lap_uniform_SF(Ax: SF, x: SF) -> None:
     check(load().moons_lap_uniform_sf(Ax.handle, x.handle))
 
 
+def interp(out: SF, f: SF, dir: int) -> None:
+    """interpO2_SF: C->N (theta weights, extrapolated ends) or N->C along dir."""
+    check(load().moons_interp(out.handle, f.handle, dir))
+
+
+def _arr(fields):
+    return (C.c_void_p * len(fields))(*[f.handle for f in fields])
+
+
+def edgeCrossFace_E(E, U_E, B, tmp) -> None:
+    """E = U_E x face2Edge_no_diag(B) (ops_advect.f90:202-218)."""
+    check(load().moons_edge_cross_face_E(_arr(E), _arr(U_E), _arr(B), _arr(tmp)))
+
+
+def advect_B(adv, U_E, B, temp_E_TF, temp_E) -> None:
+    """adv = curl(U_E x B) on faces (ops_advect.f90:186-200)."""
+    check(load().moons_advect_B(_arr(adv), _arr(U_E), _arr(B), _arr(temp_E_TF), _arr(temp_E)))
+
+
 def volume(vol: SF) -> None:
     check(load().moons_volume(vol.handle))
 
@@ -214,7 +233,7 @@ class PCG_solver_SF:
         self.prec = prec
         self.set_ISP(ISP or iter_solver_params())
 
-    def set_MG(self, max_levels: int = 16, nu1: int = 2, nu2: int = 2, coarse_sweeps: int = 40) -> None:
+    def set_MG(self, max_levels: int = 16, nu1: int = 3, nu2: int = 3, coarse_sweeps: int = 40) -> None:
         check(load().moons_pcg_set_mg(self.handle, max_levels, nu1, nu2, coarse_sweeps))
 
     def mg_info(self):
@@ -256,3 +275,46 @@ def clean_div(PCG: PCG_solver_SF, X, Xstar, phi: SF, scale: float, temp_F, temp_
     check(load().moons_clean_div(PCG.handle, arr(*[u.handle for u in X]), arr(*[u.handle for u in Xstar]), phi.handle,
                                  float(scale), arr(*[u.handle for u in temp_F]), temp_CC.handle, C.byref(info)))
     return pcg_info(info.iterations, info.exit_loop, info.res_norm0, info.res_norm, info.ms)
+
+
+class PCG_solver_VF:
+    """Jacobi-PCG for the face-centred implicit diffusion solves (init_PCG_VF,
+    code/handwritten/PCG/PCG_solver_extend.f90:117-180).
+
+    op="mom_diffusion": A = I + c lap (prec_mom_VF); op="ind_diffusion":
+    A = I + c curl(k curl) with k = sigmaInv on edges (prec_ind_VF).  x is the
+    three face components whose BCs the solver keeps.  Call update_MFP before
+    solving (the coefficient starts at 0).
+    """
+
+    OPS = {"mom_diffusion": 1, "ind_diffusion": 2}
+
+    def __init__(self, m: mesh, x, ISP: iter_solver_params | None = None, op: str = "mom_diffusion", k=None):
+        h = C.c_void_p()
+        karg = _arr(k) if k is not None else None
+        check(load().moons_pcg_vf_create(m.handle, self.OPS[op], _arr(x), karg, C.byref(h)))
+        self.handle = h
+        self.m = m
+        self.op = op
+        self.set_ISP(ISP or iter_solver_params())
+
+    def set_ISP(self, ISP: iter_solver_params) -> None:
+        self.ISP = ISP
+        check(load().moons_pcg_vf_set_isp(self.handle, ISP.iter_max, ISP.tol_rel, ISP.tol_abs, ISP.n_skip_check_res))
+
+    def update_MFP(self, coeff_implicit_time_split: float, update_preconditioner: bool = True) -> None:
+        check(load().moons_pcg_vf_update_mfp(self.handle, float(coeff_implicit_time_split),
+                                             1 if update_preconditioner else 0))
+
+    def solve(self, x, b) -> pcg_info:
+        info = PcgInfo()
+        check(load().moons_pcg_vf_solve(self.handle, _arr(x), _arr(b), C.byref(info)))
+        return pcg_info(info.iterations, info.exit_loop, info.res_norm0, info.res_norm, info.ms)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_pcg_vf_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

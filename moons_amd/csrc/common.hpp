@@ -153,5 +153,13 @@ int launch_apply_bcs(moons_field_s *u, bool implicit_);
 int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int pad, int gen);
 int launch_assign_ghost_xperiodic(moons_field_s *u, double val);
 int launch_scale(moons_field_s *u, double s);
+int launch_interp(moons_field_s *out, const moons_field_s *in, int dir);
+// Gate for the general kernels (apply BCs, del, plane assignment): while set,
+// those kernels return at once when the device flag is non-zero.
+void set_gate(const int *gate);
+int launch_plane_set(moons_field_s *u, int d, int idx, double val);
+int launch_plane_scale(moons_field_s *u, int d, int idx, double val);
+int launch_cross(moons_field_s *out, const moons_field_s *a, const moons_field_s *b, const moons_field_s *c,
+                 const moons_field_s *d, int sign);
 int launch_sub(moons_field_s *out, const moons_field_s *a, const moons_field_s *b);
 }

@@ -10,6 +10,11 @@ namespace moons {
 static thread_local std::string g_err;
 static hipStream_t g_stream = nullptr;
 static int g_device = -1;
+// Device flag that gates the general kernels launched by an iterative
+// solver: when *g_gate != 0 they return at once (the loop has exited).
+static const int *g_gate = nullptr;
+
+void set_gate(const int *gate) { g_gate = gate; }
 
 void set_error(const std::string &msg) { g_err = msg; }
 int fail(int code, const std::string &msg) {
@@ -169,7 +174,8 @@ struct Shape3 {
 // ghosts included), raw rules of code/handwritten/BCs/direct/apply_BCs_faces_raw.f90
 // and code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90.
 __global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int face, int type, int nalong,
-                                                       double bval, double dh_nhat, int implicit_) {
+                                                       double bval, double dh_nhat, int implicit_, const int *gate) {
+  if (gate && *gate) return;
   const int d = face / 2, is_max = face % 2;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
   const int na = S.s[a], nb = S.s[b];
@@ -217,7 +223,9 @@ __global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int 
 }
 
 // Assigns val to one whole plane idx along direction d.
-__global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d, int idx, double val) {
+__global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d, int idx, double val,
+                                                      const int *gate) {
+  if (gate && *gate) return;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
   const int na = S.s[a], nb = S.s[b];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,6 +235,20 @@ __global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d
   u[idx3(S.s, ii[0], ii[1], ii[2])] = val;
 }
 
+// Multiplies one whole plane idx along direction d by val.
+__global__ void __launch_bounds__(256) k_scale_plane(double *u, Shape3 S, int d, int idx, double val,
+                                                     const int *gate) {
+  if (gate && *gate) return;
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na * nb) return;
+  int ii[3];
+  ii[a] = t % na; ii[b] = t / na; ii[d] = idx;
+  const size_t q = idx3(S.s, ii[0], ii[1], ii[2]);
+  u[q] = u[q] * val;
+}
+
 // del along dir for every output element of the padded range, one thread per
 // output element.  kind: 0 stag CC->N (gt=1), 1 stag N->CC (gt=0), 2 col CC, 3 col N.
 // Follows code/handwritten/stencils/stencils.f90 element by element, including
@@ -234,7 +256,8 @@ __global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d
 __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const double *__restrict__ f, Shape3 So,
                                              Shape3 Sf, int dir, int kind, const double *__restrict__ L,
                                              const double *__restrict__ D, const double *__restrict__ U,
-                                             int gen, int pad) {
+                                             int gen, int pad, const int *gate) {
+  if (gate && *gate) return;
   const int a = (dir + 1) % 3, b = (dir + 2) % 3;
   const int lo = a < b ? a : b, hi = a < b ? b : a;
   const int nd = So.s[dir];
@@ -311,6 +334,54 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
   out[oo] = res;
 }
 
+// interpO2_SF along dir, one thread per output element: C->N with theta
+// weights and extrapolated end nodes (GF_interp.f90:19-47, GF_extrap.f90:17-46)
+// or N->C half sums (GF_interp.f90:49-74).
+__global__ void __launch_bounds__(256) k_interp(double *__restrict__ out, const double *__restrict__ in, Shape3 So,
+                                                Shape3 Si, int dir, int c2n, const double *__restrict__ tD,
+                                                const double *__restrict__ tU) {
+  const int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  const int nd = So.s[dir];
+  const long long total = (long long)nd * So.s[a] * So.s[b];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int o = (int)(t % nd);
+  const long long rest = t / nd;
+  const int pa = (int)(rest % So.s[a]), pb = (int)(rest / So.s[a]);
+  int io[3], ii[3];
+  io[dir] = o; io[a] = pa; io[b] = pb;
+  ii[dir] = 0; ii[a] = pa; ii[b] = pb;
+  const double *C = in + idx3(Si.s, ii[0], ii[1], ii[2]);
+  const size_t fs = (dir == 0) ? 1 : (dir == 1 ? (size_t)Si.s[0] : (size_t)Si.s[0] * Si.s[1]);
+  double v;
+  if (c2n) {
+    const int sC = Si.s[dir], sN = nd;
+    if (o >= 1 && o <= sC - 1) {
+      v = C[(size_t)o * fs] * tD[o - 1] + C[(size_t)(o - 1) * fs] * tU[o - 1];
+    } else if (o == 0) {
+      const double n1 = C[fs] * tD[0] + C[0] * tU[0];
+      v = 2.0 * C[0] - n1;
+    } else { // o == sN - 1
+      const double nl = C[(size_t)(sC - 1) * fs] * tD[sC - 2] + C[(size_t)(sC - 2) * fs] * tU[sC - 2];
+      v = 2.0 * C[(size_t)(sC - 1) * fs] - nl;
+      (void)sN;
+    }
+  } else {
+    v = 0.5 * (C[(size_t)o * fs] + C[(size_t)(o + 1) * fs]);
+  }
+  out[idx3(So.s, io[0], io[1], io[2])] = v;
+}
+
+// out = a*b - c*d (sign > 0) or -(a*b - c*d) (sign < 0): the cross-product
+// components of GF_cross_product.f90 (x and z: Ay Bz - Az By; y: -(Ax Bz - Az Bx)).
+__global__ void __launch_bounds__(256) k_cross(double *out, const double *a, const double *b, const double *c,
+                                               const double *d, size_t n, int sign) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double v = a[t] * b[t] - c[t] * d[t];
+  out[t] = sign > 0 ? v : -v;
+}
+
 __global__ void __launch_bounds__(256) k_scale(double *u, size_t n, double s) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) u[t] = u[t] * s;
@@ -352,7 +423,7 @@ int launch_apply_bcs(moons_field_s *u, bool implicit_) {
     int a = (d + 1) % 3, bb = (d + 2) % 3;
     size_t np = (size_t)u->s[a] * u->s[bb];
     hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
-                       (u->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0);
+                       (u->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
     MOONS_LAUNCH_CHECK();
   }
   return MOONS_OK;
@@ -366,11 +437,27 @@ int launch_assign_ghost_xperiodic(moons_field_s *u, double val) {
     int a = (d + 1) % 3, bb = (d + 2) % 3;
     size_t np = (size_t)u->s[a] * u->s[bb];
     hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, d,
-                       (face % 2) ? u->s[d] - 1 : 0, val);
+                       (face % 2) ? u->s[d] - 1 : 0, val, g_gate);
     MOONS_LAUNCH_CHECK();
   }
   return MOONS_OK;
 }
+
+static int launch_plane(moons_field_s *u, int d, int idx, double val, bool scale) {
+  if (d < 0 || d > 2 || idx < 0 || idx >= u->s[d]) return fail(MOONS_ERR_ARG, "plane index out of range");
+  Shape3 S{{u->s[0], u->s[1], u->s[2]}};
+  const int a = (d + 1) % 3, bb = (d + 2) % 3;
+  const size_t np = (size_t)u->s[a] * u->s[bb];
+  if (scale)
+    hipLaunchKernelGGL(k_scale_plane, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, d, idx, val, g_gate);
+  else
+    hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, d, idx, val, g_gate);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int launch_plane_set(moons_field_s *u, int d, int idx, double val) { return launch_plane(u, d, idx, val, false); }
+int launch_plane_scale(moons_field_s *u, int d, int idx, double val) { return launch_plane(u, d, idx, val, true); }
 
 int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int pad, int gen) {
   if (dir < 0 || dir > 2 || (n != 1 && n != 2) || gen < 1 || gen > 3 || pad < 0 || pad > 1)
@@ -390,18 +477,41 @@ int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int p
   int a = (dir + 1) % 3, b = (dir + 2) % 3;
   size_t total = (size_t)out->s[dir] * (size_t)(f->s[a] - 2 * pad) * (size_t)(f->s[b] - 2 * pad);
   hipLaunchKernelGGL(k_del, dim3(nblk(total)), dim3(256), 0, g_stream, out->d, f->d, So, Sf, dir, kind, L, D, U,
-                     gen, pad);
+                     gen, pad, g_gate);
   MOONS_LAUNCH_CHECK();
   if (gen == 1 && pad > 0) {
     // diff(): assign with padding zeroes the ghost planes of the other directions
     for (int q : {a, b}) {
       int c2 = (q + 1) % 3, c3 = (q + 2) % 3;
       size_t np = (size_t)out->s[c2] * out->s[c3];
-      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, 0, 0.0);
-      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, out->s[q] - 1, 0.0);
+      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, 0, 0.0, g_gate);
+      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, out->s[q] - 1, 0.0,
+                         g_gate);
       MOONS_LAUNCH_CHECK();
     }
   }
+  return MOONS_OK;
+}
+
+int launch_interp(moons_field_s *out, const moons_field_s *in, int dir) {
+  if (dir < 0 || dir > 2 || out->m != in->m) return fail(MOONS_ERR_ARG, "interp: bad dir or mesh");
+  const int nout = (out->dl >> dir) & 1, nin = (in->dl >> dir) & 1;
+  if (nout == nin || (out->dl & ~(1 << dir)) != (in->dl & ~(1 << dir)))
+    return fail(MOONS_ERR_ARG, "Error: bad DLs in interpO2_SF (ops_interp.f90)");
+  const DirCoefDev &c = in->m->dev.c[dir];
+  Shape3 So{{out->s[0], out->s[1], out->s[2]}}, Si{{in->s[0], in->s[1], in->s[2]}};
+  hipLaunchKernelGGL(k_interp, dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, in->d, So, Si, dir, nout, c.theta_D,
+                     c.theta_U);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int launch_cross(moons_field_s *out, const moons_field_s *a, const moons_field_s *b, const moons_field_s *c,
+                 const moons_field_s *d, int sign) {
+  if (a->n != out->n || b->n != out->n || c->n != out->n || d->n != out->n)
+    return fail(MOONS_ERR_ARG, "cross_product: fields must be collocated");
+  hipLaunchKernelGGL(k_cross, dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, a->d, b->d, c->d, d->d, out->n, sign);
+  MOONS_LAUNCH_CHECK();
   return MOONS_OK;
 }
 
@@ -692,6 +802,50 @@ int moons_curl(moons_field_t curlU, moons_field_t u, moons_field_t v, moons_fiel
       if ((st = launch_del(curlU, v, 1, 0, 0, 1))) return st;
       return launch_del(curlU, u, 1, 1, 0, 3);
   }
+}
+
+int moons_interp(moons_field_t out, moons_field_t in, int dir) {
+  if (!out || !in) return fail(MOONS_ERR_ARG, "moons_interp: null argument");
+  return launch_interp(out, in, dir);
+}
+
+// face2Edge_TF_no_diag (ops_interp.f90:531-544) + cross_product_TF
+// (TF_extend.f90:385-400).  tmp: the six face-to-edge interpolants
+// (B_x on y,z-edges, B_y on x,z-edges, B_z on x,y-edges).
+int moons_edge_cross_face_E(moons_field_t E[3], moons_field_t U_E[6], moons_field_t B[3], moons_field_t tmp[6]) {
+  if (!E || !U_E || !B || !tmp) return fail(MOONS_ERR_ARG, "moons_edge_cross_face_E: null argument");
+  for (int q = 0; q < 6; ++q)
+    if (!U_E[q] || !tmp[q] || (q < 3 && (!E[q] || !B[q]))) return fail(MOONS_ERR_ARG, "moons_edge_cross_face_E: null component");
+  const int edge_dl[3] = {MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z};
+  // tmp layout: 0 Bx@y, 1 Bx@z, 2 By@x, 3 By@z, 4 Bz@x, 5 Bz@y ; U_E: Uy@x, Uz@x, Ux@y, Uz@y, Ux@z, Uy@z
+  const int tmp_face[6] = {0, 0, 1, 1, 2, 2}, tmp_edge[6] = {1, 2, 0, 2, 0, 1};
+  const int ue_edge[6] = {0, 0, 1, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) {
+    if (tmp[q]->dl != edge_dl[tmp_edge[q]] || U_E[q]->dl != edge_dl[ue_edge[q]])
+      return fail(MOONS_ERR_ARG, "moons_edge_cross_face_E: component locations do not match");
+    if (B[tmp_face[q]]->dl != (1 << tmp_face[q])) return fail(MOONS_ERR_ARG, "moons_edge_cross_face_E: B must be on faces");
+  }
+  for (int d = 0; d < 3; ++d)
+    if (E[d]->dl != edge_dl[d]) return fail(MOONS_ERR_ARG, "moons_edge_cross_face_E: E must be on edges");
+  int st;
+  for (int q = 0; q < 6; ++q) {
+    const int orth = 3 - tmp_face[q] - tmp_edge[q];
+    if ((st = launch_interp(tmp[q], B[tmp_face[q]], orth))) return st;
+  }
+  // x: Uy@x * Bz@x - Uz@x * By@x ; y: -(Ux@y * Bz@y - Uz@y * Bx@y) ; z: Ux@z * By@z - Uy@z * Bx@z
+  if ((st = launch_cross(E[0], U_E[0], tmp[4], U_E[1], tmp[2], 1))) return st;
+  if ((st = launch_cross(E[1], U_E[2], tmp[5], U_E[3], tmp[0], -1))) return st;
+  return launch_cross(E[2], U_E[4], tmp[3], U_E[5], tmp[1], 1);
+}
+
+// advect_B (ops_advect.f90:186-200): adv = curl(U_E x face2Edge(B)).
+int moons_advect_B(moons_field_t adv[3], moons_field_t U_E[6], moons_field_t B[3], moons_field_t temp_E_TF[6],
+                   moons_field_t temp_E[3]) {
+  int st = moons_edge_cross_face_E(temp_E, U_E, B, temp_E_TF);
+  if (st) return st;
+  for (int d = 0; d < 3; ++d)
+    if ((st = moons_curl(adv[d], temp_E[0], temp_E[1], temp_E[2], d))) return st;
+  return MOONS_OK;
 }
 
 int moons_volume(moons_field_t vol) {

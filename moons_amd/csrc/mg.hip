@@ -184,6 +184,8 @@ bool dirichlet_like(int t) { return t == MOONS_BC_DIRICHLET || t == MOONS_BC_ANT
 
 } // namespace
 
+constexpr int kCoarseCells = 8;
+
 MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int coarse_sweeps, std::string *err) {
   if (x->dl != MOONS_DL_CC) {
     if (err) *err = "multigrid needs a cell-centred field";
@@ -204,6 +206,9 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     const HostLevel &f = H.back();
     bool ok = true;
     for (int d = 0; d < 3; ++d) ok = ok && (f.n[d] % 2 == 0) && (f.n[d] / 2 >= 2);
+    // a level of at most kCoarseCells per direction is solved by the
+    // single-block sweeps; coarser levels cost launches and gain nothing
+    if (std::max(f.n[0], std::max(f.n[1], f.n[2])) <= kCoarseCells) ok = false;
     if (!ok) break;
     HostLevel c;
     for (int d = 0; d < 3; ++d) {

@@ -522,7 +522,7 @@ struct moons_pcg_s {
   double *r = nullptr, *p = nullptr, *q = nullptr; // device work vectors
   double *partA = nullptr, *partB = nullptr, *partM = nullptr, *partC = nullptr;
   MGHier *mg = nullptr; // MOONS_PREC_MG_FP32 only
-  int mg_max_levels = 16, mg_nu1 = 2, mg_nu2 = 2, mg_coarse = 40;
+  int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 40;
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -115,11 +115,16 @@ class Level:
         return self.interior(f) - (self.offdiag(x) + self.diag() * self.interior(x))
 
 
-def build(hn3, bct, max_levels=16):
+COARSE_CELLS = 8  # a level of at most this many cells per direction is the coarsest
+
+
+def build(hn3, bct, max_levels=16, coarse_cells=COARSE_CELLS):
     levels = [Level([np.asarray(h, float) for h in hn3], bct)]
     while len(levels) < max_levels:
         L = levels[-1]
         if not all(n % 2 == 0 and n // 2 >= 2 for n in L.n):
+            break
+        if max(L.n) <= coarse_cells:
             break
         levels.append(Level([coarsen_nodes(h) for h in L.hn], bct))
     return levels
@@ -172,7 +177,7 @@ def prolong(F: Level, C: Level, xc, bct):
     return e
 
 
-def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=40):
+def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=40):
     """x0 = V f0 (padded arrays; f0 = r/vol on the fine level)."""
     xs = [np.zeros(L.shape()) for L in levels]
     fs = [None] * len(levels)

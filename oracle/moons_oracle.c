@@ -432,6 +432,82 @@ void orc_del(double *dfdh, int dl_out, const double *f, int dl_in, const orc_gri
 }
 
 /* ------------------------------------------------------------------------- */
+/* interpolation                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* interpO2_SF: hw/ops/ops_interp.f90:190-221 over interp_C2N_GF / interp_N2C_GF
+   (hw/grid_field/GF_interp/GF_interp.f90:19-76) and extrap_staggered_GF
+   (GF_extrap.f90:17-46).  Loops span the whole other two dimensions. */
+void orc_interp(double *out, int dl_out, const double *in, int dl_in, const orc_grid *g, int dir) {
+  int so[3], si[3];
+  orc_field_shape(g, dl_out, so);
+  orc_field_shape(g, dl_in, si);
+  int nout = (dl_out >> dir) & 1, nin = (dl_in >> dir) & 1;
+  const orc_coords *c = &g->c[dir];
+  int a = (dir + 1) % 3, b = (dir + 2) % 3;
+  size_t stro[3] = {1, (size_t)so[0], (size_t)so[0] * so[1]};
+  size_t stri[3] = {1, (size_t)si[0], (size_t)si[0] * si[1]};
+  for (int pb = 0; pb < si[b]; ++pb)
+    for (int pa = 0; pa < si[a]; ++pa) {
+      const double *C = in + (size_t)pa * stri[a] + (size_t)pb * stri[b];
+      double *N = out + (size_t)pa * stro[a] + (size_t)pb * stro[b];
+      size_t fs = stri[dir], os = stro[dir];
+      if (nout && !nin) { /* C -> N */
+        int sC = si[dir], sN = so[dir];
+        for (int i = 1; i <= sC - 1; ++i)
+          N[(size_t)i * os] = C[(size_t)i * fs] * c->theta_D[i - 1] + C[(size_t)(i - 1) * fs] * c->theta_U[i - 1];
+        N[0] = 2.0 * C[0] - N[os];
+        N[(size_t)(sN - 1) * os] = 2.0 * C[(size_t)(sC - 1) * fs] - N[(size_t)(sN - 2) * os];
+      } else { /* N -> C */
+        int sN = si[dir];
+        for (int i = 0; i < sN - 1; ++i) N[(size_t)i * os] = 0.5 * (C[(size_t)i * fs] + C[(size_t)(i + 1) * fs]);
+      }
+    }
+}
+
+/* face2Edge_SF_1 with edgeDir != faceDir (ops_interp.f90:242-270): one
+   interpolation along the remaining direction. */
+static void face2edge(double *edge, int edge_dir, const double *face, int face_dir, const orc_grid *g) {
+  int orth = 3 - edge_dir - face_dir;
+  int dl_face = 1 << face_dir;
+  int dl_edge = 7 ^ (1 << edge_dir);
+  orc_interp(edge, dl_edge, face, dl_face, g, orth);
+}
+
+/* edgeCrossFace_E (ops_advect.f90:202-218) = face2Edge_TF_no_diag
+   (ops_interp.f90:531-544) then cross_product_TF (hw/fields/TF_extend.f90:385-400,
+   GF_cross_product.f90: x = Ay Bz - Az By, y = -(Ax Bz - Az Bx), z = Ax By - Ay Bx). */
+void orc_edge_cross_face_E(double *Ex, double *Ey, double *Ez, double *const U_E[6], double *const B[3],
+                           const orc_grid *g) {
+  size_t nx = orc_field_size(g, ORC_DL_EX), ny = orc_field_size(g, ORC_DL_EY), nz = orc_field_size(g, ORC_DL_EZ);
+  /* B components on edges: Bz_x, By_x (x-edges), Bx_y, Bz_y (y-edges), Bx_z, By_z (z-edges) */
+  double *Bzx = dalloc(nx), *Byx = dalloc(nx), *Bxy = dalloc(ny), *Bzy = dalloc(ny), *Bxz = dalloc(nz), *Byz = dalloc(nz);
+  face2edge(Bxy, 1, B[0], 0, g);
+  face2edge(Bxz, 2, B[0], 0, g);
+  face2edge(Byx, 0, B[1], 1, g);
+  face2edge(Byz, 2, B[1], 1, g);
+  face2edge(Bzx, 0, B[2], 2, g);
+  face2edge(Bzy, 1, B[2], 2, g);
+  const double *Uyx = U_E[0], *Uzx = U_E[1], *Uxy = U_E[2], *Uzy = U_E[3], *Uxz = U_E[4], *Uyz = U_E[5];
+  for (size_t q = 0; q < nx; ++q) Ex[q] = Uyx[q] * Bzx[q] - Uzx[q] * Byx[q];
+  for (size_t q = 0; q < ny; ++q) Ey[q] = -(Uxy[q] * Bzy[q] - Uzy[q] * Bxy[q]);
+  for (size_t q = 0; q < nz; ++q) Ez[q] = Uxz[q] * Byz[q] - Uyz[q] * Bxz[q];
+  free(Bzx); free(Byx); free(Bxy); free(Bzy); free(Bxz); free(Byz);
+}
+
+/* advect_B (ops_advect.f90:186-200): adv = curl(edgeCrossFace_E(U_E, B)),
+   curl_VF of an edge field (ops_discrete.f90:357-365). */
+void orc_advect_B(double *adv[3], double *const U_E[6], double *const B[3], const orc_grid *g) {
+  double *E[3];
+  E[0] = dalloc(orc_field_size(g, ORC_DL_EX));
+  E[1] = dalloc(orc_field_size(g, ORC_DL_EY));
+  E[2] = dalloc(orc_field_size(g, ORC_DL_EZ));
+  orc_edge_cross_face_E(E[0], E[1], E[2], U_E, B, g);
+  for (int d = 0; d < 3; ++d) orc_curl_component(adv[d], 1 << d, E[0], E[1], E[2], 0, g, d);
+  for (int d = 0; d < 3; ++d) free(E[d]);
+}
+
+/* ------------------------------------------------------------------------- */
 /* BCs                                                                       */
 /* ------------------------------------------------------------------------- */
 
@@ -779,6 +855,210 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
   }
   if (out) { out[0] = (double)iter; out[1] = res0; out[2] = res; out[3] = (double)nh; }
   free(r); free(z); free(p); free(Ax); free(tmp); free(vol); free(Minv);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* vector-field PCG                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* lap_centered_SF_given_both (ops_discrete.f90:112-122): staggered derivative
+   out and back along each direction, accumulated into lapU. */
+static void lap_given_both(double *lapU, const double *U, int dl, const orc_grid *g) {
+  for (int d = 0; d < 3; ++d) {
+    int dlt = dl ^ (1 << d);
+    double *t = dalloc(orc_field_size(g, dlt));
+    orc_del(t, dlt, U, dl, g, 1, d, 0, 1);
+    orc_del(lapU, dl, t, dlt, g, 1, d, 0, d == 0 ? 1 : 2);
+    free(t);
+  }
+}
+
+/* mom_diffusion(_explicit) / ind_diffusion(_explicit):
+   matrix_free_operators.f90:120-146,176-202 */
+void orc_op_vf(double *Ax[3], double *x[3], int kind, double c, double *const k[3], const orc_grid *g,
+               const orc_bcs bc[3], int implicit_) {
+  if (implicit_)
+    for (int d = 0; d < 3; ++d) orc_apply_bcs(x[d], 1 << d, g, &bc[d], 1);
+  if (kind == ORC_OP_MOM_DIFFUSION) {
+    for (int d = 0; d < 3; ++d) lap_given_both(Ax[d], x[d], 1 << d, g);
+  } else {
+    const int edl[3] = {ORC_DL_EX, ORC_DL_EY, ORC_DL_EZ};
+    double *E[3];
+    for (int d = 0; d < 3; ++d) E[d] = dalloc(orc_field_size(g, edl[d]));
+    for (int d = 0; d < 3; ++d) orc_curl_component(E[d], edl[d], x[0], x[1], x[2], 0, g, d);
+    for (int d = 0; d < 3; ++d) {
+      size_t n = orc_field_size(g, edl[d]);
+      for (size_t q = 0; q < n; ++q) E[d][q] = E[d][q] * k[d][q];
+    }
+    for (int d = 0; d < 3; ++d) orc_curl_component(Ax[d], 1 << d, E[0], E[1], E[2], 0, g, d);
+    for (int d = 0; d < 3; ++d) free(E[d]);
+  }
+  for (int d = 0; d < 3; ++d) {
+    size_t n = orc_field_size(g, 1 << d);
+    for (size_t q = 0; q < n; ++q) Ax[d][q] = Ax[d][q] * c;
+    for (size_t q = 0; q < n; ++q) Ax[d][q] = Ax[d][q] + x[d][q];
+  }
+}
+
+/* curl_curl_diagonal: hw/grid_field/GF_diagonals.f90:37-71 */
+static void curl_curl_diag(double *dg[3], double *const sig[3], const orc_grid *g) {
+  const orc_coords *C[3] = {&g->c[0], &g->c[1], &g->c[2]};
+  int sd[3][3], ss[3][3];
+  for (int d = 0; d < 3; ++d) orc_field_shape(g, 1 << d, sd[d]);
+  orc_field_shape(g, ORC_DL_EX, ss[0]);
+  orc_field_shape(g, ORC_DL_EY, ss[1]);
+  orc_field_shape(g, ORC_DL_EZ, ss[2]);
+  for (int d = 0; d < 3; ++d) memset(dg[d], 0, sizeof(double) * orc_field_size(g, 1 << d));
+  /* diag_x: y and z terms with sig_z and sig_y */
+  for (int kk = 1; kk < sd[0][2] - 1; ++kk)
+    for (int j = 1; j < sd[0][1] - 1; ++j)
+      for (int i = 1; i < sd[0][0] - 1; ++i)
+        dg[0][IDX(sd[0], i, j, kk)] =
+            -C[1]->n2cc_D[j] * sig[2][IDX(ss[2], i, j, kk)] * C[1]->cc2n_U[j - 1] -
+            C[1]->n2cc_U[j] * sig[2][IDX(ss[2], i, j + 1, kk)] * C[1]->cc2n_D[j] -
+            C[2]->n2cc_D[kk] * sig[1][IDX(ss[1], i, j, kk)] * C[2]->cc2n_U[kk - 1] -
+            C[2]->n2cc_U[kk] * sig[1][IDX(ss[1], i, j, kk + 1)] * C[2]->cc2n_D[kk];
+  for (int kk = 1; kk < sd[1][2] - 1; ++kk)
+    for (int j = 1; j < sd[1][1] - 1; ++j)
+      for (int i = 1; i < sd[1][0] - 1; ++i)
+        dg[1][IDX(sd[1], i, j, kk)] =
+            -C[0]->n2cc_D[i] * sig[2][IDX(ss[2], i, j, kk)] * C[0]->cc2n_U[i - 1] -
+            C[0]->n2cc_U[i] * sig[2][IDX(ss[2], i + 1, j, kk)] * C[0]->cc2n_D[i] -
+            C[2]->n2cc_D[kk] * sig[0][IDX(ss[0], i, j, kk)] * C[2]->cc2n_U[kk - 1] -
+            C[2]->n2cc_U[kk] * sig[0][IDX(ss[0], i, j, kk + 1)] * C[2]->cc2n_D[kk];
+  for (int kk = 1; kk < sd[2][2] - 1; ++kk)
+    for (int j = 1; j < sd[2][1] - 1; ++j)
+      for (int i = 1; i < sd[2][0] - 1; ++i)
+        dg[2][IDX(sd[2], i, j, kk)] =
+            -C[0]->n2cc_D[i] * sig[1][IDX(ss[1], i, j, kk)] * C[0]->cc2n_U[i - 1] -
+            C[0]->n2cc_U[i] * sig[1][IDX(ss[1], i + 1, j, kk)] * C[0]->cc2n_D[i] -
+            C[1]->n2cc_D[j] * sig[0][IDX(ss[0], i, j, kk)] * C[1]->cc2n_U[j - 1] -
+            C[1]->n2cc_U[j] * sig[0][IDX(ss[0], i, j + 1, kk)] * C[1]->cc2n_D[j];
+}
+
+/* prec_mom_SF / prec_ind_VF: preconditioners.f90:86-129 */
+void orc_minv_vf(double *Minv[3], int kind, double c, double *const k[3], const orc_grid *g) {
+  if (kind == ORC_OP_IND_DIFFUSION) curl_curl_diag(Minv, k, g);
+  for (int d = 0; d < 3; ++d) {
+    int dl = 1 << d;
+    size_t n = orc_field_size(g, dl);
+    if (kind == ORC_OP_MOM_DIFFUSION) orc_lap_diagonal(Minv[d], dl, g);
+    double *vol = dalloc(n);
+    orc_volume(vol, dl, g);
+    for (size_t q = 0; q < n; ++q) Minv[d][q] = Minv[d][q] * c;
+    for (size_t q = 0; q < n; ++q) Minv[d][q] = Minv[d][q] + 1.0;
+    for (size_t q = 0; q < n; ++q) Minv[d][q] = Minv[d][q] * vol[q];
+    for (size_t q = 0; q < n; ++q) Minv[d][q] = 1.0 / Minv[d][q];
+    orc_assign_ghost_xperiodic(Minv[d], dl, g, NULL, 0.0);
+    free(vol);
+  }
+}
+
+/* dot_product_VF: hw/ops/ops_aux.f90:199-207 */
+static double dot_vf(double *const A[3], double *const B[3], const orc_grid *g, const orc_bcs bc[3]) {
+  double s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    int dl = 1 << d, sh[3];
+    orc_field_shape(g, dl, sh);
+    size_t n = orc_field_size(g, dl);
+    double *t = dalloc(n);
+    for (size_t q = 0; q < n; ++q) t[q] = A[d][q] * B[d][q];
+    orc_assign_wall_dirichlet(t, dl, g, &bc[d], 0.0);
+    s = s + orc_sum_pad(t, sh, 1);
+    free(t);
+  }
+  return s;
+}
+
+int orc_pcg_vf(const orc_grid *g, int kind, double c, double *const k[3], const orc_bcs bc[3], double *x[3],
+               double *const b[3], int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+               double *res_hist) {
+  if (n_skip < 1) return -1;
+  double *r[3], *z[3], *p[3], *Ax[3], *t[3], *vol[3], *Minv[3];
+  size_t n[3];
+  for (int d = 0; d < 3; ++d) {
+    n[d] = orc_field_size(g, 1 << d);
+    r[d] = dalloc(n[d]); z[d] = dalloc(n[d]); p[d] = dalloc(n[d]); Ax[d] = dalloc(n[d]);
+    t[d] = dalloc(n[d]); vol[d] = dalloc(n[d]); Minv[d] = dalloc(n[d]);
+    orc_volume(vol[d], 1 << d, g);
+  }
+  orc_minv_vf(Minv, kind, c, k, g);
+  /* modify_RHS_VF */
+  for (int d = 0; d < 3; ++d) memcpy(r[d], b[d], sizeof(double) * n[d]);
+  for (int d = 0; d < 3; ++d) {
+    memset(t[d], 0, sizeof(double) * n[d]);
+    orc_apply_bcs(t[d], 1 << d, g, &bc[d], 0);
+    assign_ghost_n_xperiodic(t[d], 1 << d, g, &bc[d], 0.0);
+  }
+  orc_op_vf(Ax, t, kind, c, k, g, bc, 0);
+  for (int d = 0; d < 3; ++d) {
+    orc_assign_ghost_xperiodic(Ax[d], 1 << d, g, &bc[d], 0.0);
+    for (size_t q = 0; q < n[d]; ++q) r[d][q] = r[d][q] - Ax[d][q];
+  }
+  orc_op_vf(Ax, x, kind, c, k, g, bc, 1);
+  for (int d = 0; d < 3; ++d) {
+    for (size_t q = 0; q < n[d]; ++q) r[d][q] = r[d][q] - Ax[d][q];
+    for (size_t q = 0; q < n[d]; ++q) r[d][q] = r[d][q] * vol[d][q];
+    assign_wall_periodic_single(r[d], 1 << d, g, &bc[d], 0.0);
+    orc_assign_wall_dirichlet(r[d], 1 << d, g, &bc[d], 0.0);
+    orc_multiply_wall_neumann(r[d], 1 << d, g, &bc[d], 0.5);
+  }
+  double res0 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double e = 0.0;
+    for (size_t q = 0; q < n[d]; ++q) e += r[d][q] * r[d][q];
+    res0 = res0 + e;
+  }
+  res0 = sqrt(res0);
+  for (int d = 0; d < 3; ++d) {
+    for (size_t q = 0; q < n[d]; ++q) z[d][q] = Minv[d][q] * r[d][q];
+    memcpy(p[d], z[d], sizeof(double) * n[d]);
+  }
+  double rhok = dot_vf(r, z, g, bc);
+  double res = sqrt(fabs(rhok));
+  int nh = 0;
+  if (res_hist) res_hist[nh++] = res;
+  int iter = 0;
+  int e2 = res < tol_abs;
+  if (!e2) {
+    for (int it = 1; it <= iter_max; ++it) {
+      orc_op_vf(Ax, p, kind, c, k, g, bc, 1);
+      for (int d = 0; d < 3; ++d) {
+        orc_multiply_wall_neumann(Ax[d], 1 << d, g, &bc[d], 0.5);
+        orc_assign_wall_dirichlet(Ax[d], 1 << d, g, &bc[d], 0.0);
+        assign_wall_periodic_single(Ax[d], 1 << d, g, &bc[d], 0.0);
+        for (size_t q = 0; q < n[d]; ++q) Ax[d][q] = Ax[d][q] * vol[d][q];
+      }
+      double alpha = rhok / dot_vf(p, Ax, g, bc);
+      for (int d = 0; d < 3; ++d) {
+        for (size_t q = 0; q < n[d]; ++q) x[d][q] = x[d][q] + p[d][q] * alpha;
+        orc_apply_bcs(x[d], 1 << d, g, &bc[d], 0);
+      }
+      iter++;
+      for (int d = 0; d < 3; ++d)
+        for (size_t q = 0; q < n[d]; ++q) r[d][q] = r[d][q] + Ax[d][q] * (-alpha);
+      if (iter % n_skip == 0) {
+        res = sqrt(fabs(dot_vf(r, r, g, bc)));
+        if (res_hist) res_hist[nh++] = res;
+        if (res / res0 < tol_rel || res < tol_abs || iter >= iter_max) break;
+      }
+      for (int d = 0; d < 3; ++d)
+        for (size_t q = 0; q < n[d]; ++q) z[d][q] = Minv[d][q] * r[d][q];
+      double rhokp1 = dot_vf(z, r, g, bc);
+      double beta = rhokp1 / rhok;
+      for (int d = 0; d < 3; ++d)
+        for (size_t q = 0; q < n[d]; ++q) p[d][q] = p[d][q] * beta + z[d][q];
+      rhok = rhokp1;
+    }
+  } else {
+    for (int d = 0; d < 3; ++d) orc_apply_bcs(x[d], 1 << d, g, &bc[d], 0);
+    iter++;
+  }
+  if (out) { out[0] = (double)iter; out[1] = res0; out[2] = res; out[3] = (double)nh; }
+  for (int d = 0; d < 3; ++d) {
+    free(r[d]); free(z[d]); free(p[d]); free(Ax[d]); free(t[d]); free(vol[d]); free(Minv[d]);
+  }
   return 0;
 }
 

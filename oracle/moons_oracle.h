@@ -93,6 +93,20 @@ void orc_curl_component(double *curlU, int dl_out, const double *u, const double
 void orc_lap_centered_sf(double *lapU, const double *U, int dl, const orc_grid *g,
                          const orc_bcs *bc_out);
 
+/* ---- interpolation (reference hw/ops/ops_interp.f90:190-221,
+   hw/grid_field/GF_interp/GF_interp.f90:19-76, GF_extrap.f90:17-46) ----
+   C->N along dir (theta weights, ghost nodes extrapolated) or N->C (half sum). */
+void orc_interp(double *out, int dl_out, const double *in, int dl_in, const orc_grid *g, int dir);
+
+/* ---- constrained transport (reference hw/ops/ops_advect.f90:186-218) ----
+   U_E: the six off-diagonal edge components of velocity, ordered
+   U%y%x, U%z%x (on x-edges), U%x%y, U%z%y (y-edges), U%x%z, U%y%z (z-edges).
+   B: the three face components.  E = U_E x face2Edge_no_diag(B) on edges;
+   advect_B: adv = curl(E) on faces. */
+void orc_edge_cross_face_E(double *Ex, double *Ey, double *Ez, double *const U_E[6], double *const B[3],
+                           const orc_grid *g);
+void orc_advect_B(double *adv[3], double *const U_E[6], double *const B[3], const orc_grid *g);
+
 /* ---- BCs (reference hw/BCs/direct and hw/BCs/implicit raw kernels) ---- */
 void orc_apply_bcs(double *u, int dl, const orc_grid *g, const orc_bcs *bc, int implicit_);
 void orc_assign_ghost_xperiodic(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val);
@@ -115,6 +129,23 @@ double orc_sum_all(const double *u, size_t n);
 int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann,
                    double *x, const double *b, int iter_max, double tol_rel, double tol_abs,
                    int n_skip_check_res, double *out, double *res_hist);
+
+/* ---- vector-field PCG (momentum and induction implicit diffusion) ----
+   Restates solve_PCG_VF (hw/solvers/PCG/PCG_solver_algorithm.f90:109-179) and
+   modify_RHS_VF (PCG_aux.f90:56-79) with
+     kind 1: mom_diffusion  A = I + c lap_centered_VF  (matrix_free_operators.f90:176-202),
+             prec_mom_VF    (preconditioners.f90:86-113)
+     kind 2: ind_diffusion  A = I + c curl(k curl)     (matrix_free_operators.f90:120-146),
+             prec_ind_VF    (preconditioners.f90:115-129), k = sigmaInv on edges.
+   x, b: the three face components (ghosts included); bc: BCs of each component. */
+#define ORC_OP_MOM_DIFFUSION 1
+#define ORC_OP_IND_DIFFUSION 2
+void orc_op_vf(double *Ax[3], double *x[3], int kind, double c, double *const k[3], const orc_grid *g,
+               const orc_bcs bc[3], int implicit_);
+int orc_pcg_vf(const orc_grid *g, int kind, double c, double *const k[3], const orc_bcs bc[3], double *x[3],
+               double *const b[3], int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+               double *res_hist);
+void orc_minv_vf(double *Minv[3], int kind, double c, double *const k[3], const orc_grid *g);
 
 /* seeded SplitMix64 -> uniform [-1,1) */
 void orc_splitmix_fill(double *out, size_t n, unsigned long long seed, double amplitude);

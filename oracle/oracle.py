@@ -67,6 +67,14 @@ def lib():
         L.orc_pcg_lap_sf.argtypes = [C.c_void_p, C.c_int, C.POINTER(BCs), C.c_int, _dp, _dp, C.c_int, C.c_double,
                                      C.c_double, C.c_int, _dp, _dp]
         L.orc_splitmix_fill.argtypes = [_dp, C.c_size_t, C.c_ulonglong, C.c_double]
+        L.orc_interp.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_void_p, C.c_int]
+        L.orc_edge_cross_face_E.argtypes = [_dp, _dp, _dp, _dp * 6, _dp * 3, C.c_void_p]
+        L.orc_advect_B.argtypes = [_dp * 3, _dp * 6, _dp * 3, C.c_void_p]
+        L.orc_op_vf.argtypes = [_dp * 3, _dp * 3, C.c_int, C.c_double, _dp * 3, C.c_void_p, BCs * 3, C.c_int]
+        L.orc_pcg_vf.restype = C.c_int
+        L.orc_pcg_vf.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp * 3, BCs * 3, _dp * 3, _dp * 3, C.c_int,
+                                 C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_minv_vf.argtypes = [_dp * 3, C.c_int, C.c_double, _dp * 3, C.c_void_p]
         _lib = L
     return _lib
 
@@ -156,6 +164,66 @@ def lap_centered(g: Grid, U, dl, bcs=None):
     out = g.zeros(dl)
     lib().orc_lap_centered_sf(_F(out), _F(U), dl, g.h, C.byref(bcs) if bcs else None)
     return out
+
+
+def interp(g: Grid, f, dl_in, dl_out, d):
+    out = g.zeros(dl_out)
+    lib().orc_interp(_F(out), dl_out, _F(f), dl_in, g.h, d)
+    return out
+
+
+def edge_cross_face_E(g: Grid, U_E, B):
+    E = [g.zeros(dl) for dl in (6, 5, 3)]
+    lib().orc_edge_cross_face_E(_F(E[0]), _F(E[1]), _F(E[2]), (_dp * 6)(*[_F(u) for u in U_E]),
+                                (_dp * 3)(*[_F(b) for b in B]), g.h)
+    return E
+
+
+def advect_B(g: Grid, U_E, B):
+    adv = [g.zeros(1 << d) for d in range(3)]
+    lib().orc_advect_B((_dp * 3)(*[_F(a) for a in adv]), (_dp * 6)(*[_F(u) for u in U_E]),
+                       (_dp * 3)(*[_F(b) for b in B]), g.h)
+    return adv
+
+
+MOM_DIFFUSION, IND_DIFFUSION = 1, 2
+
+
+def _p3(arrs):
+    return (_dp * 3)(*[_F(a) for a in arrs])
+
+
+def _k3(g, k):
+    if k is None:  # unused by mom_diffusion; any edge arrays do
+        k = [g.zeros(dl) for dl in (6, 5, 3)]
+    return k, _p3(k)
+
+
+def op_vf(g: Grid, x, kind, c, bcs3, k=None, implicit=True):
+    """Ax of mom_diffusion / ind_diffusion; x is modified (implicit BCs) when implicit."""
+    Ax = [g.zeros(1 << d) for d in range(3)]
+    k, kp = _k3(g, k)
+    lib().orc_op_vf(_p3(Ax), _p3(x), kind, c, kp, g.h, (BCs * 3)(*bcs3), 1 if implicit else 0)
+    return Ax
+
+
+def minv_vf(g: Grid, kind, c, k=None):
+    M = [g.zeros(1 << d) for d in range(3)]
+    k, kp = _k3(g, k)
+    lib().orc_minv_vf(_p3(M), kind, c, kp, g.h)
+    return M
+
+
+def pcg_vf(g: Grid, kind, c, bcs3, x, b, iter_max, tol_rel, tol_abs, n_skip=1, k=None):
+    """VF PCG; x (three F-ordered face arrays) updated in place."""
+    out = np.zeros(4)
+    hist = np.zeros(iter_max + 2)
+    k, kp = _k3(g, k)
+    st = lib().orc_pcg_vf(g.h, kind, c, kp, (BCs * 3)(*bcs3), _p3(x), _p3(b), iter_max, tol_rel, tol_abs,
+                          n_skip, p(out), p(hist))
+    if st != 0:
+        raise RuntimeError("oracle pcg_vf failed")
+    return int(out[0]), out[1], out[2], hist[: int(out[3])].copy()
 
 
 def apply_bcs(g: Grid, u, dl, bcs: BCs, implicit=False):

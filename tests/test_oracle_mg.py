@@ -74,7 +74,7 @@ def test_mg_pcg_reaches_dense_solution():
     bct = BCS["duct"]
     g = O.Grid(*hs)
     bcs = O.BCs.make(["Periodic", "Periodic"] + ["Neumann"] * 4)
-    levels = MG.build(hs, bct)
+    levels = MG.build(hs, bct, coarse_cells=2)
     assert len(levels) == 3
     L = levels[0]
     vol = O.volume(g, 0)[1:-1, 1:-1, 1:-1]
