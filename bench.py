@@ -63,13 +63,15 @@ def splitmix(n, seed, amp):
 
 
 def load_traffic(name):
+    """HBM bytes per launch of a kernel, from its committed rocprofv3 PMC summary
+    (the line `hbm_bytes_per_launch: N`), or None."""
     p = os.path.join(ROOT, "profiles", name)
-    if os.path.exists(p):
-        try:
-            with open(p) as fh:
-                return json.load(fh).get("hbm_bytes_per_launch")
-        except Exception:
-            return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        for line in fh:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
     return None
 
 
@@ -126,6 +128,12 @@ def main():
         per = {"pcg_matvec": 16, "pcg_update": 48, "pcg_pupdate": 24}
         print(json.dumps({n: {"us": round(1e3 * t, 2), "GBs": round(per[n] * N ** 3 / (t * 1e-3) / 1e9, 1)}
                           for n, t in zip(names, kms)}), flush=True)
+        # calibration for the PMC byte counters: 20 device-to-device copies of
+        # one 130^3 FP64 field (17,576,000 bytes read and written per copy)
+        c = M.SF(m, "CC")
+        for _ in range(20):
+            c.assign(b)
+        M.synchronize()
         return
 
     def timed(solver, xf):
@@ -152,6 +160,12 @@ def main():
     Pm = M.PCG_solver_SF(m, xm, M.iter_solver_params(2000, TOL, 1e-300, 1), prec="MG_FP32")
     mg_ms, mg_info = timed(Pm, xm)
     mg_levels, mg_coarsest = Pm.mg_info()
+    # the same solver with the 128^3 duct configuration's stated multigrid
+    # (4 levels, 2 pre- and 2 post-smoothing red-black sweeps)
+    xc = M.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
+    Pc = M.PCG_solver_SF(m, xc, M.iter_solver_params(2000, TOL, 1e-300, 1), prec="MG_FP32")
+    Pc.set_MG(4, 2, 2, 40)
+    mg4_ms, mg4_info = timed(Pc, xc)
     # agreement of the two solutions (free constant removed, interior cells)
     xa, xb = x.download()[1:-1, 1:-1, 1:-1], xm.download()[1:-1, 1:-1, 1:-1]
     xa, xb = xa - xa.mean(), xb - xb.mean()
@@ -168,7 +182,7 @@ def main():
     iter_us = 1e3 * sum(kms)
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(f"{dname}_128_traffic.json"),
+                "traffic": load_traffic(f"{dname}_128_traffic.txt"),
                 "per_kernel_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
                 "per_kernel_GBs": {n: round(alg[n] / (t * 1e-3) / 1e9, 1) for n, t in zip(names, kms)},
                 "iteration_GBs": round(88 * ncell / (iter_us * 1e-6) / 1e9, 1)}
@@ -213,6 +227,8 @@ def main():
         "mixed_mg_levels": mg_levels,
         "mixed_mg_coarsest_cells": list(mg_coarsest),
         "mixed_vs_fp64_solution_rel_l2": mg_vs_fp64,
+        "mixed_mg_4levels_2x2_ms": round(mg4_ms, 4),
+        "mixed_mg_4levels_2x2_iterations": mg4_info.iterations,
         "speedup_mixed_vs_plain": round(ms_step / mg_ms, 3),
         "pcie_inclusive_ms": round(pcie_ms, 3),
         "roofline": roofline,

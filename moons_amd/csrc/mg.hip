@@ -38,7 +38,18 @@ struct MGHier {
   std::vector<LevelDev> L;
   std::vector<float *> arrays; // device allocations to free
   int nu1 = 2, nu2 = 2, coarse_sweeps = 40;
+  int tail0 = 0;               // first level of the LDS-resident tail (== L.size(): none)
+  size_t tail_lds = 0;         // LDS bytes of the tail's x and f arrays
+  LevelDev *dlev = nullptr;    // level descriptors in device memory (for the tail kernel)
 };
+
+// The coarse end of the hierarchy (levels whose x and f fit in one
+// workgroup's LDS together) is cycled by a single workgroup in one launch:
+// the same smoothing, restriction, coarse sweeps and prolongation as the
+// per-level launches, in the same order, so the result is identical.
+constexpr int kTailThreads = 1024;
+constexpr size_t kTailLdsBytes = 60 * 1024;
+constexpr long long kTailMaxCells = 8192;
 
 __device__ __forceinline__ size_t lidx(const LevelDev &L, int i, int j, int k) {
   return (size_t)i + (size_t)L.s0 * ((size_t)j + (size_t)L.s1 * (size_t)k);
@@ -156,6 +167,109 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(LevelDev L, int sweeps, cons
         if (color_cell(L, t, color, i, j, k)) L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, s == 0 && color == 0);
       }
       __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ size_t padded_cells(const LevelDev &L) {
+  return (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+}
+
+// Descriptor of tail level l with x and f redirected into LDS.
+__device__ __forceinline__ LevelDev tail_level(const LevelDev *lev, int t0, int l, float *sm) {
+  size_t off = 0;
+  for (int q = t0; q < l; ++q) off += 2 * padded_cells(lev[q]);
+  LevelDev L = lev[l];
+  L.x = sm + off;
+  L.f = sm + off + padded_cells(L);
+  return L;
+}
+
+__device__ __forceinline__ void tail_smooth(const LevelDev &L, int color, bool zero_guess) {
+  const long long ncol = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
+  for (long long t = threadIdx.x; t < ncol; t += kTailThreads) {
+    int i, j, k;
+    if (color_cell(L, t, color, i, j, k)) L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, zero_guess);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, int t0, int nl, int nu1, int nu2,
+                                                          int coarse_sweeps, const int *done) {
+  if (done && *done) return;
+  extern __shared__ float sm[];
+  {
+    const LevelDev G = lev[t0];
+    const LevelDev L = tail_level(lev, t0, t0, sm);
+    const size_t n = padded_cells(G);
+    for (size_t t = threadIdx.x; t < n; t += kTailThreads) L.f[t] = G.f[t];
+    __syncthreads();
+  }
+  for (int l = t0; l < nl - 1; ++l) {
+    const LevelDev F = tail_level(lev, t0, l, sm);
+    for (int s = 0; s < nu1; ++s) {
+      tail_smooth(F, 0, s == 0);
+      tail_smooth(F, 1, false);
+    }
+    const LevelDev C = tail_level(lev, t0, l + 1, sm);
+    const long long nc = (long long)C.n0 * C.n1 * C.n2;
+    for (long long t = threadIdx.x; t < nc; t += kTailThreads) {
+      const int I = 1 + (int)(t % C.n0);
+      const int J = 1 + (int)((t / C.n0) % C.n1);
+      const int K = 1 + (int)(t / ((long long)C.n0 * C.n1));
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
+        acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
+      }
+      C.f[lidx(C, I, J, K)] = acc;
+    }
+    __syncthreads();
+  }
+  {
+    const LevelDev C = tail_level(lev, t0, nl - 1, sm);
+    for (int s = 0; s < coarse_sweeps; ++s) {
+      tail_smooth(C, 0, s == 0);
+      tail_smooth(C, 1, false);
+    }
+  }
+  for (int l = nl - 2; l >= t0; --l) {
+    const LevelDev F = tail_level(lev, t0, l, sm);
+    const LevelDev C = tail_level(lev, t0, l + 1, sm);
+    const long long nf = (long long)F.n0 * F.n1 * F.n2;
+    for (long long t = threadIdx.x; t < nf; t += kTailThreads) {
+      const int i = 1 + (int)(t % F.n0);
+      const int j = 1 + (int)((t / F.n0) % F.n1);
+      const int k = 1 + (int)(t / ((long long)F.n0 * F.n1));
+      const int I = F.pI[0][i], I2 = F.pI2[0][i];
+      const int J = F.pI[1][j], J2 = F.pI2[1][j];
+      const int K = F.pI[2][k], K2 = F.pI2[2][k];
+      const float ax = F.pa[0][i], bx = F.pb[0][i];
+      const float ay = F.pa[1][j], by = F.pb[1][j];
+      const float az = F.pa[2][k], bz = F.pb[2][k];
+      const float *c = C.x;
+      const float e = ax * (ay * (az * c[lidx(C, I, J, K)] + bz * c[lidx(C, I, J, K2)]) +
+                            by * (az * c[lidx(C, I, J2, K)] + bz * c[lidx(C, I, J2, K2)])) +
+                      bx * (ay * (az * c[lidx(C, I2, J, K)] + bz * c[lidx(C, I2, J, K2)]) +
+                            by * (az * c[lidx(C, I2, J2, K)] + bz * c[lidx(C, I2, J2, K2)]));
+      F.x[lidx(F, i, j, k)] += e;
+    }
+    __syncthreads();
+    for (int s = 0; s < nu2; ++s) {
+      tail_smooth(F, 1, false);
+      tail_smooth(F, 0, false);
+    }
+  }
+  {
+    const LevelDev G = lev[t0];
+    const LevelDev L = tail_level(lev, t0, t0, sm);
+    const long long n = (long long)G.n0 * G.n1 * G.n2;
+    for (long long t = threadIdx.x; t < n; t += kTailThreads) {
+      const int i = 1 + (int)(t % G.n0);
+      const int j = 1 + (int)((t / G.n0) % G.n1);
+      const int k = 1 + (int)(t / ((long long)G.n0 * G.n1));
+      G.x[lidx(G, i, j, k)] = L.x[lidx(L, i, j, k)];
     }
   }
 }
@@ -327,6 +441,28 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       L.pa[d] = pad; L.pb[d] = pbd; L.pI[d] = pId; L.pI2[d] = pI2d;
     }
   }
+  // LDS-resident tail: the longest coarse suffix whose x and f fit in LDS
+  h->tail0 = nl;
+  size_t bytes = 0;
+  for (int l = nl - 1; l >= 0; --l) {
+    const LevelDev &L = h->L[l];
+    const size_t b = 2 * sizeof(float) * (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+    if (bytes + b > kTailLdsBytes || (long long)L.n0 * L.n1 * L.n2 > kTailMaxCells) break;
+    bytes += b;
+    h->tail0 = l;
+  }
+  h->tail_lds = bytes;
+  {
+    float *p = dalloc((nl * sizeof(LevelDev) + sizeof(float) - 1) / sizeof(float));
+    if (!p) {
+      if (err) *err = "multigrid: hipMalloc failed";
+      mg_destroy(h);
+      return nullptr;
+    }
+    h->dlev = reinterpret_cast<LevelDev *>(p);
+    (void)hipMemcpyAsync(h->dlev, h->L.data(), nl * sizeof(LevelDev), hipMemcpyHostToDevice, stream());
+    (void)hipStreamSynchronize(stream());
+  }
   return h;
 }
 
@@ -363,8 +499,10 @@ static void smooth(const LevelDev &L, int color, bool zero_guess, const int *don
 
 int mg_vcycle(MGHier *h, const int *done) {
   const int nl = (int)h->L.size();
+  const bool tail = h->tail0 < nl;
+  const int g = tail ? h->tail0 : nl - 1; // levels [0, g) run as grid launches
   // descend
-  for (int l = 0; l < nl - 1; ++l) {
+  for (int l = 0; l < g; ++l) {
     const LevelDev &L = h->L[l];
     for (int s = 0; s < h->nu1; ++s) {
       smooth(L, 0, s == 0, done);
@@ -374,9 +512,13 @@ int mg_vcycle(MGHier *h, const int *done) {
     const long long nc = (long long)C.n0 * C.n1 * C.n2;
     hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, stream(), L, C, done);
   }
-  hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
+  if (tail)
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tail0, nl, h->nu1,
+                       h->nu2, h->coarse_sweeps, done);
+  else
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
   // ascend
-  for (int l = nl - 2; l >= 0; --l) {
+  for (int l = g - 1; l >= 0; --l) {
     const LevelDev &L = h->L[l];
     const LevelDev &C = h->L[l + 1];
     const long long nf = (long long)L.n0 * L.n1 * L.n2;

@@ -824,8 +824,10 @@ int moons_pcg_apply_prec(moons_pcg_t S, moons_field_t r, moons_field_t z) {
 
 int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   if (!S || !ms || reps < 1) return fail(MOONS_ERR_ARG, "moons_pcg_time_kernels: bad argument");
-  // Runs plain CG iterations on the solver's current p and r (which change)
-  // with the exit test disabled, updating a scratch x; events bracket each kernel.
+  // One plain iteration sets up p, q, r and the partial sums; then each
+  // kernel is launched reps times back to back between two events (the exit
+  // test disabled, a scratch x updated), so the average is the kernel's
+  // duration plus the dispatch gap between consecutive launches.
   double *xs = nullptr;
   const size_t bytes = (size_t)S->G.s0 * S->G.s1 * S->G.s2 * sizeof(double);
   MOONS_HIP(hipMalloc(&xs, bytes));
@@ -837,30 +839,23 @@ int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   h.rho[0] = h.rho[1] = 1.0;
   h.res0 = 1.0;
   MOONS_HIP(hipMemcpyAsync(S->ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, stream()));
-  hipEvent_t e[4];
-  for (int i = 0; i < 4; ++i) MOONS_HIP(hipEventCreate(&e[i]));
-  double acc[3] = {0, 0, 0};
-  for (int rep = 0; rep < reps + 1; ++rep) {
-    const int it = rep + 1;
-    int st;
+  hipEvent_t e[2];
+  for (int i = 0; i < 2; ++i) MOONS_HIP(hipEventCreate(&e[i]));
+  int st;
+  if ((st = pcg_launch(S, xs, 1, 1 | 2 | 4))) return st;
+  for (int kk = 0; kk < 3; ++kk) {
+    if ((st = pcg_launch(S, xs, 2, 1 << kk))) return st; // warm-up
     MOONS_HIP(hipEventRecord(e[0], stream()));
-    if ((st = pcg_launch(S, xs, it, 1))) return st;
+    for (int rep = 0; rep < reps; ++rep)
+      if ((st = pcg_launch(S, xs, 2, 1 << kk))) return st;
     MOONS_HIP(hipEventRecord(e[1], stream()));
-    if ((st = pcg_launch(S, xs, it, 2))) return st;
-    MOONS_HIP(hipEventRecord(e[2], stream()));
-    if ((st = pcg_launch(S, xs, it, 4))) return st;
-    MOONS_HIP(hipEventRecord(e[3], stream()));
-    MOONS_HIP(hipEventSynchronize(e[3]));
-    if (rep == 0) continue; // warm-up
-    for (int kk = 0; kk < 3; ++kk) {
-      float tms = 0.f;
-      MOONS_HIP(hipEventElapsedTime(&tms, e[kk], e[kk + 1]));
-      acc[kk] += tms;
-    }
+    MOONS_HIP(hipEventSynchronize(e[1]));
+    float tms = 0.f;
+    MOONS_HIP(hipEventElapsedTime(&tms, e[0], e[1]));
+    ms[kk] = tms / reps;
   }
-  for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+  for (int i = 0; i < 2; ++i) (void)hipEventDestroy(e[i]);
   (void)hipFree(xs);
-  for (int kk = 0; kk < 3; ++kk) ms[kk] = acc[kk] / reps;
   return MOONS_OK;
 }
 

@@ -76,7 +76,8 @@ def test_vf_pcg_matches_oracle(gpu, op, bcname, N, c):
     info, xg, (it, res0, res, xr) = solve_both(gpu, g, m, obcs, x0, b, X, B, op, c, k)
     assert info.iterations == it
     assert abs(info.res_norm0 - res0) <= 1e-12 * res0
-    assert abs(info.res_norm - res) <= 1e-6 * res + 1e-14 * res0
+    # late residuals sit near the rounding floor of the recurrence: compare on the scale of res0
+    assert abs(info.res_norm - res) <= 1e-11 * res0
     assert rel_l2(xg, xr) <= 1e-10
 
 
