@@ -79,6 +79,7 @@ typedef struct moons_mesh_s *moons_mesh_t;
 typedef struct moons_field_s *moons_field_t;
 typedef struct moons_pcg_s *moons_pcg_t;
 typedef struct moons_pcgvf_s *moons_pcgvf_t;
+typedef struct moons_mhd_s *moons_mhd_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
@@ -239,6 +240,59 @@ int moons_pcg_vf_update_mfp(moons_pcgvf_t pcg, double coeff_implicit_time_split,
  * code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:109-179): x in/out,
  * b the right-hand side, three face components each. */
 int moons_pcg_vf_solve(moons_pcgvf_t pcg, moons_field_t x[3], moons_field_t b[3], moons_pcg_info *info);
+
+/* ------------------------------------------------------- the time step */
+/* Parameters of one MOONS MHD time step (solve_MOONS,
+ * code/handwritten/MOONS/MOONS_solver.f90:47-144, one RK stage):
+ *   momentum sources F = -u.grad(u) + (1-alpha_U)/Re lap(U) + J x B
+ *     (add_all_momentum_sources, solvers/momentum/add_all_momentum_sources.f90:22-86);
+ *   alpha_U = 0: Euler_time_no_diff_AB2_sources (method 4), alpha_U = 1:
+ *     Euler_time_AB2_sources (method 6) with A = I - dt/Re lap
+ *     (solvers/time_marching_methods.f90:117-150,266-296);
+ *   induction (when induction != 0): low-Rem (Btot = B0), CT source
+ *     curl(U_E x Btot), backward-Euler curl-curl diffusion with sigma = 1,
+ *     phi cleaning; J = curl(B)/Rem, J x B scaled by N = Ha^2/Re
+ *     (induction_extend.f90:475-536, induction_sources.f90:28-84).
+ * Solver settings index 0..3 = U, p, B, phi (iter_max, tol_rel, tol_abs,
+ * n_skip_check_res); prec_p selects the PPE preconditioner
+ * (MOONS_PREC_LAP_SF or MOONS_PREC_MG_FP32). */
+typedef struct {
+  double Re, Ha, Rem, dt;
+  double alpha_U;
+  int induction;
+  int prec_p;
+  double B0[3];
+  int iter_max[4];
+  double tol_rel[4];
+  double tol_abs[4];
+  int n_skip[4];
+} moons_mhd_params;
+
+typedef struct {
+  int iterations[4]; /* of the last step's U, p, B, phi solves */
+  double ms;         /* device time of the call */
+} moons_mhd_info;
+
+#define MOONS_MHD_F 1       /* momentum sources F (3 faces) */
+#define MOONS_MHD_FNM1 2    /* previous step's F */
+#define MOONS_MHD_USTAR 3   /* intermediate velocity */
+#define MOONS_MHD_U_E 4     /* U on edges, 6 entries: Uy@x, Uz@x, Ux@y, Uz@y, Ux@z, Uy@z */
+#define MOONS_MHD_J 5       /* current density on edges */
+#define MOONS_MHD_JCROSSB 6 /* N J x B on faces */
+#define MOONS_MHD_FB 7      /* induction sources */
+
+/* Creates the step state over the caller's U (faces, with BCs), p (CC, with
+ * BCs), and for induction B (faces, with BCs) and phi (CC, with BCs); these
+ * stay the caller's and are advanced in place.  F and Fnm1 start at zero
+ * (the reference's start-up), U_E, J and J x B are computed from U and B. */
+int moons_mhd_create(moons_mesh_t m, const moons_mhd_params *params, moons_field_t U[3], moons_field_t p,
+                     moons_field_t B[3], moons_field_t phi, moons_mhd_t *out);
+int moons_mhd_destroy(moons_mhd_t s);
+/* Advances n_steps time steps; info (may be NULL) gets the last step's solver
+ * iterations and the device time of the call. */
+int moons_mhd_step(moons_mhd_t s, int n_steps, moons_mhd_info *info);
+/* Handles of internal fields (MOONS_MHD_*), 3 or 6 entries, for inspection. */
+int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
 
 #ifdef __cplusplus
 }

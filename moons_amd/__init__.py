@@ -9,6 +9,7 @@ from .core import (  # noqa: F401
     DL,
     PCG_solver_SF,
     PCG_solver_VF,
+    MHD_step,
     SF,
     Lap_uniform_SF,
     apply_BCs,

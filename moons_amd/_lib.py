@@ -26,6 +26,17 @@ class PcgInfo(C.Structure):
 
 
 # name -> (restype, argtypes); every symbol declared in include/moons_amd.h
+class MhdParams(C.Structure):
+    _fields_ = [("Re", C.c_double), ("Ha", C.c_double), ("Rem", C.c_double), ("dt", C.c_double),
+                ("alpha_U", C.c_double), ("induction", C.c_int), ("prec_p", C.c_int), ("B0", C.c_double * 3),
+                ("iter_max", C.c_int * 4), ("tol_rel", C.c_double * 4), ("tol_abs", C.c_double * 4),
+                ("n_skip", C.c_int * 4)]
+
+
+class MhdInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int * 4), ("ms", C.c_double)]
+
+
 SIGNATURES = {
     "moons_init": (C.c_int, [C.c_int]),
     "moons_finalize": (C.c_int, []),
@@ -73,6 +84,11 @@ SIGNATURES = {
     "moons_pcg_vf_set_isp": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
     "moons_pcg_vf_update_mfp": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "moons_pcg_vf_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(PcgInfo)]),
+    "moons_mhd_create": (C.c_int, [C.c_void_p, C.POINTER(MhdParams), C.POINTER(C.c_void_p), C.c_void_p,
+                                   C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_mhd_destroy": (C.c_int, [C.c_void_p]),
+    "moons_mhd_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MhdInfo)]),
+    "moons_mhd_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
 }
 
 _lib = None

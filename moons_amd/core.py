@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import PcgInfo, check, load
+from ._lib import MhdInfo, MhdParams, PcgInfo, check, load
 
 DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
 BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
@@ -129,9 +129,22 @@ class SF:
         check(load().moons_field_copy(self.handle, other.handle))
         return self
 
+    @classmethod
+    def borrow(cls, m: mesh, handle) -> "SF":
+        """View of a field owned by the library (not destroyed by this object)."""
+        f = cls.__new__(cls)
+        f.m = m
+        f.handle = C.c_void_p(handle)
+        f._borrowed = True
+        s = (C.c_int * 3)()
+        check(load().moons_field_shape(f.handle, s))
+        f.shape = (s[0], s[1], s[2])
+        f.dl = -1
+        return f
+
     def __del__(self):
         try:
-            if getattr(self, "handle", None):
+            if getattr(self, "handle", None) and not getattr(self, "_borrowed", False):
                 load().moons_field_destroy(self.handle)
                 self.handle = None
         except Exception:
@@ -315,6 +328,55 @@ class PCG_solver_VF:
         try:
             if getattr(self, "handle", None):
                 load().moons_pcg_vf_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class MHD_step:
+    """One MOONS time step on the device (solve_MOONS,
+    code/handwritten/MOONS/MOONS_solver.f90:47-144): momentum sources, AB2 time
+    march (alpha_U = 0: method 4, explicit diffusion; 1: method 6, backward
+    Euler), pressure projection; with B, low-Rem induction with the CT source,
+    implicit diffusion, phi cleaning, J and J x B.  U, p, B, phi are the
+    caller's fields (with their BCs) and are advanced in place.
+    ISPs are (iter_max, tol_rel, tol_abs, n_skip_check_res)."""
+
+    FIELDS = {"F": 1, "Fnm1": 2, "Ustar": 3, "U_E": 4, "J": 5, "jCrossB": 6, "FB": 7}
+
+    def __init__(self, m: mesh, U, p: SF, B=None, phi: SF | None = None, Re=400.0, Ha=0.0, Rem=1.0, dt=1e-3,
+                 alpha_U=1.0, B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
+                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), prec_p="Lap_SF"):
+        P = MhdParams()
+        P.Re, P.Ha, P.Rem, P.dt, P.alpha_U = Re, Ha, Rem, dt, alpha_U
+        P.induction = 1 if B is not None else 0
+        P.prec_p = PCG_solver_SF.PRECS[prec_p]
+        for d in range(3):
+            P.B0[d] = B0[d]
+        for q, isp in enumerate((isp_U, isp_p, isp_B, isp_phi)):
+            P.iter_max[q], P.tol_rel[q], P.tol_abs[q], P.n_skip[q] = isp
+        h = C.c_void_p()
+        check(load().moons_mhd_create(m.handle, C.byref(P), _arr(U), p.handle, _arr(B) if B is not None else None,
+                                      phi.handle if phi is not None else None, C.byref(h)))
+        self.handle = h
+        self.m = m
+        self._keep = (U, p, B, phi)
+
+    def step(self, n: int = 1):
+        info = MhdInfo()
+        check(load().moons_mhd_step(self.handle, n, C.byref(info)))
+        return {"iterations": dict(zip(("U", "p", "B", "phi"), list(info.iterations))), "ms": info.ms}
+
+    def field(self, which: str):
+        out = (C.c_void_p * 6)()
+        check(load().moons_mhd_field(self.handle, self.FIELDS[which], out))
+        n = 6 if which == "U_E" else 3
+        return [SF.borrow(self.m, out[q]) for q in range(n)]
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_mhd_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

@@ -1,0 +1,417 @@
+// step.hip -- one MOONS MHD time step on the device: momentum (sources, AB2
+// time march with backward-Euler or explicit diffusion, pressure projection)
+// and low magnetic Reynolds number induction (CT source curl(U_E x Btot),
+// implicit curl-curl diffusion, phi cleaning, J and J x B).
+//
+// Sequence of code/handwritten/MOONS/MOONS_solver.f90:47-144 for one RK stage:
+//   add_all_momentum_sources   solvers/momentum/add_all_momentum_sources.f90:22-86
+//   solve_momentum             governing_equations/momentum_extend.f90:326-374
+//     method 4 Euler_time_no_diff_AB2_sources  solvers/time_marching_methods.f90:266-296
+//     method 6 Euler_time_AB2_sources          solvers/time_marching_methods.f90:117-150
+//   add_all_induction_sources  solvers/induction/add_all_induction_sources.f90:24-60
+//   solve_induction (method 6) governing_equations/induction_extend.f90:475-525
+//   compute_J, compute_Btot (finite_Rem = F: Btot = B0), compute_JCrossB
+//                              solvers/induction/induction_aux.f90:80-88,
+//                              solvers/induction/induction_sources.f90:70-84
+// Every operator is a device kernel of this library; element-wise chains are
+// fused into one kernel each, keeping the reference's operation order (AB2 =
+// 1.5 F, then + (-0.5) Fnm1, + L, * dt; Dirichlet wall planes zeroed; + X), so
+// each value is rounded exactly as in the sequential passes.  The fluid and
+// induction meshes are the same single block, so embedding and extraction
+// are identities.
+#include <cstring>
+
+#include "common.hpp"
+
+using namespace moons;
+
+namespace moons {
+
+constexpr int kST = 256;
+
+// y = y + x*s
+__global__ void __launch_bounds__(kST) k_axpy(double *y, const double *x, double s, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t < n) y[t] = y[t] + x[t] * s;
+}
+
+// y = a*b
+__global__ void __launch_bounds__(kST) k_mul3(double *y, const double *a, const double *b, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t < n) y[t] = a[t] * b[t];
+}
+
+// y = y + a*b (the product rounded first, as multiply then add)
+__global__ void __launch_bounds__(kST) k_addmul(double *y, const double *a, const double *b, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t < n) {
+    const double p = a[t] * b[t];
+    y[t] = y[t] + p;
+  }
+}
+
+// out = ((((F*1.5) + Fnm1*(-0.5)) + L) * dt), zero on the Dirichlet wall
+// planes of X (normal direction d, planes 1 and s_d-2 by mask), then + X.
+// has_L = 0 skips the + L (method 4).
+__global__ void __launch_bounds__(kST) k_ab2(double *out, const double *F, const double *Fnm1, const double *L,
+                                            const double *X, double dt, int s0, int s1, int s2, int d, int wall_mask,
+                                            int has_L) {
+  const size_t n = (size_t)s0 * s1 * s2;
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t >= n) return;
+  double v = F[t] * 1.5;
+  v = v + Fnm1[t] * (-0.5);
+  if (has_L) v = v + L[t];
+  v = v * dt;
+  const int i = (int)(t % s0), j = (int)((t / s0) % s1), k = (int)(t / ((size_t)s0 * s1));
+  const int ii = d == 0 ? i : (d == 1 ? j : k);
+  const int sd = d == 0 ? s0 : (d == 1 ? s1 : s2);
+  if (((wall_mask & 1) && ii == 1) || ((wall_mask & 2) && ii == sd - 2)) v = 0.0;
+  out[t] = v + X[t];
+}
+
+static inline unsigned sblk(size_t n) { return (unsigned)((n + kST - 1) / kST); }
+
+} // namespace moons
+
+struct moons_mhd_s {
+  moons_mesh_s *m = nullptr;
+  moons_mhd_params P{};
+  moons_field_s *U[3] = {}, *p = nullptr, *B[3] = {}, *phi = nullptr; // caller's
+  moons_field_s *Ustar[3] = {}, *F[3] = {}, *Fnm1[3] = {}, *L[3] = {}, *T1[3] = {}, *T2[3] = {}, *T3[3] = {};
+  moons_field_s *UE[6] = {}, *cc = nullptr, *tcc = nullptr;
+  moons_field_s *Bstar[3] = {}, *FB[3] = {}, *FBnm1[3] = {}, *LB[3] = {}, *B0[3] = {}, *J[3] = {}, *JxB[3] = {};
+  moons_field_s *sig[3] = {}, *E[3] = {}, *ETF[6] = {}, *A6[6] = {}, *B6[6] = {};
+  moons_pcg_t pcg_p = nullptr, pcg_phi = nullptr;
+  moons_pcgvf_t pcg_U = nullptr, pcg_B = nullptr;
+  int iters[4] = {0, 0, 0, 0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+const int kEdgeDL[3] = {MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z};
+// (component, location) of the six off-diagonal tensor entries: U_y@x, U_z@x,
+// U_x@y, U_z@y, U_x@z, U_y@z (edges for U_E, faces for the J x B operands)
+const int kNoDiag[6][2] = {{1, 0}, {2, 0}, {0, 1}, {2, 1}, {0, 2}, {1, 2}};
+
+int fcreate(moons_mesh_s *m, int dl, moons_field_s **f) { return moons_field_create(m, dl, f); }
+
+void copy_bcs_of(moons_field_s *dst, const moons_field_s *src) {
+  dst->has_bcs = src->has_bcs;
+  dst->all_neumann = src->all_neumann;
+  std::memcpy(dst->bct, src->bct, sizeof dst->bct);
+  std::memcpy(dst->bval, src->bval, sizeof dst->bval);
+}
+
+void mhd_free(moons_mhd_s *S) {
+  moons_field_s **groups3[] = {S->Ustar, S->F, S->Fnm1, S->L, S->T1, S->T2, S->T3, S->Bstar, S->FB,
+                               S->FBnm1, S->LB, S->B0, S->J, S->JxB, S->sig, S->E};
+  for (auto g : groups3)
+    for (int d = 0; d < 3; ++d)
+      if (g[d]) moons_field_destroy(g[d]);
+  moons_field_s **groups6[] = {S->UE, S->ETF, S->A6, S->B6};
+  for (auto g : groups6)
+    for (int q = 0; q < 6; ++q)
+      if (g[q]) moons_field_destroy(g[q]);
+  if (S->cc) moons_field_destroy(S->cc);
+  if (S->tcc) moons_field_destroy(S->tcc);
+  if (S->pcg_p) moons_pcg_destroy(S->pcg_p);
+  if (S->pcg_phi) moons_pcg_destroy(S->pcg_phi);
+  if (S->pcg_U) moons_pcg_vf_destroy(S->pcg_U);
+  if (S->pcg_B) moons_pcg_vf_destroy(S->pcg_B);
+  if (S->ev0) (void)hipEventDestroy(S->ev0);
+  if (S->ev1) (void)hipEventDestroy(S->ev1);
+  delete S;
+}
+
+int axpy(moons_field_s *y, const moons_field_s *x, double s) {
+  hipLaunchKernelGGL(k_axpy, dim3(sblk(y->n)), dim3(kST), 0, stream(), y->d, x->d, s, y->n);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int mul3(moons_field_s *y, const moons_field_s *a, const moons_field_s *b) {
+  hipLaunchKernelGGL(k_mul3, dim3(sblk(y->n)), dim3(kST), 0, stream(), y->d, a->d, b->d, y->n);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+int addmul(moons_field_s *y, const moons_field_s *a, const moons_field_s *b) {
+  hipLaunchKernelGGL(k_addmul, dim3(sblk(y->n)), dim3(kST), 0, stream(), y->d, a->d, b->d, y->n);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+// face i -> CC (along i) -> face j (along j)
+int face2face(moons_mhd_s *S, moons_field_s *out, const moons_field_s *u, int i, int j) {
+  int st;
+  if ((st = launch_interp(S->cc, u, i))) return st;
+  return launch_interp(out, S->cc, j);
+}
+
+// U_E = face2Edge_no_diag(U) (ops/ops_interp.f90:531-544)
+int face2edge_no_diag(moons_mhd_s *S) {
+  int st;
+  for (int q = 0; q < 6; ++q) {
+    const int i = kNoDiag[q][0], j = kNoDiag[q][1];
+    if ((st = launch_interp(S->UE[q], S->U[i], 3 - i - j))) return st;
+  }
+  return MOONS_OK;
+}
+
+// T1 = u_j d/dx_j u_i (advect_U_convection1, ops/ops_advect.f90:71-127)
+int convection(moons_mhd_s *S, moons_field_s *div[3]) {
+  int st;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = launch_interp(S->cc, S->U[d], d))) return st;
+    if ((st = launch_del(S->T2[d], S->cc, 1, d, 0, 1))) return st;
+    if ((st = mul3(div[d], S->T2[d], S->U[d]))) return st;
+  }
+  // (U_E entry, derivative direction, velocity component), reference order
+  const int terms[3][2][3] = {{{4, 1, 1}, {2, 2, 2}}, {{5, 0, 0}, {0, 2, 2}}, {{3, 0, 0}, {1, 1, 1}}};
+  for (int d = 0; d < 3; ++d)
+    for (int t = 0; t < 2; ++t) {
+      const int q = terms[d][t][0], ddir = terms[d][t][1], vel = terms[d][t][2];
+      if ((st = launch_del(S->T2[d], S->UE[q], 1, ddir, 0, 1))) return st;
+      if ((st = face2face(S, S->T3[d], S->U[vel], vel, d))) return st;
+      if ((st = addmul(div[d], S->T2[d], S->T3[d]))) return st;
+    }
+  return MOONS_OK;
+}
+
+int wall_mask(const moons_field_s *X, int d) {
+  if (!X->has_bcs) return 0;
+  return (X->bct[2 * d] == MOONS_BC_DIRICHLET ? 1 : 0) | (X->bct[2 * d + 1] == MOONS_BC_DIRICHLET ? 2 : 0);
+}
+
+int ab2_rhs(moons_field_s *out[3], moons_field_s *F[3], moons_field_s *Fnm1[3], moons_field_s *L[3],
+            moons_field_s *X[3], double dt, bool has_L) {
+  for (int d = 0; d < 3; ++d) {
+    const moons_field_s *x = X[d];
+    hipLaunchKernelGGL(k_ab2, dim3(sblk(x->n)), dim3(kST), 0, stream(), out[d]->d, F[d]->d, Fnm1[d]->d,
+                       has_L ? L[d]->d : F[d]->d, x->d, dt, x->s[0], x->s[1], x->s[2], d, wall_mask(x, d),
+                       has_L ? 1 : 0);
+    MOONS_LAUNCH_CHECK();
+  }
+  return MOONS_OK;
+}
+
+// J = curl(B)/Rem; Btot = B0; JxB = N edge2Face(J) x face2Face(Btot)
+int post_induction(moons_mhd_s *S) {
+  int st;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = moons_curl(S->J[d], S->B[0], S->B[1], S->B[2], d))) return st;
+    if ((st = launch_scale(S->J[d], 1.0 / S->P.Rem))) return st;
+  }
+  for (int q = 0; q < 6; ++q) {
+    const int i = kNoDiag[q][0], j = kNoDiag[q][1];
+    if ((st = launch_interp(S->A6[q], S->J[i], 3 - i - j))) return st;  // edge2Face
+    if ((st = face2face(S, S->B6[q], S->B0[i], i, j))) return st;       // face2Face(Btot)
+  }
+  // (J x B)_x = J_y Bz - J_z By on x-faces; y negated form; z = J_x By - J_y Bx
+  if ((st = launch_cross(S->JxB[0], S->A6[0], S->B6[1], S->A6[1], S->B6[0], 1))) return st;
+  if ((st = launch_cross(S->JxB[1], S->A6[2], S->B6[3], S->A6[3], S->B6[2], -1))) return st;
+  if ((st = launch_cross(S->JxB[2], S->A6[4], S->B6[5], S->A6[5], S->B6[4], 1))) return st;
+  const double N = S->P.Ha * S->P.Ha / S->P.Re;
+  for (int d = 0; d < 3; ++d)
+    if ((st = launch_scale(S->JxB[d], N))) return st;
+  return MOONS_OK;
+}
+
+int momentum(moons_mhd_s *S) {
+  int st;
+  moons_pcg_info info;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = moons_field_copy(S->Fnm1[d], S->F[d]))) return st;
+    if ((st = moons_field_fill(S->L[d], 0.0))) return st;
+    if ((st = moons_field_fill(S->F[d], 0.0))) return st;
+  }
+  if ((st = convection(S, S->T1))) return st;
+  for (int d = 0; d < 3; ++d)
+    if ((st = axpy(S->F[d], S->T1[d], -1.0))) return st;
+  const double ce = (1.0 / S->P.Re) * (1.0 - S->P.alpha_U);
+  for (int d = 0; d < 3; ++d) {
+    if ((st = launch_del(S->T1[d], S->U[d], 2, 0, 0, 1))) return st;
+    if ((st = launch_del(S->T1[d], S->U[d], 2, 1, 0, 2))) return st;
+    if ((st = launch_del(S->T1[d], S->U[d], 2, 2, 0, 2))) return st;
+    if ((st = axpy(S->F[d], S->T1[d], ce))) return st;
+  }
+  if (S->P.induction)
+    for (int d = 0; d < 3; ++d) {
+      if ((st = moons_field_copy(S->T1[d], S->JxB[d]))) return st;
+      if ((st = launch_assign_ghost_xperiodic(S->T1[d], 0.0))) return st;
+      if ((st = axpy(S->F[d], S->T1[d], 1.0))) return st;
+    }
+  double scale;
+  if (S->P.alpha_U == 0.0) {
+    if ((st = ab2_rhs(S->Ustar, S->F, S->Fnm1, S->L, S->U, S->P.dt, false))) return st;
+    scale = 1.0;
+  } else {
+    if ((st = ab2_rhs(S->T1, S->F, S->Fnm1, S->L, S->U, S->P.dt, true))) return st;
+    for (int d = 0; d < 3; ++d)
+      if ((st = moons_field_copy(S->Ustar[d], S->U[d]))) return st;
+    if ((st = moons_pcg_vf_update_mfp(S->pcg_U, S->P.dt * 1.0 * (-(1.0 / S->P.Re) * S->P.alpha_U), 1))) return st;
+    if ((st = moons_pcg_vf_solve(S->pcg_U, S->Ustar, S->T1, &info))) return st;
+    S->iters[0] = info.iterations;
+    scale = 1.0 / S->P.dt;
+  }
+  if ((st = moons_clean_div(S->pcg_p, S->U, S->Ustar, S->p, scale, S->T2, S->tcc, &info))) return st;
+  S->iters[1] = info.iterations;
+  return face2edge_no_diag(S);
+}
+
+int induction(moons_mhd_s *S) {
+  int st;
+  moons_pcg_info info;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = moons_field_copy(S->FBnm1[d], S->FB[d]))) return st;
+    if ((st = moons_field_fill(S->LB[d], 0.0))) return st;
+    if ((st = moons_field_fill(S->FB[d], 0.0))) return st;
+  }
+  if ((st = moons_advect_B(S->T1, S->UE, S->B0, S->ETF, S->E))) return st;
+  for (int d = 0; d < 3; ++d)
+    if ((st = axpy(S->FB[d], S->T1[d], 1.0))) return st;
+  for (int d = 0; d < 3; ++d)
+    if ((st = mul3(S->E[d], S->J[d], S->sig[d]))) return st;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = moons_curl(S->T2[d], S->E[0], S->E[1], S->E[2], d))) return st;
+    if ((st = axpy(S->FB[d], S->T2[d], -0.0))) return st; // -beta, beta = 1 - alpha = 0
+  }
+  if ((st = ab2_rhs(S->T1, S->FB, S->FBnm1, S->LB, S->B, S->P.dt, true))) return st;
+  for (int d = 0; d < 3; ++d)
+    if ((st = moons_field_copy(S->Bstar[d], S->B[d]))) return st;
+  if ((st = moons_pcg_vf_update_mfp(S->pcg_B, S->P.dt * 1.0 * (1.0 / S->P.Rem), 1))) return st;
+  if ((st = moons_pcg_vf_solve(S->pcg_B, S->Bstar, S->T1, &info))) return st;
+  S->iters[2] = info.iterations;
+  if ((st = moons_clean_div(S->pcg_phi, S->B, S->Bstar, S->phi, 1.0 / S->P.dt, S->T2, S->tcc, &info))) return st;
+  S->iters[3] = info.iterations;
+  return post_induction(S);
+}
+
+} // namespace
+
+extern "C" {
+
+int moons_mhd_create(moons_mesh_t m, const moons_mhd_params *P, moons_field_t U[3], moons_field_t p,
+                     moons_field_t B[3], moons_field_t phi, moons_mhd_t *out) {
+  if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
+  if (!m || !P || !U || !p || !out) return fail(MOONS_ERR_ARG, "moons_mhd_create: null argument");
+  if (!(P->dt > 0.0) || !(P->Re > 0.0) || (P->alpha_U != 0.0 && P->alpha_U != 1.0))
+    return fail(MOONS_ERR_ARG, "moons_mhd_create: need dt > 0, Re > 0, alpha_U 0 (method 4) or 1 (method 6)");
+  if (P->induction && (!B || !phi || !(P->Rem > 0.0)))
+    return fail(MOONS_ERR_ARG, "moons_mhd_create: induction needs B, phi and Rem > 0");
+  for (int d = 0; d < 3; ++d)
+    if (!U[d] || U[d]->dl != (1 << d) || U[d]->m != m || (P->induction && (!B[d] || B[d]->dl != (1 << d))))
+      return fail(MOONS_ERR_ARG, "moons_mhd_create: U and B must be face components on the mesh");
+  if (p->dl != MOONS_DL_CC || (P->induction && phi->dl != MOONS_DL_CC))
+    return fail(MOONS_ERR_ARG, "moons_mhd_create: p and phi must be cell-centred");
+  moons_mhd_s *S = new moons_mhd_s();
+  S->m = m;
+  S->P = *P;
+  S->p = p;
+  S->phi = phi;
+  int st = MOONS_OK;
+  for (int d = 0; d < 3 && !st; ++d) {
+    S->U[d] = U[d];
+    if (P->induction) S->B[d] = B[d];
+    moons_field_s **vecs[] = {S->Ustar, S->F, S->Fnm1, S->L, S->T1, S->T2, S->T3};
+    for (auto v : vecs)
+      if (!st) st = fcreate(m, 1 << d, &v[d]);
+    if (!st) copy_bcs_of(S->Ustar[d], U[d]);
+  }
+  for (int q = 0; q < 6 && !st; ++q) st = fcreate(m, kEdgeDL[kNoDiag[q][1]], &S->UE[q]);
+  if (!st) st = fcreate(m, MOONS_DL_CC, &S->cc);
+  if (!st) st = fcreate(m, MOONS_DL_CC, &S->tcc);
+  if (!st) {
+    for (int d = 0; d < 3; ++d) st = st ? st : moons_field_fill(S->F[d], 0.0);
+  }
+  if (!st) st = moons_pcg_sf_create(m, MOONS_OP_LAP_UNIFORM_SF, P->prec_p, p, &S->pcg_p);
+  if (!st) st = moons_pcg_set_isp(S->pcg_p, P->iter_max[1], P->tol_rel[1], P->tol_abs[1], P->n_skip[1]);
+  if (!st && P->alpha_U != 0.0) {
+    st = moons_pcg_vf_create(m, MOONS_OP_MOM_DIFFUSION, S->Ustar, nullptr, &S->pcg_U);
+    if (!st) st = moons_pcg_vf_set_isp(S->pcg_U, P->iter_max[0], P->tol_rel[0], P->tol_abs[0], P->n_skip[0]);
+  }
+  if (!st && P->induction) {
+    for (int d = 0; d < 3 && !st; ++d) {
+      moons_field_s **vecs[] = {S->Bstar, S->FB, S->FBnm1, S->LB, S->B0, S->JxB};
+      for (auto v : vecs)
+        if (!st) st = fcreate(m, 1 << d, &v[d]);
+      if (!st) copy_bcs_of(S->Bstar[d], B[d]);
+      if (!st) st = moons_field_fill(S->B0[d], P->B0[d]);
+      if (!st) st = moons_field_fill(S->FB[d], 0.0);
+      if (!st) st = fcreate(m, kEdgeDL[d], &S->J[d]);
+      if (!st) st = fcreate(m, kEdgeDL[d], &S->sig[d]);
+      if (!st) st = moons_field_fill(S->sig[d], 1.0);
+      if (!st) st = fcreate(m, kEdgeDL[d], &S->E[d]);
+    }
+    // temp_E_TF of advect_B: Bx@y, Bx@z, By@x, By@z, Bz@x, Bz@y edges
+    const int etf_edge[6] = {1, 2, 0, 2, 0, 1};
+    for (int q = 0; q < 6 && !st; ++q) st = fcreate(m, kEdgeDL[etf_edge[q]], &S->ETF[q]);
+    for (int q = 0; q < 6 && !st; ++q) {
+      st = fcreate(m, 1 << kNoDiag[q][1], &S->A6[q]);
+      if (!st) st = fcreate(m, 1 << kNoDiag[q][1], &S->B6[q]);
+    }
+    if (!st) st = moons_pcg_vf_create(m, MOONS_OP_IND_DIFFUSION, S->Bstar, S->sig, &S->pcg_B);
+    if (!st) st = moons_pcg_vf_set_isp(S->pcg_B, P->iter_max[2], P->tol_rel[2], P->tol_abs[2], P->n_skip[2]);
+    if (!st) st = moons_pcg_sf_create(m, MOONS_OP_LAP_UNIFORM_SF, MOONS_PREC_LAP_SF, phi, &S->pcg_phi);
+    if (!st) st = moons_pcg_set_isp(S->pcg_phi, P->iter_max[3], P->tol_rel[3], P->tol_abs[3], P->n_skip[3]);
+    if (!st) st = post_induction(S);
+  }
+  if (!st) st = face2edge_no_diag(S);
+  if (!st && (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess))
+    st = fail(MOONS_ERR_HIP, "moons_mhd_create: event creation failed");
+  if (st) {
+    mhd_free(S);
+    return st;
+  }
+  *out = S;
+  return MOONS_OK;
+}
+
+int moons_mhd_destroy(moons_mhd_t S) {
+  if (!S) return MOONS_OK;
+  if (stream()) (void)hipStreamSynchronize(stream());
+  mhd_free(S);
+  return MOONS_OK;
+}
+
+int moons_mhd_step(moons_mhd_t S, int n_steps, moons_mhd_info *info) {
+  if (!S || n_steps < 0) return fail(MOONS_ERR_ARG, "moons_mhd_step: bad argument");
+  MOONS_HIP(hipEventRecord(S->ev0, stream()));
+  int st;
+  for (int n = 0; n < n_steps; ++n) {
+    if ((st = momentum(S))) return st;
+    if (S->P.induction && (st = induction(S))) return st;
+  }
+  MOONS_HIP(hipEventRecord(S->ev1, stream()));
+  MOONS_HIP(hipEventSynchronize(S->ev1));
+  if (info) {
+    for (int q = 0; q < 4; ++q) info->iterations[q] = S->iters[q];
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, S->ev0, S->ev1);
+    info->ms = ms;
+  }
+  return MOONS_OK;
+}
+
+int moons_mhd_field(moons_mhd_t S, int which, moons_field_t out[6]) {
+  if (!S || !out) return fail(MOONS_ERR_ARG, "moons_mhd_field: null argument");
+  moons_field_s **src = nullptr;
+  int n = 3;
+  switch (which) {
+    case MOONS_MHD_F: src = S->F; break;
+    case MOONS_MHD_FNM1: src = S->Fnm1; break;
+    case MOONS_MHD_USTAR: src = S->Ustar; break;
+    case MOONS_MHD_U_E: src = S->UE; n = 6; break;
+    case MOONS_MHD_J: src = S->J; break;
+    case MOONS_MHD_JCROSSB: src = S->JxB; break;
+    case MOONS_MHD_FB: src = S->FB; break;
+    default: return fail(MOONS_ERR_ARG, "moons_mhd_field: unknown field");
+  }
+  if (!src[0]) return fail(MOONS_ERR_STATE, "moons_mhd_field: field not present (no induction)");
+  for (int q = 0; q < n; ++q) out[q] = src[q];
+  return MOONS_OK;
+}
+
+} // extern "C"

@@ -75,6 +75,8 @@ def lib():
         L.orc_pcg_vf.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp * 3, BCs * 3, _dp * 3, _dp * 3, C.c_int,
                                  C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_minv_vf.argtypes = [_dp * 3, C.c_int, C.c_double, _dp * 3, C.c_void_p]
+        L.orc_assign_ghost_xperiodic.argtypes = [_dp, C.c_int, C.c_void_p, C.POINTER(BCs), C.c_double]
+        L.orc_assign_wall_dirichlet.argtypes = [_dp, C.c_int, C.c_void_p, C.POINTER(BCs), C.c_double]
         _lib = L
     return _lib
 

@@ -1,0 +1,245 @@
+"""FP64 restatement of one MOONS MHD time step (momentum + low-Rem induction),
+composed from the C oracle's operators.
+
+TEST INFRASTRUCTURE ONLY: the checker for the device time step.
+
+Sequence (code/handwritten/MOONS/MOONS_solver.f90:47-144, one RK stage):
+  momentum:  add_all_momentum_sources (solvers/momentum/add_all_momentum_sources.f90:22-86)
+               Fnm1 = F; L = 0; F = 0
+               F += -1 * u.grad(u)        advect_U_convection1 (ops/ops_advect.f90:71-127)
+               F += coeff_explicit lap(U) compute_add_diffusion (momentum_sources.f90:87-95),
+                                          lapUniformCoeff_VF (ops/ops_discrete.f90:265-273)
+               F += 1 * jCrossB           extract_add_JCrossB (momentum_sources.f90:111-121)
+             solve_momentum (governing_equations/momentum_extend.f90:326-374)
+               method 4: Euler_time_no_diff_AB2_sources (solvers/time_marching_methods.f90:266-296)
+               method 6: Euler_time_AB2_sources (time_marching_methods.f90:117-150)
+               U_E = face2Edge_no_diag(U)
+  induction: add_all_induction_sources (solvers/induction/add_all_induction_sources.f90:24-60)
+               Fnm1 = F; L = 0; F = 0
+               F += 1 * curl(U_E x Btot)  add_curl_U_cross_B (induction_sources.f90:28-38)
+               F += -beta * curl(sigInv J) add_curl_J (induction_sources.f90:40-49)
+             solve_induction, method 6 (induction_extend.f90:475-525)
+             J = curl(B)/Rem; Btot = B0 (finite_Rem = F); jCrossB = N edge2Face(J) x face2Face(Btot)
+                                          (induction_sources.f90:70-84)
+Embedding/extraction between the fluid and induction meshes (embedVelocity_E,
+extractFace) is the identity here: one mesh, one domain.
+Element-wise arithmetic follows the reference's order (multiply then add,
+AB2 = 1.5 Fn then + (-0.5) Fnm1).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import oracle as O
+
+EDGE_DL = (6, 5, 3)
+# (component, location) of the six off-diagonal entries, in the C ABI's order:
+# U_y@x-edges, U_z@x, U_x@y, U_z@y, U_x@z, U_y@z
+NO_DIAG = ((1, 0), (2, 0), (0, 1), (2, 1), (0, 2), (1, 2))
+
+
+def face2edge_no_diag(g, U):
+    """face2Edge_no_diag_TF (ops/ops_interp.f90:531-544): U_i onto j-edges, one
+    interpolation along the third direction."""
+    return {(i, j): O.interp(g, U[i], 1 << i, EDGE_DL[j], 3 - i - j) for (i, j) in NO_DIAG}
+
+
+def face2face(g, u, i, j):
+    """face2Face_SF (ops_interp.f90:287-295): face i -> CC -> face j."""
+    cc = O.interp(g, u, 1 << i, 0, i)
+    return O.interp(g, cc, 0, 1 << j, j)
+
+
+def edge2face(g, e, i, j):
+    """edge2Face_SF (ops_interp.f90:400-413): edge i -> face j along the third direction."""
+    return O.interp(g, e, EDGE_DL[i], 1 << j, 3 - i - j)
+
+
+def advect_U_convection(g, U, U_E):
+    """div_i = u_j d/dx_j u_i (ops/ops_advect.f90:71-127)."""
+    div = []
+    for d in range(3):
+        cc = O.interp(g, U[d], 1 << d, 0, d)
+        t = g.zeros(1 << d)
+        O.del_op(g, t, 1 << d, cc, 0, 1, d, 0, 1)
+        div.append(t * U[d])
+    # (edge entry, derivative direction, velocity component) per component, reference order
+    terms = {0: (((0, 2), 1, 1), ((0, 1), 2, 2)),
+             1: (((1, 2), 0, 0), ((1, 0), 2, 2)),
+             2: (((2, 1), 0, 0), ((2, 0), 1, 1))}
+    for d in range(3):
+        for key, ddir, vel in terms[d]:
+            t1 = g.zeros(1 << d)
+            O.del_op(g, t1, 1 << d, U_E[key], EDGE_DL[key[1]], 1, ddir, 0, 1)
+            t2 = face2face(g, U[vel], vel, d)
+            t1 = t1 * t2
+            div[d] = div[d] + t1
+    return div
+
+
+def lap_collocated(g, U):
+    """lapUniformCoeff_VF: second derivatives along x, y, z (assign, add, add)."""
+    out = []
+    for d in range(3):
+        t = g.zeros(1 << d)
+        O.del_op(g, t, 1 << d, U[d], 1 << d, 2, 0, 0, 1)
+        O.del_op(g, t, 1 << d, U[d], 1 << d, 2, 1, 0, 2)
+        O.del_op(g, t, 1 << d, U[d], 1 << d, 2, 2, 0, 2)
+        out.append(t)
+    return out
+
+
+def zero_ghosts(g, u, dl):
+    """assign_ghost_XPeriodic(u, 0) for a field with no BCs: all six ghost planes."""
+    u = u.copy(order="F")
+    lib = O.lib()
+    lib.orc_assign_ghost_xperiodic(O._F(u), dl, g.h, None, 0.0)
+    return u
+
+
+def assign_wall_dirichlet(g, u, dl, bcs):
+    u = u.copy(order="F")
+    O.lib().orc_assign_wall_dirichlet(O._F(u), dl, g.h, O.C.byref(bcs), 0.0)
+    return u
+
+
+def clean_div(g, X, Xstar, xbcs, phi, phibcs, all_neumann, scale, isp):
+    """clean_div_PCG (solvers/clean_divergence.f90:37-59)."""
+    t = O.div(g, Xstar[0], Xstar[1], Xstar[2], 0)
+    t = np.asfortranarray(t * scale)
+    it, res0, res, _ = O.pcg_lap(g, 0, phibcs, all_neumann, phi, t, *isp)
+    gr = O.grad(g, phi, 0)
+    inv = 1.0 / scale
+    for d in range(3):
+        X[d] = np.asfortranarray(Xstar[d] - gr[d] * inv)
+        O.apply_bcs(g, X[d], 1 << d, xbcs[d])
+    return it
+
+
+class MHDState:
+    """State and parameters of the coupled step; arrays are F-ordered FP64."""
+
+    def __init__(self, g, U, Ubcs, pbcs, Re, dt, alpha_U=1.0, B=None, Bbcs=None, phibcs=None, Rem=1.0, Ha=0.0,
+                 B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
+                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1)):
+        self.g = g
+        self.Re, self.dt, self.alpha_U = Re, dt, alpha_U
+        self.coeff_explicit_U = (1.0 / Re) * (1.0 - alpha_U)
+        self.coeff_implicit_U = -(1.0 / Re) * alpha_U
+        self.U = [np.asfortranarray(u) for u in U]
+        self.Ubcs, self.pbcs = Ubcs, pbcs
+        self.p = g.zeros(0)
+        self.F = [g.zeros(1 << d) for d in range(3)]
+        self.Fnm1 = [g.zeros(1 << d) for d in range(3)]
+        self.L = [g.zeros(1 << d) for d in range(3)]
+        self.isp_U, self.isp_p, self.isp_B, self.isp_phi = isp_U, isp_p, isp_B, isp_phi
+        self.induction = B is not None
+        self.U_E = face2edge_no_diag(g, self.U)
+        self.jCrossB = [g.zeros(1 << d) for d in range(3)]
+        self.iters = {}
+        if self.induction:
+            self.Rem, self.N = Rem, Ha * Ha / Re
+            self.coeff_implicit_B = 1.0 / Rem  # -coeff_natural * alpha, coeff_natural = -1/Rem, alpha = 1
+            self.B = [np.asfortranarray(b) for b in B]
+            self.Bbcs, self.phibcs = Bbcs, phibcs
+            self.phi = g.zeros(0)
+            self.B0 = [np.asfortranarray(np.full(g.shape(1 << d), B0[d])) for d in range(3)]
+            self.FB = [g.zeros(1 << d) for d in range(3)]
+            self.FBnm1 = [g.zeros(1 << d) for d in range(3)]
+            self.LB = [g.zeros(1 << d) for d in range(3)]
+            self.sigInv = [np.asfortranarray(np.ones(g.shape(dl))) for dl in EDGE_DL]
+            self.post_induction()
+
+    # ---------------------------------------------------------------- momentum
+    def momentum_sources(self):
+        g = self.g
+        self.Fnm1 = [f.copy(order="F") for f in self.F]
+        self.L = [g.zeros(1 << d) for d in range(3)]
+        F = [g.zeros(1 << d) for d in range(3)]
+        adv = advect_U_convection(g, self.U, self.U_E)
+        F = [F[d] + adv[d] * (-1.0) for d in range(3)]
+        lap = lap_collocated(g, self.U)
+        F = [F[d] + lap[d] * self.coeff_explicit_U for d in range(3)]
+        if self.induction:
+            jcb = [zero_ghosts(g, self.jCrossB[d], 1 << d) for d in range(3)]
+            F = [F[d] + jcb[d] * 1.0 for d in range(3)]
+        self.F = [np.asfortranarray(f) for f in F]
+
+    def _ab2_rhs(self, X, Xbcs, F, Fnm1, L):
+        out = []
+        for d in range(3):
+            t = F[d] * 1.5
+            t = t + Fnm1[d] * (-0.5)
+            t = t + L[d]
+            t = t * self.dt
+            t = assign_wall_dirichlet(self.g, np.asfortranarray(t), 1 << d, Xbcs[d])
+            t = t + X[d]
+            out.append(np.asfortranarray(t))
+        return out
+
+    def solve_momentum(self):
+        g = self.g
+        if self.alpha_U == 0.0:  # method 4: Euler_time_no_diff_AB2_sources (no L)
+            Ustar = []
+            for d in range(3):
+                t = self.F[d] * 1.5
+                t = t + self.Fnm1[d] * (-0.5)
+                t = t * self.dt
+                t = assign_wall_dirichlet(g, np.asfortranarray(t), 1 << d, self.Ubcs[d])
+                Ustar.append(np.asfortranarray(t + self.U[d]))
+            scale = 1.0
+        else:  # method 6: Euler_time_AB2_sources
+            rhs = self._ab2_rhs(self.U, self.Ubcs, self.F, self.Fnm1, self.L)
+            Ustar = [u.copy(order="F") for u in self.U]
+            c = self.dt * 1.0 * self.coeff_implicit_U
+            it, _, _, _ = O.pcg_vf(g, O.MOM_DIFFUSION, c, self.Ubcs, Ustar, rhs, *self.isp_U)
+            self.iters["U"] = it
+            scale = 1.0 / self.dt
+        self.iters["p"] = clean_div(g, self.U, Ustar, self.Ubcs, self.p, self.pbcs, True, scale, self.isp_p)
+        self.Ustar = Ustar
+        self.U_E = face2edge_no_diag(g, self.U)
+
+    # --------------------------------------------------------------- induction
+    def induction_sources(self):
+        g = self.g
+        self.FBnm1 = [f.copy(order="F") for f in self.FB]
+        self.LB = [g.zeros(1 << d) for d in range(3)]
+        F = [g.zeros(1 << d) for d in range(3)]
+        U_E = [self.U_E[k] for k in NO_DIAG]
+        adv = O.advect_B(g, U_E, self.Btot)
+        F = [F[d] + adv[d] * 1.0 for d in range(3)]
+        E = [np.asfortranarray(self.J[d] * self.sigInv[d]) for d in range(3)]
+        cj = [O.curl(g, E[0], E[1], E[2], 1 << d, d) for d in range(3)]
+        beta_B = 0.0
+        F = [F[d] + cj[d] * (-beta_B) for d in range(3)]
+        self.FB = [np.asfortranarray(f) for f in F]
+
+    def solve_induction(self):
+        g = self.g
+        rhs = self._ab2_rhs(self.B, self.Bbcs, self.FB, self.FBnm1, self.LB)
+        Bstar = [b.copy(order="F") for b in self.B]
+        c = self.dt * 1.0 * self.coeff_implicit_B
+        it, _, _, _ = O.pcg_vf(g, O.IND_DIFFUSION, c, self.Bbcs, Bstar, rhs, *self.isp_B, k=self.sigInv)
+        self.iters["B"] = it
+        self.iters["phi"] = clean_div(g, self.B, Bstar, self.Bbcs, self.phi, self.phibcs, False, 1.0 / self.dt,
+                                      self.isp_phi)
+        self.post_induction()
+
+    def post_induction(self):
+        g = self.g
+        self.J = [np.asfortranarray(O.curl(g, self.B[0], self.B[1], self.B[2], EDGE_DL[d], d) * (1.0 / self.Rem))
+                  for d in range(3)]
+        self.Btot = [b.copy(order="F") for b in self.B0]
+        A = {(i, j): edge2face(g, self.J[i], i, j) for (i, j) in NO_DIAG}
+        Bf = {(i, j): face2face(g, self.Btot[i], i, j) for (i, j) in NO_DIAG}
+        jx = A[(1, 0)] * Bf[(2, 0)] - A[(2, 0)] * Bf[(1, 0)]
+        jy = -(A[(0, 1)] * Bf[(2, 1)] - A[(2, 1)] * Bf[(0, 1)])
+        jz = A[(0, 2)] * Bf[(1, 2)] - A[(1, 2)] * Bf[(0, 2)]
+        self.jCrossB = [np.asfortranarray(j * self.N) for j in (jx, jy, jz)]
+
+    def step(self):
+        self.momentum_sources()
+        self.solve_momentum()
+        if self.induction:
+            self.induction_sources()
+            self.solve_induction()
