@@ -66,6 +66,10 @@ extern "C" {
  * counterpart in the reference (its V-cycle is dead code,
  * code_not_compiled/9.6.2017/MG/MG.f90:241-396); same solve/exit interface. */
 #define MOONS_PREC_MG_FP32 2
+/* Learned preconditioner: z = s N(r/vol * vol^(2/3) / s) with N the BF16
+ * U-net bound by moons_pcg_set_unet, inside the FP64 flexible PCG.  No
+ * reference counterpart. */
+#define MOONS_PREC_UNET 3
 
 /* Face-centred vector operators of the implicit diffusion solves:
  *   mom_diffusion: Ax = x + c lap(x)             (matrix_free_operators.f90:176-202)
@@ -80,12 +84,15 @@ typedef struct moons_field_s *moons_field_t;
 typedef struct moons_pcg_s *moons_pcg_t;
 typedef struct moons_pcgvf_s *moons_pcgvf_t;
 typedef struct moons_mhd_s *moons_mhd_t;
+typedef struct moons_unet_s *moons_unet_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
  * iterations = ISP%iter_per_call, res_norm0 = res_norm0%L2 (sqrt of the sum of
  * r0^2 over all cells), res_norm = last checked res_norm%L2, exit_loop =
- * ISP%exit_loop(1:3) as bits 0 (tol_rel), 1 (tol_abs), 2 (iter_max). */
+ * ISP%exit_loop(1:3) as bits 0 (tol_rel), 1 (tol_abs), 2 (iter_max); bit 3
+ * marks a MOONS_PREC_UNET solve that broke down or reached iter_max and was
+ * redone with the Jacobi preconditioner (the figures are then the redo's). */
 typedef struct {
   int iterations;
   int exit_loop;
@@ -280,6 +287,7 @@ typedef struct {
 #define MOONS_MHD_J 5       /* current density on edges */
 #define MOONS_MHD_JCROSSB 6 /* N J x B on faces */
 #define MOONS_MHD_FB 7      /* induction sources */
+#define MOONS_MHD_PPE_RHS 8 /* right-hand side of the last pressure solve (1 CC entry) */
 
 /* Creates the step state over the caller's U (faces, with BCs), p (CC, with
  * BCs), and for induction B (faces, with BCs) and phi (CC, with BCs); these
@@ -293,6 +301,37 @@ int moons_mhd_destroy(moons_mhd_t s);
 int moons_mhd_step(moons_mhd_t s, int n_steps, moons_mhd_info *info);
 /* Handles of internal fields (MOONS_MHD_*), 3 or 6 entries, for inspection. */
 int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
+
+/* ------------------------------------------- learned preconditioner */
+/* A 3-level 3D U-net (channels 16/32/64, 3x3x3 convolutions, ReLU, average
+ * pooling, nearest upsampling with skip concatenation) with BF16 weights and
+ * activations and FP32 accumulation on the MFMA units; FP32 master weights
+ * trained with Adam by the library's own forward and backward kernels.
+ * Input: 4 channels per cell (f vol^(2/3) / s and log(dh_d / vol^(1/3)));
+ * output: 1 channel.  None of these has a reference counterpart. */
+int moons_unet_create(unsigned long long seed, moons_unet_t *out);
+int moons_unet_destroy(moons_unet_t net);
+size_t moons_unet_num_params(moons_unet_t net);
+/* FP32 master weights, layer by layer: W[tap][c_in][c_out] then bias[c_out]. */
+int moons_unet_get_weights(moons_unet_t net, float *host, size_t n);
+int moons_unet_set_weights(moons_unet_t net, const float *host, size_t n);
+/* in4: 4 channel arrays of (n0+2)(n1+2)(n2+2) FP32 (column-major, halo
+ * ignored); out: one such array (interior written).  n0, n1, n2 multiples of 4. */
+int moons_unet_forward_host(moons_unet_t net, int n0, int n1, int n2, const float *in4, float *out);
+/* loss = mean over the interior of (N(in4) - target)^2 and d loss / d param
+ * (same order as the weights). */
+int moons_unet_grad_host(moons_unet_t net, int n0, int n1, int n2, const float *in4, const float *target,
+                         double *loss, float *grad);
+/* Adam training on npair samples (in4: npair x 4 arrays, target: npair
+ * arrays), batch samples per step, epochs passes in a seeded order;
+ * loss_hist[epoch] = mean loss.  The data set is copied to the device once. */
+int moons_unet_train(moons_unet_t net, int n0, int n1, int n2, int npair, const float *in4, const float *target,
+                     int epochs, int batch, double lr, unsigned long long seed, double *loss_hist);
+/* D = A B for 16x16 A and B rounded to BF16, through one MFMA with the lane
+ * layout the U-net kernels use (a self-check of that layout). */
+int moons_mfma_check(const float *A, const float *B, float *D);
+/* Binds the network to a MOONS_PREC_UNET solver (required before its first solve). */
+int moons_pcg_set_unet(moons_pcg_t pcg, moons_unet_t net);
 
 #ifdef __cplusplus
 }

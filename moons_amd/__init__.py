@@ -10,6 +10,8 @@ from .core import (  # noqa: F401
     PCG_solver_SF,
     PCG_solver_VF,
     MHD_step,
+    UNet,
+    mfma_check,
     SF,
     Lap_uniform_SF,
     apply_BCs,

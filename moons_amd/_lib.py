@@ -11,6 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmoons_amd.so")
 
+c_float_p = C.POINTER(C.c_float)
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
 
@@ -89,6 +90,18 @@ SIGNATURES = {
     "moons_mhd_destroy": (C.c_int, [C.c_void_p]),
     "moons_mhd_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MhdInfo)]),
     "moons_mhd_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_unet_create": (C.c_int, [C.c_ulonglong, C.POINTER(C.c_void_p)]),
+    "moons_unet_destroy": (C.c_int, [C.c_void_p]),
+    "moons_unet_num_params": (C.c_size_t, [C.c_void_p]),
+    "moons_unet_get_weights": (C.c_int, [C.c_void_p, c_float_p, C.c_size_t]),
+    "moons_unet_set_weights": (C.c_int, [C.c_void_p, c_float_p, C.c_size_t]),
+    "moons_unet_forward_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p]),
+    "moons_unet_grad_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p,
+                                       c_double_p, c_float_p]),
+    "moons_unet_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p, C.c_int,
+                                   C.c_int, C.c_double, C.c_ulonglong, c_double_p]),
+    "moons_mfma_check": (C.c_int, [c_float_p, c_float_p, c_float_p]),
+    "moons_pcg_set_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

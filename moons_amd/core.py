@@ -236,7 +236,7 @@ class PCG_solver_SF:
     """
 
     OP_LAP_UNIFORM_SF = 1
-    PRECS = {"Lap_SF": 1, "MG_FP32": 2}
+    PRECS = {"Lap_SF": 1, "MG_FP32": 2, "UNet": 3}
 
     def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None, prec: str = "Lap_SF"):
         h = C.c_void_p()
@@ -254,6 +254,11 @@ class PCG_solver_SF:
         n = (C.c_int * 3)()
         check(load().moons_pcg_mg_info(self.handle, C.byref(lv), n))
         return lv.value, (n[0], n[1], n[2])
+
+    def set_UNet(self, net: "UNet") -> None:
+        """Bind the learned preconditioner (prec="UNet")."""
+        check(load().moons_pcg_set_unet(self.handle, net.handle))
+        self._net = net
 
     def apply_prec(self, r: SF, z: SF) -> None:
         check(load().moons_pcg_apply_prec(self.handle, r.handle, z.handle))
@@ -342,7 +347,7 @@ class MHD_step:
     caller's fields (with their BCs) and are advanced in place.
     ISPs are (iter_max, tol_rel, tol_abs, n_skip_check_res)."""
 
-    FIELDS = {"F": 1, "Fnm1": 2, "Ustar": 3, "U_E": 4, "J": 5, "jCrossB": 6, "FB": 7}
+    FIELDS = {"F": 1, "Fnm1": 2, "Ustar": 3, "U_E": 4, "J": 5, "jCrossB": 6, "FB": 7, "ppe_rhs": 8}
 
     def __init__(self, m: mesh, U, p: SF, B=None, phi: SF | None = None, Re=400.0, Ha=0.0, Rem=1.0, dt=1e-3,
                  alpha_U=1.0, B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
@@ -370,7 +375,7 @@ class MHD_step:
     def field(self, which: str):
         out = (C.c_void_p * 6)()
         check(load().moons_mhd_field(self.handle, self.FIELDS[which], out))
-        n = 6 if which == "U_E" else 3
+        n = 6 if which == "U_E" else (1 if which == "ppe_rhs" else 3)
         return [SF.borrow(self.m, out[q]) for q in range(n)]
 
     def __del__(self):
@@ -380,3 +385,83 @@ class MHD_step:
                 self.handle = None
         except Exception:
             pass
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class UNet:
+    """Learned PPE preconditioner: 3-level 3D U-net (16/32/64 channels, 3x3x3
+    convolutions), BF16 weights/activations with FP32 accumulation on the MFMA
+    units, trained on the device by the library's forward/backward kernels.
+
+    Arrays are FP32 with the padded column-major shape of a cell-centred field:
+    inputs (4, n0+2, n1+2, n2+2) channel-major, targets/outputs (n0+2, n1+2, n2+2)."""
+
+    def __init__(self, seed: int = 0):
+        if not _initialised:
+            init(0)
+        h = C.c_void_p()
+        check(load().moons_unet_create(seed, C.byref(h)))
+        self.handle = h
+        self.nparam = int(load().moons_unet_num_params(h))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.empty(self.nparam, dtype=np.float32)
+        check(load().moons_unet_get_weights(self.handle, _fp(w), w.size))
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(load().moons_unet_set_weights(self.handle, _fp(w), w.size))
+
+    @staticmethod
+    def _flat_in(in4):
+        a = np.asarray(in4, dtype=np.float32)
+        return np.ascontiguousarray(np.stack([np.asfortranarray(a[c]).ravel(order="F") for c in range(a.shape[0])]))
+
+    def forward(self, in4: np.ndarray) -> np.ndarray:
+        n = [d - 2 for d in in4.shape[1:]]
+        x = self._flat_in(in4)
+        out = np.zeros(int(np.prod(in4.shape[1:])), dtype=np.float32)
+        check(load().moons_unet_forward_host(self.handle, n[0], n[1], n[2], _fp(x), _fp(out)))
+        return out.reshape(in4.shape[1:], order="F")
+
+    def grad(self, in4: np.ndarray, target: np.ndarray):
+        n = [d - 2 for d in in4.shape[1:]]
+        x = self._flat_in(in4)
+        t = np.ascontiguousarray(np.asarray(target, dtype=np.float32).ravel(order="F"))
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        check(load().moons_unet_grad_host(self.handle, n[0], n[1], n[2], _fp(x), _fp(t), C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, in4_list, target_list, epochs=5, batch=4, lr=1e-3, seed=0) -> np.ndarray:
+        n = [d - 2 for d in in4_list[0].shape[1:]]
+        X = np.ascontiguousarray(np.concatenate([self._flat_in(a).ravel() for a in in4_list]).astype(np.float32))
+        T = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float32).ravel(order="F")
+                                                 for t in target_list]))
+        hist = np.zeros(epochs)
+        check(load().moons_unet_train(self.handle, n[0], n[1], n[2], len(in4_list), _fp(X), _fp(T), epochs, batch,
+                                      float(lr), seed, hist.ctypes.data_as(C.POINTER(C.c_double))))
+        return hist
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_unet_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def mfma_check(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """D = A @ B (16x16, operands rounded to BF16) through one MFMA with the U-net's lane layout."""
+    if not _initialised:
+        init(0)
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    b = np.ascontiguousarray(B, dtype=np.float32)
+    d = np.zeros((16, 16), dtype=np.float32)
+    check(load().moons_mfma_check(_fp(a), _fp(b), _fp(d)))
+    return d

@@ -16,6 +16,7 @@
 // FP64 flexible PCG whose residual, dot products and BCs stay FP64.
 #include "mg.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -34,6 +35,22 @@ struct LevelDev {
   const float *pa[3], *pb[3];  // and their weights
 };
 
+// The coarse end of the hierarchy (levels whose x and f fit in one
+// workgroup's LDS together) is cycled by a single workgroup in one launch:
+// the same smoothing, restriction, coarse sweeps and prolongation as the
+// per-level launches, in the same order, so the result is identical.
+constexpr int kTailThreads = 1024;
+constexpr int kMaxTail = 8;
+constexpr size_t kTailLdsBytes = 60 * 1024;
+constexpr long long kTailMaxCells = 8192;
+
+struct TailDesc {
+  int nt;                      // tail levels
+  int per0, per1, per2;
+  int n[kMaxTail][3];
+  int xo[kMaxTail], fo[kMaxTail], co[kMaxTail], cs[kMaxTail]; // LDS float offsets; coefficient stride
+};
+
 struct MGHier {
   std::vector<LevelDev> L;
   std::vector<float *> arrays; // device allocations to free
@@ -41,15 +58,9 @@ struct MGHier {
   int tail0 = 0;               // first level of the LDS-resident tail (== L.size(): none)
   size_t tail_lds = 0;         // LDS bytes of the tail's x and f arrays
   LevelDev *dlev = nullptr;    // level descriptors in device memory (for the tail kernel)
+  TailDesc tail{};
 };
 
-// The coarse end of the hierarchy (levels whose x and f fit in one
-// workgroup's LDS together) is cycled by a single workgroup in one launch:
-// the same smoothing, restriction, coarse sweeps and prolongation as the
-// per-level launches, in the same order, so the result is identical.
-constexpr int kTailThreads = 1024;
-constexpr size_t kTailLdsBytes = 60 * 1024;
-constexpr long long kTailMaxCells = 8192;
 
 __device__ __forceinline__ size_t lidx(const LevelDev &L, int i, int j, int k) {
   return (size_t)i + (size_t)L.s0 * ((size_t)j + (size_t)L.s1 * (size_t)k);
@@ -171,108 +182,222 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(LevelDev L, int sweeps, cons
   }
 }
 
-__device__ __forceinline__ size_t padded_cells(const LevelDev &L) {
-  return (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+// ---- LDS-resident tail.  Level l of the tail keeps x, f and its 1D
+// coefficient arrays in LDS; everything is indexed off the one extern
+// __shared__ array so the accesses are LDS instructions.
+
+// coefficient c (0 wm, 1 wp, 2 dg, 3 rf, 4 pa, 5 pb, 6 pI, 7 pI2) of direction d at index i
+#define TCOEF(l, c, d, i) sm[T.co[l] + ((c) * 3 + (d)) * T.cs[l] + (i)]
+#define TCOEFI(l, c, d, i) __float_as_int(TCOEF(l, c, d, i))
+#define TIDX(l, i, j, k) ((i) + (T.n[l][0] + 2) * ((j) + (T.n[l][1] + 2) * (k)))
+
+__device__ __forceinline__ bool tail_color_cell(int n0, int n1, int n2, int t, int color, int &i, int &j, int &k) {
+  const int h0 = (n0 + 1) / 2;
+  const int per_k = h0 * n1;
+  if (t >= per_k * n2) return false;
+  k = 1 + t / per_k;
+  const int r = t % per_k;
+  j = 1 + r / h0;
+  i = 2 * (r % h0) + 1 + ((color + j + k + 1) & 1);
+  return i <= n0;
 }
 
-// Descriptor of tail level l with x and f redirected into LDS.
-__device__ __forceinline__ LevelDev tail_level(const LevelDev *lev, int t0, int l, float *sm) {
-  size_t off = 0;
-  for (int q = t0; q < l; ++q) off += 2 * padded_cells(lev[q]);
-  LevelDev L = lev[l];
-  L.x = sm + off;
-  L.f = sm + off + padded_cells(L);
-  return L;
+// neighbour indices with the periodic wrap / folded-BC clamp of gs_cell
+__device__ __forceinline__ void tail_nb(const TailDesc &T, int l, int i, int j, int k, int nb[6]) {
+  const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
+  const int im = (i == 1) ? (T.per0 ? n0 : 1) : i - 1;
+  const int ip = (i == n0) ? (T.per0 ? 1 : n0) : i + 1;
+  const int jm = (j == 1) ? (T.per1 ? n1 : 1) : j - 1;
+  const int jp = (j == n1) ? (T.per1 ? 1 : n1) : j + 1;
+  const int km = (k == 1) ? (T.per2 ? n2 : 1) : k - 1;
+  const int kp = (k == n2) ? (T.per2 ? 1 : n2) : k + 1;
+  nb[0] = TIDX(l, im, j, k); nb[1] = TIDX(l, ip, j, k);
+  nb[2] = TIDX(l, i, jm, k); nb[3] = TIDX(l, i, jp, k);
+  nb[4] = TIDX(l, i, j, km); nb[5] = TIDX(l, i, j, kp);
 }
 
-__device__ __forceinline__ void tail_smooth(const LevelDev &L, int color, bool zero_guess) {
-  const long long ncol = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
-  for (long long t = threadIdx.x; t < ncol; t += kTailThreads) {
+// One colour of red-black Gauss-Seidel on tail level l (same arithmetic as gs_cell).
+__device__ __forceinline__ void tail_smooth(const TailDesc &T, float *sm, int l, int color, bool zero_guess) {
+  const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
+  const int ncol = ((n0 + 1) / 2) * n1 * n2;
+  float *x = sm + T.xo[l];
+  const float *f = sm + T.fo[l];
+  for (int t = threadIdx.x; t < ncol; t += kTailThreads) {
     int i, j, k;
-    if (color_cell(L, t, color, i, j, k)) L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, zero_guess);
+    if (!tail_color_cell(n0, n1, n2, t, color, i, j, k)) continue;
+    const float d = TCOEF(l, 2, 0, i) + TCOEF(l, 2, 1, j) + TCOEF(l, 2, 2, k);
+    const int c = TIDX(l, i, j, k);
+    const float fv = f[c];
+    if (zero_guess) {
+      x[c] = fv / d;
+      continue;
+    }
+    int nb[6];
+    tail_nb(T, l, i, j, k, nb);
+    float s = TCOEF(l, 0, 0, i) * x[nb[0]] + TCOEF(l, 1, 0, i) * x[nb[1]];
+    s += TCOEF(l, 0, 1, j) * x[nb[2]] + TCOEF(l, 1, 1, j) * x[nb[3]];
+    s += TCOEF(l, 0, 2, k) * x[nb[4]] + TCOEF(l, 1, 2, k) * x[nb[5]];
+    x[c] = (fv - s) / d;
   }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, int t0, int nl, int nu1, int nu2,
-                                                          int coarse_sweeps, const int *done) {
-  if (done && *done) return;
-  extern __shared__ float sm[];
-  {
-    const LevelDev G = lev[t0];
-    const LevelDev L = tail_level(lev, t0, t0, sm);
-    const size_t n = padded_cells(G);
-    for (size_t t = threadIdx.x; t < n; t += kTailThreads) L.f[t] = G.f[t];
-    __syncthreads();
-  }
-  for (int l = t0; l < nl - 1; ++l) {
-    const LevelDev F = tail_level(lev, t0, l, sm);
-    for (int s = 0; s < nu1; ++s) {
-      tail_smooth(F, 0, s == 0);
-      tail_smooth(F, 1, false);
-    }
-    const LevelDev C = tail_level(lev, t0, l + 1, sm);
-    const long long nc = (long long)C.n0 * C.n1 * C.n2;
-    for (long long t = threadIdx.x; t < nc; t += kTailThreads) {
-      const int I = 1 + (int)(t % C.n0);
-      const int J = 1 + (int)((t / C.n0) % C.n1);
-      const int K = 1 + (int)(t / ((long long)C.n0 * C.n1));
-      float acc = 0.f;
+// Coarsest-level sweeps with each thread's cells (at most CPT per colour)
+// and their coefficients held in registers across the sweeps.
+template <int CPT>
+__device__ __forceinline__ void tail_coarse(const TailDesc &T, float *sm, int l, int sweeps) {
+  const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
+  float *x = sm + T.xo[l];
+  const float *f = sm + T.fo[l];
+  bool v[2][CPT];
+  int c[2][CPT], nb[2][CPT][6];
+  float fv[2][CPT], dd[2][CPT], w[2][CPT][6];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
-        acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
+  for (int col = 0; col < 2; ++col)
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      int i = 1, j = 1, k = 1;
+      v[col][q] = tail_color_cell(n0, n1, n2, threadIdx.x + q * kTailThreads, col, i, j, k);
+      if (!v[col][q]) i = j = k = 1;
+      dd[col][q] = TCOEF(l, 2, 0, i) + TCOEF(l, 2, 1, j) + TCOEF(l, 2, 2, k);
+      c[col][q] = TIDX(l, i, j, k);
+      fv[col][q] = f[c[col][q]];
+      tail_nb(T, l, i, j, k, nb[col][q]);
+      w[col][q][0] = TCOEF(l, 0, 0, i); w[col][q][1] = TCOEF(l, 1, 0, i);
+      w[col][q][2] = TCOEF(l, 0, 1, j); w[col][q][3] = TCOEF(l, 1, 1, j);
+      w[col][q][4] = TCOEF(l, 0, 2, k); w[col][q][5] = TCOEF(l, 1, 2, k);
+    }
+  for (int s = 0; s < sweeps; ++s) {
+#pragma unroll
+    for (int col = 0; col < 2; ++col) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        if (!v[col][q]) continue;
+        if (s == 0 && col == 0) {
+          x[c[col][q]] = fv[col][q] / dd[col][q];
+        } else {
+          const int *b = nb[col][q];
+          const float *ww = w[col][q];
+          float a = ww[0] * x[b[0]] + ww[1] * x[b[1]];
+          a += ww[2] * x[b[2]] + ww[3] * x[b[3]];
+          a += ww[4] * x[b[4]] + ww[5] * x[b[5]];
+          x[c[col][q]] = (fv[col][q] - a) / dd[col][q];
+        }
       }
-      C.f[lidx(C, I, J, K)] = acc;
-    }
-    __syncthreads();
-  }
-  {
-    const LevelDev C = tail_level(lev, t0, nl - 1, sm);
-    for (int s = 0; s < coarse_sweeps; ++s) {
-      tail_smooth(C, 0, s == 0);
-      tail_smooth(C, 1, false);
-    }
-  }
-  for (int l = nl - 2; l >= t0; --l) {
-    const LevelDev F = tail_level(lev, t0, l, sm);
-    const LevelDev C = tail_level(lev, t0, l + 1, sm);
-    const long long nf = (long long)F.n0 * F.n1 * F.n2;
-    for (long long t = threadIdx.x; t < nf; t += kTailThreads) {
-      const int i = 1 + (int)(t % F.n0);
-      const int j = 1 + (int)((t / F.n0) % F.n1);
-      const int k = 1 + (int)(t / ((long long)F.n0 * F.n1));
-      const int I = F.pI[0][i], I2 = F.pI2[0][i];
-      const int J = F.pI[1][j], J2 = F.pI2[1][j];
-      const int K = F.pI[2][k], K2 = F.pI2[2][k];
-      const float ax = F.pa[0][i], bx = F.pb[0][i];
-      const float ay = F.pa[1][j], by = F.pb[1][j];
-      const float az = F.pa[2][k], bz = F.pb[2][k];
-      const float *c = C.x;
-      const float e = ax * (ay * (az * c[lidx(C, I, J, K)] + bz * c[lidx(C, I, J, K2)]) +
-                            by * (az * c[lidx(C, I, J2, K)] + bz * c[lidx(C, I, J2, K2)])) +
-                      bx * (ay * (az * c[lidx(C, I2, J, K)] + bz * c[lidx(C, I2, J, K2)]) +
-                            by * (az * c[lidx(C, I2, J2, K)] + bz * c[lidx(C, I2, J2, K2)]));
-      F.x[lidx(F, i, j, k)] += e;
-    }
-    __syncthreads();
-    for (int s = 0; s < nu2; ++s) {
-      tail_smooth(F, 1, false);
-      tail_smooth(F, 0, false);
-    }
-  }
-  {
-    const LevelDev G = lev[t0];
-    const LevelDev L = tail_level(lev, t0, t0, sm);
-    const long long n = (long long)G.n0 * G.n1 * G.n2;
-    for (long long t = threadIdx.x; t < n; t += kTailThreads) {
-      const int i = 1 + (int)(t % G.n0);
-      const int j = 1 + (int)((t / G.n0) % G.n1);
-      const int k = 1 + (int)(t / ((long long)G.n0 * G.n1));
-      G.x[lidx(G, i, j, k)] = L.x[lidx(L, i, j, k)];
+      __syncthreads();
     }
   }
 }
+
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, TailDesc T, int t0, int nu1, int nu2,
+                                                          int coarse_sweeps, const int *done) {
+  if (done && *done) return;
+  extern __shared__ float sm[];
+  const int nt = T.nt;
+  // coefficients of every tail level, and f of the first, into LDS
+  for (int l = 0; l < nt; ++l) {
+    const LevelDev &G = lev[t0 + l];
+    for (int d = 0; d < 3; ++d) {
+      const int s = T.n[l][d] + 2;
+      for (int i = threadIdx.x; i < s; i += kTailThreads) {
+        TCOEF(l, 0, d, i) = G.wm[d][i];
+        TCOEF(l, 1, d, i) = G.wp[d][i];
+        TCOEF(l, 2, d, i) = G.dg[d][i];
+        TCOEF(l, 3, d, i) = G.rf[d][i];
+        TCOEF(l, 4, d, i) = G.pa[d][i];
+        TCOEF(l, 5, d, i) = G.pb[d][i];
+        TCOEF(l, 6, d, i) = __int_as_float(G.pI[d][i]);
+        TCOEF(l, 7, d, i) = __int_as_float(G.pI2[d][i]);
+      }
+    }
+  }
+  {
+    const LevelDev &G = lev[t0];
+    const int n = (T.n[0][0] + 2) * (T.n[0][1] + 2) * (T.n[0][2] + 2);
+    for (int t = threadIdx.x; t < n; t += kTailThreads) sm[T.fo[0] + t] = G.f[t];
+  }
+  __syncthreads();
+  for (int l = 0; l < nt - 1; ++l) {
+    for (int s = 0; s < nu1; ++s) {
+      tail_smooth(T, sm, l, 0, s == 0);
+      tail_smooth(T, sm, l, 1, false);
+    }
+    // restriction of the residual into level l+1 (res_cell / k_mg_restrict arithmetic)
+    const int C0 = T.n[l + 1][0], C1 = T.n[l + 1][1], C2 = T.n[l + 1][2];
+    const float *x = sm + T.xo[l];
+    const float *f = sm + T.fo[l];
+    const int nc = C0 * C1 * C2;
+    for (int t = threadIdx.x; t < nc; t += kTailThreads) {
+      const int I = 1 + t % C0, J = 1 + (t / C0) % C1, K = 1 + t / (C0 * C1);
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = 2 * I - 1 + (q & 1), j = 2 * J - 1 + ((q >> 1) & 1), k = 2 * K - 1 + (q >> 2);
+        int nb[6];
+        tail_nb(T, l, i, j, k, nb);
+        const float d = TCOEF(l, 2, 0, i) + TCOEF(l, 2, 1, j) + TCOEF(l, 2, 2, k);
+        float s = TCOEF(l, 0, 0, i) * x[nb[0]] + TCOEF(l, 1, 0, i) * x[nb[1]];
+        s += TCOEF(l, 0, 1, j) * x[nb[2]] + TCOEF(l, 1, 1, j) * x[nb[3]];
+        s += TCOEF(l, 0, 2, k) * x[nb[4]] + TCOEF(l, 1, 2, k) * x[nb[5]];
+        s += d * x[TIDX(l, i, j, k)];
+        const float r = f[TIDX(l, i, j, k)] - s;
+        acc += TCOEF(l, 3, 0, i) * TCOEF(l, 3, 1, j) * TCOEF(l, 3, 2, k) * r;
+      }
+      sm[T.fo[l + 1] + TIDX(l + 1, I, J, K)] = acc;
+    }
+    __syncthreads();
+  }
+  {
+    const int l = nt - 1;
+    const int ncol = ((T.n[l][0] + 1) / 2) * T.n[l][1] * T.n[l][2];
+    if (ncol <= kTailThreads) tail_coarse<1>(T, sm, l, coarse_sweeps);
+    else if (ncol <= 2 * kTailThreads) tail_coarse<2>(T, sm, l, coarse_sweeps);
+    else
+      for (int s = 0; s < coarse_sweeps; ++s) {
+        tail_smooth(T, sm, l, 0, s == 0);
+        tail_smooth(T, sm, l, 1, false);
+      }
+  }
+  for (int l = nt - 2; l >= 0; --l) {
+    const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
+    float *x = sm + T.xo[l];
+    const float *c = sm + T.xo[l + 1];
+    const int nf = n0 * n1 * n2;
+    for (int t = threadIdx.x; t < nf; t += kTailThreads) {
+      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      const int I = TCOEFI(l, 6, 0, i), I2 = TCOEFI(l, 7, 0, i);
+      const int J = TCOEFI(l, 6, 1, j), J2 = TCOEFI(l, 7, 1, j);
+      const int K = TCOEFI(l, 6, 2, k), K2 = TCOEFI(l, 7, 2, k);
+      const float ax = TCOEF(l, 4, 0, i), bx = TCOEF(l, 5, 0, i);
+      const float ay = TCOEF(l, 4, 1, j), by = TCOEF(l, 5, 1, j);
+      const float az = TCOEF(l, 4, 2, k), bz = TCOEF(l, 5, 2, k);
+      const int m = l + 1;
+      const float e = ax * (ay * (az * c[TIDX(m, I, J, K)] + bz * c[TIDX(m, I, J, K2)]) +
+                            by * (az * c[TIDX(m, I, J2, K)] + bz * c[TIDX(m, I, J2, K2)])) +
+                      bx * (ay * (az * c[TIDX(m, I2, J, K)] + bz * c[TIDX(m, I2, J, K2)]) +
+                            by * (az * c[TIDX(m, I2, J2, K)] + bz * c[TIDX(m, I2, J2, K2)]));
+      x[TIDX(l, i, j, k)] += e;
+    }
+    __syncthreads();
+    for (int s = 0; s < nu2; ++s) {
+      tail_smooth(T, sm, l, 1, false);
+      tail_smooth(T, sm, l, 0, false);
+    }
+  }
+  {
+    const LevelDev &G = lev[t0];
+    const int n0 = T.n[0][0], n1 = T.n[0][1], n2 = T.n[0][2];
+    const int n = n0 * n1 * n2;
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      G.x[lidx(G, i, j, k)] = sm[T.xo[0] + TIDX(0, i, j, k)];
+    }
+  }
+}
+
+#undef TCOEF
+#undef TCOEFI
+#undef TIDX
 
 // ------------------------------------------------------------------- host
 namespace {
@@ -316,10 +441,19 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     }
     H.push_back(l0);
   }
+  // red-black ordering is a proper two-colouring across a periodic boundary
+  // only for an even cell count (coarsening keeps periodic counts even)
+  for (int d = 0; d < 3; ++d)
+    if (bct[2 * d] == MOONS_BC_PERIODIC && H[0].n[d] % 2 != 0) {
+      if (err) *err = "multigrid red-black smoothing needs an even cell count along periodic directions";
+      return nullptr;
+    }
   while ((int)H.size() < max_levels) {
     const HostLevel &f = H.back();
     bool ok = true;
     for (int d = 0; d < 3; ++d) ok = ok && (f.n[d] % 2 == 0) && (f.n[d] / 2 >= 2);
+    for (int d = 0; d < 3; ++d)
+      if (bct[2 * d] == MOONS_BC_PERIODIC) ok = ok && ((f.n[d] / 2) % 2 == 0);
     // a level of at most kCoarseCells per direction is solved by the
     // single-block sweeps; coarser levels cost launches and gain nothing
     if (std::max(f.n[0], std::max(f.n[1], f.n[2])) <= kCoarseCells) ok = false;
@@ -441,17 +575,40 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       L.pa[d] = pad; L.pb[d] = pbd; L.pI[d] = pId; L.pI2[d] = pI2d;
     }
   }
-  // LDS-resident tail: the longest coarse suffix whose x and f fit in LDS
-  h->tail0 = nl;
-  size_t bytes = 0;
-  for (int l = nl - 1; l >= 0; --l) {
+  // LDS-resident tail: the longest coarse suffix whose x, f and 1D
+  // coefficients fit in LDS (floats: 2 padded arrays + 8 arrays x 3 directions)
+  auto level_floats = [&](int l) {
     const LevelDev &L = h->L[l];
-    const size_t b = 2 * sizeof(float) * (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
-    if (bytes + b > kTailLdsBytes || (long long)L.n0 * L.n1 * L.n2 > kTailMaxCells) break;
-    bytes += b;
+    const int smax = std::max(L.n0, std::max(L.n1, L.n2)) + 2;
+    return 2 * (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2) + 24 * (size_t)smax;
+  };
+  h->tail0 = nl;
+  size_t floats = 0;
+  for (int l = nl - 1; l >= 0 && nl - l <= kMaxTail; --l) {
+    const LevelDev &L = h->L[l];
+    const size_t b = level_floats(l);
+    if ((floats + b) * sizeof(float) > kTailLdsBytes || (long long)L.n0 * L.n1 * L.n2 > kTailMaxCells) break;
+    floats += b;
     h->tail0 = l;
   }
-  h->tail_lds = bytes;
+  h->tail_lds = floats * sizeof(float);
+  {
+    TailDesc &T = h->tail;
+    T.nt = nl - h->tail0;
+    T.per0 = bct[0] == MOONS_BC_PERIODIC;
+    T.per1 = bct[2] == MOONS_BC_PERIODIC;
+    T.per2 = bct[4] == MOONS_BC_PERIODIC;
+    int off = 0;
+    for (int q = 0; q < T.nt; ++q) {
+      const LevelDev &L = h->L[h->tail0 + q];
+      T.n[q][0] = L.n0; T.n[q][1] = L.n1; T.n[q][2] = L.n2;
+      const int pad = (L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+      T.xo[q] = off; off += pad;
+      T.fo[q] = off; off += pad;
+      T.cs[q] = std::max(L.n0, std::max(L.n1, L.n2)) + 2;
+      T.co[q] = off; off += 24 * T.cs[q];
+    }
+  }
   {
     float *p = dalloc((nl * sizeof(LevelDev) + sizeof(float) - 1) / sizeof(float));
     if (!p) {
@@ -513,8 +670,8 @@ int mg_vcycle(MGHier *h, const int *done) {
     hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, stream(), L, C, done);
   }
   if (tail)
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tail0, nl, h->nu1,
-                       h->nu2, h->coarse_sweeps, done);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tail, h->tail0,
+                       h->nu1, h->nu2, h->coarse_sweeps, done);
   else
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
   // ascend

@@ -24,6 +24,7 @@
 // chunks.
 #include "cc_stencil.hpp"
 #include "mg.hpp"
+#include "unet.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -522,6 +523,10 @@ struct moons_pcg_s {
   double *r = nullptr, *p = nullptr, *q = nullptr; // device work vectors
   double *partA = nullptr, *partB = nullptr, *partM = nullptr, *partC = nullptr;
   MGHier *mg = nullptr; // MOONS_PREC_MG_FP32 only
+  UnetPrec *unet = nullptr;                   // MOONS_PREC_UNET: bound network
+  bool jacobi_only = false;                   // set while a learned solve falls back to Jacobi
+  double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
+  float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
   int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 40;
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned
@@ -541,11 +546,22 @@ static int pcg_upload_ctrl(moons_pcg_s *S) {
   return MOONS_OK;
 }
 
+// flexible (non-Jacobi) preconditioners: FP32 r/vol in, FP32 z out
+static bool flex(const moons_pcg_s *S) {
+  return !S->jacobi_only && (S->mg != nullptr || S->prec == MOONS_PREC_UNET);
+}
+static float *flex_f0(moons_pcg_s *S) { return S->mg ? mg_f0(S->mg) : S->uf0; }
+static float *flex_z0(moons_pcg_s *S) { return S->mg ? mg_x0(S->mg) : S->uz0; }
+static int flex_apply(moons_pcg_s *S) {
+  if (S->mg) return mg_vcycle(S->mg, &S->ctrl->done);
+  return unet_prec_apply(S->unet, S->uf0, S->uz0, &S->ctrl->done);
+}
+
 template <int KZ>
 static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
   const dim3 grid(S->G.nblocks), block(kNT);
-  const bool mg = S->mg != nullptr;
-  float *f0 = mg ? mg_f0(S->mg) : nullptr;
+  const bool mg = flex(S);
+  float *f0 = mg ? flex_f0(S) : nullptr;
   if (which & 1) hipLaunchKernelGGL(k_pcg_matvec<KZ>, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA);
   if (which & 2) {
     if (mg)
@@ -556,9 +572,9 @@ static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
                          S->partA, S->partB, it, f0);
   }
   if (which & 4) hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
-  if (which & 8) hipLaunchKernelGGL(k_mg_zdots<KZ>, grid, block, 0, stream(), mg_x0(S->mg), S->r, S->q, S->G, S->ctrl,
+  if (which & 8) hipLaunchKernelGGL(k_mg_zdots<KZ>, grid, block, 0, stream(), flex_z0(S), S->r, S->q, S->G, S->ctrl,
                                     S->partC, it > 0 ? 1 : 0);
-  if (which & 16) hipLaunchKernelGGL(k_mg_pupdate<KZ>, grid, block, 0, stream(), S->p, mg_x0(S->mg), S->G, S->ctrl,
+  if (which & 16) hipLaunchKernelGGL(k_mg_pupdate<KZ>, grid, block, 0, stream(), S->p, flex_z0(S), S->G, S->ctrl,
                                      S->partC, it);
 }
 
@@ -581,7 +597,7 @@ static int pcg_launch_iteration_mg(moons_pcg_s *S, double *x, int it) {
   if (st) return st;
   hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kNT), 0, stream(), S->ctrl, S->partB, S->G.nblocks, it);
   MOONS_LAUNCH_CHECK();
-  if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+  if ((st = flex_apply(S))) return st;
   return pcg_launch(S, x, it, 8 | 16);
 }
 
@@ -592,8 +608,9 @@ extern "C" {
 int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons_pcg_t *out) {
   if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
   if (!m || !x || !out) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: null argument");
-  if (op != MOONS_OP_LAP_UNIFORM_SF || (prec != MOONS_PREC_LAP_SF && prec != MOONS_PREC_MG_FP32))
-    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF with prec_Lap_SF or the FP32 multigrid only");
+  if (op != MOONS_OP_LAP_UNIFORM_SF ||
+      (prec != MOONS_PREC_LAP_SF && prec != MOONS_PREC_MG_FP32 && prec != MOONS_PREC_UNET))
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF with prec_Lap_SF, the FP32 multigrid or the U-net");
   if (x->dl != MOONS_DL_CC)
     return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF solver needs a cell-centred field");
   if (x->m != m) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: field is on another mesh");
@@ -638,6 +655,15 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
       return fail(MOONS_ERR_HIP, "moons_pcg_sf_create: " + err);
     }
   }
+  if (prec == MOONS_PREC_UNET) {
+    const size_t nf = (size_t)x->n;
+    if (hipMalloc(&S->uf0, nf * sizeof(float)) != hipSuccess || hipMalloc(&S->uz0, nf * sizeof(float)) != hipSuccess) {
+      moons_pcg_destroy(S);
+      return fail(MOONS_ERR_HIP, "moons_pcg_sf_create: allocation failed");
+    }
+    (void)hipMemsetAsync(S->uf0, 0, nf * sizeof(float), stream());
+    (void)hipMemsetAsync(S->uz0, 0, nf * sizeof(float), stream());
+  }
   int st = pcg_upload_ctrl(S);
   if (st) {
     moons_pcg_destroy(S);
@@ -647,10 +673,24 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
   return MOONS_OK;
 }
 
+int moons_pcg_set_unet(moons_pcg_t S, moons_unet_t net) {
+  if (!S || !net) return fail(MOONS_ERR_ARG, "moons_pcg_set_unet: null argument");
+  if (S->prec != MOONS_PREC_UNET) return fail(MOONS_ERR_STATE, "moons_pcg_set_unet: solver was not created with MOONS_PREC_UNET");
+  if (S->unet) unet_prec_destroy(S->unet);
+  std::string err;
+  S->unet = unet_prec_create(net, &S->xbc, &err);
+  if (!S->unet) return fail(MOONS_ERR_ARG, "moons_pcg_set_unet: " + err);
+  return MOONS_OK;
+}
+
 int moons_pcg_destroy(moons_pcg_t S) {
   if (!S) return MOONS_OK;
   if (stream()) (void)hipStreamSynchronize(stream());
   if (S->mg) mg_destroy(S->mg);
+  if (S->unet) unet_prec_destroy(S->unet);
+  if (S->uf0) (void)hipFree(S->uf0);
+  if (S->uz0) (void)hipFree(S->uz0);
+  if (S->xbak) (void)hipFree(S->xbak);
   double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM, S->partC};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
@@ -674,16 +714,12 @@ int moons_pcg_set_isp(moons_pcg_t S, int iter_max, double tol_rel, double tol_ab
 
 
 
-int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pcg_info *info) {
-  if (!S || !x || !b) return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: null argument");
-  if (x->m != S->m || b->m != S->m || x->dl != S->xbc.dl || b->dl != S->xbc.dl)
-    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: x and b must match the solver's mesh and location");
+static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
   int st = pcg_upload_ctrl(S);
   if (st) return st;
   const dim3 grid(S->G.nblocks), block(kNT);
-  MOONS_HIP(hipEventRecord(S->ev0, stream()));
   const int alln = S->xbc.all_neumann ? 1 : 0;
-  const bool mg = S->mg != nullptr;
+  const bool mg = flex(S);
   if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->partM);
   if (!mg) {
     hipLaunchKernelGGL(k_pcg_rhs<false>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
@@ -693,9 +729,9 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
   } else {
     // z0 = V(r0/vol), rho0 = (z0,r0), p = z0
     hipLaunchKernelGGL(k_pcg_rhs<true>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
-                       S->partB, alln, mg_f0(S->mg));
+                       S->partB, alln, flex_f0(S));
     MOONS_LAUNCH_CHECK();
-    if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+    if ((st = flex_apply(S))) return st;
     if ((st = pcg_launch(S, x->d, 0, 8))) return st;
     hipLaunchKernelGGL(k_mg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->partC, S->G.nblocks);
     MOONS_LAUNCH_CHECK();
@@ -718,15 +754,51 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
   // apply_BCs(x) after the last update (PCG_solver_algorithm.f90:77,98)
   st = launch_apply_bcs(x, false);
   if (st) return st;
-  MOONS_HIP(hipEventRecord(S->ev1, stream()));
   MOONS_HIP(hipMemcpyAsync(S->host_ctrl, S->ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, stream()));
   MOONS_HIP(hipStreamSynchronize(stream()));
+  return MOONS_OK;
+}
+
+int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pcg_info *info) {
+  if (!S || !x || !b) return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: null argument");
+  if (x->m != S->m || b->m != S->m || x->dl != S->xbc.dl || b->dl != S->xbc.dl)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: x and b must match the solver's mesh and location");
+  if (S->prec == MOONS_PREC_UNET && !S->unet)
+    return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve: bind a network with moons_pcg_set_unet first");
+  const bool learned = S->prec == MOONS_PREC_UNET;
+  if (learned) {
+    if (!S->xbak && hipMalloc(&S->xbak, x->n * sizeof(double)) != hipSuccess)
+      return fail(MOONS_ERR_HIP, "moons_pcg_sf_solve: allocation failed");
+    MOONS_HIP(hipMemcpyAsync(S->xbak, x->d, x->n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  }
+  MOONS_HIP(hipEventRecord(S->ev0, stream()));
+  S->jacobi_only = false;
+  int st = pcg_solve_impl(S, x, b);
+  if (st) return st;
+  int fell_back = 0;
+  if (learned) {
+    // Jacobi-PCG is the fallback and the judge: a learned solve that breaks
+    // down (NaN) or stops at iter_max without meeting a tolerance is redone
+    // from the same x with the Jacobi preconditioner.
+    const PcgCtrl &h = *S->host_ctrl;
+    const bool bad = std::isnan(h.res) || std::isnan(h.res0) || std::isinf(h.res) || !(h.exit_loop & 3);
+    if (bad) {
+      MOONS_HIP(hipMemcpyAsync(x->d, S->xbak, x->n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+      S->jacobi_only = true;
+      st = pcg_solve_impl(S, x, b);
+      S->jacobi_only = false;
+      if (st) return st;
+      fell_back = 8;
+    }
+  }
+  MOONS_HIP(hipEventRecord(S->ev1, stream()));
+  MOONS_HIP(hipEventSynchronize(S->ev1));
   const PcgCtrl &h = *S->host_ctrl;
   if (std::isnan(h.res) || std::isnan(h.res0))
     return fail(MOONS_ERR_NAN, "Error: NaN in PCG residual (check_nans, PCG_solver_algorithm.f90:223-248)");
   if (info) {
     info->iterations = h.iter;
-    info->exit_loop = h.exit_loop;
+    info->exit_loop = h.exit_loop | fell_back;
     info->res_norm0 = h.res0;
     info->res_norm = h.res;
     float ms = 0.f;
@@ -808,14 +880,16 @@ int moons_pcg_apply_prec(moons_pcg_t S, moons_field_t r, moons_field_t z) {
   if (r->dl != S->xbc.dl || z->dl != S->xbc.dl || r->m != S->m || z->m != S->m)
     return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: fields must match the solver");
   const dim3 grid(S->G.nblocks), block(kNT);
-  if (S->mg) {
+  if (flex(S)) {
+    if (S->prec == MOONS_PREC_UNET && !S->unet)
+      return fail(MOONS_ERR_STATE, "moons_pcg_apply_prec: bind a network with moons_pcg_set_unet first");
     int st = pcg_upload_ctrl(S); // done = 0
     if (st) return st;
-    hipLaunchKernelGGL(k_mg_r2f, grid, block, 0, stream(), r->d, mg_f0(S->mg), S->G);
+    hipLaunchKernelGGL(k_mg_r2f, grid, block, 0, stream(), r->d, flex_f0(S), S->G);
     MOONS_LAUNCH_CHECK();
-    if ((st = mg_vcycle(S->mg, &S->ctrl->done))) return st;
+    if ((st = flex_apply(S))) return st;
     MOONS_HIP(hipMemsetAsync(z->d, 0, z->n * sizeof(double), stream()));
-    hipLaunchKernelGGL(k_mg_x2z, grid, block, 0, stream(), mg_x0(S->mg), z->d, S->G);
+    hipLaunchKernelGGL(k_mg_x2z, grid, block, 0, stream(), flex_z0(S), z->d, S->G);
     MOONS_LAUNCH_CHECK();
     return MOONS_OK;
   }

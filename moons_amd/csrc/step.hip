@@ -407,6 +407,7 @@ int moons_mhd_field(moons_mhd_t S, int which, moons_field_t out[6]) {
     case MOONS_MHD_J: src = S->J; break;
     case MOONS_MHD_JCROSSB: src = S->JxB; break;
     case MOONS_MHD_FB: src = S->FB; break;
+    case MOONS_MHD_PPE_RHS: out[0] = S->tcc; return MOONS_OK;
     default: return fail(MOONS_ERR_ARG, "moons_mhd_field: unknown field");
   }
   if (!src[0]) return fail(MOONS_ERR_STATE, "moons_mhd_field: field not present (no induction)");

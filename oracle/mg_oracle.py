@@ -124,6 +124,8 @@ def build(hn3, bct, max_levels=16, coarse_cells=COARSE_CELLS):
         L = levels[-1]
         if not all(n % 2 == 0 and n // 2 >= 2 for n in L.n):
             break
+        if any(bct[2 * d] == 3 and (L.n[d] // 2) % 2 != 0 for d in range(3)):
+            break
         if max(L.n) <= coarse_cells:
             break
         levels.append(Level([coarsen_nodes(h) for h in L.hn], bct))
