@@ -330,6 +330,12 @@ int moons_unet_train(moons_unet_t net, int n0, int n1, int n2, int npair, const 
 /* D = A B for 16x16 A and B rounded to BF16, through one MFMA with the lane
  * layout the U-net kernels use (a self-check of that layout). */
 int moons_mfma_check(const float *A, const float *B, float *D);
+/* Learned + mixed: inside a MOONS_PREC_MG_FP32 solver's V-cycle, the network
+ * gives the initial guess of multigrid level `level` (0 = finest; a level
+ * launched over the grid, with cell counts that are multiples of 4): before
+ * that level's pre-smoothing x = c N(f), c = (N, f) / (N, L N) (0 when not
+ * positive).  net = NULL removes it.  No reference counterpart. */
+int moons_pcg_set_mg_unet(moons_pcg_t pcg, moons_unet_t net, int level);
 /* Binds the network to a MOONS_PREC_UNET solver (required before its first solve). */
 int moons_pcg_set_unet(moons_pcg_t pcg, moons_unet_t net);
 

@@ -102,6 +102,7 @@ SIGNATURES = {
                                    C.c_int, C.c_double, C.c_ulonglong, c_double_p]),
     "moons_mfma_check": (C.c_int, [c_float_p, c_float_p, c_float_p]),
     "moons_pcg_set_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "moons_pcg_set_mg_unet": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@
 // Everything is FP32 (4 B/cell per array): this is the preconditioner of an
 // FP64 flexible PCG whose residual, dot products and BCs stay FP64.
 #include "mg.hpp"
+#include "unet.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -49,6 +50,7 @@ struct TailDesc {
   int per0, per1, per2;
   int n[kMaxTail][3];
   int xo[kMaxTail], fo[kMaxTail], co[kMaxTail], cs[kMaxTail]; // LDS float offsets; coefficient stride
+  int nfloat;                  // LDS floats in use
 };
 
 struct MGHier {
@@ -59,6 +61,12 @@ struct MGHier {
   size_t tail_lds = 0;         // LDS bytes of the tail's x and f arrays
   LevelDev *dlev = nullptr;    // level descriptors in device memory (for the tail kernel)
   TailDesc tail{};
+  float *tailimg = nullptr;    // device copy of the tail's initial LDS image
+  std::vector<std::vector<float>> hcoef; // host 1D coefficients: [(level*3 + d)*8 + c]
+  std::vector<std::vector<double>> hn; // node arrays (ghosts included) of every level, 3 per level
+  UnetPrec *learn = nullptr;   // learned initial guess of one grid level (or none)
+  int learn_level = -1;
+  float *gpart = nullptr;      // its Galerkin partial sums
 };
 
 
@@ -114,6 +122,7 @@ __device__ __forceinline__ bool color_cell(const LevelDev &L, long long t, int c
   return i <= L.n0;
 }
 
+
 __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int zero_guess, const int *done) {
   if (done && *done) return;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -121,6 +130,69 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
   int i, j, k;
   if (!color_cell(L, t, color, i, j, k)) return;
   L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, zero_guess != 0);
+}
+
+// Galerkin scaling of a learned initial guess n on one level: partial sums of
+// (n, f) and (n, L n) over the interior (L the level operator).
+__global__ void __launch_bounds__(256) k_mg_galerkin_part(LevelDev L, float *part, const int *done) {
+  if (done && *done) return;
+  __shared__ float sa[256], sb[256];
+  const int nc = L.n0 * L.n1 * L.n2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  float a = 0.f, bb = 0.f;
+  if (t < nc) {
+    const int i = 1 + t % L.n0, j = 1 + (t / L.n0) % L.n1, k = 1 + t / (L.n0 * L.n1);
+    const size_t c = lidx(L, i, j, k);
+    const float x = L.x[c], f = L.f[c];
+    const float Lx = f - res_cell(L, i, j, k);
+    a = x * f;
+    bb = x * Lx;
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sa[0];
+    part[2 * blockIdx.x + 1] = sb[0];
+  }
+}
+
+// x *= c with c = (n, f) / (n, L n) (the energy-optimal multiple of n as an
+// approximation of L^-1 f), or 0 when that is not a positive number; every
+// block sums the partials in the same order.
+__global__ void __launch_bounds__(256) k_mg_galerkin_scale(LevelDev L, const float *part, int nb, const int *done) {
+  if (done && *done) return;
+  __shared__ float sa[256], sb[256];
+  float a = 0.f, bb = 0.f;
+  for (int q = threadIdx.x; q < nb; q += 256) {
+    a += part[2 * q];
+    bb += part[2 * q + 1];
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sa[threadIdx.x] += sa[threadIdx.x + w];
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const float q = sb[0] != 0.f ? sa[0] / sb[0] : 0.f;
+  const float c = (q > 0.f && q < 3.0e38f) ? q : 0.f;
+  const int nc = L.n0 * L.n1 * L.n2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < nc) {
+    const int i = 1 + t % L.n0, j = 1 + (t / L.n0) % L.n1, k = 1 + t / (L.n0 * L.n1);
+    L.x[lidx(L, i, j, k)] *= c;
+  }
 }
 
 // Coarse f = sum over the 8 children of (child vol / coarse vol) * residual.
@@ -242,10 +314,15 @@ __device__ __forceinline__ void tail_smooth(const TailDesc &T, float *sm, int l,
   __syncthreads();
 }
 
-// Coarsest-level sweeps with each thread's cells (at most CPT per colour)
-// and their coefficients held in registers across the sweeps.
-template <int CPT>
-__device__ __forceinline__ void tail_coarse(const TailDesc &T, float *sm, int l, int sweeps) {
+// nsweeps red-black sweeps of tail level l (colour c_first first; the very
+// first half-sweep from a zero guess when zero_first) with each thread's cells
+// (at most CPT per colour) and their coefficients held in registers.
+// WAVE: the level has at most 64 cells per colour; wave 0 alone runs the
+// sweeps, ordered by wave-level fences instead of workgroup barriers.
+template <int CPT, bool WAVE = false>
+__device__ __forceinline__ void tail_sweeps(const TailDesc &T, float *sm, int l, int nsweeps, int c_first,
+                                            bool zero_first) {
+  if (WAVE && threadIdx.x >= 64) return;
   const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
   float *x = sm + T.xo[l];
   const float *f = sm + T.fo[l];
@@ -267,61 +344,94 @@ __device__ __forceinline__ void tail_coarse(const TailDesc &T, float *sm, int l,
       w[col][q][2] = TCOEF(l, 0, 1, j); w[col][q][3] = TCOEF(l, 1, 1, j);
       w[col][q][4] = TCOEF(l, 0, 2, k); w[col][q][5] = TCOEF(l, 1, 2, k);
     }
-  for (int s = 0; s < sweeps; ++s) {
+  for (int s = 0; s < nsweeps; ++s) {
 #pragma unroll
-    for (int col = 0; col < 2; ++col) {
+    for (int h = 0; h < 2; ++h) {
+      const int col = h == 0 ? c_first : 1 - c_first;
+      const bool zg = zero_first && s == 0 && h == 0;
 #pragma unroll
       for (int q = 0; q < CPT; ++q) {
-        if (!v[col][q]) continue;
-        if (s == 0 && col == 0) {
-          x[c[col][q]] = fv[col][q] / dd[col][q];
+        // col is uniform across the workgroup: select the register set
+        const bool vv = col ? v[1][q] : v[0][q];
+        if (!vv) continue;
+        const int cc = col ? c[1][q] : c[0][q];
+        const float ff = col ? fv[1][q] : fv[0][q];
+        const float d = col ? dd[1][q] : dd[0][q];
+        if (zg) {
+          x[cc] = ff / d;
         } else {
-          const int *b = nb[col][q];
-          const float *ww = w[col][q];
-          float a = ww[0] * x[b[0]] + ww[1] * x[b[1]];
-          a += ww[2] * x[b[2]] + ww[3] * x[b[3]];
-          a += ww[4] * x[b[4]] + ww[5] * x[b[5]];
-          x[c[col][q]] = (fv[col][q] - a) / dd[col][q];
+          float a;
+          if (col) {
+            a = w[1][q][0] * x[nb[1][q][0]] + w[1][q][1] * x[nb[1][q][1]];
+            a += w[1][q][2] * x[nb[1][q][2]] + w[1][q][3] * x[nb[1][q][3]];
+            a += w[1][q][4] * x[nb[1][q][4]] + w[1][q][5] * x[nb[1][q][5]];
+          } else {
+            a = w[0][q][0] * x[nb[0][q][0]] + w[0][q][1] * x[nb[0][q][1]];
+            a += w[0][q][2] * x[nb[0][q][2]] + w[0][q][3] * x[nb[0][q][3]];
+            a += w[0][q][4] * x[nb[0][q][4]] + w[0][q][5] * x[nb[0][q][5]];
+          }
+          x[cc] = (ff - a) / d;
         }
       }
-      __syncthreads();
+      if (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        __syncthreads();
+      }
     }
   }
 }
 
-__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, TailDesc T, int t0, int nu1, int nu2,
-                                                          int coarse_sweeps, const int *done) {
+// sweeps of tail level l by the register path when its colours fit, else the generic one
+__device__ __forceinline__ void tail_level_sweeps(const TailDesc &T, float *sm, int l, int nsweeps, int c_first,
+                                                  bool zero_first) {
+  const int ncol = ((T.n[l][0] + 1) / 2) * T.n[l][1] * T.n[l][2];
+  if (ncol <= 64) {
+    tail_sweeps<1, true>(T, sm, l, nsweeps, c_first, zero_first);
+    __syncthreads();
+  } else if (ncol <= kTailThreads) {
+    tail_sweeps<1>(T, sm, l, nsweeps, c_first, zero_first);
+  } else if (ncol <= 2 * kTailThreads) {
+    tail_sweeps<2>(T, sm, l, nsweeps, c_first, zero_first);
+  } else {
+    for (int s = 0; s < nsweeps; ++s) {
+      tail_smooth(T, sm, l, c_first, zero_first && s == 0);
+      tail_smooth(T, sm, l, 1 - c_first, false);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, const float *__restrict__ img, TailDesc T,
+                                                          int t0, int nu1, int nu2, int coarse_sweeps, const int *done) {
   if (done && *done) return;
   extern __shared__ float sm[];
   const int nt = T.nt;
-  // coefficients of every tail level, and f of the first, into LDS
-  for (int l = 0; l < nt; ++l) {
-    const LevelDev &G = lev[t0 + l];
-    for (int d = 0; d < 3; ++d) {
-      const int s = T.n[l][d] + 2;
-      for (int i = threadIdx.x; i < s; i += kTailThreads) {
-        TCOEF(l, 0, d, i) = G.wm[d][i];
-        TCOEF(l, 1, d, i) = G.wp[d][i];
-        TCOEF(l, 2, d, i) = G.dg[d][i];
-        TCOEF(l, 3, d, i) = G.rf[d][i];
-        TCOEF(l, 4, d, i) = G.pa[d][i];
-        TCOEF(l, 5, d, i) = G.pb[d][i];
-        TCOEF(l, 6, d, i) = __int_as_float(G.pI[d][i]);
-        TCOEF(l, 7, d, i) = __int_as_float(G.pI2[d][i]);
+  // the tail's LDS image (coefficients in place, x and f zero), then f of the
+  // first tail level: independent coalesced loads, all in flight together
+  {
+    const int nimg = T.nfloat;
+    for (int t0i = 0; t0i < nimg; t0i += 4 * kTailThreads) {
+      float r[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = t0i + q * kTailThreads + threadIdx.x;
+        r[q] = t < nimg ? img[t] : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = t0i + q * kTailThreads + threadIdx.x;
+        if (t < nimg) sm[t] = r[q];
       }
     }
-  }
-  {
-    const LevelDev &G = lev[t0];
+    __syncthreads();
+    const float *f0 = lev[t0].f;
     const int n = (T.n[0][0] + 2) * (T.n[0][1] + 2) * (T.n[0][2] + 2);
-    for (int t = threadIdx.x; t < n; t += kTailThreads) sm[T.fo[0] + t] = G.f[t];
+    for (int t = threadIdx.x; t < n; t += kTailThreads) sm[T.fo[0] + t] = f0[t];
   }
   __syncthreads();
   for (int l = 0; l < nt - 1; ++l) {
-    for (int s = 0; s < nu1; ++s) {
-      tail_smooth(T, sm, l, 0, s == 0);
-      tail_smooth(T, sm, l, 1, false);
-    }
+    tail_level_sweeps(T, sm, l, nu1, 0, true);
     // restriction of the residual into level l+1 (res_cell / k_mg_restrict arithmetic)
     const int C0 = T.n[l + 1][0], C1 = T.n[l + 1][1], C2 = T.n[l + 1][2];
     const float *x = sm + T.xo[l];
@@ -347,17 +457,7 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, T
     }
     __syncthreads();
   }
-  {
-    const int l = nt - 1;
-    const int ncol = ((T.n[l][0] + 1) / 2) * T.n[l][1] * T.n[l][2];
-    if (ncol <= kTailThreads) tail_coarse<1>(T, sm, l, coarse_sweeps);
-    else if (ncol <= 2 * kTailThreads) tail_coarse<2>(T, sm, l, coarse_sweeps);
-    else
-      for (int s = 0; s < coarse_sweeps; ++s) {
-        tail_smooth(T, sm, l, 0, s == 0);
-        tail_smooth(T, sm, l, 1, false);
-      }
-  }
+  tail_level_sweeps(T, sm, nt - 1, coarse_sweeps, 0, true);
   for (int l = nt - 2; l >= 0; --l) {
     const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
     float *x = sm + T.xo[l];
@@ -379,10 +479,7 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, T
       x[TIDX(l, i, j, k)] += e;
     }
     __syncthreads();
-    for (int s = 0; s < nu2; ++s) {
-      tail_smooth(T, sm, l, 1, false);
-      tail_smooth(T, sm, l, 0, false);
-    }
+    tail_level_sweeps(T, sm, l, nu2, 1, false);
   }
   {
     const LevelDev &G = lev[t0];
@@ -423,7 +520,7 @@ bool dirichlet_like(int t) { return t == MOONS_BC_DIRICHLET || t == MOONS_BC_ANT
 
 } // namespace
 
-constexpr int kCoarseCells = 8;
+constexpr int kCoarseCells = 4;
 
 MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int coarse_sweeps, std::string *err) {
   if (x->dl != MOONS_DL_CC) {
@@ -466,11 +563,14 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     H.push_back(c);
   }
   MGHier *h = new MGHier();
+  for (const HostLevel &hl : H)
+    for (int d = 0; d < 3; ++d) h->hn.push_back(hl.hn[d]);
   h->nu1 = nu1;
   h->nu2 = nu2;
   h->coarse_sweeps = coarse_sweeps;
   const int nl = (int)H.size();
   h->L.resize(nl);
+  h->hcoef.resize((size_t)nl * 3 * 8);
   auto dalloc = [&](size_t n) -> float * {
     float *p = nullptr;
     if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) return nullptr;
@@ -552,6 +652,15 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
           pb[i] = (float)bw;
         }
       }
+      {
+        std::vector<float> pIf(s), pI2f(s);
+        for (int i = 0; i < s; ++i) {
+          std::memcpy(&pIf[i], &pI[i], 4);
+          std::memcpy(&pI2f[i], &pI2[i], 4);
+        }
+        const std::vector<float> *arr[8] = {&wm, &wp, &dg, &rf, &pa, &pb, &pIf, &pI2f};
+        for (int c = 0; c < 8; ++c) h->hcoef[(l * 3 + d) * 8 + c] = *arr[c];
+      }
       float *wmd = dalloc(s), *wpd = dalloc(s), *dgd = dalloc(s), *rfd = dalloc(s), *pad = dalloc(s), *pbd = dalloc(s);
       int *pId = nullptr, *pI2d = nullptr;
       float *tmp1 = dalloc(s), *tmp2 = dalloc(s);
@@ -608,6 +717,22 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       T.cs[q] = std::max(L.n0, std::max(L.n1, L.n2)) + 2;
       T.co[q] = off; off += 24 * T.cs[q];
     }
+    T.nfloat = off;
+    // initial LDS image: the coefficient blocks in place, zeros elsewhere
+    std::vector<float> img((size_t)std::max(off, 1), 0.f);
+    for (int q = 0; q < T.nt; ++q)
+      for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < 8; ++c) {
+          const std::vector<float> &a = h->hcoef[((h->tail0 + q) * 3 + d) * 8 + c];
+          std::copy(a.begin(), a.end(), img.begin() + T.co[q] + (c * 3 + d) * T.cs[q]);
+        }
+    h->tailimg = dalloc(img.size());
+    if (!h->tailimg) {
+      if (err) *err = "multigrid: hipMalloc failed";
+      mg_destroy(h);
+      return nullptr;
+    }
+    (void)hipMemcpyAsync(h->tailimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice, stream());
   }
   {
     float *p = dalloc((nl * sizeof(LevelDev) + sizeof(float) - 1) / sizeof(float));
@@ -626,6 +751,8 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
 void mg_destroy(MGHier *h) {
   if (!h) return;
   if (stream()) (void)hipStreamSynchronize(stream());
+  if (h->learn) unet_prec_destroy(h->learn);
+  if (h->gpart) (void)hipFree(h->gpart);
   for (float *p : h->arrays) (void)hipFree(p);
   delete h;
 }
@@ -641,13 +768,42 @@ int mg_level_cells(const MGHier *h, int l, int n[3]) {
 float *mg_f0(MGHier *h) { return h->L[0].f; }
 float *mg_x0(MGHier *h) { return h->L[0].x; }
 
+static unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
+int mg_set_learned(MGHier *h, moons_unet_s *net, int level, std::string *err) {
+  if (h->learn) unet_prec_destroy(h->learn);
+  h->learn = nullptr;
+  h->learn_level = -1;
+  if (!net) return MOONS_OK;
+  const int g = h->tail0 < (int)h->L.size() ? h->tail0 : (int)h->L.size() - 1;
+  if (level < 0 || level >= g) {
+    if (err) *err = "the learned level must be one of the grid-launch levels (above the LDS-resident coarse end)";
+    return MOONS_ERR_ARG;
+  }
+  const std::vector<double> hn[3] = {h->hn[3 * level], h->hn[3 * level + 1], h->hn[3 * level + 2]};
+  h->learn = unet_prec_create_nodes(net, hn, err);
+  if (!h->learn) return MOONS_ERR_ARG;
+  const LevelDev &L = h->L[level];
+  const size_t nb = blocks_for((long long)L.n0 * L.n1 * L.n2);
+  if (!h->gpart && hipMalloc(&h->gpart, 2 * 65536 * sizeof(float)) != hipSuccess) {
+    if (err) *err = "hipMalloc failed";
+    return MOONS_ERR_HIP;
+  }
+  if (nb > 65536) {
+    if (err) *err = "learned level too large";
+    return MOONS_ERR_ARG;
+  }
+  h->learn_level = level;
+  return MOONS_OK;
+}
+
 void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps) {
   h->nu1 = nu1;
   h->nu2 = nu2;
   h->coarse_sweeps = coarse_sweeps;
 }
 
-static unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
+
 
 static void smooth(const LevelDev &L, int color, bool zero_guess, const int *done) {
   const long long nt = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
@@ -660,9 +816,19 @@ int mg_vcycle(MGHier *h, const int *done) {
   const int g = tail ? h->tail0 : nl - 1; // levels [0, g) run as grid launches
   // descend
   for (int l = 0; l < g; ++l) {
-    const LevelDev &L = h->L[l];
+    LevelDev &L = h->L[l];
+    const bool learned = h->learn && l == h->learn_level;
+    if (learned) {
+      // learned initial guess x = c N(f), then smoothing from it
+      int st = unet_prec_apply(h->learn, L.f, L.x, done);
+      if (st) return st;
+      const int nc = L.n0 * L.n1 * L.n2;
+      const unsigned nb = blocks_for(nc);
+      hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
+      hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
+    }
     for (int s = 0; s < h->nu1; ++s) {
-      smooth(L, 0, s == 0, done);
+      smooth(L, 0, s == 0 && !learned, done);
       smooth(L, 1, false, done);
     }
     const LevelDev &C = h->L[l + 1];
@@ -670,13 +836,13 @@ int mg_vcycle(MGHier *h, const int *done) {
     hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, stream(), L, C, done);
   }
   if (tail)
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tail, h->tail0,
-                       h->nu1, h->nu2, h->coarse_sweeps, done);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tailimg, h->tail,
+                       h->tail0, h->nu1, h->nu2, h->coarse_sweeps, done);
   else
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
   // ascend
   for (int l = g - 1; l >= 0; --l) {
-    const LevelDev &L = h->L[l];
+    LevelDev &L = h->L[l];
     const LevelDev &C = h->L[l + 1];
     const long long nf = (long long)L.n0 * L.n1 * L.n2;
     hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(nf)), dim3(256), 0, stream(), L, C, done);

@@ -26,5 +26,8 @@ float *mg_x0(MGHier *h);
 // when *done (device) is non-zero.
 int mg_vcycle(MGHier *h, const int *done);
 void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
+// Learned initial guess on grid level `level` (net == nullptr removes it):
+// before that level's pre-smoothing, x = c N(f) with the Galerkin factor c.
+int mg_set_learned(MGHier *h, moons_unet_s *net, int level, std::string *err);
 
 } // namespace moons

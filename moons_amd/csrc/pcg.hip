@@ -525,6 +525,8 @@ struct moons_pcg_s {
   MGHier *mg = nullptr; // MOONS_PREC_MG_FP32 only
   UnetPrec *unet = nullptr;                   // MOONS_PREC_UNET: bound network
   bool jacobi_only = false;                   // set while a learned solve falls back to Jacobi
+  moons_unet_s *mg_net = nullptr;             // learned initial guess inside the V-cycle (MG solvers)
+  int mg_net_level = -1;
   double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
   float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
   int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 40;
@@ -670,6 +672,17 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
     return st;
   }
   *out = S;
+  return MOONS_OK;
+}
+
+int moons_pcg_set_mg_unet(moons_pcg_t S, moons_unet_t net, int level) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_set_mg_unet: null solver");
+  if (!S->mg) return fail(MOONS_ERR_STATE, "moons_pcg_set_mg_unet: solver was not created with MOONS_PREC_MG_FP32");
+  std::string err;
+  const int st = mg_set_learned(S->mg, net, level, &err);
+  if (st) return fail(st, "moons_pcg_set_mg_unet: " + err);
+  S->mg_net = net;
+  S->mg_net_level = level;
   return MOONS_OK;
 }
 
@@ -864,6 +877,11 @@ int moons_pcg_set_mg(moons_pcg_t S, int max_levels, int nu1, int nu2, int coarse
   std::string err;
   S->mg = mg_create(&S->xbc, max_levels, nu1, nu2, coarse_sweeps, &err);
   if (!S->mg) return fail(MOONS_ERR_HIP, "moons_pcg_set_mg: " + err);
+  if (S->mg_net && mg_set_learned(S->mg, S->mg_net, S->mg_net_level, &err)) {
+    S->mg_net = nullptr;
+    S->mg_net_level = -1;
+    return fail(MOONS_ERR_ARG, "moons_pcg_set_mg: the new hierarchy cannot hold the learned level: " + err);
+  }
   return MOONS_OK;
 }
 

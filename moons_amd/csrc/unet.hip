@@ -841,28 +841,33 @@ int unet_prec_apply(UnetPrec *P, const float *f0, float *z0, const int *done) {
 
 // Geometry of a cell-centred field's mesh for the network input:
 // vol^(2/3) and log(dh_d / vol^(1/3)), d = x, y, z, per padded cell.
-UnetPrec *unet_prec_create(moons_unet_s *net, const moons_field_s *x, std::string *err) {
+UnetPrec *unet_prec_create_nodes(moons_unet_s *net, const std::vector<double> hn[3], std::string *err) {
   UnetPrec *P = new UnetPrec();
   P->net = net;
-  for (int d = 0; d < 3; ++d) P->n[d] = x->s[d] - 2;
-  if (P->n[0] % 4 || P->n[1] % 4 || P->n[2] % 4) {
+  int s[3];
+  std::vector<double> dh[3];
+  for (int d = 0; d < 3; ++d) {
+    s[d] = (int)hn[d].size() - 1; // padded cells
+    P->n[d] = s[d] - 2;
+    dh[d].resize(s[d]);
+    for (int i = 0; i < s[d]; ++i) dh[d][i] = hn[d][i + 1] - hn[d][i];
+  }
+  if (P->n[0] % 4 || P->n[1] % 4 || P->n[2] % 4 || P->n[0] < 4 || P->n[1] < 4 || P->n[2] < 4) {
     if (err) *err = "the learned preconditioner needs cell counts that are multiples of 4";
     delete P;
     return nullptr;
   }
-  const int s0 = x->s[0], s1 = x->s[1], s2 = x->s[2];
-  const size_t np = (size_t)s0 * s1 * s2;
+  const size_t np = (size_t)s[0] * s[1] * s[2];
   std::vector<float> v23(np), geom(3 * np);
-  const HostCoords *C = x->m->hc;
-  for (int k = 0; k < s2; ++k)
-    for (int j = 0; j < s1; ++j)
-      for (int i = 0; i < s0; ++i) {
-        const size_t o = (size_t)i + (size_t)s0 * ((size_t)j + (size_t)s1 * k);
-        const double dh[3] = {C[0].dhn[i], C[1].dhn[j], C[2].dhn[k]};
-        const double vol = dh[0] * dh[1] * dh[2];
+  for (int k = 0; k < s[2]; ++k)
+    for (int j = 0; j < s[1]; ++j)
+      for (int i = 0; i < s[0]; ++i) {
+        const size_t o = (size_t)i + (size_t)s[0] * ((size_t)j + (size_t)s[1] * k);
+        const double w[3] = {dh[0][i], dh[1][j], dh[2][k]};
+        const double vol = w[0] * w[1] * w[2];
         const double h = std::cbrt(vol);
         v23[o] = (float)(h * h);
-        for (int d = 0; d < 3; ++d) geom[d * np + o] = (float)std::log(dh[d] / h);
+        for (int d = 0; d < 3; ++d) geom[d * np + o] = (float)std::log(w[d] / h);
       }
   bool ok = hipMalloc(&P->vol23, np * 4) == hipSuccess && hipMalloc(&P->geom, 3 * np * 4) == hipSuccess &&
             hipMalloc(&P->scale, 2 * 4) == hipSuccess &&
@@ -876,6 +881,11 @@ UnetPrec *unet_prec_create(moons_unet_s *net, const moons_field_s *x, std::strin
   (void)hipMemcpyAsync(P->geom, geom.data(), 3 * np * 4, hipMemcpyHostToDevice, stream());
   (void)hipStreamSynchronize(stream());
   return P;
+}
+
+UnetPrec *unet_prec_create(moons_unet_s *net, const moons_field_s *x, std::string *err) {
+  const std::vector<double> hn[3] = {x->m->hc[0].hn, x->m->hc[1].hn, x->m->hc[2].hn};
+  return unet_prec_create_nodes(net, hn, err);
 }
 
 void unet_prec_destroy(UnetPrec *P) {

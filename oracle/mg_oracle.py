@@ -115,7 +115,7 @@ class Level:
         return self.interior(f) - (self.offdiag(x) + self.diag() * self.interior(x))
 
 
-COARSE_CELLS = 8  # a level of at most this many cells per direction is the coarsest
+COARSE_CELLS = 4  # a level of at most this many cells per direction is the coarsest
 
 
 def build(hn3, bct, max_levels=16, coarse_cells=COARSE_CELLS):

@@ -23,6 +23,12 @@
  *   PCG (PPE)                      dense numpy solve of the assembled system on
  *                                   4^3-8^3; residual-drop to 1e-12; textbook PCG
  *                                   recurrence on the assembled matrix
+ *   interp, face2Edge, E = U x B,  exactness on linear fields (theta weights on
+ *   advect_B                        stretched grids); sign and pairing of the cross
+ *                                   product on axis-aligned fields; div(curl E) = 0
+ *   VF operators, VF PCG           symmetry and positivity of vol*A with the wall
+ *                                   factors; Jacobi diagonal = dense diagonal off the
+ *                                   walls; PCG = dense solve; lid BC through the RHS
  */
 #include "moons_oracle.h"
 
