@@ -105,6 +105,11 @@ typedef struct {
 /* No reference counterpart (the reference has no device): selects the GPU and
  * creates the library stream.  Must precede every other call. */
 int moons_init(int device);
+/* Extra HIP streams (default priority) for replaying captured steps of
+ * independent states concurrently; synchronize(NULL) waits for the library stream. */
+int moons_stream_create(void **out);
+int moons_stream_destroy(void *hip_stream);
+int moons_stream_synchronize(void *hip_stream);
 int moons_finalize(void);
 const char *moons_last_error(void);
 int moons_synchronize(void);
@@ -299,6 +304,16 @@ int moons_mhd_destroy(moons_mhd_t s);
 /* Advances n_steps time steps; info (may be NULL) gets the last step's solver
  * iterations and the device time of the call. */
 int moons_mhd_step(moons_mhd_t s, int n_steps, moons_mhd_info *info);
+/* Captures one time step into a HIP graph (no reference counterpart).  Inside
+ * the graph every PCG solve runs exactly its iter_max iterations, each
+ * iteration returning at once on the device after convergence, and nothing is
+ * read back to the host; the implicit-diffusion preconditioners (which
+ * depend on dt only) are built before the capture.  Not with the learned PPE
+ * solver. */
+int moons_mhd_graph_capture(moons_mhd_t s);
+/* Replays the captured step n_steps times on hip_stream (NULL: the library
+ * stream), asynchronously. */
+int moons_mhd_graph_launch(moons_mhd_t s, int n_steps, void *hip_stream);
 /* Handles of internal fields (MOONS_MHD_*), 3 or 6 entries, for inspection. */
 int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
 
@@ -336,6 +351,8 @@ int moons_mfma_check(const float *A, const float *B, float *D);
  * that level's pre-smoothing x = c N(f), c = (N, f) / (N, L N) (0 when not
  * positive).  net = NULL removes it.  No reference counterpart. */
 int moons_pcg_set_mg_unet(moons_pcg_t pcg, moons_unet_t net, int level);
+/* Enables (default) or disables the Jacobi redo of failed MOONS_PREC_UNET solves. */
+int moons_pcg_set_fallback(moons_pcg_t pcg, int enable);
 /* Binds the network to a MOONS_PREC_UNET solver (required before its first solve). */
 int moons_pcg_set_unet(moons_pcg_t pcg, moons_unet_t net);
 

@@ -10,6 +10,7 @@ from .core import (  # noqa: F401
     PCG_solver_SF,
     PCG_solver_VF,
     MHD_step,
+    Stream,
     UNet,
     mfma_check,
     SF,

@@ -90,6 +90,11 @@ SIGNATURES = {
     "moons_mhd_destroy": (C.c_int, [C.c_void_p]),
     "moons_mhd_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MhdInfo)]),
     "moons_mhd_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_mhd_graph_capture": (C.c_int, [C.c_void_p]),
+    "moons_mhd_graph_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "moons_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "moons_stream_destroy": (C.c_int, [C.c_void_p]),
+    "moons_stream_synchronize": (C.c_int, [C.c_void_p]),
     "moons_unet_create": (C.c_int, [C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "moons_unet_destroy": (C.c_int, [C.c_void_p]),
     "moons_unet_num_params": (C.c_size_t, [C.c_void_p]),
@@ -103,6 +108,7 @@ SIGNATURES = {
     "moons_mfma_check": (C.c_int, [c_float_p, c_float_p, c_float_p]),
     "moons_pcg_set_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_pcg_set_mg_unet": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "moons_pcg_set_fallback": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 _lib = None

@@ -83,11 +83,14 @@ def build_fortran_shim() -> str | None:
         o = os.path.join(FORTRAN_BUILD, os.path.basename(s).replace(".f90", ".o"))
         _run([FLANG, "-O2", "-fPIC", "-module-dir", FORTRAN_BUILD, "-c", s, "-o", o])
         objs.append(o)
-    drv = os.path.join(FORTRAN_DIR, "ppe_driver.f90")
-    exe = os.path.join(FORTRAN_BUILD, "ppe_driver")
-    if os.path.exists(drv):
-        _run([FLANG, "-O2", "-I", FORTRAN_BUILD, "-module-dir", FORTRAN_BUILD, drv, *objs,
-              "-L" + PKG, "-lmoons_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../..", "-o", exe])
+    exe = None
+    for name in ("ppe_driver", "vf_driver"):
+        drv = os.path.join(FORTRAN_DIR, name + ".f90")
+        out = os.path.join(FORTRAN_BUILD, name)
+        if os.path.exists(drv):
+            _run([FLANG, "-O2", "-I", FORTRAN_BUILD, "-module-dir", FORTRAN_BUILD, drv, *objs,
+                  "-L" + PKG, "-lmoons_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../..", "-o", out])
+            exe = exe or out
     return exe
 
 

@@ -260,6 +260,10 @@ class PCG_solver_SF:
         check(load().moons_pcg_set_mg_unet(self.handle, net.handle if net is not None else None, level))
         self._mg_net = net
 
+    def set_fallback(self, enable: bool) -> None:
+        """Jacobi redo of failed learned solves (on by default)."""
+        check(load().moons_pcg_set_fallback(self.handle, 1 if enable else 0))
+
     def set_UNet(self, net: "UNet") -> None:
         """Bind the learned preconditioner (prec="UNet")."""
         check(load().moons_pcg_set_unet(self.handle, net.handle))
@@ -377,6 +381,14 @@ class MHD_step:
         check(load().moons_mhd_step(self.handle, n, C.byref(info)))
         return {"iterations": dict(zip(("U", "p", "B", "phi"), list(info.iterations))), "ms": info.ms}
 
+    def capture(self) -> None:
+        """Capture one step into a HIP graph (solves run exactly their iter_max iterations)."""
+        check(load().moons_mhd_graph_capture(self.handle))
+
+    def launch(self, n: int = 1, stream=None) -> None:
+        """Replay the captured step n times on `stream` (a Stream, or the library stream)."""
+        check(load().moons_mhd_graph_launch(self.handle, n, stream.handle if stream is not None else None))
+
     def field(self, which: str):
         out = (C.c_void_p * 6)()
         check(load().moons_mhd_field(self.handle, self.FIELDS[which], out))
@@ -470,3 +482,25 @@ def mfma_check(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     d = np.zeros((16, 16), dtype=np.float32)
     check(load().moons_mfma_check(_fp(a), _fp(b), _fp(d)))
     return d
+
+
+class Stream:
+    """A HIP stream of the library (default priority)."""
+
+    def __init__(self):
+        if not _initialised:
+            init(0)
+        h = C.c_void_p()
+        check(load().moons_stream_create(C.byref(h)))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        check(load().moons_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_stream_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -16,6 +16,8 @@ namespace moons {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 hipStream_t stream();
+bool sync_free();
+void set_sync_free(bool on);
 
 #define MOONS_HIP(call)                                                                  \
   do {                                                                                   \

@@ -23,6 +23,13 @@ int fail(int code, const std::string &msg) {
 }
 hipStream_t stream() { return g_stream; }
 
+// Set while a caller captures a HIP graph: solvers then launch exactly
+// iter_max iterations (each returns at once after convergence, on the
+// device) and read nothing back to the host.
+static bool g_sync_free = false;
+bool sync_free() { return g_sync_free; }
+void set_sync_free(bool on) { g_sync_free = on; }
+
 // ------------------------------------------------------------------ 1D grids
 static void add_ghost_nodes(const std::vector<double> &h, double *out) {
   size_t np = h.size();
@@ -543,6 +550,26 @@ int moons_init(int device) {
   if (g_stream) MOONS_HIP(hipStreamDestroy(g_stream));
   MOONS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
+  return MOONS_OK;
+}
+
+int moons_stream_create(void **out) {
+  if (!out) return fail(MOONS_ERR_ARG, "moons_stream_create: null argument");
+  hipStream_t st = nullptr;
+  MOONS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *out = st;
+  return MOONS_OK;
+}
+
+int moons_stream_destroy(void *st) {
+  if (!st) return MOONS_OK;
+  MOONS_HIP(hipStreamSynchronize(static_cast<hipStream_t>(st)));
+  MOONS_HIP(hipStreamDestroy(static_cast<hipStream_t>(st)));
+  return MOONS_OK;
+}
+
+int moons_stream_synchronize(void *st) {
+  MOONS_HIP(hipStreamSynchronize(st ? static_cast<hipStream_t>(st) : g_stream));
   return MOONS_OK;
 }
 

@@ -525,26 +525,34 @@ struct moons_pcg_s {
   MGHier *mg = nullptr; // MOONS_PREC_MG_FP32 only
   UnetPrec *unet = nullptr;                   // MOONS_PREC_UNET: bound network
   bool jacobi_only = false;                   // set while a learned solve falls back to Jacobi
+  bool fallback = true;                       // learned solves redo failures with Jacobi
   moons_unet_s *mg_net = nullptr;             // learned initial guess inside the V-cycle (MG solvers)
   int mg_net_level = -1;
   double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
   float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
   int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 40;
   PcgCtrl *ctrl = nullptr;
-  PcgCtrl *host_ctrl = nullptr; // pinned
+  PcgCtrl *host_ctrl = nullptr; // pinned, read-backs
+  PcgCtrl *host_init = nullptr; // pinned, initial control block (the source of every upload)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+// The initial control block lives in its own pinned buffer, written only
+// here, so a captured copy of it replays the same values.
 static int pcg_upload_ctrl(moons_pcg_s *S) {
-  PcgCtrl &h = *S->host_ctrl;
+  PcgCtrl h;
   std::memset(&h, 0, sizeof(PcgCtrl));
   h.iter_max = S->iter_max;
   h.n_skip = S->n_skip;
   h.tol_rel = S->tol_rel;
   h.tol_abs = S->tol_abs;
   h.volsum = S->volsum;
-  MOONS_HIP(hipMemcpyAsync(S->ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, stream()));
-  MOONS_HIP(hipStreamSynchronize(stream()));
+  if (std::memcmp(&h, S->host_init, sizeof(PcgCtrl)) != 0) {
+    // a copy of the old contents may still be pending
+    if (!sync_free()) MOONS_HIP(hipStreamSynchronize(stream()));
+    *S->host_init = h;
+  }
+  MOONS_HIP(hipMemcpyAsync(S->ctrl, S->host_init, sizeof(PcgCtrl), hipMemcpyHostToDevice, stream()));
   return MOONS_OK;
 }
 
@@ -641,11 +649,13 @@ int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons
             hipMalloc(&S->partC, 2 * nb * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->ctrl, sizeof(PcgCtrl)) == hipSuccess &&
             hipHostMalloc(&S->host_ctrl, sizeof(PcgCtrl), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&S->host_init, sizeof(PcgCtrl), hipHostMallocDefault) == hipSuccess &&
             hipEventCreate(&S->ev0) == hipSuccess && hipEventCreate(&S->ev1) == hipSuccess;
   if (!ok) {
     moons_pcg_destroy(S);
     return fail(MOONS_ERR_HIP, "moons_pcg_sf_create: allocation failed");
   }
+  std::memset(S->host_init, 0xff, sizeof(PcgCtrl)); // differs from any real block: first upload writes it
   (void)hipMemsetAsync(S->r, 0, bytes, stream());
   (void)hipMemsetAsync(S->p, 0, bytes, stream());
   (void)hipMemsetAsync(S->q, 0, bytes, stream());
@@ -686,6 +696,12 @@ int moons_pcg_set_mg_unet(moons_pcg_t S, moons_unet_t net, int level) {
   return MOONS_OK;
 }
 
+int moons_pcg_set_fallback(moons_pcg_t S, int enable) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_set_fallback: null solver");
+  S->fallback = enable != 0;
+  return MOONS_OK;
+}
+
 int moons_pcg_set_unet(moons_pcg_t S, moons_unet_t net) {
   if (!S || !net) return fail(MOONS_ERR_ARG, "moons_pcg_set_unet: null argument");
   if (S->prec != MOONS_PREC_UNET) return fail(MOONS_ERR_STATE, "moons_pcg_set_unet: solver was not created with MOONS_PREC_UNET");
@@ -709,6 +725,7 @@ int moons_pcg_destroy(moons_pcg_t S) {
     if (b) (void)hipFree(b);
   if (S->ctrl) (void)hipFree(S->ctrl);
   if (S->host_ctrl) (void)hipHostFree(S->host_ctrl);
+  if (S->host_init) (void)hipHostFree(S->host_init);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   delete S;
@@ -750,6 +767,14 @@ static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
     MOONS_LAUNCH_CHECK();
     if ((st = pcg_launch(S, x->d, 0, 16))) return st;
   }
+  if (sync_free()) {
+    // exactly iter_max iterations, gated on the device; nothing read back
+    for (int it = 1; it <= S->iter_max; ++it) {
+      st = mg ? pcg_launch_iteration_mg(S, x->d, it) : pcg_launch_iteration(S, x->d, it);
+      if (st) return st;
+    }
+    return launch_apply_bcs(x, false);
+  }
   // Chunks of iterations; every kernel returns at once after the exit decision.
   int launched = 0;
   for (;;) {
@@ -784,12 +809,24 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
       return fail(MOONS_ERR_HIP, "moons_pcg_sf_solve: allocation failed");
     MOONS_HIP(hipMemcpyAsync(S->xbak, x->d, x->n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   }
+  if (sync_free()) {
+    if (learned) return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve: the learned solver cannot run inside a graph");
+    S->jacobi_only = false;
+    int st = pcg_solve_impl(S, x, b);
+    if (st) return st;
+    if (info) {
+      info->iterations = -1;
+      info->exit_loop = 0;
+      info->res_norm0 = info->res_norm = info->ms = 0.0;
+    }
+    return MOONS_OK;
+  }
   MOONS_HIP(hipEventRecord(S->ev0, stream()));
   S->jacobi_only = false;
   int st = pcg_solve_impl(S, x, b);
   if (st) return st;
   int fell_back = 0;
-  if (learned) {
+  if (learned && S->fallback) {
     // Jacobi-PCG is the fallback and the judge: a learned solve that breaks
     // down (NaN) or stops at iter_max without meeting a tolerance is redone
     // from the same x with the Jacobi preconditioner.

@@ -20,6 +20,7 @@
 // induction meshes are the same single block, so embedding and extraction
 // are identities.
 #include <cstring>
+#include <string>
 
 #include "common.hpp"
 
@@ -86,6 +87,8 @@ struct moons_mhd_s {
   moons_pcgvf_t pcg_U = nullptr, pcg_B = nullptr;
   int iters[4] = {0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -122,6 +125,8 @@ void mhd_free(moons_mhd_s *S) {
   if (S->pcg_B) moons_pcg_vf_destroy(S->pcg_B);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
   delete S;
 }
 
@@ -392,6 +397,44 @@ int moons_mhd_step(moons_mhd_t S, int n_steps, moons_mhd_info *info) {
     (void)hipEventElapsedTime(&ms, S->ev0, S->ev1);
     info->ms = ms;
   }
+  return MOONS_OK;
+}
+
+int moons_mhd_graph_capture(moons_mhd_t S) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_mhd_graph_capture: null state");
+  if (S->P.prec_p == MOONS_PREC_UNET) return fail(MOONS_ERR_STATE, "moons_mhd_graph_capture: not with the learned PPE solver");
+  int st;
+  // the implicit-diffusion preconditioners depend on dt only: built here, outside the graph
+  if (S->pcg_U && (st = moons_pcg_vf_update_mfp(S->pcg_U, S->P.dt * 1.0 * (-(1.0 / S->P.Re) * S->P.alpha_U), 1)))
+    return st;
+  if (S->pcg_B && (st = moons_pcg_vf_update_mfp(S->pcg_B, S->P.dt * 1.0 * (1.0 / S->P.Rem), 1))) return st;
+  MOONS_HIP(hipStreamSynchronize(stream()));
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  S->exec = nullptr;
+  S->graph = nullptr;
+  MOONS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+  set_sync_free(true);
+  st = momentum(S);
+  if (!st && S->P.induction) st = induction(S);
+  set_sync_free(false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream(), &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("moons_mhd_graph_capture: ") + hipGetErrorString(e));
+  S->graph = g;
+  MOONS_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+  return MOONS_OK;
+}
+
+int moons_mhd_graph_launch(moons_mhd_t S, int n_steps, void *hip_stream) {
+  if (!S || n_steps < 0) return fail(MOONS_ERR_ARG, "moons_mhd_graph_launch: bad argument");
+  if (!S->exec) return fail(MOONS_ERR_STATE, "moons_mhd_graph_launch: capture the step first");
+  hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : stream();
+  for (int n = 0; n < n_steps; ++n) MOONS_HIP(hipGraphLaunch(S->exec, st));
   return MOONS_OK;
 }
 

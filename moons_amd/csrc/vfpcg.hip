@@ -193,7 +193,8 @@ struct moons_pcgvf_s {
   double *part = nullptr;
   int nbmax = 0;
   int excl = 0;
-  VfCtrl *ctrl = nullptr, *host = nullptr;
+  VfCtrl *ctrl = nullptr, *host = nullptr, *host_init = nullptr; // pinned: read-backs, initial block
+  bool minv_ready = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -208,6 +209,7 @@ static void vf_free(moons_pcgvf_s *S) {
   if (S->part) (void)hipFree(S->part);
   if (S->ctrl) (void)hipFree(S->ctrl);
   if (S->host) (void)hipHostFree(S->host);
+  if (S->host_init) (void)hipHostFree(S->host_init);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   delete S;
@@ -406,12 +408,14 @@ int moons_pcg_vf_create(moons_mesh_t m, int op, moons_field_t x[3], moons_field_
   bool ok = hipMalloc(&S->part, 3 * (size_t)S->nbmax * sizeof(double)) == hipSuccess &&
             hipMalloc(&S->ctrl, sizeof(VfCtrl)) == hipSuccess &&
             hipHostMalloc(&S->host, sizeof(VfCtrl), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&S->host_init, sizeof(VfCtrl), hipHostMallocDefault) == hipSuccess &&
             hipEventCreate(&S->ev0) == hipSuccess && hipEventCreate(&S->ev1) == hipSuccess;
   if (!ok) {
     vf_free(S);
     return fail(MOONS_ERR_HIP, "moons_pcg_vf_create: allocation failed");
   }
   (void)hipMemsetAsync(S->ctrl, 0, sizeof(VfCtrl), stream());
+  std::memset(S->host_init, 0xff, sizeof(VfCtrl));
   *out = S;
   return MOONS_OK;
 }
@@ -434,8 +438,16 @@ int moons_pcg_vf_set_isp(moons_pcgvf_t S, int iter_max, double tol_rel, double t
 
 int moons_pcg_vf_update_mfp(moons_pcgvf_t S, double coeff_implicit_time_split, int update_preconditioner) {
   if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_vf_update_mfp: null solver");
+  const bool changed = S->c != coeff_implicit_time_split;
   S->c = coeff_implicit_time_split;
-  if (update_preconditioner) return vf_minv(S);
+  // Minv depends on c and the mesh only: rebuilding it for an unchanged c
+  // would reproduce the same array
+  if (update_preconditioner && (changed || !S->minv_ready)) {
+    if (sync_free()) return fail(MOONS_ERR_STATE, "moons_pcg_vf_update_mfp: a new coefficient cannot be set inside a graph");
+    const int st = vf_minv(S);
+    if (st) return st;
+    S->minv_ready = true;
+  }
   return MOONS_OK;
 }
 
@@ -444,14 +456,19 @@ int moons_pcg_vf_solve(moons_pcgvf_t S, moons_field_t x[3], moons_field_t b[3], 
   for (int d = 0; d < 3; ++d)
     if (!x[d] || !b[d] || x[d]->dl != (1 << d) || b[d]->dl != (1 << d) || x[d]->m != S->m || b[d]->m != S->m)
       return fail(MOONS_ERR_ARG, "moons_pcg_vf_solve: x and b must be face components on the solver's mesh");
-  VfCtrl &h = *S->host;
+  VfCtrl h;
   std::memset(&h, 0, sizeof(VfCtrl));
   h.iter_max = S->iter_max;
   h.n_skip = S->n_skip;
   h.tol_rel = S->tol_rel;
   h.tol_abs = S->tol_abs;
-  MOONS_HIP(hipMemcpyAsync(S->ctrl, &h, sizeof(VfCtrl), hipMemcpyHostToDevice, stream()));
-  MOONS_HIP(hipEventRecord(S->ev0, stream()));
+  if (std::memcmp(&h, S->host_init, sizeof(VfCtrl)) != 0) {
+    if (!sync_free()) MOONS_HIP(hipStreamSynchronize(stream()));
+    *S->host_init = h;
+  }
+  MOONS_HIP(hipMemcpyAsync(S->ctrl, S->host_init, sizeof(VfCtrl), hipMemcpyHostToDevice, stream()));
+  const bool nosync = sync_free();
+  if (!nosync) MOONS_HIP(hipEventRecord(S->ev0, stream()));
   // the solve's x carries its own BCs; keep a copy with x's BCs for the implicit application
   moons_field_s xs[3];
   moons_field_s *xp[3];
@@ -493,10 +510,14 @@ int moons_pcg_vf_solve(moons_pcgvf_t S, moons_field_t x[3], moons_field_t b[3], 
   {
     int launched = 0;
     for (;;) {
-      MOONS_HIP(hipMemcpyAsync(S->host, S->ctrl, sizeof(VfCtrl), hipMemcpyDeviceToHost, stream()));
-      MOONS_HIP(hipStreamSynchronize(stream()));
-      if (S->host->done || launched >= S->iter_max) break;
-      int chunk = std::min(std::max(2, std::min(16, launched / 4)), S->iter_max - launched);
+      if (nosync) {
+        if (launched >= S->iter_max) break;
+      } else {
+        MOONS_HIP(hipMemcpyAsync(S->host, S->ctrl, sizeof(VfCtrl), hipMemcpyDeviceToHost, stream()));
+        MOONS_HIP(hipStreamSynchronize(stream()));
+        if (S->host->done || launched >= S->iter_max) break;
+      }
+      int chunk = nosync ? S->iter_max : std::min(std::max(2, std::min(16, launched / 4)), S->iter_max - launched);
       for (int c = 0; c < chunk; ++c) {
         const int it = launched + c + 1;
         if ((st = vf_op(S, S->Ax, S->p, true))) goto out;
@@ -518,6 +539,14 @@ int moons_pcg_vf_solve(moons_pcgvf_t S, moons_field_t x[3], moons_field_t b[3], 
   // apply_BCs(x) after the last update (PCG_solver_algorithm.f90:149,170)
   for (int d = 0; d < 3; ++d)
     if ((st = launch_apply_bcs(x[d], false))) return st;
+  if (nosync) {
+    if (info) {
+      info->iterations = -1;
+      info->exit_loop = 0;
+      info->res_norm0 = info->res_norm = info->ms = 0.0;
+    }
+    return MOONS_OK;
+  }
   MOONS_HIP(hipEventRecord(S->ev1, stream()));
   MOONS_HIP(hipMemcpyAsync(S->host, S->ctrl, sizeof(VfCtrl), hipMemcpyDeviceToHost, stream()));
   MOONS_HIP(hipStreamSynchronize(stream()));

@@ -15,8 +15,8 @@ module moons_amd_PCG_mod
   implicit none
   private
 
-  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu
-  public :: init, solve, delete, clean_div
+  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu
+  public :: init, solve, delete, clean_div, update_MFP
   public :: init_CC, init_Face, set_BCs
 
   type mesh_gpu
@@ -38,10 +38,20 @@ module moons_amd_PCG_mod
     type(SF_gpu) :: x, b ! device mirrors of the solve's x and b
   end type
 
-  interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu; end interface
-  interface solve;  module procedure solve_PCG_SF_gpu; end interface
+  ! init_PCG_VF / solve_PCG_VF / update_MFP_PCG_VF over three face arrays
+  ! (code/handwritten/PCG/PCG_solver_extend.f90:117-180,195-206,281-291)
+  type PCG_solver_VF_gpu
+    type(c_ptr) :: h = c_null_ptr
+    type(moons_pcg_info) :: info
+    type(SF_gpu) :: x(3), b(3), k(3)
+  end type
+
+  interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu
+                    module procedure init_PCG_VF_gpu; end interface
+  interface solve;  module procedure solve_PCG_SF_gpu; module procedure solve_PCG_VF_gpu; end interface
+  interface update_MFP; module procedure update_MFP_PCG_VF_gpu; end interface
   interface delete; module procedure delete_mesh_gpu; module procedure delete_SF_gpu;
-                    module procedure delete_PCG_SF_gpu; end interface
+                    module procedure delete_PCG_SF_gpu; module procedure delete_PCG_VF_gpu; end interface
 
 contains
 
@@ -174,6 +184,95 @@ contains
       call delete(X(d)); call delete(Xs(d)); call delete(T(d))
     enddo
     call delete(tcc)
+  end subroutine
+
+  ! init_PCG_VF: op = MOONS_OP_MOM_DIFFUSION (prec_mom_VF) or
+  ! MOONS_OP_IND_DIFFUSION (prec_ind_VF, with sigmaInv on the x, y, z edges).
+  ! X_types(6, 3), X_vals(6, 3): the BCs of the three face components.
+  subroutine init_PCG_VF_gpu(PCG, m, op, X_types, X_vals, iter_max, tol_rel, tol_abs, n_skip_check_res, &
+                             sig_x, sig_y, sig_z)
+    type(PCG_solver_VF_gpu), intent(inout) :: PCG
+    type(mesh_gpu), intent(in) :: m
+    integer(c_int), intent(in) :: op
+    integer(c_int), intent(in) :: X_types(6, 3)
+    real(c_double), intent(in) :: X_vals(6, 3)
+    integer, intent(in) :: iter_max, n_skip_check_res
+    real(c_double), intent(in) :: tol_rel, tol_abs
+    real(c_double), intent(in), optional, contiguous :: sig_x(:, :, :), sig_y(:, :, :), sig_z(:, :, :)
+    type(c_ptr) :: hx(3), hk(3)
+    integer(c_int) :: edl(3)
+    integer :: d
+    edl = [MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z]
+    do d = 1, 3
+      call init_Face(PCG%x(d), m, d)
+      call init_Face(PCG%b(d), m, d)
+      call set_BCs(PCG%x(d), X_types(:, d), X_vals(:, d), .false.)
+      hx(d) = PCG%x(d)%h
+      hk(d) = c_null_ptr
+    enddo
+    if (op .eq. MOONS_OP_IND_DIFFUSION) then
+      if (.not. (present(sig_x) .and. present(sig_y) .and. present(sig_z))) &
+        error stop 'Error: ind_diffusion needs sigmaInv on the edges in init_PCG_VF_gpu'
+      do d = 1, 3
+        call init_field(PCG%k(d), m, edl(d))
+        hk(d) = PCG%k(d)%h
+      enddo
+      call moons_check(moons_field_upload(PCG%k(1)%h, sig_x, int(size(sig_x), c_size_t)), 'upload sigmaInv')
+      call moons_check(moons_field_upload(PCG%k(2)%h, sig_y, int(size(sig_y), c_size_t)), 'upload sigmaInv')
+      call moons_check(moons_field_upload(PCG%k(3)%h, sig_z, int(size(sig_z), c_size_t)), 'upload sigmaInv')
+    endif
+    call moons_check(moons_pcg_vf_create(m%h, op, hx, hk, PCG%h), 'init_PCG_VF')
+    call moons_check(moons_pcg_vf_set_isp(PCG%h, int(iter_max, c_int), tol_rel, tol_abs, &
+                                          int(n_skip_check_res, c_int)), 'ISP')
+  end subroutine
+
+  subroutine update_MFP_PCG_VF_gpu(PCG, coeff_implicit_time_split, update_preconditioner)
+    type(PCG_solver_VF_gpu), intent(inout) :: PCG
+    real(c_double), intent(in) :: coeff_implicit_time_split
+    logical, intent(in) :: update_preconditioner
+    integer(c_int) :: up
+    up = 0
+    if (update_preconditioner) up = 1
+    call moons_check(moons_pcg_vf_update_mfp(PCG%h, coeff_implicit_time_split, up), 'update_MFP_PCG_VF')
+  end subroutine
+
+  ! solve_PCG_VF on host face arrays (ghosts included), x in/out
+  subroutine solve_PCG_VF_gpu(PCG, x1, x2, x3, b1, b2, b3, compute_norms)
+    type(PCG_solver_VF_gpu), intent(inout) :: PCG
+    real(c_double), intent(inout), contiguous :: x1(:, :, :), x2(:, :, :), x3(:, :, :)
+    real(c_double), intent(in), contiguous :: b1(:, :, :), b2(:, :, :), b3(:, :, :)
+    logical, intent(in) :: compute_norms
+    type(c_ptr) :: hx(3), hb(3)
+    integer :: d
+    call moons_check(moons_field_upload(PCG%x(1)%h, x1, int(size(x1), c_size_t)), 'upload x')
+    call moons_check(moons_field_upload(PCG%x(2)%h, x2, int(size(x2), c_size_t)), 'upload x')
+    call moons_check(moons_field_upload(PCG%x(3)%h, x3, int(size(x3), c_size_t)), 'upload x')
+    call moons_check(moons_field_upload(PCG%b(1)%h, b1, int(size(b1), c_size_t)), 'upload b')
+    call moons_check(moons_field_upload(PCG%b(2)%h, b2, int(size(b2), c_size_t)), 'upload b')
+    call moons_check(moons_field_upload(PCG%b(3)%h, b3, int(size(b3), c_size_t)), 'upload b')
+    do d = 1, 3
+      hx(d) = PCG%x(d)%h
+      hb(d) = PCG%b(d)%h
+    enddo
+    call moons_check(moons_pcg_vf_solve(PCG%h, hx, hb, PCG%info), 'solve_PCG_VF')
+    call moons_check(moons_field_download(PCG%x(1)%h, x1, int(size(x1), c_size_t)), 'download x')
+    call moons_check(moons_field_download(PCG%x(2)%h, x2, int(size(x2), c_size_t)), 'download x')
+    call moons_check(moons_field_download(PCG%x(3)%h, x3, int(size(x3), c_size_t)), 'download x')
+    if (compute_norms) then
+      write(*, '(a,i8,a,es12.4)') ' PCG_VF iterations = ', PCG%info%iterations, &
+        '   rel. residual = ', PCG%info%res_norm / PCG%info%res_norm0
+    endif
+  end subroutine
+
+  subroutine delete_PCG_VF_gpu(PCG)
+    type(PCG_solver_VF_gpu), intent(inout) :: PCG
+    integer :: d
+    if (c_associated(PCG%h)) call moons_check(moons_pcg_vf_destroy(PCG%h), 'delete PCG_VF')
+    PCG%h = c_null_ptr
+    do d = 1, 3
+      call delete(PCG%x(d)); call delete(PCG%b(d))
+      if (c_associated(PCG%k(d)%h)) call delete(PCG%k(d))
+    enddo
   end subroutine
 
   subroutine delete_mesh_gpu(m)

@@ -14,6 +14,26 @@ module moons_amd_bindings
   integer(c_int), parameter, public :: MOONS_BC_PERIODIC = 3
   integer(c_int), parameter, public :: MOONS_OP_LAP_UNIFORM_SF = 1
   integer(c_int), parameter, public :: MOONS_PREC_LAP_SF = 1, MOONS_PREC_MG_FP32 = 2
+  integer(c_int), parameter, public :: MOONS_OP_MOM_DIFFUSION = 1, MOONS_OP_IND_DIFFUSION = 2
+  integer(c_int), parameter, public :: MOONS_DL_EDGE_X = 6, MOONS_DL_EDGE_Y = 5, MOONS_DL_EDGE_Z = 3
+
+  ! moons_mhd_params / moons_mhd_info of include/moons_amd.h
+  type, bind(C), public :: moons_mhd_params
+    real(c_double) :: Re, Ha, Rem, dt
+    real(c_double) :: alpha_U
+    integer(c_int) :: induction
+    integer(c_int) :: prec_p
+    real(c_double) :: B0(3)
+    integer(c_int) :: iter_max(4)
+    real(c_double) :: tol_rel(4)
+    real(c_double) :: tol_abs(4)
+    integer(c_int) :: n_skip(4)
+  end type
+
+  type, bind(C), public :: moons_mhd_info
+    integer(c_int) :: iterations(4)
+    real(c_double) :: ms
+  end type
 
   type, bind(C), public :: moons_pcg_info
     integer(c_int) :: iterations
@@ -32,6 +52,10 @@ module moons_amd_bindings
   public :: moons_apply_bcs, moons_grad, moons_div
   public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
   public :: moons_clean_div
+  public :: moons_field_copy, moons_apply_bcs_implicit, moons_curl, moons_interp, moons_advect_B
+  public :: moons_pcg_vf_create, moons_pcg_vf_destroy, moons_pcg_vf_set_isp, moons_pcg_vf_update_mfp
+  public :: moons_pcg_vf_solve
+  public :: moons_mhd_create, moons_mhd_destroy, moons_mhd_step, moons_mhd_graph_capture, moons_mhd_graph_launch
 
   interface
     integer(c_int) function moons_init(device) bind(C, name="moons_init")
@@ -143,6 +167,84 @@ module moons_amd_bindings
       type(c_ptr), intent(in) :: X(3), Xstar(3), temp_F(3)
       real(c_double), value :: scale
       type(moons_pcg_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_field_copy(dst, src) bind(C, name="moons_field_copy")
+      import c_int, c_ptr
+      type(c_ptr), value :: dst, src
+    end function
+    integer(c_int) function moons_apply_bcs_implicit(f) bind(C, name="moons_apply_bcs_implicit")
+      import c_int, c_ptr
+      type(c_ptr), value :: f
+    end function
+    integer(c_int) function moons_curl(c, u, v, w, dir) bind(C, name="moons_curl")
+      import c_int, c_ptr
+      type(c_ptr), value :: c, u, v, w
+      integer(c_int), value :: dir
+    end function
+    integer(c_int) function moons_interp(out, f, dir) bind(C, name="moons_interp")
+      import c_int, c_ptr
+      type(c_ptr), value :: out, f
+      integer(c_int), value :: dir
+    end function
+    integer(c_int) function moons_advect_B(adv, U_E, B, temp_E_TF, temp_E) bind(C, name="moons_advect_B")
+      import c_int, c_ptr
+      type(c_ptr), intent(in) :: adv(3), U_E(6), B(3), temp_E_TF(6), temp_E(3)
+    end function
+    integer(c_int) function moons_pcg_vf_create(m, op, x, k, pcg) bind(C, name="moons_pcg_vf_create")
+      import c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: op
+      type(c_ptr), intent(in) :: x(3), k(3)
+      type(c_ptr), intent(out) :: pcg
+    end function
+    integer(c_int) function moons_pcg_vf_destroy(pcg) bind(C, name="moons_pcg_vf_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: pcg
+    end function
+    integer(c_int) function moons_pcg_vf_set_isp(pcg, iter_max, tol_rel, tol_abs, n_skip) &
+        bind(C, name="moons_pcg_vf_set_isp")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: pcg
+      integer(c_int), value :: iter_max, n_skip
+      real(c_double), value :: tol_rel, tol_abs
+    end function
+    integer(c_int) function moons_pcg_vf_update_mfp(pcg, coeff, update_prec) bind(C, name="moons_pcg_vf_update_mfp")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: pcg
+      real(c_double), value :: coeff
+      integer(c_int), value :: update_prec
+    end function
+    integer(c_int) function moons_pcg_vf_solve(pcg, x, b, info) bind(C, name="moons_pcg_vf_solve")
+      import c_int, c_ptr, moons_pcg_info
+      type(c_ptr), value :: pcg
+      type(c_ptr), intent(in) :: x(3), b(3)
+      type(moons_pcg_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_mhd_create(m, params, U, p, B, phi, s) bind(C, name="moons_mhd_create")
+      import c_int, c_ptr, moons_mhd_params
+      type(c_ptr), value :: m, p, phi
+      type(moons_mhd_params), intent(in) :: params
+      type(c_ptr), intent(in) :: U(3), B(3)
+      type(c_ptr), intent(out) :: s
+    end function
+    integer(c_int) function moons_mhd_destroy(s) bind(C, name="moons_mhd_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: s
+    end function
+    integer(c_int) function moons_mhd_step(s, n, info) bind(C, name="moons_mhd_step")
+      import c_int, c_ptr, moons_mhd_info
+      type(c_ptr), value :: s
+      integer(c_int), value :: n
+      type(moons_mhd_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_mhd_graph_capture(s) bind(C, name="moons_mhd_graph_capture")
+      import c_int, c_ptr
+      type(c_ptr), value :: s
+    end function
+    integer(c_int) function moons_mhd_graph_launch(s, n, stream) bind(C, name="moons_mhd_graph_launch")
+      import c_int, c_ptr
+      type(c_ptr), value :: s, stream
+      integer(c_int), value :: n
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import c_size_t, c_ptr

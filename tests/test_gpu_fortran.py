@@ -40,3 +40,50 @@ def test_fortran_driver_ppe(tmp_path, prec, tol, bar):
         xri = xri - (xri * vol).sum() / vol.sum()
     err = np.linalg.norm(xi - xri) / np.linalg.norm(xri)
     assert err < bar, err
+
+
+VF_DRIVER = os.path.join(ROOT, "moons_amd", "fortran", "_build", "vf_driver")
+
+
+@pytest.mark.parametrize("op,c", [(1, -0.02), (2, 0.03)])
+def test_fortran_driver_vf(tmp_path, op, c):
+    """init_PCG_VF / update_MFP / solve from Fortran host arrays: momentum with
+    the lid-driven-cavity BCs, induction with pseudo-vacuum BCs and sigmaInv."""
+    if not os.path.exists(VF_DRIVER):
+        pytest.fail("Fortran VF driver not built (run __graft_entry__.build())")
+    N, beta = 12, 1.1
+    hx = O.uniform_nodes(0.0, 1.0, N)
+    hy = O.roberts_both_nodes(0.0, 1.0, N, beta)
+    g = O.Grid(hx, hy, hy)
+    rng = np.random.default_rng(op)
+    x0 = [np.asfortranarray(0.1 * rng.standard_normal(g.shape(1 << d))) for d in range(3)]
+    b = [np.asfortranarray(rng.standard_normal(g.shape(1 << d))) for d in range(3)]
+    paths = {k: tmp_path / f"{k}.bin" for k in ("x0", "b", "x", "sig")}
+    np.concatenate([a.ravel(order="F") for a in x0]).tofile(paths["x0"])
+    np.concatenate([a.ravel(order="F") for a in b]).tofile(paths["b"])
+    if op == 1:
+        bcs = [O.BCs.make(["Dirichlet"] * 6, [0, 0, 0, 1.0, 0, 0]), O.BCs.make(["Dirichlet"] * 6),
+               O.BCs.make(["Dirichlet"] * 6)]
+        k = None
+    else:
+        bcs = [O.BCs.make(["Neumann", "Neumann"] + ["Dirichlet"] * 4),
+               O.BCs.make(["Dirichlet"] * 2 + ["Neumann", "Neumann"] + ["Dirichlet"] * 2),
+               O.BCs.make(["Dirichlet"] * 4 + ["Neumann", "Neumann"])]
+        k = [np.asfortranarray(1.0 + 0.5 * rng.random(g.shape(dl))) for dl in (6, 5, 3)]
+        np.concatenate([a.ravel(order="F") for a in k]).tofile(paths["sig"])
+    args = [VF_DRIVER, str(N), str(beta), str(op), repr(c), "400", "1e-10", str(paths["x0"]), str(paths["b"]),
+            str(paths["x"])] + ([str(paths["sig"])] if op == 2 else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "PCG_VF iterations" in out.stdout
+    flat = np.fromfile(paths["x"], dtype=np.float64)
+    xs, off = [], 0
+    for d in range(3):
+        n = int(np.prod(g.shape(1 << d)))
+        xs.append(flat[off:off + n].reshape(g.shape(1 << d), order="F"))
+        off += n
+    xr = [a.copy(order="F") for a in x0]
+    O.pcg_vf(g, op, c, bcs, xr, b, 400, 1e-10, 1e-13, 1, k=k)
+    num = sum(float(np.sum((p - q) ** 2)) for p, q in zip(xs, xr))
+    den = sum(float(np.sum(q ** 2)) for q in xr)
+    assert np.sqrt(num / den) <= 1e-10
