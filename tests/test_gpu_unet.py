@@ -1,9 +1,9 @@
 """The learned preconditioner on the GPU: the MFMA lane layout, the forward
 pass and every parameter gradient against the NumPy restatement (BF16 values
 rounded alike, so the two differ only by FP32-vs-float64 summation and the
-rare BF16 rounding flip that causes), training lowers the loss, and the
-trained network inside the flexible PCG beats the same method without it
-(untrained network) and the Jacobi PCG."""
+rare BF16 rounding flip that causes), training lowers the loss, and an
+ablation of the trained network against an untrained one, with and without a
+V-cycle around it."""
 import numpy as np
 import pytest
 
