@@ -164,6 +164,14 @@ def main():
         per = {"pcg_matvec": 16, "pcg_update": 48, "pcg_pupdate": 24}
         print(json.dumps({n: {"us": round(1e3 * t, 2), "GBs": round(per[n] * N ** 3 / (t * 1e-3) / 1e9, 1)}
                           for n, t in zip(names, kms)}), flush=True)
+        # the multigrid kernels of the mixed solver, 50 launches each
+        xk = M.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
+        Pk = M.PCG_solver_SF(m, xk, M.iter_solver_params(2, 0.0, 0.0, 1), prec="MG_FP32")
+        Pk.solve(xk, b)
+        mk = Pk.time_MG(50)
+        print(json.dumps({"mg_smooth_us": round(1e3 * mk[0], 2), "mg_restrict_us": round(1e3 * mk[1], 2),
+                          "mg_prolong_us": round(1e3 * mk[2], 2), "mg_tail_us": round(1e3 * mk[3], 2),
+                          "vcycle_us": round(1e3 * mk[4], 2)}), flush=True)
         # calibration for the PMC byte counters: 20 device-to-device copies of
         # one 130^3 FP64 field (17,576,000 bytes read and written per copy)
         c = M.SF(m, "CC")
@@ -245,12 +253,26 @@ def main():
     dname = names[dom]
     achieved = alg[dname] / (kms[dom] * 1e-3) / 1e9
     iter_us = 1e3 * sum(kms)
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(f"{dname}_128_traffic.txt"),
-                "per_kernel_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
-                "per_kernel_GBs": {n: round(alg[n] / (t * 1e-3) / 1e9, 1) for n, t in zip(names, kms)},
-                "iteration_GBs": round(88 * ncell / (iter_us * 1e-6) / 1e9, 1)}
+    fp64_roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(f"{dname}_128_traffic.txt"),
+                     "per_kernel_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
+                     "per_kernel_GBs": {n: round(alg[n] / (t * 1e-3) / 1e9, 1) for n, t in zip(names, kms)},
+                     "iteration_GBs": round(88 * ncell / (iter_us * 1e-6) / 1e9, 1)}
+    # the mixed solver's dominant kernel: the finest-level red-black
+    # half-sweep (2 (nu1 + nu2) launches per V-cycle), 6 B per cell: the
+    # other colour's x (2 B), this colour's f (2 B) and x written (2 B), FP32
+    mk = Pm.time_MG(50)
+    sm_bytes = 6 * ncell
+    sm_gbs = sm_bytes / (mk[0] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "mg_smooth (128^3 half-sweep)", "achieved": round(sm_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sm_gbs / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic("mg_smooth_128_traffic.txt"),
+                "algorithmic_bytes": sm_bytes,
+                "per_kernel_us": {"mg_smooth_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
+                                  "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2)},
+                "vcycle_us": round(1e3 * mk[4], 2),
+                "mixed_iteration_us": round(1e3 * mg_ms / max(mg_info.iterations, 1), 1)}
 
     # PCIe-inclusive rate of the host-buffer boundary (upload b, solve, download x)
     t2 = time.perf_counter()
@@ -300,6 +322,7 @@ def main():
         "learned_mixed": learned_out,
         "pcie_inclusive_ms": round(pcie_ms, 3),
         "roofline": roofline,
+        "plain_fp64_roofline": fp64_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -233,6 +233,14 @@ int moons_pcg_apply_prec(moons_pcg_t pcg, moons_field_t r, moons_field_t z);
  * receives the per-kernel averages.  No reference counterpart (profiling). */
 int moons_pcg_time_kernels(moons_pcg_t pcg, int reps, double ms[3]);
 
+/* Times the multigrid kernels of a MOONS_PREC_MG_FP32 solver with HIP events
+ * on the library stream, reps launches back to back each: ms[0] one
+ * half-sweep (one colour) of the finest level, ms[1] restriction of its
+ * residual, ms[2] prolongation into it, ms[3] the coarse end, ms[4] a whole
+ * V-cycle.  Overwrites the multigrid arrays.  No reference counterpart
+ * (profiling). */
+int moons_pcg_time_mg(moons_pcg_t pcg, int reps, double ms[5]);
+
 /* --------------------------------------------- vector (VF) Jacobi-PCG */
 /* init_PCG_VF (code/handwritten/PCG/PCG_solver_extend.f90:117-180) for
  * op = MOONS_OP_MOM_DIFFUSION with prec_mom_VF, or MOONS_OP_IND_DIFFUSION

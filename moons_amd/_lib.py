@@ -76,6 +76,7 @@ SIGNATURES = {
     "moons_clean_div": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p, C.c_double,
                                   C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(PcgInfo)]),
     "moons_pcg_time_kernels": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "moons_pcg_time_mg": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "moons_pcg_set_mg": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "moons_pcg_mg_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "moons_pcg_apply_prec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

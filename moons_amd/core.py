@@ -286,6 +286,13 @@ class PCG_solver_SF:
         check(load().moons_pcg_time_kernels(self.handle, reps, ms))
         return [ms[0], ms[1], ms[2]]
 
+    def time_MG(self, reps: int = 20):
+        """ms per launch of the finest half-sweep, restriction, prolongation,
+        coarse end, and a whole V-cycle (multigrid solvers only)."""
+        ms = (C.c_double * 5)()
+        check(load().moons_pcg_time_mg(self.handle, reps, ms))
+        return list(ms)
+
     def __del__(self):
         try:
             if getattr(self, "handle", None):

@@ -27,7 +27,8 @@
 namespace moons {
 
 struct LevelDev {
-  int n0, n1, n2, s0, s1;
+  int n0, n1, n2;
+  int h0, half;                // cells per colour and row ((n0+1)/2), per colour (h0*n1*n2)
   int per0, per1, per2;
   float *x, *f;
   const float *wm[3], *wp[3], *dg[3];
@@ -66,12 +67,32 @@ struct MGHier {
   std::vector<std::vector<double>> hn; // node arrays (ghosts included) of every level, 3 per level
   UnetPrec *learn = nullptr;   // learned initial guess of one grid level (or none)
   int learn_level = -1;
+  float *lpf = nullptr, *lpx = nullptr; // its padded (n+2)^3 input and output images
   float *gpart = nullptr;      // its Galerkin partial sums
 };
 
 
-__device__ __forceinline__ size_t lidx(const LevelDev &L, int i, int j, int k) {
-  return (size_t)i + (size_t)L.s0 * ((size_t)j + (size_t)L.s1 * (size_t)k);
+// Colour-split storage of a level's x and f: the cells of colour
+// c = (i+j+k) % 2 fill the c-th half of the array, each half x-fastest over
+// the pair index (i-1)/2, so that a red-black half-sweep reads and writes
+// whole cache lines of one colour.  No ghost cells: the smoother and the
+// residual fold the BCs into their weights.  mg_create keeps every level
+// below 2^31 cells, so 32-bit arithmetic is exact.
+__device__ __forceinline__ int lidx(const LevelDev &L, int i, int j, int k) {
+  return ((i + j + k) & 1) * L.half + ((i - 1) >> 1) + L.h0 * ((j - 1) + L.n1 * (k - 1));
+}
+
+// Per-level grid kernels run on tiles of blockDim.x (x) by blockDim.y (y)
+// threads, each thread marching kz planes along z.  The linear tile order (x
+// fastest) is remapped so that each XCD gets a contiguous range of z-slabs,
+// which keeps the z-neighbour planes of a tile in that XCD's L2.
+__device__ __forceinline__ void tile_origin(int kz, int &i0, int &j, int &k0) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int b = xcd_remap(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  const int r = b / gx;
+  i0 = (b - r * gx) * blockDim.x + threadIdx.x;
+  j = 1 + (r % gy) * blockDim.y + threadIdx.y;
+  k0 = 1 + (r / gy) * kz;
 }
 
 // Gauss-Seidel update of one cell: x = (f - sum_nb w x_nb) / diag.
@@ -123,13 +144,66 @@ __device__ __forceinline__ bool color_cell(const LevelDev &L, long long t, int c
 }
 
 
-__global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int zero_guess, const int *done) {
+// One half-sweep: the cells of one colour, (i+j+k) % 2 == color.  Thread x
+// of the tile is the pair index p of row (j, k), i.e. the cell of this colour
+// among (2p+1, 2p+2).  The arithmetic is gs_cell's; the neighbours are cells
+// of the other colour (a periodic wrap keeps that, every count along a
+// periodic direction being even), except a wall's clamp to the cell itself.
+__global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int zero_guess, int kz, const int *done) {
   if (done && *done) return;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const long long t = (long long)b * blockDim.x + threadIdx.x;
-  int i, j, k;
-  if (!color_cell(L, t, color, i, j, k)) return;
-  L.x[lidx(L, i, j, k)] = gs_cell(L, L.x, i, j, k, zero_guess != 0);
+  int p, j, k0;
+  tile_origin(kz, p, j, k0);
+  if (p >= L.h0 || j > L.n1) return;
+  float *__restrict__ xc = L.x + color * L.half;
+  const float *__restrict__ xo = L.x + (1 - color) * L.half;
+  const float *__restrict__ fc = L.f + color * L.half;
+  const int h0 = L.h0, n1 = L.n1;
+  const bool jlo = j == 1 && !L.per1, jhi = j == n1 && !L.per1;
+  const int jm = (j == 1) ? n1 : j - 1, jp = (j == n1) ? 1 : j + 1;
+  const float wmj = L.wm[1][j], wpj = L.wp[1][j], dgj = L.dg[1][j];
+  const int kend = min(k0 + kz, L.n2 + 1);
+  for (int k = k0; k < kend; ++k) {
+    const int i = 2 * p + 1 + ((color + j + k + 1) & 1);
+    if (i > L.n0) continue;
+    const int row = h0 * ((j - 1) + n1 * (k - 1));
+    const int c = row + p;
+    const float d = L.dg[0][i] + dgj + L.dg[2][k];
+    const float fv = fc[c];
+    if (zero_guess) {
+      xc[c] = fv / d;
+      continue;
+    }
+    const float xim = (i == 1) ? (L.per0 ? xo[row + h0 - 1] : xc[c]) : xo[row + ((i - 2) >> 1)];
+    const float xip = (i == L.n0) ? (L.per0 ? xo[row] : xc[c]) : xo[row + (i >> 1)];
+    const float xjm = jlo ? xc[c] : xo[h0 * ((jm - 1) + n1 * (k - 1)) + p];
+    const float xjp = jhi ? xc[c] : xo[h0 * ((jp - 1) + n1 * (k - 1)) + p];
+    const int km = (k == 1) ? L.n2 : k - 1, kp = (k == L.n2) ? 1 : k + 1;
+    const float xkm = (k == 1 && !L.per2) ? xc[c] : xo[h0 * ((j - 1) + n1 * (km - 1)) + p];
+    const float xkp = (k == L.n2 && !L.per2) ? xc[c] : xo[h0 * ((j - 1) + n1 * (kp - 1)) + p];
+    float s = L.wm[0][i] * xim + L.wp[0][i] * xip;
+    s += wmj * xjm + wpj * xjp;
+    s += L.wm[2][k] * xkm + L.wp[2][k] * xkp;
+    xc[c] = (fv - s) / d;
+  }
+}
+
+// Level arrays <-> a padded (n+2)^3 FP32 image (the learned initial guess
+// reads and writes that layout); ghosts of the image are left as they are.
+__global__ void __launch_bounds__(256) k_mg_to_padded(LevelDev L, float *__restrict__ pf, const int *done) {
+  if (done && *done) return;
+  const int nc = L.n0 * L.n1 * L.n2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nc) return;
+  const int i = 1 + t % L.n0, j = 1 + (t / L.n0) % L.n1, k = 1 + t / (L.n0 * L.n1);
+  pf[i + (L.n0 + 2) * (j + (L.n1 + 2) * k)] = L.f[lidx(L, i, j, k)];
+}
+__global__ void __launch_bounds__(256) k_mg_from_padded(LevelDev L, const float *__restrict__ px, const int *done) {
+  if (done && *done) return;
+  const int nc = L.n0 * L.n1 * L.n2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nc) return;
+  const int i = 1 + t % L.n0, j = 1 + (t / L.n0) % L.n1, k = 1 + t / (L.n0 * L.n1);
+  L.x[lidx(L, i, j, k)] = px[i + (L.n0 + 2) * (j + (L.n1 + 2) * k)];
 }
 
 // Galerkin scaling of a learned initial guess n on one level: partial sums of
@@ -196,46 +270,87 @@ __global__ void __launch_bounds__(256) k_mg_galerkin_scale(LevelDev L, const flo
 }
 
 // Coarse f = sum over the 8 children of (child vol / coarse vol) * residual.
-__global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, const int *done) {
+__global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, int kz, const int *done) {
   if (done && *done) return;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const long long t = (long long)b * blockDim.x + threadIdx.x;
-  const long long nc = (long long)C.n0 * C.n1 * C.n2;
-  if (t >= nc) return;
-  const int I = 1 + (int)(t % C.n0);
-  const int J = 1 + (int)((t / C.n0) % C.n1);
-  const int K = 1 + (int)(t / ((long long)C.n0 * C.n1));
-  float acc = 0.f;
+  int I, J, K0;
+  tile_origin(kz, I, J, K0);
+  ++I;
+  if (I > C.n0 || J > C.n1) return;
+  const int Kend = min(K0 + kz, C.n2 + 1);
+  for (int K = K0; K < Kend; ++K) {
+    float acc = 0.f;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
-    acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
+    for (int c = 0; c < 8; ++c) {
+      const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
+      acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
+    }
+    C.f[lidx(C, I, J, K)] = acc;
   }
-  C.f[lidx(C, I, J, K)] = acc;
 }
 
-// x_fine += trilinear interpolation of x_coarse.
-__global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, const int *done) {
+// x_fine += trilinear interpolation of x_coarse, one thread per coarse cell
+// (I,J,K) and its 8 children (2I-1..2I, 2J-1..2J, 2K-1..2K).  The two
+// x-children have the same pair index I-1 in their colour halves, so a wave's
+// read-modify-writes are coalesced, and the 27 coarse values the children
+// interpolate from are loaded once.  Each child's arithmetic is the
+// per-fine-cell formula (source I and pI2, weights pa and pb per direction).
+__global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, int kz, const int *done) {
   if (done && *done) return;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const long long t = (long long)b * blockDim.x + threadIdx.x;
-  const long long nf = (long long)F.n0 * F.n1 * F.n2;
-  if (t >= nf) return;
-  const int i = 1 + (int)(t % F.n0);
-  const int j = 1 + (int)((t / F.n0) % F.n1);
-  const int k = 1 + (int)(t / ((long long)F.n0 * F.n1));
-  const int I = F.pI[0][i], I2 = F.pI2[0][i];
-  const int J = F.pI[1][j], J2 = F.pI2[1][j];
-  const int K = F.pI[2][k], K2 = F.pI2[2][k];
-  const float ax = F.pa[0][i], bx = F.pb[0][i];
-  const float ay = F.pa[1][j], by = F.pb[1][j];
-  const float az = F.pa[2][k], bz = F.pb[2][k];
+  int I, J, K0;
+  tile_origin(kz, I, J, K0);
+  ++I;
+  if (I > C.n0 || J > C.n1) return;
   const float *c = C.x;
-  const float e = ax * (ay * (az * c[lidx(C, I, J, K)] + bz * c[lidx(C, I, J, K2)]) +
-                        by * (az * c[lidx(C, I, J2, K)] + bz * c[lidx(C, I, J2, K2)])) +
-                  bx * (ay * (az * c[lidx(C, I2, J, K)] + bz * c[lidx(C, I2, J, K2)]) +
-                        by * (az * c[lidx(C, I2, J2, K)] + bz * c[lidx(C, I2, J2, K2)]));
-  F.x[lidx(F, i, j, k)] += e;
+  int xs[3], ys[3];
+  float ax[2], bx[2], ay[2], by[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int i = 2 * I - 1 + a, j = 2 * J - 1 + a;
+    xs[2 * a] = F.pI2[0][i];
+    ys[2 * a] = F.pI2[1][j];
+    ax[a] = F.pa[0][i]; bx[a] = F.pb[0][i];
+    ay[a] = F.pa[1][j]; by[a] = F.pb[1][j];
+  }
+  xs[1] = I;
+  ys[1] = J;
+  const int Kend = min(K0 + kz, C.n2 + 1);
+  for (int K = K0; K < Kend; ++K) {
+    int zs[3];
+    float az[2], bz[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int k = 2 * K - 1 + a;
+      zs[2 * a] = F.pI2[2][k];
+      az[a] = F.pa[2][k]; bz[a] = F.pb[2][k];
+    }
+    zs[1] = K;
+    float v[3][3][3], xf[2][2][2];
+#pragma unroll
+    for (int z = 0; z < 3; ++z)
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) v[z][y][x] = c[lidx(C, xs[x], ys[y], zs[z])];
+#pragma unroll
+    for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx) xf[cz][cy][cx] = F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)];
+#pragma unroll
+    for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx) {
+          const int z2 = 2 * cz, y2 = 2 * cy, x2 = 2 * cx; // the pI2 source of this child
+          const float e = ax[cx] * (ay[cy] * (az[cz] * v[1][1][1] + bz[cz] * v[z2][1][1]) +
+                                    by[cy] * (az[cz] * v[1][y2][1] + bz[cz] * v[z2][y2][1])) +
+                          bx[cx] * (ay[cy] * (az[cz] * v[1][1][x2] + bz[cz] * v[z2][1][x2]) +
+                                    by[cy] * (az[cz] * v[1][y2][x2] + bz[cz] * v[z2][y2][x2]));
+          F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)] = xf[cz][cy][cx] + e;
+        }
+  }
 }
 
 // Coarsest level: one workgroup, `sweeps` red-black sweeps from x = 0.
@@ -425,9 +540,12 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, c
       }
     }
     __syncthreads();
-    const float *f0 = lev[t0].f;
-    const int n = (T.n[0][0] + 2) * (T.n[0][1] + 2) * (T.n[0][2] + 2);
-    for (int t = threadIdx.x; t < n; t += kTailThreads) sm[T.fo[0] + t] = f0[t];
+    const LevelDev &G = lev[t0];
+    const int n0 = T.n[0][0], n1 = T.n[0][1], n = n0 * n1 * T.n[0][2];
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      sm[T.fo[0] + TIDX(0, i, j, k)] = G.f[lidx(G, i, j, k)];
+    }
   }
   __syncthreads();
   for (int l = 0; l < nt - 1; ++l) {
@@ -582,11 +700,17 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     const HostLevel &hl = H[l];
     LevelDev &L = h->L[l];
     L.n0 = hl.n[0]; L.n1 = hl.n[1]; L.n2 = hl.n[2];
-    L.s0 = L.n0 + 2; L.s1 = L.n1 + 2;
+    L.h0 = (L.n0 + 1) / 2;
+    L.half = (int)((size_t)L.h0 * L.n1 * L.n2);
     L.per0 = bct[0] == MOONS_BC_PERIODIC;
     L.per1 = bct[2] == MOONS_BC_PERIODIC;
     L.per2 = bct[4] == MOONS_BC_PERIODIC;
-    const size_t ncell = (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+    const size_t ncell = (size_t)2 * ((L.n0 + 1) / 2) * L.n1 * L.n2;
+    if (ncell >= ((size_t)1 << 31)) {
+      if (err) *err = "multigrid: a level of 2^31 or more cells is not supported";
+      mg_destroy(h);
+      return nullptr;
+    }
     L.x = dalloc(ncell);
     L.f = dalloc(ncell);
     if (!L.x || !L.f) {
@@ -753,6 +877,8 @@ void mg_destroy(MGHier *h) {
   if (stream()) (void)hipStreamSynchronize(stream());
   if (h->learn) unet_prec_destroy(h->learn);
   if (h->gpart) (void)hipFree(h->gpart);
+  if (h->lpf) (void)hipFree(h->lpf);
+  if (h->lpx) (void)hipFree(h->lpx);
   for (float *p : h->arrays) (void)hipFree(p);
   delete h;
 }
@@ -766,6 +892,11 @@ int mg_level_cells(const MGHier *h, int l, int n[3]) {
 }
 
 float *mg_f0(MGHier *h) { return h->L[0].f; }
+void mg_f32_map(const MGHier *h, int map[3]) {
+  map[0] = h->L[0].h0;
+  map[1] = h->L[0].n1;
+  map[2] = h->L[0].half;
+}
 float *mg_x0(MGHier *h) { return h->L[0].x; }
 
 static unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
@@ -793,6 +924,16 @@ int mg_set_learned(MGHier *h, moons_unet_s *net, int level, std::string *err) {
     if (err) *err = "learned level too large";
     return MOONS_ERR_ARG;
   }
+  const size_t npad = (size_t)(L.n0 + 2) * (L.n1 + 2) * (L.n2 + 2);
+  if (h->lpf) (void)hipFree(h->lpf);
+  if (h->lpx) (void)hipFree(h->lpx);
+  h->lpf = h->lpx = nullptr;
+  if (hipMalloc(&h->lpf, npad * sizeof(float)) != hipSuccess || hipMalloc(&h->lpx, npad * sizeof(float)) != hipSuccess) {
+    if (err) *err = "hipMalloc failed";
+    return MOONS_ERR_HIP;
+  }
+  (void)hipMemsetAsync(h->lpf, 0, npad * sizeof(float), stream());
+  (void)hipMemsetAsync(h->lpx, 0, npad * sizeof(float), stream());
   h->learn_level = level;
   return MOONS_OK;
 }
@@ -805,9 +946,45 @@ void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps) {
 
 
 
+// Launch shape of a per-level grid kernel over nx by ny by nz points: a
+// 256-thread tile no wider than the row needs (so narrow levels leave no lanes
+// idle), and the largest z-march (1, 2 or 4 planes) that still gives at least
+// 1024 tiles (4 per CU).
+struct TileLaunch {
+  dim3 grid, block;
+  int kz;
+};
+
+static TileLaunch tiles(int nx, int ny, int nz) {
+  int bx = 64;
+  while (bx > 8 && bx / 2 >= nx) bx /= 2;
+  const int by = 256 / bx;
+  TileLaunch t;
+  t.block = dim3(bx, by);
+  auto grid = [&](int kz) { return dim3((nx + bx - 1) / bx, (ny + by - 1) / by, (nz + kz - 1) / kz); };
+  t.kz = 1;
+  while (t.kz < 4) {
+    const dim3 g = grid(2 * t.kz);
+    if ((long long)g.x * g.y * g.z < 1024) break;
+    t.kz *= 2;
+  }
+  t.grid = grid(t.kz);
+  return t;
+}
+
 static void smooth(const LevelDev &L, int color, bool zero_guess, const int *done) {
-  const long long nt = (long long)((L.n0 + 1) / 2) * L.n1 * L.n2;
-  hipLaunchKernelGGL(k_mg_smooth, dim3(blocks_for(nt)), dim3(256), 0, stream(), L, color, zero_guess ? 1 : 0, done);
+  const TileLaunch t = tiles((L.n0 + 1) / 2, L.n1, L.n2);
+  hipLaunchKernelGGL(k_mg_smooth, t.grid, t.block, 0, stream(), L, color, zero_guess ? 1 : 0, t.kz, done);
+}
+
+static void restrict_to(const LevelDev &F, const LevelDev &C, const int *done) {
+  const TileLaunch t = tiles(C.n0, C.n1, C.n2);
+  hipLaunchKernelGGL(k_mg_restrict, t.grid, t.block, 0, stream(), F, C, t.kz, done);
+}
+
+static void prolong_from(const LevelDev &F, const LevelDev &C, const int *done) {
+  const TileLaunch t = tiles(C.n0, C.n1, C.n2);
+  hipLaunchKernelGGL(k_mg_prolong, t.grid, t.block, 0, stream(), F, C, t.kz, done);
 }
 
 int mg_vcycle(MGHier *h, const int *done) {
@@ -820,10 +997,12 @@ int mg_vcycle(MGHier *h, const int *done) {
     const bool learned = h->learn && l == h->learn_level;
     if (learned) {
       // learned initial guess x = c N(f), then smoothing from it
-      int st = unet_prec_apply(h->learn, L.f, L.x, done);
-      if (st) return st;
       const int nc = L.n0 * L.n1 * L.n2;
       const unsigned nb = blocks_for(nc);
+      hipLaunchKernelGGL(k_mg_to_padded, dim3(nb), dim3(256), 0, stream(), L, h->lpf, done);
+      int st = unet_prec_apply(h->learn, h->lpf, h->lpx, done);
+      if (st) return st;
+      hipLaunchKernelGGL(k_mg_from_padded, dim3(nb), dim3(256), 0, stream(), L, h->lpx, done);
       hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
       hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
     }
@@ -831,9 +1010,7 @@ int mg_vcycle(MGHier *h, const int *done) {
       smooth(L, 0, s == 0 && !learned, done);
       smooth(L, 1, false, done);
     }
-    const LevelDev &C = h->L[l + 1];
-    const long long nc = (long long)C.n0 * C.n1 * C.n2;
-    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, stream(), L, C, done);
+    restrict_to(L, h->L[l + 1], done);
   }
   if (tail)
     hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tailimg, h->tail,
@@ -843,9 +1020,7 @@ int mg_vcycle(MGHier *h, const int *done) {
   // ascend
   for (int l = g - 1; l >= 0; --l) {
     LevelDev &L = h->L[l];
-    const LevelDev &C = h->L[l + 1];
-    const long long nf = (long long)L.n0 * L.n1 * L.n2;
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(nf)), dim3(256), 0, stream(), L, C, done);
+    prolong_from(L, h->L[l + 1], done);
     for (int s = 0; s < h->nu2; ++s) {
       smooth(L, 1, false, done);
       smooth(L, 0, false, done);
@@ -854,6 +1029,49 @@ int mg_vcycle(MGHier *h, const int *done) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));
   return MOONS_OK;
+}
+
+int mg_time_kernels(MGHier *h, int reps, double ms[5]) {
+  const int nl = (int)h->L.size();
+  hipEvent_t e[2];
+  for (int i = 0; i < 2; ++i)
+    if (hipEventCreate(&e[i]) != hipSuccess) return fail(MOONS_ERR_HIP, "hipEventCreate failed");
+  int st = MOONS_OK;
+  for (int w = 0; w < 5 && st == MOONS_OK; ++w) {
+    ms[w] = 0.0;
+    if ((w == 1 || w == 2) && nl < 2) continue;
+    auto launch = [&]() -> int {
+      switch (w) {
+        case 0: smooth(h->L[0], 0, false, nullptr); return MOONS_OK;
+        case 1: restrict_to(h->L[0], h->L[1], nullptr); return MOONS_OK;
+        case 2: prolong_from(h->L[0], h->L[1], nullptr); return MOONS_OK;
+        case 3:
+          if (h->tail0 < nl)
+            hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tailimg,
+                               h->tail, h->tail0, h->nu1, h->nu2, h->coarse_sweeps, nullptr);
+          else
+            hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, nullptr);
+          return MOONS_OK;
+        default: return mg_vcycle(h, nullptr);
+      }
+    };
+    for (int r = 0; r < 2 && st == MOONS_OK; ++r) st = launch(); // warm-up
+    if (st) break;
+    (void)hipEventRecord(e[0], stream());
+    for (int r = 0; r < reps && st == MOONS_OK; ++r) st = launch();
+    (void)hipEventRecord(e[1], stream());
+    if (st) break;
+    hipError_t he = hipEventSynchronize(e[1]);
+    float t = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&t, e[0], e[1]);
+    if (he != hipSuccess) {
+      st = fail(MOONS_ERR_HIP, std::string("multigrid kernel timing: ") + hipGetErrorString(he));
+      break;
+    }
+    ms[w] = (double)t / reps;
+  }
+  for (int i = 0; i < 2; ++i) (void)hipEventDestroy(e[i]);
+  return st;
 }
 
 } // namespace moons

@@ -18,14 +18,21 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
 void mg_destroy(MGHier *h);
 int mg_levels(const MGHier *h);
 int mg_level_cells(const MGHier *h, int l, int n[3]);
-// Fine-level right-hand side f0 = r/vol and output x0, FP32, with the padded
-// shape of the cell-centred field.
+// Fine-level right-hand side f0 = r/vol and output x0, FP32, in the
+// colour-split layout: cell (i,j,k) (1-based interior indices) at
+// ((i+j+k) & 1) * map[2] + (i-1)/2 + map[0] * ((j-1) + map[1] * (k-1)).
 float *mg_f0(MGHier *h);
 float *mg_x0(MGHier *h);
+void mg_f32_map(const MGHier *h, int map[3]);
 // One V-cycle x0 = V f0 on the library stream; every kernel returns at once
 // when *done (device) is non-zero.
 int mg_vcycle(MGHier *h, const int *done);
 void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
+// Average duration (ms) of one launch, reps launches back to back on the
+// library stream: [0] a half-sweep of level 0, [1] restriction 0 -> 1,
+// [2] prolongation 1 -> 0 (0 when there is one level), [3] the coarse end,
+// [4] a whole V-cycle.  Overwrites the level arrays.
+int mg_time_kernels(MGHier *h, int reps, double ms[5]);
 // Learned initial guess on grid level `level` (net == nullptr removes it):
 // before that level's pre-smoothing, x = c N(f) with the Galerkin factor c.
 int mg_set_learned(MGHier *h, moons_unet_s *net, int level, std::string *err);

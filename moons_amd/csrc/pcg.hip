@@ -42,6 +42,15 @@ struct PcgCtrl {
 
 // ------------------------------------------------------------------ kernels
 
+// Where an FP32 preconditioner vector keeps cell (i,j,k): at the padded index
+// o of the field (split == 0) or in the multigrid's colour-split layout.
+struct F32Map {
+  int split, h0, n1, half;
+};
+__device__ __forceinline__ size_t f32_at(const F32Map &M, size_t o, int i, int j, int k) {
+  return M.split ? (size_t)(((i + j + k) & 1) * M.half + ((i - 1) >> 1) + M.h0 * ((j - 1) + M.n1 * (k - 1))) : o;
+}
+
 // Partial sums of b*vol over interior cells (subtract_phys_mean_vol_SF,
 // code/handwritten/ops/ops_aux.f90:146-167).
 __global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, CCGeom G, double *partM) {
@@ -64,7 +73,7 @@ template <bool MG>
 __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, const double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ p, CCGeom G,
                                                 const PcgCtrl *ctrl, const double *partM, double *partB,
-                                                int all_neumann, float *__restrict__ f0) {
+                                                int all_neumann, float *__restrict__ f0, F32Map M) {
   __shared__ double sh[kNT / kWave];
   double mean = 0.0;
   if (all_neumann) mean = prologue_sum<kNT>(partM, gridDim.x, 1, 0, sh) / ctrl->volsum;
@@ -109,7 +118,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
       r[o] = rv;
       a_rr += rv * rv;
       if (MG) {
-        f0[o] = (float)(rv / vol);
+        f0[f32_at(M, o, i, j, k)] = (float)(rv / vol);
       } else {
         const double z = minv_at<double>(G, dxy, axy, k) * rv;
         p[o] = z;
@@ -183,7 +192,7 @@ template <int KZ, bool MG>
 __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, const double *__restrict__ p,
                                                    double *__restrict__ r, const double *__restrict__ q, CCGeom G,
                                                    PcgCtrl *ctrl, const double *partA, double *partB, int it,
-                                                   float *__restrict__ f0) {
+                                                   float *__restrict__ f0, F32Map M) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
   const TileIdx t = tile_of(G);
@@ -223,7 +232,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_update(double *__restrict__ x, cons
         r[o] = rn;
         a_rr += rn * rn;
         if (MG) {
-          f0[o] = (float)(rn / (dxy * G.c[2].dh[t.k0 + u]));
+          f0[f32_at(M, o, i, j, t.k0 + u)] = (float)(rn / (dxy * G.c[2].dh[t.k0 + u]));
         } else {
           const double z = minv_at<double>(G, dxy, axy, t.k0 + u) * rn;
           a_rz += z * rn;
@@ -342,7 +351,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_check(PcgCtrl *ctrl, const double *
 template <int KZ>
 __global__ void __launch_bounds__(kNT) k_mg_zdots(const float *__restrict__ z32, const double *__restrict__ r,
                                                  const double *__restrict__ q, CCGeom G, const PcgCtrl *ctrl,
-                                                 double *partC, int use_q) {
+                                                 double *partC, int use_q, F32Map M) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
   const TileIdx t = tile_of(G);
@@ -355,7 +364,7 @@ __global__ void __launch_bounds__(kNT) k_mg_zdots(const float *__restrict__ z32,
     for (int u = 0; u < KZ; ++u) {
       if (u < nk) {
         const size_t o = base + (size_t)u * sxy;
-        const double z = (double)z32[o];
+        const double z = (double)z32[f32_at(M, o, t.i, t.j, t.k0 + u)];
         a_zr += z * r[o];
         if (use_q) a_zq += z * q[o];
       }
@@ -391,7 +400,7 @@ __global__ void __launch_bounds__(kNT) k_mg_init(PcgCtrl *ctrl, const double *pa
 // p = p*beta + z (it >= 1) or p = z (it == 0), then the face ghosts of p.
 template <int KZ>
 __global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, const float *__restrict__ z32, CCGeom G,
-                                                   PcgCtrl *ctrl, const double *partC, int it) {
+                                                   PcgCtrl *ctrl, const double *partC, int it, F32Map M) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
   const TileIdx t = tile_of(G);
@@ -404,7 +413,7 @@ __global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, cons
 #pragma unroll
   for (int u = 0; u < KZ; ++u) {
     if (u < nk) {
-      zv[u] = z32[base + (size_t)u * sxy];
+      zv[u] = z32[f32_at(M, base + (size_t)u * sxy, i, j, t.k0 + u)];
       pv[u] = it > 0 ? p[base + (size_t)u * sxy] : 0.0;
     }
   }
@@ -435,23 +444,25 @@ __global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, cons
 }
 
 // f0 = r/vol (FP32) for one preconditioner application; z = x0 back to FP64.
-__global__ void __launch_bounds__(kNT) k_mg_r2f(const double *__restrict__ r, float *__restrict__ f0, CCGeom G) {
+__global__ void __launch_bounds__(kNT) k_mg_r2f(const double *__restrict__ r, float *__restrict__ f0, CCGeom G,
+                                               F32Map M) {
   const TileIdx t = tile_of(G);
   if (!t.active) return;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
   const double dxy = G.c[0].dh[t.i] * G.c[1].dh[t.j];
   for (int k = t.k0; k < t.k1; ++k) {
     const size_t o = (size_t)t.i + (size_t)t.j * sx + (size_t)k * sxy;
-    f0[o] = (float)(r[o] / (dxy * G.c[2].dh[k]));
+    f0[f32_at(M, o, t.i, t.j, k)] = (float)(r[o] / (dxy * G.c[2].dh[k]));
   }
 }
-__global__ void __launch_bounds__(kNT) k_mg_x2z(const float *__restrict__ x0, double *__restrict__ z, CCGeom G) {
+__global__ void __launch_bounds__(kNT) k_mg_x2z(const float *__restrict__ x0, double *__restrict__ z, CCGeom G,
+                                               F32Map M) {
   const TileIdx t = tile_of(G);
   if (!t.active) return;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
   for (int k = t.k0; k < t.k1; ++k) {
     const size_t o = (size_t)t.i + (size_t)t.j * sx + (size_t)k * sxy;
-    z[o] = (double)x0[o];
+    z[o] = (double)x0[f32_at(M, o, t.i, t.j, k)];
   }
 }
 
@@ -562,6 +573,15 @@ static bool flex(const moons_pcg_s *S) {
 }
 static float *flex_f0(moons_pcg_s *S) { return S->mg ? mg_f0(S->mg) : S->uf0; }
 static float *flex_z0(moons_pcg_s *S) { return S->mg ? mg_x0(S->mg) : S->uz0; }
+static F32Map flex_map(const moons_pcg_s *S) {
+  F32Map M{0, 0, 0, 0};
+  if (S->mg) {
+    int m[3];
+    mg_f32_map(S->mg, m);
+    M = F32Map{1, m[0], m[1], m[2]};
+  }
+  return M;
+}
 static int flex_apply(moons_pcg_s *S) {
   if (S->mg) return mg_vcycle(S->mg, &S->ctrl->done);
   return unet_prec_apply(S->unet, S->uf0, S->uz0, &S->ctrl->done);
@@ -572,20 +592,21 @@ static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
   const dim3 grid(S->G.nblocks), block(kNT);
   const bool mg = flex(S);
   float *f0 = mg ? flex_f0(S) : nullptr;
+  const F32Map M = flex_map(S);
   if (which & 1) hipLaunchKernelGGL(k_pcg_matvec<KZ>, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA);
   if (which & 2) {
     if (mg)
       hipLaunchKernelGGL((k_pcg_update<KZ, true>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
-                         S->partA, S->partB, it, f0);
+                         S->partA, S->partB, it, f0, M);
     else
       hipLaunchKernelGGL((k_pcg_update<KZ, false>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
-                         S->partA, S->partB, it, f0);
+                         S->partA, S->partB, it, f0, M);
   }
   if (which & 4) hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
   if (which & 8) hipLaunchKernelGGL(k_mg_zdots<KZ>, grid, block, 0, stream(), flex_z0(S), S->r, S->q, S->G, S->ctrl,
-                                    S->partC, it > 0 ? 1 : 0);
+                                    S->partC, it > 0 ? 1 : 0, M);
   if (which & 16) hipLaunchKernelGGL(k_mg_pupdate<KZ>, grid, block, 0, stream(), S->p, flex_z0(S), S->G, S->ctrl,
-                                     S->partC, it);
+                                     S->partC, it, M);
 }
 
 // which: bit 0 matvec, 1 update, 2 p-update (Jacobi), 3 (z,r),(z,q) dots, 4 p-update (multigrid)
@@ -753,13 +774,13 @@ static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
   if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->partM);
   if (!mg) {
     hipLaunchKernelGGL(k_pcg_rhs<false>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
-                       S->partB, alln, (float *)nullptr);
+                       S->partB, alln, (float *)nullptr, flex_map(S));
     hipLaunchKernelGGL(k_pcg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->G.nblocks);
     MOONS_LAUNCH_CHECK();
   } else {
     // z0 = V(r0/vol), rho0 = (z0,r0), p = z0
     hipLaunchKernelGGL(k_pcg_rhs<true>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
-                       S->partB, alln, flex_f0(S));
+                       S->partB, alln, flex_f0(S), flex_map(S));
     MOONS_LAUNCH_CHECK();
     if ((st = flex_apply(S))) return st;
     if ((st = pcg_launch(S, x->d, 0, 8))) return st;
@@ -940,15 +961,21 @@ int moons_pcg_apply_prec(moons_pcg_t S, moons_field_t r, moons_field_t z) {
       return fail(MOONS_ERR_STATE, "moons_pcg_apply_prec: bind a network with moons_pcg_set_unet first");
     int st = pcg_upload_ctrl(S); // done = 0
     if (st) return st;
-    hipLaunchKernelGGL(k_mg_r2f, grid, block, 0, stream(), r->d, flex_f0(S), S->G);
+    hipLaunchKernelGGL(k_mg_r2f, grid, block, 0, stream(), r->d, flex_f0(S), S->G, flex_map(S));
     MOONS_LAUNCH_CHECK();
     if ((st = flex_apply(S))) return st;
     MOONS_HIP(hipMemsetAsync(z->d, 0, z->n * sizeof(double), stream()));
-    hipLaunchKernelGGL(k_mg_x2z, grid, block, 0, stream(), flex_z0(S), z->d, S->G);
+    hipLaunchKernelGGL(k_mg_x2z, grid, block, 0, stream(), flex_z0(S), z->d, S->G, flex_map(S));
     MOONS_LAUNCH_CHECK();
     return MOONS_OK;
   }
   return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: the Jacobi preconditioner is applied inside the PCG kernels only");
+}
+
+int moons_pcg_time_mg(moons_pcg_t S, int reps, double ms[5]) {
+  if (!S || !ms || reps < 1) return fail(MOONS_ERR_ARG, "moons_pcg_time_mg: bad argument");
+  if (!S->mg) return fail(MOONS_ERR_ARG, "moons_pcg_time_mg: the solver has no multigrid preconditioner");
+  return mg_time_kernels(S->mg, reps, ms);
 }
 
 int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
