@@ -216,7 +216,8 @@ int moons_clean_div(moons_pcg_t pcg, moons_field_t X[3], moons_field_t Xstar[3],
 /* Multigrid parameters of a MOONS_PREC_MG_FP32 solver (rebuilds the level
  * hierarchy): at most max_levels levels, nu1/nu2 red-black Gauss-Seidel sweeps
  * before/after the coarse correction, coarse_sweeps sweeps on the coarsest
- * level.  No reference counterpart (see MOONS_PREC_MG_FP32). */
+ * level.  Defaults 16, 3, 3, 16.  No reference counterpart (see
+ * MOONS_PREC_MG_FP32). */
 int moons_pcg_set_mg(moons_pcg_t pcg, int max_levels, int nu1, int nu2, int coarse_sweeps);
 /* Number of multigrid levels and the coarsest level's cells per direction. */
 int moons_pcg_mg_info(moons_pcg_t pcg, int *levels, int n_coarsest[3]);

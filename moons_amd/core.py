@@ -246,7 +246,7 @@ class PCG_solver_SF:
         self.prec = prec
         self.set_ISP(ISP or iter_solver_params())
 
-    def set_MG(self, max_levels: int = 16, nu1: int = 3, nu2: int = 3, coarse_sweeps: int = 40) -> None:
+    def set_MG(self, max_levels: int = 16, nu1: int = 3, nu2: int = 3, coarse_sweeps: int = 16) -> None:
         check(load().moons_pcg_set_mg(self.handle, max_levels, nu1, nu2, coarse_sweeps))
 
     def mg_info(self):

@@ -46,12 +46,18 @@ constexpr int kMaxTail = 8;
 constexpr size_t kTailLdsBytes = 60 * 1024;
 constexpr long long kTailMaxCells = 8192;
 
+constexpr int kTailLoads = 8; // per-thread loads of f and of the coefficients at entry
+
 struct TailDesc {
   int nt;                      // tail levels
   int per0, per1, per2;
   int n[kMaxTail][3];
   int xo[kMaxTail], fo[kMaxTail], co[kMaxTail], cs[kMaxTail]; // LDS float offsets; coefficient stride
+  int coff[kMaxTail];          // offset of each level's coefficient block in the packed device copy
+  int ncoef;                   // floats of coefficients in all
   int nfloat;                  // LDS floats in use
+  const float *f0;             // f of the first tail level (colour-split, in the level arrays)
+  float *x0;                   // and its x, written at exit
 };
 
 struct MGHier {
@@ -60,9 +66,8 @@ struct MGHier {
   int nu1 = 2, nu2 = 2, coarse_sweeps = 40;
   int tail0 = 0;               // first level of the LDS-resident tail (== L.size(): none)
   size_t tail_lds = 0;         // LDS bytes of the tail's x and f arrays
-  LevelDev *dlev = nullptr;    // level descriptors in device memory (for the tail kernel)
   TailDesc tail{};
-  float *tailimg = nullptr;    // device copy of the tail's initial LDS image
+  float *tailimg = nullptr;    // the tail's coefficient blocks, packed
   std::vector<std::vector<float>> hcoef; // host 1D coefficients: [(level*3 + d)*8 + c]
   std::vector<std::vector<double>> hn; // node arrays (ghosts included) of every level, 3 per level
   UnetPrec *learn = nullptr;   // learned initial guess of one grid level (or none)
@@ -150,7 +155,8 @@ __device__ __forceinline__ bool color_cell(const LevelDev &L, long long t, int c
 // of the other colour (a periodic wrap keeps that, every count along a
 // periodic direction being even), except a wall's clamp to the cell itself.
 __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int zero_guess, int kz, const int *done) {
-  if (done && *done) return;
+  // *done is tested at the stores, so that its load overlaps the operand loads
+  const bool skip = done && *done;
   int p, j, k0;
   tile_origin(kz, p, j, k0);
   if (p >= L.h0 || j > L.n1) return;
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
     const float d = L.dg[0][i] + dgj + L.dg[2][k];
     const float fv = fc[c];
     if (zero_guess) {
-      xc[c] = fv / d;
+      if (!skip) xc[c] = fv / d;
       continue;
     }
     const float xim = (i == 1) ? (L.per0 ? xo[row + h0 - 1] : xc[c]) : xo[row + ((i - 2) >> 1)];
@@ -183,7 +189,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
     float s = L.wm[0][i] * xim + L.wp[0][i] * xip;
     s += wmj * xjm + wpj * xjp;
     s += L.wm[2][k] * xkm + L.wp[2][k] * xkp;
-    xc[c] = (fv - s) / d;
+    if (!skip) xc[c] = (fv - s) / d;
   }
 }
 
@@ -271,7 +277,7 @@ __global__ void __launch_bounds__(256) k_mg_galerkin_scale(LevelDev L, const flo
 
 // Coarse f = sum over the 8 children of (child vol / coarse vol) * residual.
 __global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, int kz, const int *done) {
-  if (done && *done) return;
+  const bool skip = done && *done; // tested at the store (overlaps the loads)
   int I, J, K0;
   tile_origin(kz, I, J, K0);
   ++I;
@@ -284,7 +290,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, int
       const int i = 2 * I - 1 + (c & 1), j = 2 * J - 1 + ((c >> 1) & 1), k = 2 * K - 1 + (c >> 2);
       acc += F.rf[0][i] * F.rf[1][j] * F.rf[2][k] * res_cell(F, i, j, k);
     }
-    C.f[lidx(C, I, J, K)] = acc;
+    if (!skip) C.f[lidx(C, I, J, K)] = acc;
   }
 }
 
@@ -295,7 +301,7 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, int
 // interpolate from are loaded once.  Each child's arithmetic is the
 // per-fine-cell formula (source I and pI2, weights pa and pb per direction).
 __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, int kz, const int *done) {
-  if (done && *done) return;
+  const bool skip = done && *done; // tested at the stores (overlaps the loads)
   int I, J, K0;
   tile_origin(kz, I, J, K0);
   ++I;
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, int 
                                     by[cy] * (az[cz] * v[1][y2][1] + bz[cz] * v[z2][y2][1])) +
                           bx[cx] * (ay[cy] * (az[cz] * v[1][1][x2] + bz[cz] * v[z2][1][x2]) +
                                     by[cy] * (az[cz] * v[1][y2][x2] + bz[cz] * v[z2][y2][x2]));
-          F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)] = xf[cz][cy][cx] + e;
+          if (!skip) F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)] = xf[cz][cy][cx] + e;
         }
   }
 }
@@ -517,34 +523,41 @@ __device__ __forceinline__ void tail_level_sweeps(const TailDesc &T, float *sm, 
   }
 }
 
-__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, const float *__restrict__ img, TailDesc T,
-                                                          int t0, int nu1, int nu2, int coarse_sweeps, const int *done) {
-  if (done && *done) return;
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(const float *__restrict__ cimg, TailDesc T, int nu1,
+                                                          int nu2, int coarse_sweeps, const int *done) {
+  const bool skip = done && *done; // tested once the entry loads are in flight
   extern __shared__ float sm[];
   const int nt = T.nt;
-  // the tail's LDS image (coefficients in place, x and f zero), then f of the
-  // first tail level: independent coalesced loads, all in flight together
-  {
-    const int nimg = T.nfloat;
-    for (int t0i = 0; t0i < nimg; t0i += 4 * kTailThreads) {
-      float r[4];
+  // f of the first tail level (from the colour-split level array) and the
+  // coefficient blocks: independent loads, all in flight while LDS is cleared
+  const int n0 = T.n[0][0], n1 = T.n[0][1], nf0 = n0 * n1 * T.n[0][2];
+  const int h0 = (n0 + 1) / 2, half = h0 * n1 * T.n[0][2];
+  float fr[kTailLoads], cr[kTailLoads];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int t = t0i + q * kTailThreads + threadIdx.x;
-        r[q] = t < nimg ? img[t] : 0.f;
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int t = t0i + q * kTailThreads + threadIdx.x;
-        if (t < nimg) sm[t] = r[q];
-      }
-    }
-    __syncthreads();
-    const LevelDev &G = lev[t0];
-    const int n0 = T.n[0][0], n1 = T.n[0][1], n = n0 * n1 * T.n[0][2];
-    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+  for (int q = 0; q < kTailLoads; ++q) {
+    const int t = q * kTailThreads + threadIdx.x;
+    fr[q] = 0.f;
+    cr[q] = 0.f;
+    if (t < nf0) {
       const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
-      sm[T.fo[0] + TIDX(0, i, j, k)] = G.f[lidx(G, i, j, k)];
+      fr[q] = T.f0[((i + j + k) & 1) * half + ((i - 1) >> 1) + h0 * ((j - 1) + n1 * (k - 1))];
+    }
+    if (t < T.ncoef) cr[q] = cimg[t];
+  }
+  for (int t = threadIdx.x; t < T.nfloat; t += kTailThreads) sm[t] = 0.f;
+  __syncthreads();
+  if (skip) return;
+#pragma unroll
+  for (int q = 0; q < kTailLoads; ++q) {
+    const int t = q * kTailThreads + threadIdx.x;
+    if (t < nf0) {
+      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      sm[T.fo[0] + TIDX(0, i, j, k)] = fr[q];
+    }
+    if (t < T.ncoef) {
+      int l = 0;
+      while (l + 1 < nt && t >= T.coff[l + 1]) ++l;
+      sm[T.co[l] + (t - T.coff[l])] = cr[q];
     }
   }
   __syncthreads();
@@ -599,14 +612,9 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const LevelDev *lev, c
     __syncthreads();
     tail_level_sweeps(T, sm, l, nu2, 1, false);
   }
-  {
-    const LevelDev &G = lev[t0];
-    const int n0 = T.n[0][0], n1 = T.n[0][1], n2 = T.n[0][2];
-    const int n = n0 * n1 * n2;
-    for (int t = threadIdx.x; t < n; t += kTailThreads) {
-      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
-      G.x[lidx(G, i, j, k)] = sm[T.xo[0] + TIDX(0, i, j, k)];
-    }
+  for (int t = threadIdx.x; t < nf0; t += kTailThreads) {
+    const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+    T.x0[((i + j + k) & 1) * half + ((i - 1) >> 1) + h0 * ((j - 1) + n1 * (k - 1))] = sm[T.xo[0] + TIDX(0, i, j, k)];
   }
 }
 
@@ -842,14 +850,30 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       T.co[q] = off; off += 24 * T.cs[q];
     }
     T.nfloat = off;
-    // initial LDS image: the coefficient blocks in place, zeros elsewhere
-    std::vector<float> img((size_t)std::max(off, 1), 0.f);
+    // the coefficient blocks of the tail levels, packed (the kernel places
+    // each at its LDS offset co)
+    int nco = 0;
+    for (int q = 0; q < T.nt; ++q) {
+      T.coff[q] = nco;
+      nco += 24 * T.cs[q];
+    }
+    T.ncoef = nco;
+    if (T.n[0][0] * T.n[0][1] * T.n[0][2] > kTailLoads * kTailThreads || nco > kTailLoads * kTailThreads) {
+      if (err) *err = "multigrid: internal error (tail larger than its entry loads)";
+      mg_destroy(h);
+      return nullptr;
+    }
+    std::vector<float> img((size_t)std::max(nco, 1), 0.f);
     for (int q = 0; q < T.nt; ++q)
       for (int d = 0; d < 3; ++d)
         for (int c = 0; c < 8; ++c) {
           const std::vector<float> &a = h->hcoef[((h->tail0 + q) * 3 + d) * 8 + c];
-          std::copy(a.begin(), a.end(), img.begin() + T.co[q] + (c * 3 + d) * T.cs[q]);
+          std::copy(a.begin(), a.end(), img.begin() + T.coff[q] + (c * 3 + d) * T.cs[q]);
         }
+    if (T.nt > 0) {
+      T.f0 = h->L[h->tail0].f;
+      T.x0 = h->L[h->tail0].x;
+    }
     h->tailimg = dalloc(img.size());
     if (!h->tailimg) {
       if (err) *err = "multigrid: hipMalloc failed";
@@ -857,17 +881,7 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       return nullptr;
     }
     (void)hipMemcpyAsync(h->tailimg, img.data(), img.size() * sizeof(float), hipMemcpyHostToDevice, stream());
-  }
-  {
-    float *p = dalloc((nl * sizeof(LevelDev) + sizeof(float) - 1) / sizeof(float));
-    if (!p) {
-      if (err) *err = "multigrid: hipMalloc failed";
-      mg_destroy(h);
-      return nullptr;
-    }
-    h->dlev = reinterpret_cast<LevelDev *>(p);
-    (void)hipMemcpyAsync(h->dlev, h->L.data(), nl * sizeof(LevelDev), hipMemcpyHostToDevice, stream());
-    (void)hipStreamSynchronize(stream());
+    (void)hipStreamSynchronize(stream()); // img and the coefficient vectors go out of scope
   }
   return h;
 }
@@ -1013,8 +1027,8 @@ int mg_vcycle(MGHier *h, const int *done) {
     restrict_to(L, h->L[l + 1], done);
   }
   if (tail)
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tailimg, h->tail,
-                       h->tail0, h->nu1, h->nu2, h->coarse_sweeps, done);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail, h->nu1,
+                       h->nu2, h->coarse_sweeps, done);
   else
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
   // ascend
@@ -1047,8 +1061,8 @@ int mg_time_kernels(MGHier *h, int reps, double ms[5]) {
         case 2: prolong_from(h->L[0], h->L[1], nullptr); return MOONS_OK;
         case 3:
           if (h->tail0 < nl)
-            hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->dlev, h->tailimg,
-                               h->tail, h->tail0, h->nu1, h->nu2, h->coarse_sweeps, nullptr);
+            hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail,
+                               h->nu1, h->nu2, h->coarse_sweeps, nullptr);
           else
             hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, nullptr);
           return MOONS_OK;

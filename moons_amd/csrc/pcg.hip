@@ -21,7 +21,7 @@
 // value while the other blocks of that kernel still read the old one.  Every
 // kernel returns at once after the exit decision; the host launches chunks of
 // iterations (passing the iteration number) and reads the exit flag between
-// chunks.
+// chunks, sized by the residual's observed rate of decrease.
 #include "cc_stencil.hpp"
 #include "mg.hpp"
 #include "unet.hpp"
@@ -541,12 +541,30 @@ struct moons_pcg_s {
   int mg_net_level = -1;
   double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
   float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
-  int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 40;
+  int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 16;
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned, read-backs
   PcgCtrl *host_init = nullptr; // pinned, initial control block (the source of every upload)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+// Iterations to launch before the next read-back of the control block: the
+// count at which the relative residual, decreasing at its average rate so
+// far, reaches tol_rel, within [lo, hi] (lo without a usable rate).  Each
+// read-back idles the GPU for a host round trip, and each iteration launched
+// after convergence still costs its kernels' launches, so the estimate aims
+// at the iteration the exit test will stop at.
+static int next_chunk(const PcgCtrl &h, double tol_rel, int lo, int hi) {
+  int chunk = lo;
+  if (h.iter >= 2 && h.res0 > 0.0 && tol_rel > 0.0) {
+    const double rel = h.res / h.res0;
+    if (rel > tol_rel && rel < 1.0) {
+      const double rate = std::pow(rel, 1.0 / h.iter);
+      if (rate < 0.999) chunk = (int)std::ceil(std::log(tol_rel / rel) / std::log(rate));
+    }
+  }
+  return std::max(lo, std::min(hi, chunk));
+}
 
 // The initial control block lives in its own pinned buffer, written only
 // here, so a captured copy of it replays the same values.
@@ -802,7 +820,8 @@ static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
     MOONS_HIP(hipMemcpyAsync(S->host_ctrl, S->ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, stream()));
     MOONS_HIP(hipStreamSynchronize(stream()));
     if (S->host_ctrl->done || launched >= S->iter_max) break;
-    int chunk = mg ? std::max(2, std::min(8, launched / 2)) : std::max(4, std::min(64, launched / 4));
+    int chunk = launched == 0 ? (mg ? 3 : 16)
+                              : next_chunk(*S->host_ctrl, S->tol_rel, mg ? 1 : 4, mg ? 32 : 512);
     chunk = std::min(chunk, S->iter_max - launched);
     for (int c = 0; c < chunk; ++c) {
       st = mg ? pcg_launch_iteration_mg(S, x->d, launched + c + 1) : pcg_launch_iteration(S, x->d, launched + c + 1);

@@ -179,9 +179,13 @@ def prolong(F: Level, C: Level, xc, bct):
     return e
 
 
-def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=40):
-    """x0 = V f0 (padded arrays; f0 = r/vol on the fine level)."""
+def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=16, x0=None):
+    """x0 = V f0 (padded arrays; f0 = r/vol on the fine level).  With x0 given,
+    the fine level's pre-smoothing starts from it (the learned initial guess)
+    instead of from zero."""
     xs = [np.zeros(L.shape()) for L in levels]
+    if x0 is not None:
+        xs[0] = np.array(x0, dtype=float)
     fs = [None] * len(levels)
     fs[0] = f0
 
@@ -194,7 +198,7 @@ def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=40):
                 L.smooth(x, f, 1, False)
             return
         for s in range(nu1):
-            L.smooth(x, f, 0, s == 0)
+            L.smooth(x, f, 0, s == 0 and not (l == 0 and x0 is not None))
             L.smooth(x, f, 1, False)
         C = levels[l + 1]
         fc = np.zeros(C.shape())

@@ -7,6 +7,7 @@ V-cycle around it."""
 import numpy as np
 import pytest
 
+from oracle import mg_oracle as MGO
 from oracle import oracle as O
 from oracle import unet_oracle as UO
 
@@ -79,14 +80,21 @@ def test_training_lowers_the_loss(gpu):
 
 
 def test_learned_preconditioner_ablation(gpu):
-    """32^3 all-Neumann PPE, held-out right-hand side.  The trained network
-    approximates the exact solution (up to a constant factor) far better than
-    the untrained one; as the fine-level initial guess of a two-level V-cycle
-    it needs fewer flexible-PCG iterations than the untrained network and no
-    more than the cycle without a network (at 32^3 the multigrid already
-    removes what the network models).  Alone in the flexible PCG it does not
-    reach the tolerance (it models the smooth part of the error only), and
-    the Jacobi redo then meets it."""
+    """32^3 all-Neumann PPE, held-out right-hand side f with exact solution e.
+
+    * The trained network approximates e (up to its best constant factor) far
+      better than the untrained one.
+    * One V-cycle of a two-level cycle (32^3 fine, 16^3 coarse by 10 sweeps)
+      with the trained network as the fine level's initial guess leaves less
+      error than the same cycle without a network and than with the untrained
+      network: the network supplies the smooth part of e that two red-black
+      sweeps and a rough coarse solve leave.
+    * Every placement converges to 1e-8 inside the flexible PCG, the
+      standalone learned solver through its Jacobi redo.  Their iteration
+      counts are printed only: at this size the multigrid cycle alone already
+      converges in about as many iterations, and which of the cycles with and
+      without the network needs one or two fewer varies with the training run.
+    """
     print("unet ablation", flush=True)
     from moons_amd import learned
 
@@ -104,31 +112,56 @@ def test_learned_preconditioner_ablation(gpu):
     e = learned.solve_targets(m, types, [f])[0]
     x_in, t = learned.make_sample(f, e, hs)
     inner = (slice(1, -1),) * 3
-    err = {}
-    for name, net in (("trained", trained), ("untrained", untrained)):
-        o = net.forward(x_in)[inner].astype(np.float64)
-        tt = t[inner].astype(np.float64)
-        c = (o * tt).sum() / max((o * o).sum(), 1e-300)
-        err[name] = float(np.linalg.norm(c * o - tt) / np.linalg.norm(tt))
-    print("relative error after scaling", err, flush=True)
+
+    def scaled_error(o, ref):
+        o = o - o.mean()
+        ref = ref - ref.mean()
+        c = (o * ref).sum() / max((o * o).sum(), 1e-300)
+        return float(np.linalg.norm(c * o - ref) / np.linalg.norm(ref))
+
+    err = {name: scaled_error(net.forward(x_in)[inner].astype(np.float64), t[inner].astype(np.float64))
+           for name, net in (("trained", trained), ("untrained", untrained))}
+    print("network: relative error after scaling", err, flush=True)
     assert err["trained"] < 0.75 * err["untrained"], err
     b = gpu.SF(m, "CC").upload(f)
-    # the network as the initial guess of the fine level of a two-level V-cycle
-    # (coarse 16^3 level solved by 10 sweeps only): iterations to 1e-8
-    its = {}
+    vol = float(np.prod([np.diff(h)[1] for h in hs]))
+    r = gpu.SF(m, "CC").upload(f * vol)  # the PCG's residual form vol * f
+    verr, its, zs = {}, {}, {}
     for name, net in (("mg2", None), ("mg2+trained", trained), ("mg2+untrained", untrained)):
         x = gpu.SF(m, "CC").fill(0.0).set_BCs(types, None, all_Neumann=True)
         P = gpu.PCG_solver_SF(m, x, gpu.iter_solver_params(500, 1e-8, 1e-300, 1), prec="MG_FP32")
         P.set_MG(2, 2, 2, 10)
         if net is not None:
             P.set_MG_UNet(net, 0)
+        z = gpu.SF(m, "CC").fill(0.0)
+        P.apply_prec(r, z)
+        zs[name] = z.download()[inner]
+        verr[name] = scaled_error(zs[name], e[inner])
         info = P.solve(x, b)
         assert info.res_norm / info.res_norm0 < 1e-8, (name, info)
         its[name] = info.iterations
-    print("two-level V-cycle iterations", its, flush=True)
-    # the trained network is no worse than the cycle without it and better than an untrained one
-    assert its["mg2+trained"] <= its["mg2"] and its["mg2+trained"] < its["mg2+untrained"], its
-    # standalone in the flexible PCG: the Jacobi fallback returns a converged x
+    print("one two-level V-cycle: relative error after scaling", verr, flush=True)
+    print("flexible-PCG iterations to 1e-8", its, flush=True)
+    assert verr["mg2+trained"] < verr["mg2"] and verr["mg2+trained"] < verr["mg2+untrained"], verr
+    # parity of the learned level: the restated cycle (network in BF16, input
+    # f vol^(2/3)/s and output times s, Galerkin factor (n,f)/(n,Ln), then the
+    # two-level V-cycle from c n) gives the GPU's output, which differs from
+    # the cycle without the network by far more than the tolerance
+    levels = MGO.build(hs, [2] * 6, max_levels=2)
+    L0 = levels[0]
+    v23, _ = learned.geometry(hs)
+    s = float(np.sqrt(np.mean((f * v23)[inner] ** 2)))
+    n = np.zeros(f.shape)
+    n[inner] = s * UO.from_internal(UO.Net(trained.get_weights(), bf=True).forward(UO.to_internal(x_in)))
+    Ln = f[inner] - L0.residual(n, f)
+    c = float((n[inner] * f[inner]).sum() / (n[inner] * Ln).sum())
+    zr = MGO.vcycle(levels, f, [2] * 6, nu1=2, nu2=2, coarse_sweeps=10, x0=c * n)[inner]
+    zn = MGO.vcycle(levels, f, [2] * 6, nu1=2, nu2=2, coarse_sweeps=10)[inner]
+    d_net = rel(zs["mg2+trained"], zr)
+    print("learned-level V-cycle vs restatement", d_net, "; without network", rel(zs["mg2"], zn),
+          "; effect of the network", rel(zr, zn), flush=True)
+    assert d_net < 1e-3 and rel(zs["mg2"], zn) < 2e-5 and rel(zr, zn) > 0.05
+    # standalone in the flexible PCG: the Jacobi redo returns a converged x
     x = gpu.SF(m, "CC").fill(0.0).set_BCs(types, None, all_Neumann=True)
     P = gpu.PCG_solver_SF(m, x, gpu.iter_solver_params(400, 1e-8, 1e-300, 1), prec="UNet")
     P.set_UNet(trained)
