@@ -127,6 +127,40 @@ def run_ensemble(M, args):
     print(json.dumps(out), flush=True)
 
 
+def run_mhd(M, args):
+    """MHD lid-driven cavity time steps at N^3, eager on the library stream
+    (no graph): the 64^3 MHD configuration's parameters (Re 400, Ha 10, Rem 1
+    low-Rem induction, B0 = (0, 1, 0), dt 5e-4, 1e-3 seeded velocity noise,
+    insulating B walls, backward-Euler momentum diffusion); the PPE solved by
+    the mixed solver to 1e-8, the momentum, induction and phi solves with 5
+    iterations.  One step = one time step."""
+    from moons_amd.ensemble import PSEUDO_VACUUM, noise_velocity
+
+    N = args.mhd
+    m = M.mesh(*[M.nodes_uniform(0.0, 1.0, N)] * 3)
+    U = noise_velocity(m, 2)
+    p = M.SF(m, "CC").fill(0.0).set_BCs(["Neumann"] * 6, None, all_Neumann=True)
+    B = [M.SF(m, 1 << d).fill(0.0).set_BCs(*PSEUDO_VACUUM[d]) for d in range(3)]
+    phi = M.SF(m, "CC").fill(0.0).set_BCs(["Dirichlet"] * 6)
+    isp = (5, 1e-6, 1e-13, 1)
+    st = M.MHD_step(m, U, p, B=B, phi=phi, Re=400.0, Ha=10.0, Rem=1.0, dt=5e-4, alpha_U=1.0, B0=(0.0, 1.0, 0.0),
+                    isp_U=isp, isp_p=(10000, TOL, 1e-13, 1), isp_B=isp, isp_phi=isp, prec_p="MG_FP32")
+    st.step(args.warmup)
+    M.synchronize()
+    t0 = time.perf_counter()
+    info = st.step(args.steps)
+    M.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    out = {"metric": "MHD time steps per second", "value": round(1e3 / ms, 2), "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"mhd_cavity_{N}", "cells": f"{N}^3", "Re": 400, "Ha": 10, "Rem": 1, "dt": 5e-4,
+                      "ppe": "mixed FP32 V-cycle in FP64 FPCG to 1e-8", "other_solves": "5 iterations",
+                      "execution": "eager, one stream"},
+           "last_step_iterations": info["iterations"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +170,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="batched-ensemble benchmark instead: 64 MHD cavities of sizes {32,48,64,96}^3 advanced by "
                          "HIP-graph replays on 4 streams")
+    ap.add_argument("--mhd", type=int, default=0, metavar="N",
+                    help="MHD time-step benchmark instead: one lid-driven MHD cavity at N^3, eager")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training and timing the learned component (U-net initial guess in the V-cycle)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -152,6 +188,9 @@ def main():
     M.init(0)
     if args.ensemble:
         run_ensemble(M, args)
+        return
+    if args.mhd:
+        run_mhd(M, args)
         return
     m, x, b, types, hs, bh = make_inputs(M)
     P = M.PCG_solver_SF(m, x, M.iter_solver_params(20000, TOL, 1e-300, 1))
