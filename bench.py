@@ -208,9 +208,9 @@ def main():
         Pk = M.PCG_solver_SF(m, xk, M.iter_solver_params(2, 0.0, 0.0, 1), prec="MG_FP32")
         Pk.solve(xk, b)
         mk = Pk.time_MG(50)
-        print(json.dumps({"mg_smooth_us": round(1e3 * mk[0], 2), "mg_restrict_us": round(1e3 * mk[1], 2),
-                          "mg_prolong_us": round(1e3 * mk[2], 2), "mg_tail_us": round(1e3 * mk[3], 2),
-                          "vcycle_us": round(1e3 * mk[4], 2)}), flush=True)
+        names = ["half_sweep_128", "restrict_128_to_64", "prolong_64_to_128", "tail", "vcycle", "fused_sweep_128",
+                 "half_sweep_64", "fused_sweep_64", "half_sweep_32", "fused_sweep_32"]
+        print(json.dumps({n + "_us": round(1e3 * v, 2) for n, v in zip(names, mk)}), flush=True)
         # calibration for the PMC byte counters: 20 device-to-device copies of
         # one 130^3 FP64 field (17,576,000 bytes read and written per copy)
         c = M.SF(m, "CC")
@@ -309,7 +309,8 @@ def main():
                 "traffic": load_traffic("mg_smooth_128_traffic.txt"),
                 "algorithmic_bytes": sm_bytes,
                 "per_kernel_us": {"mg_smooth_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
-                                  "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2)},
+                                  "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2),
+                                  "mg_fused_sweep_64": round(1e3 * mk[7], 2), "mg_fused_sweep_32": round(1e3 * mk[9], 2)},
                 "vcycle_us": round(1e3 * mk[4], 2),
                 "mixed_iteration_us": round(1e3 * mg_ms / max(mg_info.iterations, 1), 1)}
 

@@ -238,9 +238,11 @@ int moons_pcg_time_kernels(moons_pcg_t pcg, int reps, double ms[3]);
  * on the library stream, reps launches back to back each: ms[0] one
  * half-sweep (one colour) of the finest level, ms[1] restriction of its
  * residual, ms[2] prolongation into it, ms[3] the coarse end, ms[4] a whole
- * V-cycle.  Overwrites the multigrid arrays.  No reference counterpart
- * (profiling). */
-int moons_pcg_time_mg(moons_pcg_t pcg, int reps, double ms[5]);
+ * V-cycle, ms[5] one fused whole sweep (both colours, one launch) of the
+ * finest level, ms[6]/ms[7] a half-sweep / fused sweep of level 1 and
+ * ms[8]/ms[9] of level 2 (0 where the level does not exist).  Overwrites the
+ * multigrid arrays.  No reference counterpart (profiling). */
+int moons_pcg_time_mg(moons_pcg_t pcg, int reps, double ms[10]);
 
 /* --------------------------------------------- vector (VF) Jacobi-PCG */
 /* init_PCG_VF (code/handwritten/PCG/PCG_solver_extend.f90:117-180) for

@@ -288,8 +288,9 @@ class PCG_solver_SF:
 
     def time_MG(self, reps: int = 20):
         """ms per launch of the finest half-sweep, restriction, prolongation,
-        coarse end, and a whole V-cycle (multigrid solvers only)."""
-        ms = (C.c_double * 5)()
+        coarse end, a whole V-cycle, the finest fused sweep, and the half-sweep
+        and fused sweep of levels 1 and 2 (multigrid solvers only)."""
+        ms = (C.c_double * 10)()
         check(load().moons_pcg_time_mg(self.handle, reps, ms))
         return list(ms)
 

@@ -31,6 +31,7 @@ struct LevelDev {
   int h0, half;                // cells per colour and row ((n0+1)/2), per colour (h0*n1*n2)
   int per0, per1, per2;
   float *x, *f;
+  float *xalt;                 // the other buffer of x for the fused sweeps (swapped with x on the host)
   const float *wm[3], *wp[3], *dg[3];
   const float *rf[3];          // restriction fractions into the next coarser level
   const int *pI[3], *pI2[3];   // prolongation sources in the next coarser level
@@ -190,6 +191,58 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
     s += wmj * xjm + wpj * xjp;
     s += L.wm[2][k] * xkm + L.wp[2][k] * xkp;
     if (!skip) xc[c] = (fv - s) / d;
+  }
+}
+
+// One whole red-black sweep (colour c_first, then the other) in one launch,
+// from x = L.x into xn, every value read from L.x: a thread owns the pair
+// (2p+1, 2p+2) of row (j, k), computes the new value of its c_first cell,
+// then that of its other cell from the new values of that cell's neighbours,
+// each recomputed from L.x (its pair partner's is the value just computed).
+// The arithmetic is that of the two half-sweeps (gs_cell, the same operand
+// order), so the result is identical; the separate output buffer removes the
+// hazard between tiles, and the recomputation costs loads the L1 and L2 serve.
+// A neighbour that a wall clamps to the cell itself is read from L.x, as the
+// second half-sweep reads the cell's own old value.
+__global__ void __launch_bounds__(256) k_mg_sweep2(LevelDev L, float *__restrict__ xn, int c_first, int zero_first,
+                                                   int kz, const int *done) {
+  const bool skip = done && *done; // tested at the stores (overlaps the loads)
+  int p, j, k0;
+  tile_origin(kz, p, j, k0);
+  if (p >= L.h0 || j > L.n1) return;
+  const float *xo = L.x;
+  const bool zg = zero_first != 0;
+  const int kend = min(k0 + kz, L.n2 + 1);
+  for (int k = k0; k < kend; ++k) {
+    const int ia = 2 * p + 1 + ((c_first + j + k + 1) & 1); // the pair's cell of colour c_first
+    const int ib = 2 * p + 1 + ((c_first + j + k) & 1);     // and its cell of the other colour
+    const bool has_a = ia <= L.n0, has_b = ib <= L.n0;
+    const float va = has_a ? gs_cell(L, xo, ia, j, k, zg) : 0.f;
+    float vb = 0.f;
+    if (has_b) {
+      const int im = (ib == 1) ? (L.per0 ? L.n0 : 1) : ib - 1;
+      const int ip = (ib == L.n0) ? (L.per0 ? 1 : L.n0) : ib + 1;
+      const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1;
+      const int jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
+      const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1;
+      const int kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
+      // the value the second half-sweep reads at neighbour (ii, jj, kk) of ib
+      auto nv = [&](int ii, int jj, int kk) -> float {
+        if (ii == ib && jj == j && kk == k) return xo[lidx(L, ib, j, k)];
+        if (ii == ia && jj == j && kk == k) return va;
+        return gs_cell(L, xo, ii, jj, kk, zg);
+      };
+      const float d = L.dg[0][ib] + L.dg[1][j] + L.dg[2][k];
+      const float fv = L.f[lidx(L, ib, j, k)];
+      float s = L.wm[0][ib] * nv(im, j, k) + L.wp[0][ib] * nv(ip, j, k);
+      s += L.wm[1][j] * nv(ib, jm, k) + L.wp[1][j] * nv(ib, jp, k);
+      s += L.wm[2][k] * nv(ib, j, km) + L.wp[2][k] * nv(ib, j, kp);
+      vb = (fv - s) / d;
+    }
+    if (!skip) {
+      if (has_a) xn[lidx(L, ia, j, k)] = va;
+      if (has_b) xn[lidx(L, ib, j, k)] = vb;
+    }
   }
 }
 
@@ -720,8 +773,9 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
       return nullptr;
     }
     L.x = dalloc(ncell);
+    L.xalt = dalloc(ncell);
     L.f = dalloc(ncell);
-    if (!L.x || !L.f) {
+    if (!L.x || !L.xalt || !L.f) {
       if (err) *err = "multigrid: hipMalloc failed";
       mg_destroy(h);
       return nullptr;
@@ -991,6 +1045,27 @@ static void smooth(const LevelDev &L, int color, bool zero_guess, const int *don
   hipLaunchKernelGGL(k_mg_smooth, t.grid, t.block, 0, stream(), L, color, zero_guess ? 1 : 0, t.kz, done);
 }
 
+// Levels of at most this many cells are smoothed by fused whole sweeps (one
+// launch each, x swapped between its two buffers); larger levels by two
+// in-place half-sweep launches, whose loads the recomputation would multiply.
+constexpr long long kFusedSweepMaxCells = 64LL * 64 * 64;
+
+static void sweep2(LevelDev &L, int c_first, bool zero_first, const int *done) {
+  const TileLaunch t = tiles((L.n0 + 1) / 2, L.n1, L.n2);
+  hipLaunchKernelGGL(k_mg_sweep2, t.grid, t.block, 0, stream(), L, L.xalt, c_first, zero_first ? 1 : 0, t.kz, done);
+  std::swap(L.x, L.xalt);
+}
+
+// One red-black sweep of level L, colour c_first first.
+static void sweep(LevelDev &L, int c_first, bool zero_first, const int *done) {
+  if ((long long)L.n0 * L.n1 * L.n2 <= kFusedSweepMaxCells) {
+    sweep2(L, c_first, zero_first, done);
+  } else {
+    smooth(L, c_first, zero_first, done);
+    smooth(L, 1 - c_first, false, done);
+  }
+}
+
 static void restrict_to(const LevelDev &F, const LevelDev &C, const int *done) {
   const TileLaunch t = tiles(C.n0, C.n1, C.n2);
   hipLaunchKernelGGL(k_mg_restrict, t.grid, t.block, 0, stream(), F, C, t.kz, done);
@@ -1020,10 +1095,7 @@ int mg_vcycle(MGHier *h, const int *done) {
       hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
       hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
     }
-    for (int s = 0; s < h->nu1; ++s) {
-      smooth(L, 0, s == 0 && !learned, done);
-      smooth(L, 1, false, done);
-    }
+    for (int s = 0; s < h->nu1; ++s) sweep(L, 0, s == 0 && !learned, done);
     restrict_to(L, h->L[l + 1], done);
   }
   if (tail)
@@ -1035,30 +1107,33 @@ int mg_vcycle(MGHier *h, const int *done) {
   for (int l = g - 1; l >= 0; --l) {
     LevelDev &L = h->L[l];
     prolong_from(L, h->L[l + 1], done);
-    for (int s = 0; s < h->nu2; ++s) {
-      smooth(L, 1, false, done);
-      smooth(L, 0, false, done);
-    }
+    for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));
   return MOONS_OK;
 }
 
-int mg_time_kernels(MGHier *h, int reps, double ms[5]) {
+int mg_time_kernels(MGHier *h, int reps, double ms[10]) {
   const int nl = (int)h->L.size();
   hipEvent_t e[2];
   for (int i = 0; i < 2; ++i)
     if (hipEventCreate(&e[i]) != hipSuccess) return fail(MOONS_ERR_HIP, "hipEventCreate failed");
   int st = MOONS_OK;
-  for (int w = 0; w < 5 && st == MOONS_OK; ++w) {
+  for (int w = 0; w < 10 && st == MOONS_OK; ++w) {
     ms[w] = 0.0;
     if ((w == 1 || w == 2) && nl < 2) continue;
+    // 5, 7, 9: a fused whole sweep of level 0, 1, 2; 6, 8: a half-sweep of level 1, 2
+    const int lw = w == 5 ? 0 : (w == 6 || w == 7) ? 1 : 2;
+    const int g = h->tail0 < nl ? h->tail0 : nl - 1; // grid-launched levels
+    if (w >= 5 && lw >= g) continue;
     auto launch = [&]() -> int {
       switch (w) {
         case 0: smooth(h->L[0], 0, false, nullptr); return MOONS_OK;
         case 1: restrict_to(h->L[0], h->L[1], nullptr); return MOONS_OK;
         case 2: prolong_from(h->L[0], h->L[1], nullptr); return MOONS_OK;
+        case 5: case 7: case 9: sweep2(h->L[lw], 0, false, nullptr); return MOONS_OK;
+        case 6: case 8: smooth(h->L[lw], 0, false, nullptr); return MOONS_OK;
         case 3:
           if (h->tail0 < nl)
             hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail,
@@ -1066,7 +1141,8 @@ int mg_time_kernels(MGHier *h, int reps, double ms[5]) {
           else
             hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, nullptr);
           return MOONS_OK;
-        default: return mg_vcycle(h, nullptr);
+        case 4: return mg_vcycle(h, nullptr);
+        default: return MOONS_OK;
       }
     };
     for (int r = 0; r < 2 && st == MOONS_OK; ++r) st = launch(); // warm-up
@@ -1083,6 +1159,8 @@ int mg_time_kernels(MGHier *h, int reps, double ms[5]) {
       break;
     }
     ms[w] = (double)t / reps;
+    // the fused sweeps swapped x of level lw (reps + 2) times: restore its buffer
+    if ((w == 5 || w == 7 || w == 9) && (reps % 2) != 0) std::swap(h->L[lw].x, h->L[lw].xalt);
   }
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(e[i]);
   return st;

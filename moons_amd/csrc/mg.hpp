@@ -31,8 +31,10 @@ void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
 // Average duration (ms) of one launch, reps launches back to back on the
 // library stream: [0] a half-sweep of level 0, [1] restriction 0 -> 1,
 // [2] prolongation 1 -> 0 (0 when there is one level), [3] the coarse end,
-// [4] a whole V-cycle.  Overwrites the level arrays.
-int mg_time_kernels(MGHier *h, int reps, double ms[5]);
+// [4] a whole V-cycle, [5] a fused whole sweep of level 0, [6]/[7] a
+// half-sweep / fused sweep of level 1, [8]/[9] of level 2 (0 for a level
+// that does not exist).  Overwrites the level arrays.
+int mg_time_kernels(MGHier *h, int reps, double ms[10]);
 // Learned initial guess on grid level `level` (net == nullptr removes it):
 // before that level's pre-smoothing, x = c N(f) with the Galerkin factor c.
 int mg_set_learned(MGHier *h, moons_unet_s *net, int level, std::string *err);

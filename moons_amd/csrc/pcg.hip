@@ -991,7 +991,7 @@ int moons_pcg_apply_prec(moons_pcg_t S, moons_field_t r, moons_field_t z) {
   return fail(MOONS_ERR_ARG, "moons_pcg_apply_prec: the Jacobi preconditioner is applied inside the PCG kernels only");
 }
 
-int moons_pcg_time_mg(moons_pcg_t S, int reps, double ms[5]) {
+int moons_pcg_time_mg(moons_pcg_t S, int reps, double ms[10]) {
   if (!S || !ms || reps < 1) return fail(MOONS_ERR_ARG, "moons_pcg_time_mg: bad argument");
   if (!S->mg) return fail(MOONS_ERR_ARG, "moons_pcg_time_mg: the solver has no multigrid preconditioner");
   return mg_time_kernels(S->mg, reps, ms);
