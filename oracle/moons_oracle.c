@@ -36,6 +36,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Threads of the OpenMP loops (the CPU timing baseline).  With 1 (the
+   default) every loop runs sequentially.  With more, the stencil pencils and
+   element-wise loops split over threads with unchanged results, and the
+   padded sums add per-plane partial sums in plane order, which changes their
+   last bits (used for timing only). */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 1 ? n : 1; }
+
 #define IDX(s, i, j, k) ((size_t)(i) + (size_t)(s)[0] * ((size_t)(j) + (size_t)(s)[1] * (size_t)(k)))
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -418,6 +426,7 @@ void orc_del(double *dfdh, int dl_out, const double *f, int dl_in, const orc_gri
   size_t stro[3] = {1, (size_t)so[0], (size_t)so[0] * so[1]};
   int a = (dir + 1) % 3, b = (dir + 2) % 3;
   int lo = (a < b) ? a : b, hi = (a < b) ? b : a; /* reference loops the outer dims in k,j / k,i / j,i */
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
   for (int q = pad; q < sf[hi] - pad; ++q)
     for (int p = pad; p < sf[lo] - pad; ++p) {
       size_t offf = (size_t)p * strf[lo] + (size_t)q * strf[hi];
@@ -742,6 +751,20 @@ void orc_lap_diagonal(double *diag, int dl, const orc_grid *g) {
 /* sum_GF_pad: hw/grid_field/GF_aux.f90:140-158 (sequential order k, j, i) */
 double orc_sum_pad(const double *u, const int s[3], int pad) {
   double m = 0.0;
+  if (g_threads > 1) {
+    const int nk = s[2] - 2 * pad;
+    double *part = (double *)calloc(nk > 0 ? (size_t)nk : 1, sizeof(double));
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = pad; k < s[2] - pad; ++k) {
+      double a = 0.0;
+      for (int j = pad; j < s[1] - pad; ++j)
+        for (int i = pad; i < s[0] - pad; ++i) a += u[IDX(s, i, j, k)];
+      part[k - pad] = a;
+    }
+    for (int k = 0; k < nk; ++k) m += part[k];
+    free(part);
+    return m;
+  }
   for (int k = pad; k < s[2] - pad; ++k)
     for (int j = pad; j < s[1] - pad; ++j)
       for (int i = pad; i < s[0] - pad; ++i) m += u[IDX(s, i, j, k)];
@@ -759,6 +782,7 @@ static double dot_sf(const double *A, const double *B, int dl, const orc_grid *g
   int s[3];
   orc_field_shape(g, dl, s);
   size_t n = orc_field_size(g, dl);
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
   for (size_t q = 0; q < n; ++q) tmp[q] = A[q] * B[q];
   orc_assign_wall_dirichlet(tmp, dl, g, bcx, 0.0);
   return orc_sum_pad(tmp, s, 1);
@@ -793,8 +817,10 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
   /* modify_RHS_SF: PCG_aux.f90:30-54 */
   memcpy(r, b, sizeof(double) * n);
   if (all_neumann) { /* subtract_phys_mean_vol_SF: ops_aux.f90:146-167 */
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
     for (size_t q = 0; q < n; ++q) tmp[q] = r[q] * vol[q];
     double mean = orc_sum_pad(tmp, s, 1) / volsum;
+#pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
     for (size_t q = 0; q < n; ++q) r[q] = r[q] - mean;
     orc_assign_ghost_xperiodic(r, dl, g, NULL, 0.0);
   }
@@ -837,22 +863,27 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
       orc_multiply_wall_neumann(Ax, dl, g, bc, 0.5);
       orc_assign_wall_dirichlet(Ax, dl, g, bc, 0.0);
       assign_wall_periodic_single(Ax, dl, g, bc, 0.0);
-      for (size_t q = 0; q < n; ++q) Ax[q] = Ax[q] * vol[q];
+  #pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (size_t q = 0; q < n; ++q) Ax[q] = Ax[q] * vol[q];
       double alpha = rhok / dot_sf(p, Ax, dl, g, bc, tmp);
-      for (size_t q = 0; q < n; ++q) x[q] = x[q] + p[q] * alpha;
+  #pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (size_t q = 0; q < n; ++q) x[q] = x[q] + p[q] * alpha;
       orc_apply_bcs(x, dl, g, bc, 0);
       iter++;
-      for (size_t q = 0; q < n; ++q) r[q] = r[q] + Ax[q] * (-alpha);
+  #pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (size_t q = 0; q < n; ++q) r[q] = r[q] + Ax[q] * (-alpha);
       if (iter % n_skip == 0) {
         res = sqrt(fabs(dot_sf(r, r, dl, g, bc, tmp)));
         if (res_hist) res_hist[nh++] = res;
         e1 = res / res0 < tol_rel; e2 = res < tol_abs; e3 = iter >= iter_max;
         if (e1 || e2 || e3) break;
       }
-      for (size_t q = 0; q < n; ++q) z[q] = Minv[q] * r[q];
+  #pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (size_t q = 0; q < n; ++q) z[q] = Minv[q] * r[q];
       double rhokp1 = dot_sf(z, r, dl, g, bc, tmp);
       double beta = rhokp1 / rhok;
-      for (size_t q = 0; q < n; ++q) p[q] = p[q] * beta + z[q];
+  #pragma omp parallel for if (g_threads > 1) num_threads(g_threads) schedule(static)
+    for (size_t q = 0; q < n; ++q) p[q] = p[q] * beta + z[q];
       rhok = rhokp1;
     }
   } else {

@@ -117,6 +117,8 @@ void orc_multiply_wall_neumann(double *u, int dl, const orc_grid *g, const orc_b
 void orc_volume(double *vol, int dl, const orc_grid *g);
 void orc_lap_diagonal(double *diag, int dl, const orc_grid *g);
 double orc_sum_pad(const double *u, const int s[3], int pad);
+/* Threads of the OpenMP loops (1, the default: sequential, the exact order). */
+void orc_set_threads(int n);
 double orc_sum_all(const double *u, size_t n);
 
 /* ---- PCG for the scalar Laplacian (PPE / phi cleaning) ----

@@ -244,6 +244,14 @@ def lap_diagonal(g: Grid, dl):
     return out
 
 
+def set_threads(n: int) -> None:
+    """Threads of the C oracle's OpenMP loops.  1 (the default) keeps every
+    sum in its sequential order; more threads split the stencils and the
+    element-wise loops (unchanged results) and add padded sums per plane (last
+    bits differ), for the CPU timing baseline."""
+    lib().orc_set_threads(int(n))
+
+
 def pcg_lap(g: Grid, dl, bcs: BCs, all_neumann, x, b, iter_max, tol_rel, tol_abs, n_skip=1):
     """Returns (iterations, res0, res, res_history); x is updated in place."""
     out = np.zeros(4)
