@@ -85,6 +85,7 @@ typedef struct moons_pcg_s *moons_pcg_t;
 typedef struct moons_pcgvf_s *moons_pcgvf_t;
 typedef struct moons_mhd_s *moons_mhd_t;
 typedef struct moons_unet_s *moons_unet_t;
+typedef struct moons_fft_s *moons_fft_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
@@ -212,6 +213,31 @@ int moons_pcg_sf_solve(moons_pcg_t pcg, moons_field_t x, moons_field_t b, moons_
 int moons_clean_div(moons_pcg_t pcg, moons_field_t X[3], moons_field_t Xstar[3], moons_field_t phi,
                     double scale, moons_field_t temp_F[3], moons_field_t temp_CC,
                     moons_pcg_info *info);
+
+/* ------------------------------------------ cosine-transform direct solver */
+
+/* init(FFT, u, m, direction, var_name)
+ * (code/handwritten/FFT_solver/FFT_solver_extend.f90:43-126): a direct solver
+ * of the cell-centred Poisson problem in which every plane normal to dir
+ * (1, 2, 3) is an independent 2D problem in the transverse directions, solved
+ * by staggered DCT-II transforms (code/handwritten/ops/ops_dct.f90:36-70).
+ * The transverse spacings must be uniform (MOONS_ERR_ARG otherwise).  The
+ * eigenvalues are those of the 5-point Neumann Laplacian for any transverse
+ * spacings; the reference's factor equals them for equal spacings only, and
+ * for dir = 1 it guards the zero mode with the wrong indices (see fft.hip). */
+int moons_fft_create(moons_mesh_t m, int dir, moons_fft_t *fft);
+int moons_fft_destroy(moons_fft_t fft);
+/* solve(FFT, u, f, m, compute_norms) (FFT_solver_extend.f90:128-137;
+ * solve_FFT_SF_x/y/z, code/handwritten/FFT_solver/FFT_algorithm.f90:27-210):
+ * u's interior from the transforms of f's interior, the zero transverse mode
+ * of every plane set to 0 ("pin down pressure"), then apply_BCs(u).  u and f
+ * are cell-centred fields of the solver's mesh. */
+int moons_fft_solve(moons_fft_t fft, moons_field_t u, moons_field_t f);
+/* clean_div_FFT (code/handwritten/solvers/clean_divergence.f90:61-83):
+ *   temp_CC = scale*div(Xstar); solve(FFT, phi, temp_CC);
+ *   X = Xstar - grad(phi)/scale; apply_BCs(X). */
+int moons_clean_div_fft(moons_fft_t fft, moons_field_t X[3], moons_field_t Xstar[3], moons_field_t phi,
+                        double scale, moons_field_t temp_F[3], moons_field_t temp_CC);
 
 /* Multigrid parameters of a MOONS_PREC_MG_FP32 solver (rebuilds the level
  * hierarchy): at most max_levels levels, nu1/nu2 red-black Gauss-Seidel sweeps

@@ -7,6 +7,7 @@ from ._lib import LIB_PATH, MoonsError, load  # noqa: F401
 from .core import (  # noqa: F401
     BC,
     DL,
+    FFT_solver_SF,
     PCG_solver_SF,
     PCG_solver_VF,
     MHD_step,
@@ -18,6 +19,7 @@ from .core import (  # noqa: F401
     apply_BCs,
     apply_BCs_implicit,
     clean_div,
+    clean_div_FFT,
     curl,
     del_op,
     div,

@@ -77,6 +77,11 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(PcgInfo)]),
     "moons_pcg_time_kernels": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "moons_pcg_time_mg": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "moons_fft_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_fft_destroy": (C.c_int, [C.c_void_p]),
+    "moons_fft_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_clean_div_fft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                      C.c_void_p]),
     "moons_pcg_set_mg": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "moons_pcg_mg_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "moons_pcg_apply_prec": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

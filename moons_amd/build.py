@@ -63,7 +63,8 @@ def build_library() -> str:
         _run([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lrocblas",
+          "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

@@ -312,6 +312,35 @@ def clean_div(PCG: PCG_solver_SF, X, Xstar, phi: SF, scale: float, temp_F, temp_
     return pcg_info(info.iterations, info.exit_loop, info.res_norm0, info.res_norm, info.ms)
 
 
+class FFT_solver_SF:
+    """The cosine-transform direct solver (init_FFT / solve,
+    code/handwritten/FFT_solver/FFT_solver_extend.f90:43-137): every plane
+    normal to `direction` (1, 2, 3) is an independent 2D Poisson problem solved
+    by DCT-II transforms; the transverse spacings must be uniform."""
+
+    def __init__(self, m: mesh, direction: int):
+        h = C.c_void_p()
+        check(load().moons_fft_create(m.handle, int(direction), C.byref(h)))
+        self.handle = h
+        self.m = m
+
+    def solve(self, u: SF, f: SF) -> None:
+        check(load().moons_fft_solve(self.handle, u.handle, f.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_fft_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def clean_div_FFT(FFT: FFT_solver_SF, X, Xstar, phi: SF, scale: float, temp_F, temp_CC: SF) -> None:
+    """clean_div_FFT (code/handwritten/solvers/clean_divergence.f90:61-83)."""
+    check(load().moons_clean_div_fft(FFT.handle, _arr(X), _arr(Xstar), phi.handle, scale, _arr(temp_F),
+                                     temp_CC.handle))
+
+
 class PCG_solver_VF:
     """Jacobi-PCG for the face-centred implicit diffusion solves (init_PCG_VF,
     code/handwritten/PCG/PCG_solver_extend.f90:117-180).
