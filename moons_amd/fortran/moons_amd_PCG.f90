@@ -5,6 +5,9 @@
 !   init(PCG, ...)          init_PCG_SF   (code/handwritten/PCG/PCG_solver_extend.f90:51-107)
 !   solve(PCG, x, b, ...)   solve_PCG_SF  (code/handwritten/PCG/PCG_solver_extend.f90:182-193)
 !   clean_div(PCG, ...)     clean_div_PCG (code/handwritten/solvers/clean_divergence.f90:37-59)
+!   init(FFT, m, dir), solve(FFT, u, f), clean_div(FFT, ...)
+!                           init_FFT / solve (code/handwritten/FFT_solver/FFT_solver_extend.f90:43-137)
+!                           and clean_div_FFT (clean_divergence.f90:61-83)
 ! Fields are the host's ghost-padded, column-major real(8) arrays (the layout
 ! of grid_field%f, code/generated_code/grid_field/grid_field.f90:39-43); they
 ! are copied to the device at solve entry and back at exit (the PCIe-inclusive
@@ -15,7 +18,7 @@ module moons_amd_PCG_mod
   implicit none
   private
 
-  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu
+  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu, FFT_solver_SF_gpu
   public :: init, solve, delete, clean_div, update_MFP
   public :: init_CC, init_Face, set_BCs
 
@@ -46,12 +49,22 @@ module moons_amd_PCG_mod
     type(SF_gpu) :: x(3), b(3), k(3)
   end type
 
+  ! init_FFT / solve / clean_div_FFT: the cosine-transform direct solver of
+  ! the planes normal to dir (its u carries the BCs)
+  type FFT_solver_SF_gpu
+    type(c_ptr) :: h = c_null_ptr
+    type(SF_gpu) :: u, f
+  end type
+
   interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu
-                    module procedure init_PCG_VF_gpu; end interface
-  interface solve;  module procedure solve_PCG_SF_gpu; module procedure solve_PCG_VF_gpu; end interface
+                    module procedure init_PCG_VF_gpu; module procedure init_FFT_gpu; end interface
+  interface solve;  module procedure solve_PCG_SF_gpu; module procedure solve_PCG_VF_gpu
+                    module procedure solve_FFT_gpu; end interface
+  interface clean_div; module procedure clean_div_PCG_gpu; module procedure clean_div_FFT_gpu; end interface
   interface update_MFP; module procedure update_MFP_PCG_VF_gpu; end interface
   interface delete; module procedure delete_mesh_gpu; module procedure delete_SF_gpu;
-                    module procedure delete_PCG_SF_gpu; module procedure delete_PCG_VF_gpu; end interface
+                    module procedure delete_PCG_SF_gpu; module procedure delete_PCG_VF_gpu
+                    module procedure delete_FFT_gpu; end interface
 
 contains
 
@@ -151,7 +164,7 @@ contains
 
   ! clean_div_PCG on host face arrays: X = Xstar - grad(phi)/scale with
   ! lap(phi) = scale*div(Xstar); X carries its BCs (types, vals).
-  subroutine clean_div(PCG, m, X1, X2, X3, Xs1, Xs2, Xs3, X_types, X_vals, phi, scale)
+  subroutine clean_div_PCG_gpu(PCG, m, X1, X2, X3, Xs1, Xs2, Xs3, X_types, X_vals, phi, scale)
     type(PCG_solver_SF_gpu), intent(inout) :: PCG
     type(mesh_gpu), intent(in) :: m
     real(c_double), intent(inout), contiguous :: X1(:, :, :), X2(:, :, :), X3(:, :, :)
@@ -184,6 +197,76 @@ contains
       call delete(X(d)); call delete(Xs(d)); call delete(T(d))
     enddo
     call delete(tcc)
+  end subroutine
+
+  ! init_FFT: direction dir (1, 2, 3); u's BCs (types, vals)
+  subroutine init_FFT_gpu(FFT, m, dir, u_types, u_vals)
+    type(FFT_solver_SF_gpu), intent(inout) :: FFT
+    type(mesh_gpu), intent(in) :: m
+    integer, intent(in) :: dir
+    integer(c_int), intent(in) :: u_types(6)
+    real(c_double), intent(in) :: u_vals(6)
+    call init_CC(FFT%u, m)
+    call init_CC(FFT%f, m)
+    call set_BCs(FFT%u, u_types, u_vals, .false.)
+    call moons_check(moons_fft_create(m%h, int(dir, c_int), FFT%h), 'init_FFT')
+  end subroutine
+
+  ! solve(FFT, u, f) on host arrays (ghosts included)
+  subroutine solve_FFT_gpu(FFT, u, f)
+    type(FFT_solver_SF_gpu), intent(inout) :: FFT
+    real(c_double), intent(inout), contiguous :: u(:, :, :)
+    real(c_double), intent(in), contiguous :: f(:, :, :)
+    integer(c_size_t) :: n
+    n = int(size(u), c_size_t)
+    if (any(shape(u) .ne. FFT%u%s) .or. any(shape(f) .ne. FFT%u%s)) then
+      error stop 'Error: u and f must have the cell-centred shape in solve_FFT_gpu'
+    endif
+    call moons_check(moons_field_upload(FFT%f%h, f, n), 'upload f')
+    call moons_check(moons_fft_solve(FFT%h, FFT%u%h, FFT%f%h), 'solve_FFT')
+    call moons_check(moons_field_download(FFT%u%h, u, n), 'download u')
+  end subroutine
+
+  ! clean_div_FFT on host face arrays: X = Xstar - grad(phi)/scale with
+  ! phi from the cosine-transform solve of scale*div(Xstar)
+  subroutine clean_div_FFT_gpu(FFT, m, X1, X2, X3, Xs1, Xs2, Xs3, X_types, X_vals, phi, scale)
+    type(FFT_solver_SF_gpu), intent(inout) :: FFT
+    type(mesh_gpu), intent(in) :: m
+    real(c_double), intent(inout), contiguous :: X1(:, :, :), X2(:, :, :), X3(:, :, :)
+    real(c_double), intent(in), contiguous :: Xs1(:, :, :), Xs2(:, :, :), Xs3(:, :, :)
+    integer(c_int), intent(in) :: X_types(6)
+    real(c_double), intent(in) :: X_vals(6)
+    real(c_double), intent(inout), contiguous :: phi(:, :, :)
+    real(c_double), intent(in) :: scale
+    type(SF_gpu) :: X(3), Xs(3), T(3)
+    type(c_ptr) :: hX(3), hXs(3), hT(3)
+    integer :: d
+    do d = 1, 3
+      call init_Face(X(d), m, d)
+      call init_Face(Xs(d), m, d)
+      call init_Face(T(d), m, d)
+      call set_BCs(X(d), X_types, X_vals, .false.)
+      hX(d) = X(d)%h; hXs(d) = Xs(d)%h; hT(d) = T(d)%h
+    enddo
+    call moons_check(moons_field_upload(Xs(1)%h, Xs1, int(size(Xs1), c_size_t)), 'upload')
+    call moons_check(moons_field_upload(Xs(2)%h, Xs2, int(size(Xs2), c_size_t)), 'upload')
+    call moons_check(moons_field_upload(Xs(3)%h, Xs3, int(size(Xs3), c_size_t)), 'upload')
+    call moons_check(moons_clean_div_fft(FFT%h, hX, hXs, FFT%u%h, scale, hT, FFT%f%h), 'clean_div_FFT')
+    call moons_check(moons_field_download(X(1)%h, X1, int(size(X1), c_size_t)), 'download')
+    call moons_check(moons_field_download(X(2)%h, X2, int(size(X2), c_size_t)), 'download')
+    call moons_check(moons_field_download(X(3)%h, X3, int(size(X3), c_size_t)), 'download')
+    call moons_check(moons_field_download(FFT%u%h, phi, int(size(phi), c_size_t)), 'download')
+    do d = 1, 3
+      call delete(X(d)); call delete(Xs(d)); call delete(T(d))
+    enddo
+  end subroutine
+
+  subroutine delete_FFT_gpu(FFT)
+    type(FFT_solver_SF_gpu), intent(inout) :: FFT
+    if (c_associated(FFT%h)) call moons_check(moons_fft_destroy(FFT%h), 'fft destroy')
+    FFT%h = c_null_ptr
+    call delete(FFT%u)
+    call delete(FFT%f)
   end subroutine
 
   ! init_PCG_VF: op = MOONS_OP_MOM_DIFFUSION (prec_mom_VF) or

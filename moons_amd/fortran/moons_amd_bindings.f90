@@ -52,6 +52,7 @@ module moons_amd_bindings
   public :: moons_apply_bcs, moons_grad, moons_div
   public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
   public :: moons_clean_div
+  public :: moons_fft_create, moons_fft_destroy, moons_fft_solve, moons_clean_div_fft
   public :: moons_field_copy, moons_apply_bcs_implicit, moons_curl, moons_interp, moons_advect_B
   public :: moons_pcg_vf_create, moons_pcg_vf_destroy, moons_pcg_vf_set_isp, moons_pcg_vf_update_mfp
   public :: moons_pcg_vf_solve
@@ -167,6 +168,27 @@ module moons_amd_bindings
       type(c_ptr), intent(in) :: X(3), Xstar(3), temp_F(3)
       real(c_double), value :: scale
       type(moons_pcg_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_fft_create(m, dir, fft) bind(C, name="moons_fft_create")
+      import c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: dir
+      type(c_ptr), intent(out) :: fft
+    end function
+    integer(c_int) function moons_fft_destroy(fft) bind(C, name="moons_fft_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: fft
+    end function
+    integer(c_int) function moons_fft_solve(fft, u, f) bind(C, name="moons_fft_solve")
+      import c_int, c_ptr
+      type(c_ptr), value :: fft, u, f
+    end function
+    integer(c_int) function moons_clean_div_fft(fft, X, Xstar, phi, scale, temp_F, temp_CC) &
+        bind(C, name="moons_clean_div_fft")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: fft, phi, temp_CC
+      type(c_ptr), intent(in) :: X(3), Xstar(3), temp_F(3)
+      real(c_double), value :: scale
     end function
     integer(c_int) function moons_field_copy(dst, src) bind(C, name="moons_field_copy")
       import c_int, c_ptr
