@@ -2,6 +2,8 @@
 ! through moons_amd_PCG_mod, as a MOONS host would at clean_div/solve.
 !
 ! usage: ppe_driver N beta prec iter_max tol b.bin x.bin
+!   prec: MOONS_PREC_* of the PCG, or 0 for the cosine-transform direct
+!         solver of the x-planes (init_FFT / solve; needs beta <= 1)
 !   mesh: x uniform on [0,1], y and z Roberts beta (uniform when beta <= 1)
 !   BCs:  homogeneous Neumann on all faces, all_Neumann
 !   b.bin: the right-hand side, real(8), ghost-padded (N+2)^3, column-major
@@ -18,6 +20,7 @@ program ppe_driver
   real(c_double), allocatable :: hx(:), hy(:), x(:, :, :), b(:, :, :)
   type(mesh_gpu) :: m
   type(PCG_solver_SF_gpu) :: PCG
+  type(FFT_solver_SF_gpu) :: FFT
   integer(c_int) :: types(6)
   real(c_double) :: vals(6)
 
@@ -46,12 +49,19 @@ program ppe_driver
 
   types = MOONS_BC_NEUMANN
   vals = 0.0_c_double
-  call init(PCG, m, types, vals, .true., iter_max, tol, 1.0d-300, 1, int(prec_i, c_int))
-  call solve(PCG, x, b, .true.)
+  if (prec_i .eq. 0) then
+    call init(FFT, m, 1, types, vals)
+    call solve(FFT, x, b)
+    write(*, '(a)') ' FFT solve done'
+    call delete(FFT)
+  else
+    call init(PCG, m, types, vals, .true., iter_max, tol, 1.0d-300, 1, int(prec_i, c_int))
+    call solve(PCG, x, b, .true.)
+    call delete(PCG)
+  endif
 
   open(newunit=u, file=trim(xpath), access='stream', form='unformatted', status='replace')
   write(u) x
   close(u)
-  call delete(PCG)
   call delete(m)
 end program

@@ -42,6 +42,32 @@ def test_fortran_driver_ppe(tmp_path, prec, tol, bar):
     assert err < bar, err
 
 
+def test_fortran_driver_fft(tmp_path):
+    """init_FFT / solve from Fortran host arrays (ppe_driver prec 0): the x-plane
+    cosine-transform solve equals the restatement."""
+    from oracle import fft_oracle as FO
+
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    N = 16
+    hx = O.uniform_nodes(0.0, 1.0, N)
+    g = O.Grid(hx, hx, hx)
+    shape = g.shape(0)
+    b = np.asfortranarray(O.splitmix(int(np.prod(shape)), 32).reshape(shape, order="F"))
+    bpath, xpath = tmp_path / "b.bin", tmp_path / "x.bin"
+    b.ravel(order="F").tofile(bpath)
+    out = subprocess.run([DRIVER, str(N), "0", "0", "1", "1e-8", str(bpath), str(xpath)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "FFT solve done" in out.stdout
+    x = np.fromfile(xpath, dtype=np.float64).reshape(shape, order="F")
+    h = float(np.diff(hx)[1])
+    xr = np.zeros(shape, order="F")
+    xr[1:-1, 1:-1, 1:-1] = FO.solve_interior(b[1:-1, 1:-1, 1:-1], [h, h, h], 1)
+    O.apply_bcs(g, xr, 0, O.BCs.make(["Neumann"] * 6))
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+
+
 VF_DRIVER = os.path.join(ROOT, "moons_amd", "fortran", "_build", "vf_driver")
 
 
