@@ -180,15 +180,10 @@ struct Shape3 {
 // One face of apply_BCs / apply_BCs_implicit over the full plane (tangential
 // ghosts included), raw rules of code/handwritten/BCs/direct/apply_BCs_faces_raw.f90
 // and code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90.
-__global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int face, int type, int nalong,
-                                                       double bval, double dh_nhat, int implicit_, const int *gate) {
-  if (gate && *gate) return;
+__device__ __forceinline__ void apply_bc_point(double *u, const Shape3 &S, int face, int type, int nalong, double bval,
+                                               double dh_nhat, int implicit_, int ja, int jb) {
   const int d = face / 2, is_max = face % 2;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
-  const int na = S.s[a], nb = S.s[b];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= na * nb) return;
-  const int ja = t % na, jb = t / na;
   const int s = S.s[d];
   int G, B, I, Io;
   if (!nalong) {
@@ -227,6 +222,36 @@ __global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int 
     case MOONS_BC_ANTISYMMETRIC: u[ig] = -u[iin]; break;
     default: break;
   }
+}
+
+__global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int face, int type, int nalong,
+                                                       double bval, double dh_nhat, int implicit_, const int *gate) {
+  if (gate && *gate) return;
+  const int d = face / 2;
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na * nb) return;
+  apply_bc_point(u, S, face, type, nalong, bval, dh_nhat, implicit_, t % na, t / na);
+}
+
+// Both faces of direction d in one launch: the first na*nb threads the min
+// face, the others the max face.  Used when the direction has at least 6
+// points, where the planes one face writes are none of those the other reads,
+// so the result equals the min-then-max sequence.
+__global__ void __launch_bounds__(256) k_apply_bc_pair(double *u, Shape3 S, int d, int type0, int type1, int nalong,
+                                                       double bval0, double bval1, double dh0, double dh1,
+                                                       int implicit_, const int *gate) {
+  if (gate && *gate) return;
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int np = na * nb;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * np) return;
+  const int side = t >= np ? 1 : 0;
+  const int q = t - side * np;
+  apply_bc_point(u, S, 2 * d + side, side ? type1 : type0, nalong, side ? bval1 : bval0, side ? dh1 : dh0, implicit_,
+                 q % na, q / na);
 }
 
 // Assigns val to one whole plane idx along direction d.
@@ -269,12 +294,13 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
   const int lo = a < b ? a : b, hi = a < b ? b : a;
   const int nd = So.s[dir];
   const int nlo = Sf.s[lo] - 2 * pad, nhi = Sf.s[hi] - 2 * pad;
-  const long long total = (long long)nd * nlo * nhi;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // 32-bit: the launcher rejects fields of 2^31 or more elements
+  const int total = nd * nlo * nhi;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total || nlo <= 0 || nhi <= 0) return;
-  const int o = (int)(t % nd);
-  const long long rest = t / nd;
-  const int plo = pad + (int)(rest % nlo), phi = pad + (int)(rest / nlo);
+  const int o = t % nd;
+  const int rest = t / nd;
+  const int plo = pad + rest % nlo, phi = pad + rest / nlo;
   int io[3];
   io[dir] = o; io[lo] = plo; io[hi] = phi;
   const size_t oo = idx3(So.s, io[0], io[1], io[2]);
@@ -349,12 +375,13 @@ __global__ void __launch_bounds__(256) k_interp(double *__restrict__ out, const 
                                                 const double *__restrict__ tU) {
   const int a = (dir + 1) % 3, b = (dir + 2) % 3;
   const int nd = So.s[dir];
-  const long long total = (long long)nd * So.s[a] * So.s[b];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // 32-bit: the launcher rejects fields of 2^31 or more elements
+  const int total = nd * So.s[a] * So.s[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int o = (int)(t % nd);
-  const long long rest = t / nd;
-  const int pa = (int)(rest % So.s[a]), pb = (int)(rest / So.s[a]);
+  const int o = t % nd;
+  const int rest = t / nd;
+  const int pa = rest % So.s[a], pb = rest / So.s[a];
   int io[3], ii[3];
   io[dir] = o; io[a] = pa; io[b] = pb;
   ii[dir] = 0; ii[a] = pa; ii[b] = pb;
@@ -424,14 +451,24 @@ int launch_apply_bcs(moons_field_s *u, bool implicit_) {
   if (!u->has_bcs) return MOONS_OK;
   Shape3 S{{u->s[0], u->s[1], u->s[2]}};
   BCDev b = make_bcdev(u);
-  for (int face = 0; face < 6; ++face) {
-    if (b.type[face] == MOONS_BC_NONE) continue;
-    int d = face / 2;
-    int a = (d + 1) % 3, bb = (d + 2) % 3;
-    size_t np = (size_t)u->s[a] * u->s[bb];
-    hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
-                       (u->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
-    MOONS_LAUNCH_CHECK();
+  for (int d = 0; d < 3; ++d) {
+    const int a = (d + 1) % 3, bb = (d + 2) % 3;
+    const size_t np = (size_t)u->s[a] * u->s[bb];
+    const int nalong = (u->dl >> d) & 1;
+    const int f0 = 2 * d, f1 = 2 * d + 1;
+    if (b.type[f0] != MOONS_BC_NONE && b.type[f1] != MOONS_BC_NONE && u->s[d] >= 6) {
+      hipLaunchKernelGGL(k_apply_bc_pair, dim3(nblk(2 * np)), dim3(256), 0, g_stream, u->d, S, d, b.type[f0],
+                         b.type[f1], nalong, b.val[f0], b.val[f1], b.dh_nhat[f0], b.dh_nhat[f1], implicit_ ? 1 : 0,
+                         g_gate);
+      MOONS_LAUNCH_CHECK();
+      continue;
+    }
+    for (int face : {f0, f1}) {
+      if (b.type[face] == MOONS_BC_NONE) continue;
+      hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
+                         nalong, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
+      MOONS_LAUNCH_CHECK();
+    }
   }
   return MOONS_OK;
 }
@@ -483,6 +520,8 @@ int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int p
   Shape3 So{{out->s[0], out->s[1], out->s[2]}}, Sf{{f->s[0], f->s[1], f->s[2]}};
   int a = (dir + 1) % 3, b = (dir + 2) % 3;
   size_t total = (size_t)out->s[dir] * (size_t)(f->s[a] - 2 * pad) * (size_t)(f->s[b] - 2 * pad);
+  if (total >= ((size_t)1 << 31) || out->n >= ((size_t)1 << 31) || f->n >= ((size_t)1 << 31))
+    return fail(MOONS_ERR_ARG, "del: fields of 2^31 or more elements are not supported");
   hipLaunchKernelGGL(k_del, dim3(nblk(total)), dim3(256), 0, g_stream, out->d, f->d, So, Sf, dir, kind, L, D, U,
                      gen, pad, g_gate);
   MOONS_LAUNCH_CHECK();
@@ -507,6 +546,8 @@ int launch_interp(moons_field_s *out, const moons_field_s *in, int dir) {
     return fail(MOONS_ERR_ARG, "Error: bad DLs in interpO2_SF (ops_interp.f90)");
   const DirCoefDev &c = in->m->dev.c[dir];
   Shape3 So{{out->s[0], out->s[1], out->s[2]}}, Si{{in->s[0], in->s[1], in->s[2]}};
+  if (out->n >= ((size_t)1 << 31) || in->n >= ((size_t)1 << 31))
+    return fail(MOONS_ERR_ARG, "interp: fields of 2^31 or more elements are not supported");
   hipLaunchKernelGGL(k_interp, dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, in->d, So, Si, dir, nout, c.theta_D,
                      c.theta_U);
   MOONS_LAUNCH_CHECK();

@@ -104,12 +104,14 @@ __global__ void __launch_bounds__(kVT) k_vf_dot(VF3 a, VF3 b, int excl, double *
   __shared__ double sh[kVT / kWave];
   const int d = blockIdx.y;
   const int s0 = a.s[d][0], s1 = a.s[d][1], s2 = a.s[d][2];
-  const long long ni = (long long)(s0 - 2) * (s1 - 2) * (s2 - 2);
+  const int n0 = s0 - 2, n01 = (s0 - 2) * (s1 - 2);
+  const int ni = n01 * (s2 - 2); // fields of the VF solver stay below 2^31 elements (checked at creation)
   double acc = 0.0;
-  for (long long t = (long long)blockIdx.x * kVT + threadIdx.x; t < ni; t += (long long)gridDim.x * kVT) {
-    const int i = 1 + (int)(t % (s0 - 2));
-    const int j = 1 + (int)((t / (s0 - 2)) % (s1 - 2));
-    const int k = 1 + (int)(t / ((long long)(s0 - 2) * (s1 - 2)));
+  for (int t = blockIdx.x * kVT + threadIdx.x; t < ni; t += gridDim.x * kVT) {
+    const int k = 1 + t / n01;
+    const int r = t - (k - 1) * n01;
+    const int j = 1 + r / n0;
+    const int i = 1 + (r - (j - 1) * n0);
     const int ii[3] = {i, j, k};
     const int sd = a.s[d][d];
     if (((excl >> (2 * d)) & 1) && ii[d] == 1) continue;
@@ -364,6 +366,9 @@ int moons_pcg_vf_create(moons_mesh_t m, int op, moons_field_t x[3], moons_field_
   for (int d = 0; d < 3; ++d)
     if (!x[d] || x[d]->dl != (1 << d) || x[d]->m != m)
       return fail(MOONS_ERR_ARG, "moons_pcg_vf_create: x must be the three face components on this mesh");
+  for (int d = 0; d < 3; ++d)
+    if (x[d]->n >= ((size_t)1 << 31))
+      return fail(MOONS_ERR_ARG, "moons_pcg_vf_create: fields of 2^31 or more elements are not supported");
   if (op == MOONS_OP_IND_DIFFUSION) {
     const int edl[3] = {MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z};
     if (!k) return fail(MOONS_ERR_ARG, "moons_pcg_vf_create: ind_diffusion needs sigmaInv on edges");
