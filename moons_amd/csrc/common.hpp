@@ -159,6 +159,7 @@ int launch_interp(moons_field_s *out, const moons_field_s *in, int dir);
 // Gate for the general kernels (apply BCs, del, plane assignment): while set,
 // those kernels return at once when the device flag is non-zero.
 void set_gate(const int *gate);
+const int *current_gate();
 int launch_plane_set(moons_field_s *u, int d, int idx, double val);
 int launch_plane_scale(moons_field_s *u, int d, int idx, double val);
 int launch_cross(moons_field_s *out, const moons_field_s *a, const moons_field_s *b, const moons_field_s *c,

@@ -15,6 +15,7 @@ static int g_device = -1;
 static const int *g_gate = nullptr;
 
 void set_gate(const int *gate) { g_gate = gate; }
+const int *current_gate() { return g_gate; }
 
 void set_error(const std::string &msg) { g_err = msg; }
 int fail(int code, const std::string &msg) {

@@ -61,6 +61,80 @@ __global__ void __launch_bounds__(kVT) k_vf_scale_add(VF3 y, VF3 x, double c, co
   }
 }
 
+// The staggered 1D stencils of the three directions (cc2n: CC -> N, n2cc: N -> CC).
+struct StagCoefs {
+  const double *cc2n_D[3], *cc2n_U[3], *n2cc_D[3], *n2cc_U[3];
+};
+
+// The value k_del (staggered kinds, gen 1, pad 0) stores at position q along
+// a pencil of the first pass: kind 0 CC -> N reads input q-1, q; kind 1 N -> CC
+// reads q, q+1; entries without both inputs are the zeroed ends.
+__device__ __forceinline__ double stag_first(const double *__restrict__ fp, int fs, int q, int s_in, int kind,
+                                             const double *__restrict__ D, const double *__restrict__ U) {
+  const int i = q - (kind == 0 ? 1 : 0);
+  if (i < 0 || i > s_in - 2) return 0.0;
+  const double x0 = fp[i * fs] * D[i];
+  const double x1 = fp[(i + 1) * fs] * U[i];
+  return x0 + x1;
+}
+
+// Ax = x + c lap(x) for the three face components: mom_diffusion
+// (code/handwritten/solvers/matrix_free_operators.f90:176-202) over
+// lap_centered_SF_given_both (code/handwritten/ops/ops_discrete.f90:112-122),
+// one thread per element of Ax_d.  For each direction e in turn, the staggered
+// derivative out and back (two del passes through a temporary) is evaluated
+// from x: the second pass's two inputs are the first pass's values recomputed
+// here.  The arithmetic, the zeroed end entries and the accumulation of the
+// two passes per direction (assign for x, add for y and z, the add starting
+// from zero at the pencil ends) and of k_vf_scale_add are those of the
+// separate launches, so the result is identical.
+__global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, double c, const int *gate) {
+  if (gate && *gate) return;
+  const int d = blockIdx.y;
+  const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
+  if (t >= (int)Ax.n[d]) return;
+  const int s0 = x.s[d][0], s1 = x.s[d][1];
+  const int s[3] = {s0, s1, x.s[d][2]};
+  const int io[3] = {t % s0, (t / s0) % s1, t / (s0 * s1)};
+  const int str[3] = {1, s0, s0 * s1};
+  const double *__restrict__ xp = x.p[d];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const bool n_along = (e == d); // face-d data is node-located along d
+    const int kind1 = n_along ? 1 : 0, kind2 = 1 - kind1;
+    const double *D1 = kind1 == 0 ? C.cc2n_D[e] : C.n2cc_D[e];
+    const double *U1 = kind1 == 0 ? C.cc2n_U[e] : C.n2cc_U[e];
+    const double *D2 = kind2 == 0 ? C.cc2n_D[e] : C.n2cc_D[e];
+    const double *U2 = kind2 == 0 ? C.cc2n_U[e] : C.n2cc_U[e];
+    const int se = s[e];
+    const int nt = kind1 == 0 ? se + 1 : se - 1; // temporary's length along e
+    const int o = io[e];
+    const double *fp = xp + (t - o * str[e]); // start of the pencil along e
+    const int i2 = o - (kind2 == 0 ? 1 : 0);
+    const bool written = (i2 >= 0 && i2 <= nt - 2);
+    const bool zero_end = (o == 0 || o == se - 1);
+    double v = 0.0;
+    if (written) {
+      const double t0 = stag_first(fp, str[e], i2, se, kind1, D1, U1);
+      const double t1 = stag_first(fp, str[e], i2 + 1, se, kind1, D1, U1);
+      const double x0 = t0 * D2[i2];
+      const double x1 = t1 * U2[i2];
+      if (e == 0) {
+        v = x0 + x1;
+      } else {
+        const double cur = zero_end ? 0.0 : acc;
+        v = cur + x0 + x1;
+      }
+    } else {
+      v = (e == 0 || zero_end) ? 0.0 : acc;
+    }
+    acc = v;
+  }
+  const double w = acc * c;
+  Ax.p[d][t] = w + xp[t];
+}
+
 // y = y - x
 __global__ void __launch_bounds__(kVT) k_vf_sub(VF3 y, VF3 x, const int *done) {
   if (*done) return;
@@ -301,11 +375,17 @@ static int vf_op(moons_pcgvf_s *S, moons_field_s *Ax[3], moons_field_s *x[3], bo
     for (int d = 0; d < 3; ++d)
       if ((st = launch_apply_bcs(x[d], true))) return st;
   if (S->kind == 1) {
-    for (int d = 0; d < 3; ++d)
-      for (int e = 0; e < 3; ++e) {
-        if ((st = launch_del(S->tmp[d][e], x[d], 1, e, 0, 1))) return st;
-        if ((st = launch_del(Ax[d], S->tmp[d][e], 1, e, 0, e == 0 ? 1 : 2))) return st;
-      }
+    // lap and x + c lap in one launch (the del passes fused; same arithmetic)
+    StagCoefs C;
+    for (int e = 0; e < 3; ++e) {
+      const DirCoefDev &cd = S->m->dev.c[e];
+      C.cc2n_D[e] = cd.cc2n_D; C.cc2n_U[e] = cd.cc2n_U;
+      C.n2cc_D[e] = cd.n2cc_D; C.n2cc_U[e] = cd.n2cc_U;
+    }
+    hipLaunchKernelGGL(k_vf_mom_op, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(x), C, S->c,
+                       current_gate());
+    MOONS_LAUNCH_CHECK();
+    return MOONS_OK;
   } else {
     for (int d = 0; d < 3; ++d)
       if ((st = moons_curl(S->E[d], x[0], x[1], x[2], d))) return st;
