@@ -135,6 +135,63 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
   Ax.p[d][t] = w + xp[t];
 }
 
+// out_d = curl(in)_d, then (mode 0) out_d = curl * aux_d (k_vf_mul) or
+// (mode 1) out_d = curl * c + aux_d (k_vf_scale_add): curl_SF component d
+// (code/handwritten/ops/ops_discrete.f90:184-202) assigns the derivative of
+// in[(d+2)%3] along (d+1)%3 and subtracts that of in[(d+1)%3] along (d+2)%3,
+// both staggered (mode 0: faces to edges, CC -> N; mode 1: edges to faces,
+// N -> CC).  One thread per element of out_d with the arithmetic and zeroed
+// end entries of the two del launches and of the following element-wise
+// kernel, so the result is identical.
+__global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C, VF3 aux, double c, int mode,
+                                                    const int *gate) {
+  if (gate && *gate) return;
+  const int d = blockIdx.y;
+  const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
+  if (t >= (int)out.n[d]) return;
+  const int s0 = out.s[d][0], s1 = out.s[d][1];
+  const int io[3] = {t % s0, (t / s0) % s1, t / (s0 * s1)};
+  const int kind = mode == 0 ? 0 : 1;
+  double v = 0.0;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int ci = q == 0 ? (d + 2) % 3 : (d + 1) % 3; // input component
+    const int e = q == 0 ? (d + 1) % 3 : (d + 2) % 3;  // derivative direction
+    const int *si = in.s[ci];
+    const int stri[3] = {1, si[0], si[0] * si[1]};
+    const double *D = kind == 0 ? C.cc2n_D[e] : C.n2cc_D[e];
+    const double *U = kind == 0 ? C.cc2n_U[e] : C.n2cc_U[e];
+    const int o = io[e];
+    const int nd = out.s[d][e], s_in = si[e];
+    int base = 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      if (r != e) base += io[r] * stri[r];
+    const double *fp = in.p[ci] + base;
+    const int i = o - (kind == 0 ? 1 : 0);
+    const bool written = (i >= 0 && i <= s_in - 2);
+    const bool zero_end = (o == 0 || o == nd - 1);
+    if (q == 0) {
+      v = written ? fp[i * stri[e]] * D[i] + fp[(i + 1) * stri[e]] * U[i] : 0.0;
+    } else {
+      const double cur = zero_end ? 0.0 : v;
+      if (written) {
+        const double x0 = fp[i * stri[e]] * D[i];
+        const double x1 = fp[(i + 1) * stri[e]] * U[i];
+        v = cur - (x0 + x1);
+      } else {
+        v = cur;
+      }
+    }
+  }
+  if (mode == 0) {
+    out.p[d][t] = v * aux.p[d][t];
+  } else {
+    const double w = v * c;
+    out.p[d][t] = w + aux.p[d][t];
+  }
+}
+
 // y = y - x
 __global__ void __launch_bounds__(kVT) k_vf_sub(VF3 y, VF3 x, const int *done) {
   if (*done) return;
@@ -387,13 +444,21 @@ static int vf_op(moons_pcgvf_s *S, moons_field_s *Ax[3], moons_field_s *x[3], bo
     MOONS_LAUNCH_CHECK();
     return MOONS_OK;
   } else {
-    for (int d = 0; d < 3; ++d)
-      if ((st = moons_curl(S->E[d], x[0], x[1], x[2], d))) return st;
-    VF3 e = vf3(S->E), kk = vf3(S->k);
-    hipLaunchKernelGGL(k_vf_mul, dim3(vblk(vmax(S->E)), 3), dim3(kVT), 0, stream(), e, kk, &S->ctrl->done);
+    // E = k curl(x), then Ax = x + c curl(E): two launches (the curls'
+    // del passes and the element-wise products fused; same arithmetic)
+    StagCoefs C;
+    for (int e = 0; e < 3; ++e) {
+      const DirCoefDev &cd = S->m->dev.c[e];
+      C.cc2n_D[e] = cd.cc2n_D; C.cc2n_U[e] = cd.cc2n_U;
+      C.n2cc_D[e] = cd.n2cc_D; C.n2cc_U[e] = cd.n2cc_U;
+    }
+    const int *gate = current_gate();
+    hipLaunchKernelGGL(k_vf_curl_op, dim3(vblk(vmax(S->E)), 3), dim3(kVT), 0, stream(), vf3(S->E), vf3(x), C,
+                       vf3(S->k), 0.0, 0, gate);
+    hipLaunchKernelGGL(k_vf_curl_op, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(S->E), C, vf3(x),
+                       S->c, 1, gate);
     MOONS_LAUNCH_CHECK();
-    for (int d = 0; d < 3; ++d)
-      if ((st = moons_curl(Ax[d], S->E[0], S->E[1], S->E[2], d))) return st;
+    return MOONS_OK;
   }
   hipLaunchKernelGGL(k_vf_scale_add, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(x), S->c,
                      &S->ctrl->done);
