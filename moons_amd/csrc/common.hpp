@@ -152,6 +152,7 @@ struct moons_field_s {
 namespace moons {
 BCDev make_bcdev(const moons_field_s *f);
 int launch_apply_bcs(moons_field_s *u, bool implicit_);
+int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_); // three fields (e.g. a VF)
 int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int pad, int gen);
 int launch_assign_ghost_xperiodic(moons_field_s *u, double val);
 int launch_scale(moons_field_s *u, double s);

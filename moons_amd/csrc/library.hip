@@ -2,6 +2,7 @@
 // operators of libmoons_amd.  gfx950 only.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -180,6 +181,16 @@ struct Shape3 {
 
 // One face of apply_BCs / apply_BCs_implicit over the full plane (tangential
 // ghosts included), raw rules of code/handwritten/BCs/direct/apply_BCs_faces_raw.f90
+// Up to three fields' faces of one direction (k_apply_bc_multi).
+struct BcMulti {
+  int on[3];
+  double *u[3];
+  Shape3 S[3];
+  int nalong[3];
+  int type[3][2];
+  double val[3][2], dh[3][2];
+};
+
 // and code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90.
 __device__ __forceinline__ void apply_bc_point(double *u, const Shape3 &S, int face, int type, int nalong, double bval,
                                                double dh_nhat, int implicit_, int ja, int jb) {
@@ -468,6 +479,66 @@ int launch_apply_bcs(moons_field_s *u, bool implicit_) {
       if (b.type[face] == MOONS_BC_NONE) continue;
       hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
                          nalong, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
+      MOONS_LAUNCH_CHECK();
+    }
+  }
+  return MOONS_OK;
+}
+
+// apply_BCs of three fields: per direction, the faces of every field whose
+// direction has at least 6 points in one launch (fields are independent, so
+// only each field's own direction order matters), the others face by face.
+__global__ void __launch_bounds__(256) k_apply_bc_multi(BcMulti M, int d, int implicit_, const int *gate) {
+  if (gate && *gate) return;
+  const int f = blockIdx.y;
+  if (!M.on[f]) return;
+  const Shape3 &S = M.S[f];
+  const int a = (d + 1) % 3, b = (d + 2) % 3;
+  const int na = S.s[a], nb = S.s[b];
+  const int np = na * nb;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * np) return;
+  const int side = t >= np ? 1 : 0;
+  const int q = t - side * np;
+  apply_bc_point(M.u[f], S, 2 * d + side, M.type[f][side], M.nalong[f], M.val[f][side], M.dh[f][side], implicit_,
+                 q % na, q / na);
+}
+
+int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_) {
+  for (int d = 0; d < 3; ++d) {
+    BcMulti M{};
+    size_t npmax = 0;
+    for (int f = 0; f < 3; ++f) {
+      const moons_field_s *x = u[f];
+      if (!x->has_bcs) continue;
+      BCDev b = make_bcdev(x);
+      const int f0 = 2 * d, f1 = 2 * d + 1;
+      const int a = (d + 1) % 3, bb = (d + 2) % 3;
+      const size_t np = (size_t)x->s[a] * x->s[bb];
+      if (b.type[f0] != MOONS_BC_NONE && b.type[f1] != MOONS_BC_NONE && x->s[d] >= 6) {
+        M.on[f] = 1;
+        M.u[f] = x->d;
+        M.S[f] = Shape3{{x->s[0], x->s[1], x->s[2]}};
+        M.nalong[f] = (x->dl >> d) & 1;
+        for (int side = 0; side < 2; ++side) {
+          M.type[f][side] = b.type[2 * d + side];
+          M.val[f][side] = b.val[2 * d + side];
+          M.dh[f][side] = b.dh_nhat[2 * d + side];
+        }
+        npmax = std::max(npmax, np);
+      } else {
+        Shape3 S{{x->s[0], x->s[1], x->s[2]}};
+        for (int face : {f0, f1}) {
+          if (b.type[face] == MOONS_BC_NONE) continue;
+          hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, x->d, S, face, b.type[face],
+                             (x->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
+          MOONS_LAUNCH_CHECK();
+        }
+      }
+    }
+    if (npmax > 0) {
+      hipLaunchKernelGGL(k_apply_bc_multi, dim3(nblk(2 * npmax), 3), dim3(256), 0, g_stream, M, d, implicit_ ? 1 : 0,
+                         g_gate);
       MOONS_LAUNCH_CHECK();
     }
   }

@@ -66,6 +66,31 @@ struct StagCoefs {
   const double *cc2n_D[3], *cc2n_U[3], *n2cc_D[3], *n2cc_U[3];
 };
 
+// The wall-plane rules of the operator's output in the PCG loop
+// (multiply_wall_Neumann 0.5, assign_wall_Dirichlet 0, assign_wall_Periodic_single
+// 0 on the boundary-normal planes 1 and s-2 of each component; vf_wall_fix),
+// then the product with the cell volume: applied by the operator kernels to
+// each element they write when `on` is set, with the same arithmetic.
+struct WallVol {
+  int on;
+  const double *vol[3];
+  int lo[3], hi[3]; // 0 none, 1 Dirichlet (set 0), 2 Neumann (times 0.5)
+  int per[3];       // periodic pair: plane s-2 set to 0
+};
+
+__device__ __forceinline__ double wall_vol(const WallVol &W, int d, int pd, int sd, int t, double v) {
+  if (pd == 1) {
+    if (W.lo[d] == 1) v = 0.0;
+    else if (W.lo[d] == 2) v = v * 0.5;
+  }
+  if (pd == sd - 2) {
+    if (W.hi[d] == 1) v = 0.0;
+    else if (W.hi[d] == 2) v = v * 0.5;
+    if (W.per[d]) v = 0.0;
+  }
+  return v * W.vol[d][t];
+}
+
 // The value k_del (staggered kinds, gen 1, pad 0) stores at position q along
 // a pencil of the first pass: kind 0 CC -> N reads input q-1, q; kind 1 N -> CC
 // reads q, q+1; entries without both inputs are the zeroed ends.
@@ -88,7 +113,8 @@ __device__ __forceinline__ double stag_first(const double *__restrict__ fp, int 
 // two passes per direction (assign for x, add for y and z, the add starting
 // from zero at the pencil ends) and of k_vf_scale_add are those of the
 // separate launches, so the result is identical.
-__global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, double c, const int *gate) {
+__global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, double c, WallVol W,
+                                                   const int *gate) {
   if (gate && *gate) return;
   const int d = blockIdx.y;
   const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
@@ -132,7 +158,9 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
     acc = v;
   }
   const double w = acc * c;
-  Ax.p[d][t] = w + xp[t];
+  double res = w + xp[t];
+  if (W.on) res = wall_vol(W, d, io[d], s[d], t, res);
+  Ax.p[d][t] = res;
 }
 
 // out_d = curl(in)_d, then (mode 0) out_d = curl * aux_d (k_vf_mul) or
@@ -144,7 +172,7 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
 // end entries of the two del launches and of the following element-wise
 // kernel, so the result is identical.
 __global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C, VF3 aux, double c, int mode,
-                                                    const int *gate) {
+                                                    WallVol W, const int *gate) {
   if (gate && *gate) return;
   const int d = blockIdx.y;
   const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
@@ -188,7 +216,9 @@ __global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C
     out.p[d][t] = v * aux.p[d][t];
   } else {
     const double w = v * c;
-    out.p[d][t] = w + aux.p[d][t];
+    double res = w + aux.p[d][t];
+    if (W.on) res = wall_vol(W, d, io[d], out.s[d][d], t, res);
+    out.p[d][t] = res;
   }
 }
 
@@ -200,13 +230,16 @@ __global__ void __launch_bounds__(kVT) k_vf_sub(VF3 y, VF3 x, const int *done) {
   if (t < y.n[d]) y.p[d][t] = y.p[d][t] - x.p[d][t];
 }
 
-// which 0: y = y + x*alpha ; which 1: y = y + x*(-alpha)
-__global__ void __launch_bounds__(kVT) k_vf_axpy(VF3 y, VF3 x, const VfCtrl *ctrl, int which) {
+// x = x + p*alpha and r = r + Ax*(-alpha) in one pass (two k_vf_axpy)
+__global__ void __launch_bounds__(kVT) k_vf_axpy2(VF3 x, VF3 p, VF3 r, VF3 Ax, const VfCtrl *ctrl) {
   if (ctrl->done) return;
   const int d = blockIdx.y;
   const size_t t = (size_t)blockIdx.x * kVT + threadIdx.x;
-  const double a = which ? -ctrl->alpha : ctrl->alpha;
-  if (t < y.n[d]) y.p[d][t] = y.p[d][t] + x.p[d][t] * a;
+  const double a = ctrl->alpha, ma = -ctrl->alpha;
+  if (t < x.n[d]) {
+    x.p[d][t] = x.p[d][t] + p.p[d][t] * a;
+    r.p[d][t] = r.p[d][t] + Ax.p[d][t] * ma;
+  }
 }
 
 // z = Minv*r (all cells)
@@ -426,42 +459,43 @@ static inline size_t vmax(moons_field_s *const f[3]) { return std::max(f[0]->n, 
 
 // Ax = op(x): implicit BCs on x when implicit_, then lap_centered_VF or
 // curl(k curl x), then Ax = Ax*c + x.
-static int vf_op(moons_pcgvf_s *S, moons_field_s *Ax[3], moons_field_s *x[3], bool implicit_) {
+// The operator (implicit BCs on x first when implicit_); with wall_vol the
+// loop's wall-plane rules and volume product are applied to Ax as well.
+static int vf_op(moons_pcgvf_s *S, moons_field_s *Ax[3], moons_field_s *x[3], bool implicit_, bool wall_vol_ = false) {
   int st;
-  if (implicit_)
-    for (int d = 0; d < 3; ++d)
-      if ((st = launch_apply_bcs(x[d], true))) return st;
+  if (implicit_ && (st = launch_apply_bcs3(x, true))) return st;
+  StagCoefs C;
+  for (int e = 0; e < 3; ++e) {
+    const DirCoefDev &cd = S->m->dev.c[e];
+    C.cc2n_D[e] = cd.cc2n_D; C.cc2n_U[e] = cd.cc2n_U;
+    C.n2cc_D[e] = cd.n2cc_D; C.n2cc_U[e] = cd.n2cc_U;
+  }
+  WallVol W{};
+  if (wall_vol_) {
+    W.on = 1;
+    for (int d = 0; d < 3; ++d) {
+      const moons_field_s &b = S->bcs[d];
+      W.vol[d] = S->vol[d]->d;
+      const int s = x[d]->s[d];
+      auto rule = [](int t) { return t == MOONS_BC_DIRICHLET ? 1 : (t == MOONS_BC_NEUMANN ? 2 : 0); };
+      W.lo[d] = rule(b.bct[2 * d]);
+      W.hi[d] = rule(b.bct[2 * d + 1]);
+      W.per[d] = b.bct[2 * d] == MOONS_BC_PERIODIC && b.bct[2 * d + 1] == MOONS_BC_PERIODIC && s > 4;
+    }
+  }
+  const int *gate = current_gate();
   if (S->kind == 1) {
     // lap and x + c lap in one launch (the del passes fused; same arithmetic)
-    StagCoefs C;
-    for (int e = 0; e < 3; ++e) {
-      const DirCoefDev &cd = S->m->dev.c[e];
-      C.cc2n_D[e] = cd.cc2n_D; C.cc2n_U[e] = cd.cc2n_U;
-      C.n2cc_D[e] = cd.n2cc_D; C.n2cc_U[e] = cd.n2cc_U;
-    }
-    hipLaunchKernelGGL(k_vf_mom_op, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(x), C, S->c,
-                       current_gate());
-    MOONS_LAUNCH_CHECK();
-    return MOONS_OK;
+    hipLaunchKernelGGL(k_vf_mom_op, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(x), C, S->c, W,
+                       gate);
   } else {
     // E = k curl(x), then Ax = x + c curl(E): two launches (the curls'
     // del passes and the element-wise products fused; same arithmetic)
-    StagCoefs C;
-    for (int e = 0; e < 3; ++e) {
-      const DirCoefDev &cd = S->m->dev.c[e];
-      C.cc2n_D[e] = cd.cc2n_D; C.cc2n_U[e] = cd.cc2n_U;
-      C.n2cc_D[e] = cd.n2cc_D; C.n2cc_U[e] = cd.n2cc_U;
-    }
-    const int *gate = current_gate();
     hipLaunchKernelGGL(k_vf_curl_op, dim3(vblk(vmax(S->E)), 3), dim3(kVT), 0, stream(), vf3(S->E), vf3(x), C,
-                       vf3(S->k), 0.0, 0, gate);
+                       vf3(S->k), 0.0, 0, WallVol{}, gate);
     hipLaunchKernelGGL(k_vf_curl_op, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(S->E), C, vf3(x),
-                       S->c, 1, gate);
-    MOONS_LAUNCH_CHECK();
-    return MOONS_OK;
+                       S->c, 1, W, gate);
   }
-  hipLaunchKernelGGL(k_vf_scale_add, dim3(vblk(vmax(Ax)), 3), dim3(kVT), 0, stream(), vf3(Ax), vf3(x), S->c,
-                     &S->ctrl->done);
   MOONS_LAUNCH_CHECK();
   return MOONS_OK;
 }
@@ -670,12 +704,9 @@ int moons_pcg_vf_solve(moons_pcgvf_t S, moons_field_t x[3], moons_field_t b[3], 
       int chunk = nosync ? S->iter_max : std::min(std::max(2, std::min(16, launched / 4)), S->iter_max - launched);
       for (int c = 0; c < chunk; ++c) {
         const int it = launched + c + 1;
-        if ((st = vf_op(S, S->Ax, S->p, true))) goto out;
-        if ((st = vf_wall_fix(S, S->Ax, false))) goto out;
-        hipLaunchKernelGGL(k_vf_mul, g3, blk, 0, stream(), vf3(S->Ax), vf3(S->vol), done);
+        if ((st = vf_op(S, S->Ax, S->p, true, true))) goto out; // with the wall rules and vol
         if ((st = vf_dot(S, S->p, S->Ax, OP_ALPHA, it))) goto out;
-        hipLaunchKernelGGL(k_vf_axpy, g3, blk, 0, stream(), vf3(xp), vf3(S->p), S->ctrl, 0);
-        hipLaunchKernelGGL(k_vf_axpy, g3, blk, 0, stream(), vf3(S->r), vf3(S->Ax), S->ctrl, 1);
+        hipLaunchKernelGGL(k_vf_axpy2, g3, blk, 0, stream(), vf3(xp), vf3(S->p), vf3(S->r), vf3(S->Ax), S->ctrl);
         if ((st = vf_dot(S, S->r, S->r, OP_CHECK, it))) goto out;
         hipLaunchKernelGGL(k_vf_minv, g3, blk, 0, stream(), vf3(S->z), vf3(S->Minv), vf3(S->r), done);
         if ((st = vf_dot(S, S->z, S->r, OP_BETA, it))) goto out;
