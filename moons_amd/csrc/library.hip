@@ -279,6 +279,38 @@ __global__ void __launch_bounds__(256) k_assign_plane(double *u, Shape3 S, int d
   u[idx3(S.s, ii[0], ii[1], ii[2])] = val;
 }
 
+// Assigns val to the ghost planes of the faces in mask (bit f: face f = 2d +
+// side, plane 0 or s_d - 1) in one launch: thread t walks the faces' planes
+// one after another.  Edges shared by two faces get the same value twice.
+__global__ void __launch_bounds__(256) k_assign_faces(double *u, Shape3 S, int mask, double val, const int *gate) {
+  if (gate && *gate) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int f = 0; f < 6; ++f) {
+    if (!((mask >> f) & 1)) continue;
+    const int d = f / 2, a = (d + 1) % 3, b = (d + 2) % 3;
+    const int np = S.s[a] * S.s[b];
+    if (t < np) {
+      int ii[3];
+      ii[a] = t % S.s[a]; ii[b] = t / S.s[a]; ii[d] = (f & 1) ? S.s[d] - 1 : 0;
+      u[idx3(S.s, ii[0], ii[1], ii[2])] = val;
+      return;
+    }
+    t -= np;
+  }
+}
+
+static int launch_assign_faces(moons_field_s *u, int mask, double val) {
+  if (!mask) return MOONS_OK;
+  Shape3 S{{u->s[0], u->s[1], u->s[2]}};
+  size_t n = 0;
+  for (int f = 0; f < 6; ++f)
+    if ((mask >> f) & 1) n += (size_t)u->s[((f / 2) + 1) % 3] * u->s[((f / 2) + 2) % 3];
+  hipLaunchKernelGGL(k_assign_faces, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g_stream, u->d, S, mask, val,
+                     g_gate);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
 // Multiplies one whole plane idx along direction d by val.
 __global__ void __launch_bounds__(256) k_scale_plane(double *u, Shape3 S, int d, int idx, double val,
                                                      const int *gate) {
@@ -546,17 +578,10 @@ int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_) {
 }
 
 int launch_assign_ghost_xperiodic(moons_field_s *u, double val) {
-  Shape3 S{{u->s[0], u->s[1], u->s[2]}};
-  for (int face = 0; face < 6; ++face) {
-    if (u->has_bcs && u->bct[face] == MOONS_BC_PERIODIC) continue;
-    int d = face / 2;
-    int a = (d + 1) % 3, bb = (d + 2) % 3;
-    size_t np = (size_t)u->s[a] * u->s[bb];
-    hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, d,
-                       (face % 2) ? u->s[d] - 1 : 0, val, g_gate);
-    MOONS_LAUNCH_CHECK();
-  }
-  return MOONS_OK;
+  int mask = 0;
+  for (int face = 0; face < 6; ++face)
+    if (!(u->has_bcs && u->bct[face] == MOONS_BC_PERIODIC)) mask |= 1 << face;
+  return launch_assign_faces(u, mask, val);
 }
 
 static int launch_plane(moons_field_s *u, int d, int idx, double val, bool scale) {
@@ -599,14 +624,8 @@ int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int p
   MOONS_LAUNCH_CHECK();
   if (gen == 1 && pad > 0) {
     // diff(): assign with padding zeroes the ghost planes of the other directions
-    for (int q : {a, b}) {
-      int c2 = (q + 1) % 3, c3 = (q + 2) % 3;
-      size_t np = (size_t)out->s[c2] * out->s[c3];
-      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, 0, 0.0, g_gate);
-      hipLaunchKernelGGL(k_assign_plane, dim3(nblk(np)), dim3(256), 0, g_stream, out->d, So, q, out->s[q] - 1, 0.0,
-                         g_gate);
-      MOONS_LAUNCH_CHECK();
-    }
+    int st = launch_assign_faces(out, (3 << (2 * a)) | (3 << (2 * b)), 0.0);
+    if (st) return st;
   }
   return MOONS_OK;
 }
