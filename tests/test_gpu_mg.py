@@ -55,6 +55,38 @@ def test_vcycle_matches_restatement(gpu, case):
     assert P.mg_info()[0] == len(levels)
 
 
+@pytest.mark.parametrize("bct", [[3, 3, 2, 2, 2, 2], [2, 1, 3, 3, 2, 2]])
+def test_vcycle_half_sweep_levels_match_restatement(gpu, bct):
+    """A mesh whose finest level (80 x 64 x 64 cells, above the fused-sweep
+    size) is smoothed by the in-place half-sweeps and whose next level by the
+    fused whole sweeps: one V-cycle against the restatement, periodic along x
+    or along y."""
+    print(f"vcycle 80x64x64 {bct}", flush=True)
+    hx = O.uniform_nodes(0.0, 1.25, 80)
+    hy = O.roberts_both_nodes(0.0, 1.0, 64, 1.05) if bct[2] != 3 else O.uniform_nodes(0.0, 1.0, 64)
+    hz = O.roberts_both_nodes(0.0, 1.0, 64, 1.1)
+    hs = [hx, hy, hz]
+    g, m = O.Grid(*hs), gpu.mesh(*hs)
+    x = gpu.SF(m, "CC").set_BCs([NAMES[t] for t in bct], None, all_Neumann=(1 not in bct))
+    P = mg_solver(gpu, m, x)
+    rng = np.random.default_rng(8)
+    r = np.zeros(g.shape(0), order="F")
+    r[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(n - 2 for n in g.shape(0)))
+    R = gpu.SF(m, "CC").upload(r)
+    Z = gpu.SF(m, "CC")
+    P.apply_prec(R, Z)
+    z = Z.download()
+    levels = MGO.build(hs, bct)
+    assert np.prod(levels[0].n) > 64 ** 3 >= np.prod(levels[1].n)
+    vol = O.volume(g, 0)
+    f = np.zeros(g.shape(0))
+    f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
+    zr = MGO.vcycle(levels, f, bct)
+    zi, zri = z[1:-1, 1:-1, 1:-1], zr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(zi - zri) / np.linalg.norm(zri)
+    assert err < 2e-5, err
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_mixed_pcg_matches_fp64_oracle(gpu, case):
     N, beta, bct = CASES[case]
