@@ -50,17 +50,6 @@ __global__ void __launch_bounds__(kVT) k_vf_mul(VF3 y, VF3 a, const int *done) {
   if (t < y.n[d]) y.p[d][t] = y.p[d][t] * a.p[d][t];
 }
 
-// y = y*c; y = y + x  (multiply then add, as mom/ind_diffusion_explicit)
-__global__ void __launch_bounds__(kVT) k_vf_scale_add(VF3 y, VF3 x, double c, const int *done) {
-  if (*done) return;
-  const int d = blockIdx.y;
-  const size_t t = (size_t)blockIdx.x * kVT + threadIdx.x;
-  if (t < y.n[d]) {
-    const double v = y.p[d][t] * c;
-    y.p[d][t] = v + x.p[d][t];
-  }
-}
-
 // The staggered 1D stencils of the three directions (cc2n: CC -> N, n2cc: N -> CC).
 struct StagCoefs {
   const double *cc2n_D[3], *cc2n_U[3], *n2cc_D[3], *n2cc_U[3];
@@ -111,8 +100,9 @@ __device__ __forceinline__ double stag_first(const double *__restrict__ fp, int 
 // from x: the second pass's two inputs are the first pass's values recomputed
 // here.  The arithmetic, the zeroed end entries and the accumulation of the
 // two passes per direction (assign for x, add for y and z, the add starting
-// from zero at the pencil ends) and of k_vf_scale_add are those of the
-// separate launches, so the result is identical.
+// from zero at the pencil ends) and the final Ax = lap*c + x (multiply then
+// add, as mom_diffusion_explicit) are those of the separate launches, so the
+// result is identical.
 __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, double c, WallVol W,
                                                    const int *gate) {
   if (gate && *gate) return;
@@ -163,8 +153,9 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
   Ax.p[d][t] = res;
 }
 
-// out_d = curl(in)_d, then (mode 0) out_d = curl * aux_d (k_vf_mul) or
-// (mode 1) out_d = curl * c + aux_d (k_vf_scale_add): curl_SF component d
+// out_d = curl(in)_d, then (mode 0) out_d = curl * aux_d (as k_vf_mul) or
+// (mode 1) out_d = curl * c + aux_d (multiply then add, as
+// ind_diffusion_explicit): curl_SF component d
 // (code/handwritten/ops/ops_discrete.f90:184-202) assigns the derivative of
 // in[(d+2)%3] along (d+1)%3 and subtracts that of in[(d+1)%3] along (d+2)%3,
 // both staggered (mode 0: faces to edges, CC -> N; mode 1: edges to faces,
@@ -355,7 +346,6 @@ struct moons_pcgvf_s {
   moons_field_s *r[3] = {}, *p[3] = {}, *Ax[3] = {}, *z[3] = {}, *xbc[3] = {}, *vol[3] = {}, *Minv[3] = {};
   moons_field_s *k[3] = {};                 // sigmaInv on edges (induction)
   moons_field_s *E[3] = {};                 // edge work (induction)
-  moons_field_s *tmp[3][3] = {};            // staggered derivative work (momentum)
   double *part = nullptr;
   int nbmax = 0;
   int excl = 0;
@@ -369,9 +359,6 @@ static void vf_free(moons_pcgvf_s *S) {
   for (auto arr : all)
     for (int d = 0; d < 3; ++d)
       if (arr[d]) moons_field_destroy(arr[d]);
-  for (int d = 0; d < 3; ++d)
-    for (int e = 0; e < 3; ++e)
-      if (S->tmp[d][e]) moons_field_destroy(S->tmp[d][e]);
   if (S->part) (void)hipFree(S->part);
   if (S->ctrl) (void)hipFree(S->ctrl);
   if (S->host) (void)hipHostFree(S->host);
@@ -570,8 +557,6 @@ int moons_pcg_vf_create(moons_mesh_t m, int op, moons_field_t x[3], moons_field_
       copy_bcs(S->Ax[d], *x[d]);
       st = moons_volume(S->vol[d]);
     }
-    if (!st && op == MOONS_OP_MOM_DIFFUSION)
-      for (int e = 0; e < 3 && !st; ++e) st = moons_field_create(m, (1 << d) ^ (1 << e), &S->tmp[d][e]);
   }
   if (!st && op == MOONS_OP_IND_DIFFUSION) {
     const int edl[3] = {MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z};
