@@ -34,6 +34,7 @@ extern "C" {
 #define MOONS_ERR_HIP 2
 #define MOONS_ERR_STATE 3
 #define MOONS_ERR_NAN 4
+#define MOONS_ERR_IO 5 /* a file could not be opened, written or parsed */
 
 /* --------------------------------------------------------- data location
  * 3-bit mask, bit d set when the field is node-centred along direction d
@@ -153,6 +154,41 @@ int moons_field_copy(moons_field_t dst, moons_field_t src);
 /* Device address of the field's array, for callers that share memory with
  * the library on its stream (e.g. torch tensors over the same buffers). */
 int moons_field_device_ptr(moons_field_t f, void **dptr);
+
+/* ------------------------------------------ restart primitives, raw export */
+/* Text files in the reference's list-directed layout (reals with the fewest
+ * digits that read back exactly; F form for 0.095 <= |x| < 9.5e14, E form
+ * otherwise; records of at most 79 columns).  They read back, in the
+ * reference or here, to the same doubles bit for bit, and the readers accept
+ * any list-directed text.  Host arrays are column-major, ghost cells included.
+ *
+ * grid_field restart primitives (export_primitives_grid_field /
+ * import_primitives_grid_field, code/generated_code/grid_field/grid_field.f90:123-156):
+ * shape, "f     = ", the values, "s     = ", shape, "s_1D  = ", size. */
+int moons_gf_write(const char *path, const double *f, const int s[3]);
+/* Reads the shape into s_out; with f non-NULL also the values (n must equal
+ * the size on file).  A file of an unallocated array (shape 0) leaves f alone. */
+int moons_gf_read(const char *path, double *f, size_t n, int s_out[3]);
+/* Tecplot raw export of one component on a one-block mesh (export_raw ->
+ * export_3D_1C_steady/_unsteady, code/handwritten/IO/export_raw_processed.f90:44-60,
+ * code/handwritten/IO/IO_export.f90:265-292; exp_3D_1C_GF,
+ * code/handwritten/grid_field/GF_export_import/GF_export.f90:70-89): file
+ * dir name <c|n|f_x|...|e_z> [_t=<t as F40.20>] .dat, header, zone, one
+ * "x y z value" record per point, pad cells left out at each end.  hn_* are
+ * the node coordinates with ghosts (sn = N+3); t NULL: the steady name. */
+int moons_raw_write(const double *hn_x, int sn_x, const double *hn_y, int sn_y, const double *hn_z, int sn_z,
+                    int dl, const double *f, const char *dir, const char *name, int pad, const double *t);
+/* import_raw -> import_3D_1C (code/handwritten/IO/import_raw.f90:23-31,
+ * code/handwritten/IO/IO_import.f90:20-33, imp_3D_1C_GF
+ * code/handwritten/grid_field/GF_export_import/GF_import.f90:69-89): reads
+ * the values of the padded range into f (shape s); pad cells keep theirs. */
+int moons_raw_read(const int s[3], int dl, double *f, const char *dir, const char *name, int pad);
+/* The same on device fields: the array is copied between HBM and the host
+ * once per call (export: download then write; import: read then upload). */
+int moons_field_export(moons_field_t f, const char *path);
+int moons_field_import(moons_field_t f, const char *path);
+int moons_export_raw(moons_field_t f, const char *dir, const char *name, int pad, const double *t);
+int moons_import_raw(moons_field_t f, const char *dir, const char *name, int pad);
 
 /* --------------------------------------------------------------- BCs */
 /* apply_BCs / apply_BCs_implicit (code/handwritten/BCs/apply/apply_BCs.f90:42-64,

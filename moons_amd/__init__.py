@@ -34,4 +34,10 @@ from .core import (  # noqa: F401
     interp,
     edgeCrossFace_E,
     advect_B,
+    export_raw,
+    import_raw,
+    gf_write,
+    gf_read,
+    raw_write,
+    raw_read,
 )

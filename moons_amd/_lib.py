@@ -58,6 +58,15 @@ SIGNATURES = {
     "moons_field_fill": (C.c_int, [C.c_void_p, C.c_double]),
     "moons_field_copy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_field_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_gf_write": (C.c_int, [C.c_char_p, c_double_p, c_int_p]),
+    "moons_gf_read": (C.c_int, [C.c_char_p, c_double_p, C.c_size_t, c_int_p]),
+    "moons_raw_write": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_int, c_double_p, C.c_int, C.c_int, c_double_p,
+                                  C.c_char_p, C.c_char_p, C.c_int, c_double_p]),
+    "moons_raw_read": (C.c_int, [c_int_p, C.c_int, c_double_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "moons_field_export": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "moons_field_import": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "moons_export_raw": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, c_double_p]),
+    "moons_import_raw": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
     "moons_apply_bcs": (C.c_int, [C.c_void_p]),
     "moons_apply_bcs_implicit": (C.c_int, [C.c_void_p]),
     "moons_del": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),

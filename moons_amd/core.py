@@ -10,6 +10,7 @@ compute runs in the HIP library on the GPU; this module only holds handles.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -129,6 +130,17 @@ class SF:
         check(load().moons_field_copy(self.handle, other.handle))
         return self
 
+    def export(self, path: str) -> None:
+        """Writes the field's grid_field restart primitives
+        (export_primitives_grid_field, code/generated_code/grid_field/grid_field.f90:123-139)."""
+        check(load().moons_field_export(self.handle, os.fsencode(path)))
+
+    def import_(self, path: str) -> "SF":
+        """Reads grid_field restart primitives of this field's shape
+        (import_primitives_grid_field, code/generated_code/grid_field/grid_field.f90:141-156)."""
+        check(load().moons_field_import(self.handle, os.fsencode(path)))
+        return self
+
     @classmethod
     def borrow(cls, m: mesh, handle) -> "SF":
         """View of a field owned by the library (not destroyed by this object)."""
@@ -149,6 +161,68 @@ class SF:
                 self.handle = None
         except Exception:
             pass
+
+
+# ------------------------------------------ restart primitives, raw export
+def gf_write(path: str, f: np.ndarray) -> None:
+    """grid_field restart primitives of a host array (column-major order)."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    s = (C.c_int * 3)(*f.shape)
+    flat = f.ravel(order="F")
+    check(load().moons_gf_write(os.fsencode(path), _dp(flat), s))
+
+
+def gf_read(path: str) -> np.ndarray:
+    """The array of a grid_field restart-primitives file (shape from the file)."""
+    s = (C.c_int * 3)()
+    check(load().moons_gf_read(os.fsencode(path), None, 0, s))
+    shape = (s[0], s[1], s[2])
+    if min(shape) <= 0:
+        return np.zeros((0, 0, 0))
+    flat = np.empty(int(np.prod(shape)))
+    check(load().moons_gf_read(os.fsencode(path), _dp(flat), flat.size, s))
+    return flat.reshape(shape, order="F")
+
+
+def _raw_fields(x):
+    return list(x) if isinstance(x, (list, tuple)) else [x]
+
+
+def export_raw(m: mesh, x, dir: str, name: str, pad: int, t: float | None = None) -> None:
+    """Tecplot raw export of an SF, or of each component of a staggered VF
+    given as three SFs (er_steady_SF / er_steady_VF / the unsteady forms with
+    t, code/handwritten/IO/export_raw_processed.f90:44-84)."""
+    tt = C.c_double(t) if t is not None else None
+    for f in _raw_fields(x):
+        check(load().moons_export_raw(f.handle, os.fsencode(dir), name.encode(), int(pad),
+                                      C.byref(tt) if tt is not None else None))
+
+
+def import_raw(m: mesh, x, dir: str, name: str, pad: int) -> None:
+    """Raw Tecplot import into an SF or each of three SFs (ir_steady_SF /
+    ir_steady_VF, code/handwritten/IO/import_raw.f90:23-43)."""
+    for f in _raw_fields(x):
+        check(load().moons_import_raw(f.handle, os.fsencode(dir), name.encode(), int(pad)))
+
+
+def raw_write(hn, dl, f: np.ndarray, dir: str, name: str, pad: int, t: float | None = None) -> None:
+    """export_raw of a host array f on the grid of node arrays hn (with ghosts)."""
+    hx, hy, hz = (np.ascontiguousarray(h, dtype=np.float64) for h in hn)
+    flat = np.asfortranarray(f, dtype=np.float64).ravel(order="F")
+    d = DL[dl] if isinstance(dl, str) else int(dl)
+    tt = C.c_double(t) if t is not None else None
+    check(load().moons_raw_write(_dp(hx), hx.size, _dp(hy), hy.size, _dp(hz), hz.size, d, _dp(flat),
+                                 os.fsencode(dir), name.encode(), int(pad), C.byref(tt) if tt is not None else None))
+
+
+def raw_read(f: np.ndarray, dl, dir: str, name: str, pad: int) -> np.ndarray:
+    """import_raw into a copy of the host array f (its pad cells kept)."""
+    a = np.array(f, dtype=np.float64, order="F")
+    s = (C.c_int * 3)(*a.shape)
+    flat = a.ravel(order="F")
+    d = DL[dl] if isinstance(dl, str) else int(dl)
+    check(load().moons_raw_read(s, d, _dp(flat), os.fsencode(dir), name.encode(), int(pad)))
+    return flat.reshape(a.shape, order="F")
 
 
 # ------------------------------------------------------------------ operators

@@ -8,6 +8,13 @@
 !   init(FFT, m, dir), solve(FFT, u, f), clean_div(FFT, ...)
 !                           init_FFT / solve (code/handwritten/FFT_solver/FFT_solver_extend.f90:43-137)
 !                           and clean_div_FFT (clean_divergence.f90:61-83)
+!   export(f, path), import(f, path)
+!                           grid_field restart primitives of a device field
+!                           (code/generated_code/grid_field/grid_field.f90:108-156;
+!                           a file path in place of an open unit)
+!   export_raw(m, f, dir, name, pad), import_raw(...)
+!                           Tecplot raw files (code/handwritten/IO/export_raw_processed.f90:44-60,
+!                           code/handwritten/IO/import_raw.f90:23-31)
 ! Fields are the host's ghost-padded, column-major real(8) arrays (the layout
 ! of grid_field%f, code/generated_code/grid_field/grid_field.f90:39-43); they
 ! are copied to the device at solve entry and back at exit (the PCIe-inclusive
@@ -21,6 +28,7 @@ module moons_amd_PCG_mod
   public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu, FFT_solver_SF_gpu
   public :: init, solve, delete, clean_div, update_MFP
   public :: init_CC, init_Face, set_BCs
+  public :: export, import, export_raw, import_raw
 
   type mesh_gpu
     type(c_ptr) :: h = c_null_ptr
@@ -62,6 +70,10 @@ module moons_amd_PCG_mod
                     module procedure solve_FFT_gpu; end interface
   interface clean_div; module procedure clean_div_PCG_gpu; module procedure clean_div_FFT_gpu; end interface
   interface update_MFP; module procedure update_MFP_PCG_VF_gpu; end interface
+  interface export; module procedure export_SF_gpu; end interface
+  interface import; module procedure import_SF_gpu; end interface
+  interface export_raw; module procedure export_raw_SF_gpu; end interface
+  interface import_raw; module procedure import_raw_SF_gpu; end interface
   interface delete; module procedure delete_mesh_gpu; module procedure delete_SF_gpu;
                     module procedure delete_PCG_SF_gpu; module procedure delete_PCG_VF_gpu
                     module procedure delete_FFT_gpu; end interface
@@ -356,6 +368,38 @@ contains
       call delete(PCG%x(d)); call delete(PCG%b(d))
       if (c_associated(PCG%k(d)%h)) call delete(PCG%k(d))
     enddo
+  end subroutine
+
+  subroutine export_SF_gpu(f, path)
+    type(SF_gpu), intent(in) :: f
+    character(len=*), intent(in) :: path
+    call moons_check(moons_field_export(f%h, trim(path)//c_null_char), 'export')
+  end subroutine
+
+  subroutine import_SF_gpu(f, path)
+    type(SF_gpu), intent(inout) :: f
+    character(len=*), intent(in) :: path
+    call moons_check(moons_field_import(f%h, trim(path)//c_null_char), 'import')
+  end subroutine
+
+  ! export_raw(m, x, dir, name, pad): file dir//name//<c|n|f_x|...>//'.dat'
+  subroutine export_raw_SF_gpu(m, f, dir, name, pad)
+    type(mesh_gpu), intent(in) :: m
+    type(SF_gpu), intent(in) :: f
+    character(len=*), intent(in) :: dir, name
+    integer, intent(in) :: pad
+    if (.not. c_associated(m%h)) stop 'Error: export_raw on a mesh that was not initialised'
+    call moons_check(moons_export_raw(f%h, dir//c_null_char, name//c_null_char, int(pad, c_int), c_null_ptr), &
+                     'export_raw')
+  end subroutine
+
+  subroutine import_raw_SF_gpu(m, f, dir, name, pad)
+    type(mesh_gpu), intent(in) :: m
+    type(SF_gpu), intent(inout) :: f
+    character(len=*), intent(in) :: dir, name
+    integer, intent(in) :: pad
+    if (.not. c_associated(m%h)) stop 'Error: import_raw on a mesh that was not initialised'
+    call moons_check(moons_import_raw(f%h, dir//c_null_char, name//c_null_char, int(pad, c_int)), 'import_raw')
   end subroutine
 
   subroutine delete_mesh_gpu(m)

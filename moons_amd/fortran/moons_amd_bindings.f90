@@ -57,6 +57,7 @@ module moons_amd_bindings
   public :: moons_pcg_vf_create, moons_pcg_vf_destroy, moons_pcg_vf_set_isp, moons_pcg_vf_update_mfp
   public :: moons_pcg_vf_solve
   public :: moons_mhd_create, moons_mhd_destroy, moons_mhd_step, moons_mhd_graph_capture, moons_mhd_graph_launch
+  public :: moons_field_export, moons_field_import, moons_export_raw, moons_import_raw
 
   interface
     integer(c_int) function moons_init(device) bind(C, name="moons_init")
@@ -122,6 +123,28 @@ module moons_amd_bindings
       type(c_ptr), value :: f
       real(c_double), intent(out) :: host(*)
       integer(c_size_t), value :: n
+    end function
+    integer(c_int) function moons_field_export(f, path) bind(C, name="moons_field_export")
+      import c_int, c_ptr, c_char
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function moons_field_import(f, path) bind(C, name="moons_field_import")
+      import c_int, c_ptr, c_char
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function moons_export_raw(f, dir, name, pad, t) bind(C, name="moons_export_raw")
+      import c_int, c_ptr, c_char
+      type(c_ptr), value :: f, t
+      character(kind=c_char), intent(in) :: dir(*), name(*)
+      integer(c_int), value :: pad
+    end function
+    integer(c_int) function moons_import_raw(f, dir, name, pad) bind(C, name="moons_import_raw")
+      import c_int, c_ptr, c_char
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: dir(*), name(*)
+      integer(c_int), value :: pad
     end function
     integer(c_int) function moons_field_fill(f, v) bind(C, name="moons_field_fill")
       import c_int, c_double, c_ptr

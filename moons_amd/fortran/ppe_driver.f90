@@ -1,13 +1,16 @@
 ! ppe_driver.f90 -- Fortran host driving the GPU pressure-Poisson solve
 ! through moons_amd_PCG_mod, as a MOONS host would at clean_div/solve.
 !
-! usage: ppe_driver N beta prec iter_max tol b.bin x.bin
+! usage: ppe_driver N beta prec iter_max tol b.bin x.bin [rawdir]
 !   prec: MOONS_PREC_* of the PCG, or 0 for the cosine-transform direct
 !         solver of the x-planes (init_FFT / solve; needs beta <= 1)
 !   mesh: x uniform on [0,1], y and z Roberts beta (uniform when beta <= 1)
 !   BCs:  homogeneous Neumann on all faces, all_Neumann
 !   b.bin: the right-hand side, real(8), ghost-padded (N+2)^3, column-major
 !   x.bin: the solution written back (x0 = 0)
+!   rawdir: (PCG only) the device solution is also written as the raw
+!         Tecplot file rawdir//'xc.dat' and the restart primitives
+!         rawdir//'x_restart.txt'
 ! prints: iterations and the relative residual.
 program ppe_driver
   use iso_c_binding
@@ -16,7 +19,7 @@ program ppe_driver
   implicit none
   integer :: N, iter_max, prec_i, u
   real(c_double) :: beta, tol
-  character(len=512) :: arg, bpath, xpath
+  character(len=512) :: arg, bpath, xpath, rawdir
   real(c_double), allocatable :: hx(:), hy(:), x(:, :, :), b(:, :, :)
   type(mesh_gpu) :: m
   type(PCG_solver_SF_gpu) :: PCG
@@ -31,6 +34,8 @@ program ppe_driver
   call get_command_argument(5, arg); read(arg, *) tol
   call get_command_argument(6, bpath)
   call get_command_argument(7, xpath)
+  rawdir = ''
+  if (command_argument_count() .ge. 8) call get_command_argument(8, rawdir)
 
   allocate(hx(N + 3), hy(N + 3))
   call moons_check(moons_nodes_uniform(0.0_c_double, 1.0_c_double, int(N, c_int), hx), 'nodes')
@@ -57,6 +62,11 @@ program ppe_driver
   else
     call init(PCG, m, types, vals, .true., iter_max, tol, 1.0d-300, 1, int(prec_i, c_int))
     call solve(PCG, x, b, .true.)
+    if (len_trim(rawdir) .gt. 0) then
+      ! the device copy of the solution as a raw Tecplot file and as restart primitives
+      call export_raw(m, PCG%x, trim(rawdir), 'x', 0)
+      call export(PCG%x, trim(rawdir)//'x_restart.txt')
+    endif
     call delete(PCG)
   endif
 

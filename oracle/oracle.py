@@ -282,6 +282,12 @@ def ref():
         R.ref_interp_stencil.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, _dp, _ip]
         R.ref_pencil.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
         R.ref_bc_plane.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        R.ref_gf_export.argtypes = [_dp, _ip, C.c_char_p, C.c_int]
+        R.ref_gf_import.argtypes = [C.c_char_p, C.c_int, _dp, C.c_int, _ip]
+        raw = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_char_p, C.c_int,
+               C.c_char_p, C.c_int, C.c_int]
+        R.ref_export_raw.argtypes = raw
+        R.ref_import_raw.argtypes = raw
         _ref = R
     return _ref
 
@@ -327,3 +333,40 @@ def ref_bc_plane(kind, ug, ub, ui, bvals, dh_nhat):
     x, y = ug.shape
     ref().ref_bc_plane(kind, p(ug), p(ub), p(ui), p(bvals), dh_nhat, x, y)
     return ug, ub
+
+
+def ref_gf_export(path: str, f: np.ndarray) -> None:
+    """The reference's grid_field export of the 3D array f (column-major)."""
+    f = np.asfortranarray(f, dtype=np.float64)
+    s = (C.c_int * 3)(*f.shape)
+    b = path.encode()
+    ref().ref_gf_export(f.ctypes.data_as(_dp), s, b, len(b))
+
+
+def ref_gf_import(path: str, n: int):
+    """The reference's grid_field import: (shape, flat column-major values)."""
+    f = np.zeros(max(n, 1))
+    s = (C.c_int * 3)()
+    b = path.encode()
+    ref().ref_gf_import(b, len(b), p(f), n, s)
+    return tuple(s), f[:n]
+
+
+def _raw_args(hn, dl, f, directory, name, pad):
+    hx, hy, hz = (np.ascontiguousarray(h, dtype=np.float64) for h in hn)
+    fl = np.asfortranarray(f, dtype=np.float64).ravel(order="F").copy()
+    d, nm = directory.encode(), name.encode()
+    return [p(hx), len(hx), p(hy), len(hy), p(hz), len(hz), dl, p(fl), fl.size, d, len(d), nm, len(nm), pad], fl
+
+
+def ref_export_raw(hn, dl, f, directory, name, pad) -> None:
+    """The reference's steady raw Tecplot export of f on the grid of node arrays hn."""
+    args, _ = _raw_args(hn, dl, f, directory, name, pad)
+    ref().ref_export_raw(*args)
+
+
+def ref_import_raw(hn, dl, f, directory, name, pad) -> np.ndarray:
+    """The reference's raw Tecplot import into a copy of f (pad cells kept)."""
+    args, fl = _raw_args(hn, dl, f, directory, name, pad)
+    ref().ref_import_raw(*args)
+    return fl.reshape(np.shape(f), order="F")

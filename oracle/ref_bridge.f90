@@ -4,8 +4,10 @@
 ! TEST INFRASTRUCTURE ONLY: lets tests/ check the C restatement in
 ! oracle/moons_oracle.c against the reference's own compiled code for the
 ! parts that compile stand-alone: 1D grid distributions, derivative and
-! interpolation stencil coefficients, the pencil kernels of stencils.f90 and
-! the raw ghost-plane BC kernels.  Nothing here is used by the product.
+! interpolation stencil coefficients, the pencil kernels of stencils.f90,
+! the raw ghost-plane BC kernels, and the grid_field restart primitives and
+! Tecplot raw files written and read by the reference's own routines.
+! Nothing here is used by the product.
 module moons_ref_bridge
   use iso_c_binding
   use current_precision_mod
@@ -16,8 +18,134 @@ module moons_ref_bridge
   use stencils_mod
   use apply_BCs_faces_raw_mod
   use apply_BCs_faces_raw_implicit_mod
+  use string_mod, only: string, str, delete
+  use IO_tools_mod, only: new_and_open
+  use grid_mod, only: grid
+  use grid_extend_mod, only: init, get_shape
+  use data_location_mod, only: data_location
+  use data_location_extend_mod, only: init_CC, init_Node, init_Face, init_Edge
+  use grid_field_mod, only: grid_field, export, import
+  use exp_Tecplot_Header_mod, only: exp_Header_3D_1C
+  use construct_suffix_mod, only: construct_suffix
+  use GF_export_mod, only: exp_3D_1C_GF
+  use GF_import_mod, only: imp_3D_1C_GF
   implicit none
 contains
+
+  function c2f(c, n) result(f)
+    integer(c_int), intent(in) :: n
+    character(kind=c_char), intent(in) :: c(n)
+    character(len=n) :: f
+    integer :: i
+    do i = 1, n
+      f(i:i) = c(i)
+    end do
+  end function
+
+  ! Data location by the library's code: bit d set = node-centred along d+1.
+  subroutine dl_from_code(DL, code)
+    type(data_location), intent(inout) :: DL
+    integer(c_int), intent(in) :: code
+    select case (code)
+    case (0); call init_CC(DL)
+    case (7); call init_Node(DL)
+    case (1); call init_Face(DL, 1)
+    case (2); call init_Face(DL, 2)
+    case (4); call init_Face(DL, 3)
+    case (6); call init_Edge(DL, 1)
+    case (5); call init_Edge(DL, 2)
+    case (3); call init_Edge(DL, 3)
+    end select
+  end subroutine
+
+  ! export(grid_field, un) of generated_code/grid_field/grid_field.f90 on a
+  ! grid_field holding f(s(1),s(2),s(3))
+  subroutine ref_gf_export(f, s, path, plen) bind(C, name="ref_gf_export")
+    integer(c_int), intent(in) :: s(3)
+    real(c_double), intent(in) :: f(s(1), s(2), s(3))
+    integer(c_int), value :: plen
+    character(kind=c_char), intent(in) :: path(plen)
+    type(grid_field) :: GF
+    integer :: un
+    allocate(GF%f(s(1), s(2), s(3)))
+    GF%f = f
+    GF%s = s
+    GF%s_1D = s(1) * s(2) * s(3)
+    open(newunit=un, file=c2f(path, plen), status='replace', action='write')
+    call export(GF, un)
+    close(un)
+  end subroutine
+
+  ! import(grid_field, un): s gets the shape read, f (n entries) the values
+  subroutine ref_gf_import(path, plen, f, n, s) bind(C, name="ref_gf_import")
+    integer(c_int), value :: plen, n
+    character(kind=c_char), intent(in) :: path(plen)
+    real(c_double), intent(out) :: f(n)
+    integer(c_int), intent(out) :: s(3)
+    type(grid_field) :: GF
+    integer :: un
+    open(newunit=un, file=c2f(path, plen), status='old', action='read')
+    call import(GF, un)
+    close(un)
+    s = 0
+    if (allocated(GF%f)) then
+      s = shape(GF%f)
+      if (size(GF%f) == n) f = reshape(GF%f, (/ n /))
+    end if
+  end subroutine
+
+  ! export_3D_1C_steady (handwritten/IO/IO_export.f90:265-277) with
+  ! exp_3D_1C (handwritten/IO/base_export.f90:150-159) on a one-block grid
+  subroutine ref_export_raw(hx, sx, hy, sy, hz, sz, code, f, nf, dir, dlen, name, nlen, pad) &
+      bind(C, name="ref_export_raw")
+    integer(c_int), value :: sx, sy, sz, code, nf, dlen, nlen, pad
+    real(c_double), intent(in) :: hx(sx), hy(sy), hz(sz), f(nf)
+    character(kind=c_char), intent(in) :: dir(dlen), name(nlen)
+    type(grid) :: g
+    type(data_location) :: DL
+    type(grid_field) :: GF
+    type(string) :: s
+    integer :: sh(3), un
+    call init(g, hx, hy, hz)
+    call dl_from_code(DL, code)
+    sh = get_shape(g, DL)
+    allocate(GF%f(sh(1), sh(2), sh(3)))
+    GF%f = reshape(f, sh)
+    call construct_suffix(s, c2f(name, nlen), DL)
+    un = new_and_open(c2f(dir, dlen), str(s))
+    call exp_Header_3D_1C(un, str(s))
+    call exp_3D_1C_GF(g, DL, 1, pad, un, GF)
+    close(un)
+    call delete(s)
+  end subroutine
+
+  ! import_3D_1C (handwritten/IO/IO_import.f90:20-33) with imp_3D_1C
+  ! (handwritten/IO/base_import.f90:142-150) on a one-block grid; f holds the
+  ! field before the call (its pad cells are kept)
+  subroutine ref_import_raw(hx, sx, hy, sy, hz, sz, code, f, nf, dir, dlen, name, nlen, pad) &
+      bind(C, name="ref_import_raw")
+    integer(c_int), value :: sx, sy, sz, code, nf, dlen, nlen, pad
+    real(c_double), intent(in) :: hx(sx), hy(sy), hz(sz)
+    real(c_double), intent(inout) :: f(nf)
+    character(kind=c_char), intent(in) :: dir(dlen), name(nlen)
+    type(grid) :: g
+    type(data_location) :: DL
+    type(grid_field) :: GF
+    type(string) :: s
+    integer :: sh(3), un
+    call init(g, hx, hy, hz)
+    call dl_from_code(DL, code)
+    sh = get_shape(g, DL)
+    allocate(GF%f(sh(1), sh(2), sh(3)))
+    GF%f = reshape(f, sh)
+    call construct_suffix(s, c2f(name, nlen), DL)
+    un = new_and_open(c2f(dir, dlen), str(s))
+    read(un, *); read(un, *)
+    call imp_3D_1C_GF(g, DL, pad, un, GF)
+    close(un)
+    call delete(s)
+    f = reshape(GF%f, (/ nf /))
+  end subroutine
 
   ! hn(1:N+1) = uniform(hmin,hmax,N) / robertsBoth / robertsLeft / robertsRight
   subroutine ref_grid(kind, hmin, hmax, N, beta, hn) bind(C, name="ref_grid")

@@ -26,11 +26,16 @@ def test_fortran_driver_ppe(tmp_path, prec, tol, bar):
     b = np.asfortranarray(O.splitmix(int(np.prod(shape)), 31).reshape(shape, order="F"))
     bpath, xpath = tmp_path / "b.bin", tmp_path / "x.bin"
     b.ravel(order="F").tofile(bpath)
-    out = subprocess.run([DRIVER, str(N), str(beta), str(prec), "3000", repr(tol), str(bpath), str(xpath)],
-                         capture_output=True, text=True, timeout=120)
+    out = subprocess.run([DRIVER, str(N), str(beta), str(prec), "3000", repr(tol), str(bpath), str(xpath),
+                          str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "PCG iterations" in out.stdout
     x = np.fromfile(xpath, dtype=np.float64).reshape(shape, order="F")
+    # export_raw / export of the device solution from Fortran read back bit for bit
+    import moons_amd as M
+    xb = x.view(np.int64)
+    np.testing.assert_array_equal(M.raw_read(np.zeros(shape), "CC", str(tmp_path) + "/", "x", 0).view(np.int64), xb)
+    np.testing.assert_array_equal(M.gf_read(str(tmp_path / "x_restart.txt")).view(np.int64), xb)
     xr = np.zeros(shape, order="F")
     O.pcg_lap(g, 0, O.BCs.make(["Neumann"] * 6), True, xr, b, 3000, 1e-10 if prec == 1 else 1e-13, 1e-300)
     vol = O.volume(g, 0)[1:-1, 1:-1, 1:-1]
