@@ -206,6 +206,57 @@ def run_fft(M, args):
     print(json.dumps(out), flush=True)
 
 
+def run_energy(M, args):
+    """The energy equation at N^3 on a uniform grid, inputs resident in HBM:
+    one step = sources (advection by a seeded velocity, scale -1) and method 3
+    (AB2, backward-Euler I + c div(k grad) with k = 1 on every face, Pe 100,
+    dt 5e-4) solved by the nrg_diffusion Jacobi-PCG to 1e-8.  Also times the
+    nrg_diffusion matvec kernel (read p and the three face arrays of k, write
+    q: 40 B/cell) against the HBM peak."""
+    from moons_amd.ensemble import noise_velocity
+
+    N = args.energy
+    hs = [M.nodes_uniform(0.0, 1.0, N)] * 3
+    m = M.mesh(*hs)
+    U = noise_velocity(m, 2)
+    T_bcs = (["Dirichlet", "Dirichlet", "Neumann", "Neumann", "Neumann", "Neumann"], [1.0, 0.0, 0, 0, 0, 0])
+    x = np.linspace(0.0, 1.0, N + 2)
+    T0 = np.asfortranarray(np.broadcast_to((1.0 - x)[:, None, None], (N + 2,) * 3))
+    T = M.SF(m, "CC").upload(T0).set_BCs(*T_bcs)
+    k = [M.SF(m, 1 << d).fill(1.0) for d in range(3)]
+    st = M.Energy_step(m, T, U, k, dt=5e-4, method=3, Pe=100.0, advection=(True, -1.0), diffusion=(False, 0.0),
+                       isp_T=(1000, TOL, 1e-300, 1))
+    st.step(args.warmup)
+    M.synchronize()
+    t0 = time.perf_counter()
+    info = st.step(args.steps)
+    M.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    # the nrg_diffusion PCG's kernels, 50 timed launches each after 4 iterations
+    xk = M.SF(m, "CC").upload(T0).set_BCs(*T_bcs)
+    P = M.PCG_solver_SF(m, xk, M.iter_solver_params(4, 0.0, 0.0, 1), op="nrg_diffusion")
+    P.set_k(k)
+    P.update_MFP(5e-4 * (-1.0 / 100.0))
+    P.solve(xk, M.SF(m, "CC").upload(T0))
+    kms = P.time_kernels(50)
+    cells = N ** 3
+    per = {"nrg_matvec": 40, "pcg_update": 48, "pcg_pupdate": 24}
+    kern = {n: {"us": round(1e3 * t, 2), "GBs": round(per[n] * cells / (t * 1e-3) / 1e9, 1)}
+            for n, t in zip(per, kms)}
+    gbs = per["nrg_matvec"] * cells / (kms[0] * 1e-3) / 1e9
+    out = {"metric": "energy time steps per second", "value": round(1e3 / ms, 2), "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"energy_{N}", "cells": f"{N}^3", "method": 3, "Pe": 100, "dt": 5e-4,
+                      "k": "1 on every face", "T_solve": "nrg_diffusion Jacobi-PCG to 1e-8",
+                      "sources": "advection (scale -1) by a seeded velocity"},
+           "last_step_iterations": info["iterations"], "kernels": kern,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(gbs / 8000.0, 4), "traffic": None, "kernel": "nrg_matvec",
+                        "bytes_per_cell": 40}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +270,8 @@ def main():
                     help="MHD time-step benchmark instead: one lid-driven MHD cavity at N^3, eager")
     ap.add_argument("--fft", type=int, default=0, metavar="N",
                     help="cosine-transform direct PPE solver benchmark instead, at N^3")
+    ap.add_argument("--energy", type=int, default=0, metavar="N",
+                    help="energy-equation step benchmark instead (method 3, k = 1), at N^3")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training and timing the learned component (U-net initial guess in the V-cycle)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -241,6 +294,9 @@ def main():
         return
     if args.fft:
         run_fft(M, args)
+        return
+    if args.energy:
+        run_energy(M, args)
         return
     m, x, b, types, hs, bh = make_inputs(M)
     P = M.PCG_solver_SF(m, x, M.iter_solver_params(20000, TOL, 1e-300, 1))

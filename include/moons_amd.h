@@ -62,6 +62,11 @@ extern "C" {
 
 /* ----------------------------------------------------------- operators */
 #define MOONS_OP_LAP_UNIFORM_SF 1 /* code/handwritten/solvers/matrix_free_operators.f90:35-62 */
+/* nrg_diffusion: Ax = c div(k grad x) + x on a cell-centred x with k on the
+ * faces (code/handwritten/solvers/matrix_free_operators.f90:148-174), the
+ * energy equation's implicit operator; with MOONS_PREC_LAP_SF only (the
+ * reference binds prec_lap_SF, code/handwritten/governing_equations/energy_extend.f90:129-130). */
+#define MOONS_OP_NRG_DIFFUSION 3
 #define MOONS_PREC_LAP_SF 1       /* code/handwritten/solvers/preconditioners/preconditioners.f90:55-71 */
 /* FP32 geometric multigrid V-cycle inside an FP64 flexible PCG.  No compiled
  * counterpart in the reference (its V-cycle is dead code,
@@ -87,6 +92,7 @@ typedef struct moons_pcgvf_s *moons_pcgvf_t;
 typedef struct moons_mhd_s *moons_mhd_t;
 typedef struct moons_unet_s *moons_unet_t;
 typedef struct moons_fft_s *moons_fft_t;
+typedef struct moons_nrg_s *moons_nrg_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
@@ -238,6 +244,13 @@ int moons_pcg_destroy(moons_pcg_t pcg);
  * (code/handwritten/var_set/iter_solver_params_extend.f90:44-58). */
 int moons_pcg_set_isp(moons_pcg_t pcg, int iter_max, double tol_rel, double tol_abs,
                       int n_skip_check_res);
+/* The k of an nrg_diffusion solver: init(PCG%k, k) of init_PCG_SF
+ * (code/handwritten/PCG/PCG_solver_extend.f90:77); k[0..2] are the x-, y- and
+ * z-face fields of the mesh, copied.  Required before its first solve. */
+int moons_pcg_sf_set_k(moons_pcg_t pcg, moons_field_t k[3]);
+/* update_MFP_PCG_SF (code/handwritten/PCG/PCG_solver_extend.f90:269-279): the
+ * coefficient c of an nrg_diffusion solver; prec_Lap_SF does not depend on it. */
+int moons_pcg_sf_update_mfp(moons_pcg_t pcg, double coeff_implicit_time_split, int update_preconditioner);
 /* solve_PCG_SF (code/handwritten/PCG/PCG_solver_extend.f90:182-193): x in/out,
  * b the right-hand side, both device fields of the solver's location.  The
  * whole loop runs on the device; info may be NULL. */
@@ -389,6 +402,60 @@ int moons_mhd_graph_capture(moons_mhd_t s);
 int moons_mhd_graph_launch(moons_mhd_t s, int n_steps, void *hip_stream);
 /* Handles of internal fields (MOONS_MHD_*), 3 or 6 entries, for inspection. */
 int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
+
+/* ---------------------------------------------------- energy equation */
+/* One energy step (solve_energy after add_all_energy_sources,
+ * code/handwritten/governing_equations/energy_extend.f90:236-262,
+ * code/handwritten/solvers/energy/add_all_energy_sources.f90:21-80), the
+ * first equation of each time step of solve_MOONS
+ * (code/handwritten/MOONS/MOONS_solver.f90:80-83):
+ *   Fnm1 = F; F = 0;
+ *   advection (add_advection, code/handwritten/solvers/energy/energy_sources.f90:26-38):
+ *     F += advection_scale * (-div(cellCenter2Face(T) * U));
+ *   diffusion (add_diffusion, energy_sources.f90:40-48):
+ *     F += diffusion_scale * lapUniformCoeff(T);
+ *   method 1: T += dt F; method 2: T += dt AB2(F, Fnm1); method 3: Tnm1 = T,
+ *     solve (I + c div(k grad)) T = T + dt AB2(F, Fnm1) with c = dt coeff_implicit
+ *     (code/handwritten/solvers/time_marching_methods_SF.f90:71-98,135-170).
+ * coeff_implicit is the MFP's (-alpha/Pe; code/handwritten/var_set/var_set_extend.f90:82).
+ * The reference's presets scale advection by -1 and diffusion by 1/Pe
+ * (e.g. code/handwritten/sim_params/sim_params_default_MHD_Shercliff/sim_params_init_flow_config.f90:71-72). */
+typedef struct {
+  double dt;
+  int method;
+  double coeff_implicit;
+  int advection;
+  double advection_scale;
+  int diffusion;
+  double diffusion_scale;
+  int iter_max;
+  double tol_rel, tol_abs;
+  int n_skip;
+} moons_nrg_params;
+
+typedef struct {
+  int iterations; /* of the last step's T solve (method 3) */
+  double ms;      /* device time of the call */
+} moons_nrg_info;
+
+#define MOONS_NRG_F 1    /* energy sources F */
+#define MOONS_NRG_FNM1 2 /* previous step's F */
+#define MOONS_NRG_TNM1 3 /* T before the last implicit solve */
+
+/* Creates the energy state over the caller's T (cell-centred, with BCs) and
+ * face velocity U (read at each step: the momentum state's U); k are the
+ * face fields bound to the implicit operator (copied).  The reference binds
+ * a zero-filled temporary there (energy_extend.f90:103,129-130), which makes
+ * its implicit operator the identity; pass zero fields to reproduce that, or
+ * the face conductivity (cellCenter2Face of k_cc, energy_extend.f90:121-124)
+ * for I + c div(k grad).  F and Fnm1 start at zero, Tnm1 = T. */
+int moons_nrg_create(moons_mesh_t m, const moons_nrg_params *params, moons_field_t T, moons_field_t U[3],
+                     moons_field_t k[3], moons_nrg_t *out);
+int moons_nrg_destroy(moons_nrg_t s);
+/* Advances n_steps energy steps with the current U. */
+int moons_nrg_step(moons_nrg_t s, int n_steps, moons_nrg_info *info);
+/* Handle of an internal field (MOONS_NRG_*). */
+int moons_nrg_field(moons_nrg_t s, int which, moons_field_t *out);
 
 /* ------------------------------------------- learned preconditioner */
 /* A 3-level 3D U-net (channels 16/32/64, 3x3x3 convolutions, ReLU, average

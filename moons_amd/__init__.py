@@ -11,6 +11,7 @@ from .core import (  # noqa: F401
     PCG_solver_SF,
     PCG_solver_VF,
     MHD_step,
+    Energy_step,
     Stream,
     UNet,
     mfma_check,

@@ -38,6 +38,16 @@ class MhdInfo(C.Structure):
     _fields_ = [("iterations", C.c_int * 4), ("ms", C.c_double)]
 
 
+class NrgParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("method", C.c_int), ("coeff_implicit", C.c_double), ("advection", C.c_int),
+                ("advection_scale", C.c_double), ("diffusion", C.c_int), ("diffusion_scale", C.c_double),
+                ("iter_max", C.c_int), ("tol_rel", C.c_double), ("tol_abs", C.c_double), ("n_skip", C.c_int)]
+
+
+class NrgInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("ms", C.c_double)]
+
+
 SIGNATURES = {
     "moons_init": (C.c_int, [C.c_int]),
     "moons_finalize": (C.c_int, []),
@@ -81,6 +91,13 @@ SIGNATURES = {
     "moons_pcg_sf_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "moons_pcg_destroy": (C.c_int, [C.c_void_p]),
     "moons_pcg_set_isp": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "moons_pcg_sf_set_k": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_pcg_sf_update_mfp": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "moons_nrg_create": (C.c_int, [C.c_void_p, C.POINTER(NrgParams), C.c_void_p, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "moons_nrg_destroy": (C.c_int, [C.c_void_p]),
+    "moons_nrg_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NrgInfo)]),
+    "moons_nrg_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "moons_pcg_sf_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgInfo)]),
     "moons_clean_div": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p, C.c_double,
                                   C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(PcgInfo)]),

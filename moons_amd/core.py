@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import MhdInfo, MhdParams, PcgInfo, check, load
+from ._lib import MhdInfo, MhdParams, NrgInfo, NrgParams, PcgInfo, check, load
 
 DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
 BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
@@ -303,22 +303,33 @@ class pcg_info:
 
 
 class PCG_solver_SF:
-    """PCG for the cell-centred Laplacian (op Lap_uniform_SF).
+    """PCG for a cell-centred field.
 
+    op="Lap_uniform_SF": the Laplacian (pressure Poisson, phi cleaning).
+    op="nrg_diffusion": I + c div(k grad) of the energy equation; set k with
+    set_k and c with update_MFP; prec="Lap_SF" only.
     prec="Lap_SF": the reference's Jacobi preconditioner (prec_Lap_SF), FP64.
     prec="MG_FP32": one FP32 multigrid V-cycle inside an FP64 flexible PCG.
     """
 
-    OP_LAP_UNIFORM_SF = 1
+    OPS = {"Lap_uniform_SF": 1, "nrg_diffusion": 3}
     PRECS = {"Lap_SF": 1, "MG_FP32": 2, "UNet": 3}
 
-    def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None, prec: str = "Lap_SF"):
+    def __init__(self, m: mesh, x: SF, ISP: iter_solver_params | None = None, prec: str = "Lap_SF",
+                 op: str = "Lap_uniform_SF"):
         h = C.c_void_p()
-        check(load().moons_pcg_sf_create(m.handle, self.OP_LAP_UNIFORM_SF, self.PRECS[prec], x.handle, C.byref(h)))
+        check(load().moons_pcg_sf_create(m.handle, self.OPS[op], self.PRECS[prec], x.handle, C.byref(h)))
         self.handle = h
         self.m = m
         self.prec = prec
         self.set_ISP(ISP or iter_solver_params())
+
+    def set_k(self, k) -> None:
+        """The nrg_diffusion operator's face coefficients (three face SFs, copied)."""
+        check(load().moons_pcg_sf_set_k(self.handle, _arr(k)))
+
+    def update_MFP(self, coeff_implicit_time_split: float, update_preconditioner: bool = True) -> None:
+        check(load().moons_pcg_sf_update_mfp(self.handle, coeff_implicit_time_split, 1 if update_preconditioner else 0))
 
     def set_MG(self, max_levels: int = 16, nu1: int = 3, nu2: int = 3, coarse_sweeps: int = 16) -> None:
         check(load().moons_pcg_set_mg(self.handle, max_levels, nu1, nu2, coarse_sweeps))
@@ -510,6 +521,55 @@ class MHD_step:
         try:
             if getattr(self, "handle", None):
                 load().moons_mhd_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Energy_step:
+    """The energy equation's step (solve_energy after add_all_energy_sources,
+    code/handwritten/governing_equations/energy_extend.f90:236-262): sources
+    advection (-div(face(T) U) times advection_scale) and diffusion
+    (lapUniformCoeff(T) times diffusion_scale), then method 1 (Euler), 2
+    (explicit AB2) or 3 (AB2 with backward-Euler I + c div(k grad),
+    c = dt coeff_implicit).  T (with BCs) is advanced in place; U is read at
+    each step.  k: three face SFs bound to the implicit operator; None gives
+    zeros, as the reference's init binds (its operator is then I)."""
+
+    FIELDS = {"F": 1, "Fnm1": 2, "Tnm1": 3}
+
+    def __init__(self, m: mesh, T: SF, U=None, k=None, dt=1e-3, method=3, Pe=100.0, coeff_implicit=None,
+                 advection=(True, -1.0), diffusion=(False, None), isp_T=(5, 1e-6, 1e-13, 1)):
+        P = NrgParams()
+        P.dt, P.method = dt, method
+        P.coeff_implicit = -1.0 / Pe if coeff_implicit is None else coeff_implicit
+        P.advection, P.advection_scale = (1 if advection[0] else 0), advection[1]
+        P.diffusion = 1 if diffusion[0] else 0
+        P.diffusion_scale = (1.0 / Pe) if diffusion[1] is None else diffusion[1]
+        P.iter_max, P.tol_rel, P.tol_abs, P.n_skip = isp_T
+        if k is None:
+            k = [SF(m, 1 << d).fill(0.0) for d in range(3)]
+        h = C.c_void_p()
+        check(load().moons_nrg_create(m.handle, C.byref(P), T.handle, _arr(U) if U is not None else None, _arr(k),
+                                      C.byref(h)))
+        self.handle = h
+        self.m = m
+        self._keep = (T, U, k)
+
+    def step(self, n: int = 1):
+        info = NrgInfo()
+        check(load().moons_nrg_step(self.handle, n, C.byref(info)))
+        return {"iterations": info.iterations, "ms": info.ms}
+
+    def field(self, which: str) -> SF:
+        out = C.c_void_p()
+        check(load().moons_nrg_field(self.handle, self.FIELDS[which], C.byref(out)))
+        return SF.borrow(self.m, out.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_nrg_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

@@ -2,9 +2,11 @@
 // (pressure-Poisson and phi-cleaning solves), the whole loop on the device.
 //
 // Restates solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107)
-// with op = Lap_uniform_SF (code/handwritten/solvers/matrix_free_operators.f90:35-62),
-// prec = prec_Lap_SF (code/handwritten/solvers/preconditioners/preconditioners.f90:55-71)
-// and modify_RHS_SF (code/handwritten/solvers/PCG/PCG_aux.f90:30-54).
+// with op = Lap_uniform_SF (code/handwritten/solvers/matrix_free_operators.f90:35-62)
+// or nrg_diffusion (the energy equation's I + c div(k grad),
+// matrix_free_operators.f90:148-174), prec = prec_Lap_SF
+// (code/handwritten/solvers/preconditioners/preconditioners.f90:55-71) and
+// modify_RHS_SF (code/handwritten/solvers/PCG/PCG_aux.f90:30-54).
 //
 // One iteration is three HBM-bound kernels over the interior cells:
 //   k_pcg_matvec   q = vol*lap(p), partial (p,q)              read p, write q        16 B/cell
@@ -51,6 +53,36 @@ __device__ __forceinline__ size_t f32_at(const F32Map &M, size_t o, int i, int j
   return M.split ? (size_t)(((i + j + k) & 1) * M.half + ((i - 1) >> 1) + M.h0 * ((j - 1) + M.n1 * (k - 1))) : o;
 }
 
+// The nrg_diffusion operator's face coefficients k (x-, y-, z-face arrays of
+// the mesh) and c = coeff_implicit_time_split.
+struct NrgK {
+  const double *k[3];
+  double c;
+};
+
+// c div(k grad x) + x at an interior cell, in nrg_diffusion_explicit's order
+// (matrix_free_operators.f90:148-161): the face gradients of lap7, each times
+// its face k, the divergence summed as in lap7, times c, plus x.
+__device__ __forceinline__ double nrg7(const NrgK &K, const CCGeom &G, double c0, double xm, double xp, double ym,
+                                       double yp, double zm, double zp, int i, int j, int k) {
+  const size_t s0 = G.s0, s1 = G.s1;
+  const size_t ox = (size_t)i + (s0 + 1) * ((size_t)j + s1 * (size_t)k); // x-faces: (s0+1) x s1 x s2
+  const size_t oy = (size_t)i + s0 * ((size_t)j + (s1 + 1) * (size_t)k); // y-faces: s0 x (s1+1) x s2
+  const size_t oz = (size_t)i + s0 * ((size_t)j + s1 * (size_t)k);       // z-faces: s0 x s1 x (s2+1)
+  const Coef1 &X = G.c[0], &Y = G.c[1], &Z = G.c[2];
+  const double gxm = (xm * X.cD[i - 1] + c0 * X.cU[i - 1]) * K.k[0][ox];
+  const double gxp = (c0 * X.cD[i] + xp * X.cU[i]) * K.k[0][ox + 1];
+  double v = gxm * X.nD[i] + gxp * X.nU[i];
+  const double gym = (ym * Y.cD[j - 1] + c0 * Y.cU[j - 1]) * K.k[1][oy];
+  const double gyp = (c0 * Y.cD[j] + yp * Y.cU[j]) * K.k[1][oy + s0];
+  v = v + gym * Y.nD[j] + gyp * Y.nU[j];
+  const double gzm = (zm * Z.cD[k - 1] + c0 * Z.cU[k - 1]) * K.k[2][oz];
+  const double gzp = (c0 * Z.cD[k] + zp * Z.cU[k]) * K.k[2][oz + s0 * s1];
+  v = v + gzm * Z.nD[k] + gzp * Z.nU[k];
+  v = v * K.c;
+  return v + c0;
+}
+
 // Partial sums of b*vol over interior cells (subtract_phys_mean_vol_SF,
 // code/handwritten/ops/ops_aux.f90:146-167).
 __global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, CCGeom G, double *partM) {
@@ -69,11 +101,11 @@ __global__ void __launch_bounds__(kNT) k_pcg_mean(const double *__restrict__ b, 
 
 // modify_RHS_SF + PCG set-up (PCG_aux.f90:30-54; PCG_solver_algorithm.f90:57-66):
 // r = vol*((b - mean) - A x_BC - A x), z = Minv r, p = z; partials of (r,r), (r,z).
-template <bool MG>
+template <bool MG, bool NRG>
 __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, const double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ p, CCGeom G,
                                                 const PcgCtrl *ctrl, const double *partM, double *partB,
-                                                int all_neumann, float *__restrict__ f0, F32Map M) {
+                                                int all_neumann, float *__restrict__ f0, F32Map M, NrgK K) {
   __shared__ double sh[kNT / kWave];
   double mean = 0.0;
   if (all_neumann) mean = prologue_sum<kNT>(partM, gridDim.x, 1, 0, sh) / ctrl->volsum;
@@ -107,8 +139,11 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
           nbc[f] = (ty == MOONS_BC_NONE) ? 0.0 : ghost_cc(ty, false, G.bval[f], G.dh_nhat[f], 0.0, 0.0, 0.0);
         }
       }
-      const double axbc = lap7(0.0, nbc[0], nbc[1], nbc[2], nbc[3], nbc[4], nbc[5], G.c[0], i, G.c[1], j, G.c[2], k);
-      const double ax = lap7(c, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], G.c[0], i, G.c[1], j, G.c[2], k);
+      const double axbc =
+          NRG ? nrg7(K, G, 0.0, nbc[0], nbc[1], nbc[2], nbc[3], nbc[4], nbc[5], i, j, k)
+              : lap7(0.0, nbc[0], nbc[1], nbc[2], nbc[3], nbc[4], nbc[5], G.c[0], i, G.c[1], j, G.c[2], k);
+      const double ax = NRG ? nrg7(K, G, c, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], i, j, k)
+                            : lap7(c, nb[0], nb[1], nb[2], nb[3], nb[4], nb[5], G.c[0], i, G.c[1], j, G.c[2], k);
       double rv = b[o];
       if (all_neumann) rv = rv - mean;
       rv = rv - axbc;
@@ -156,11 +191,11 @@ __global__ void __launch_bounds__(kNT) k_pcg_init(PcgCtrl *ctrl, const double *p
   }
 }
 
-// q = vol*lap(p) over the interior (p's face ghosts hold the implicit BC
-// values) and partial (p,q).
-template <int KZ>
+// q = vol*A(p) over the interior (A = lap, or nrg_diffusion when NRG; p's
+// face ghosts hold the implicit BC values) and partial (p,q).
+template <int KZ, bool NRG>
 __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p, double *__restrict__ q, CCGeom G,
-                                                   const PcgCtrl *ctrl, double *partA) {
+                                                   const PcgCtrl *ctrl, double *partA, NrgK K) {
   if (ctrl->done) return;
   __shared__ double sh[kNT / kWave];
   const TileIdx t = tile_of(G);
@@ -175,7 +210,9 @@ __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p
     for (int k = t.k0; k < t.k1; ++k) {
       const size_t ok = (size_t)k * sxy;
       const double pp = pc[ok + sxy];
-      const double v = lap7(p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, G.c[0], i, G.c[1], j, G.c[2], k);
+      const double v = NRG ? nrg7(K, G, p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, i, j, k)
+                           : lap7(p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, G.c[0], i, G.c[1],
+                                  j, G.c[2], k);
       const double qv = v * (dxy * G.c[2].dh[k]);
       q[i + (size_t)j * sx + ok] = qv;
       acc += p0 * qv;
@@ -542,6 +579,8 @@ struct moons_pcg_s {
   double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
   float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
   int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 16;
+  double *kf[3] = {nullptr, nullptr, nullptr}; // nrg_diffusion: the solver's copy of k (PCG%k)
+  double cn = 0.0;                             // nrg_diffusion: coeff_implicit_time_split
   PcgCtrl *ctrl = nullptr;
   PcgCtrl *host_ctrl = nullptr; // pinned, read-backs
   PcgCtrl *host_init = nullptr; // pinned, initial control block (the source of every upload)
@@ -605,13 +644,26 @@ static int flex_apply(moons_pcg_s *S) {
   return unet_prec_apply(S->unet, S->uf0, S->uz0, &S->ctrl->done);
 }
 
+static NrgK nrg_k(const moons_pcg_s *S) {
+  NrgK K;
+  for (int d = 0; d < 3; ++d) K.k[d] = S->kf[d];
+  K.c = S->cn;
+  return K;
+}
+
 template <int KZ>
 static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
   const dim3 grid(S->G.nblocks), block(kNT);
   const bool mg = flex(S);
   float *f0 = mg ? flex_f0(S) : nullptr;
   const F32Map M = flex_map(S);
-  if (which & 1) hipLaunchKernelGGL(k_pcg_matvec<KZ>, grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA);
+  const NrgK K = nrg_k(S);
+  if (which & 1) {
+    if (S->op == MOONS_OP_NRG_DIFFUSION)
+      hipLaunchKernelGGL((k_pcg_matvec<KZ, true>), grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA, K);
+    else
+      hipLaunchKernelGGL((k_pcg_matvec<KZ, false>), grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA, K);
+  }
   if (which & 2) {
     if (mg)
       hipLaunchKernelGGL((k_pcg_update<KZ, true>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
@@ -657,11 +709,14 @@ extern "C" {
 int moons_pcg_sf_create(moons_mesh_t m, int op, int prec, moons_field_t x, moons_pcg_t *out) {
   if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
   if (!m || !x || !out) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: null argument");
-  if (op != MOONS_OP_LAP_UNIFORM_SF ||
-      (prec != MOONS_PREC_LAP_SF && prec != MOONS_PREC_MG_FP32 && prec != MOONS_PREC_UNET))
-    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF with prec_Lap_SF, the FP32 multigrid or the U-net");
+  const bool lap_ok = op == MOONS_OP_LAP_UNIFORM_SF &&
+                      (prec == MOONS_PREC_LAP_SF || prec == MOONS_PREC_MG_FP32 || prec == MOONS_PREC_UNET);
+  const bool nrg_ok = op == MOONS_OP_NRG_DIFFUSION && prec == MOONS_PREC_LAP_SF;
+  if (!lap_ok && !nrg_ok)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF with prec_Lap_SF, the FP32 multigrid or the "
+                               "U-net, or nrg_diffusion with prec_Lap_SF");
   if (x->dl != MOONS_DL_CC)
-    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: Lap_uniform_SF solver needs a cell-centred field");
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: the solver needs a cell-centred field");
   if (x->m != m) return fail(MOONS_ERR_ARG, "moons_pcg_sf_create: field is on another mesh");
   moons_pcg_s *S = new moons_pcg_s();
   S->m = m;
@@ -759,7 +814,7 @@ int moons_pcg_destroy(moons_pcg_t S) {
   if (S->uf0) (void)hipFree(S->uf0);
   if (S->uz0) (void)hipFree(S->uz0);
   if (S->xbak) (void)hipFree(S->xbak);
-  double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM, S->partC};
+  double *bufs[] = {S->r, S->p, S->q, S->partA, S->partB, S->partM, S->partC, S->kf[0], S->kf[1], S->kf[2]};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (S->ctrl) (void)hipFree(S->ctrl);
@@ -768,6 +823,31 @@ int moons_pcg_destroy(moons_pcg_t S) {
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
   delete S;
+  return MOONS_OK;
+}
+
+int moons_pcg_sf_set_k(moons_pcg_t S, moons_field_t k[3]) {
+  if (!S || !k) return fail(MOONS_ERR_ARG, "moons_pcg_sf_set_k: null argument");
+  if (S->op != MOONS_OP_NRG_DIFFUSION) return fail(MOONS_ERR_STATE, "moons_pcg_sf_set_k: not an nrg_diffusion solver");
+  for (int d = 0; d < 3; ++d)
+    if (!k[d] || k[d]->m != S->m || k[d]->dl != (1 << d))
+      return fail(MOONS_ERR_ARG, "moons_pcg_sf_set_k: k must be the x-, y- and z-face fields of the solver's mesh");
+  for (int d = 0; d < 3; ++d) {
+    if (!S->kf[d] && hipMalloc(&S->kf[d], k[d]->n * sizeof(double)) != hipSuccess)
+      return fail(MOONS_ERR_HIP, "moons_pcg_sf_set_k: allocation failed");
+    MOONS_HIP(hipMemcpyAsync(S->kf[d], k[d]->d, k[d]->n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  }
+  return MOONS_OK;
+}
+
+int moons_pcg_sf_update_mfp(moons_pcg_t S, double coeff_implicit_time_split, int update_preconditioner) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_sf_update_mfp: null solver");
+  if (S->op != MOONS_OP_NRG_DIFFUSION)
+    return fail(MOONS_ERR_STATE, "moons_pcg_sf_update_mfp: Lap_uniform_SF has no implicit coefficient");
+  // prec_Lap_SF does not depend on the coefficient: Minv is rebuilt in
+  // registers from the 1D arrays either way
+  (void)update_preconditioner;
+  S->cn = coeff_implicit_time_split;
   return MOONS_OK;
 }
 
@@ -791,14 +871,18 @@ static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
   const bool mg = flex(S);
   if (alln) hipLaunchKernelGGL(k_pcg_mean, grid, block, 0, stream(), b->d, S->G, S->partM);
   if (!mg) {
-    hipLaunchKernelGGL(k_pcg_rhs<false>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
-                       S->partB, alln, (float *)nullptr, flex_map(S));
+    if (S->op == MOONS_OP_NRG_DIFFUSION)
+      hipLaunchKernelGGL((k_pcg_rhs<false, true>), grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl,
+                         S->partM, S->partB, alln, (float *)nullptr, flex_map(S), nrg_k(S));
+    else
+      hipLaunchKernelGGL((k_pcg_rhs<false, false>), grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl,
+                         S->partM, S->partB, alln, (float *)nullptr, flex_map(S), nrg_k(S));
     hipLaunchKernelGGL(k_pcg_init, dim3(1), block, 0, stream(), S->ctrl, S->partB, S->G.nblocks);
     MOONS_LAUNCH_CHECK();
   } else {
     // z0 = V(r0/vol), rho0 = (z0,r0), p = z0
-    hipLaunchKernelGGL(k_pcg_rhs<true>, grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl, S->partM,
-                       S->partB, alln, flex_f0(S), flex_map(S));
+    hipLaunchKernelGGL((k_pcg_rhs<true, false>), grid, block, 0, stream(), b->d, x->d, S->r, S->p, S->G, S->ctrl,
+                       S->partM, S->partB, alln, flex_f0(S), flex_map(S), nrg_k(S));
     MOONS_LAUNCH_CHECK();
     if ((st = flex_apply(S))) return st;
     if ((st = pcg_launch(S, x->d, 0, 8))) return st;
@@ -843,6 +927,8 @@ int moons_pcg_sf_solve(moons_pcg_t S, moons_field_t x, moons_field_t b, moons_pc
     return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve: x and b must match the solver's mesh and location");
   if (S->prec == MOONS_PREC_UNET && !S->unet)
     return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve: bind a network with moons_pcg_set_unet first");
+  if (S->op == MOONS_OP_NRG_DIFFUSION && !S->kf[0])
+    return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve: set k with moons_pcg_sf_set_k first");
   const bool learned = S->prec == MOONS_PREC_UNET;
   if (learned) {
     if (!S->xbak && hipMalloc(&S->xbak, x->n * sizeof(double)) != hipSuccess)
@@ -999,6 +1085,8 @@ int moons_pcg_time_mg(moons_pcg_t S, int reps, double ms[10]) {
 
 int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   if (!S || !ms || reps < 1) return fail(MOONS_ERR_ARG, "moons_pcg_time_kernels: bad argument");
+  if (S->op == MOONS_OP_NRG_DIFFUSION && !S->kf[0])
+    return fail(MOONS_ERR_STATE, "moons_pcg_time_kernels: set k with moons_pcg_sf_set_k first");
   // One plain iteration sets up p, q, r and the partial sums; then each
   // kernel is launched reps times back to back between two events (the exit
   // test disabled, a scratch x updated), so the average is the kernel's

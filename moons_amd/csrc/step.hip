@@ -459,3 +459,172 @@ int moons_mhd_field(moons_mhd_t S, int which, moons_field_t out[6]) {
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// The energy equation: sources (advection, diffusion) and one of the three
+// time-marching methods of solve_energy, composed from the kernels above.
+// ---------------------------------------------------------------------------
+
+struct moons_nrg_s {
+  moons_mesh_s *m = nullptr;
+  moons_nrg_params P{};
+  moons_field_s *T = nullptr, *U[3] = {}; // caller's
+  moons_field_s *F = nullptr, *Fnm1 = nullptr, *Tnm1 = nullptr, *rhs = nullptr, *tcc = nullptr, *tF[3] = {};
+  moons_pcg_t pcg = nullptr;
+  int iters = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void nrg_free(moons_nrg_s *S) {
+  moons_field_s *fs[] = {S->F, S->Fnm1, S->Tnm1, S->rhs, S->tcc, S->tF[0], S->tF[1], S->tF[2]};
+  for (auto f : fs)
+    if (f) moons_field_destroy(f);
+  if (S->pcg) moons_pcg_destroy(S->pcg);
+  if (S->ev0) (void)hipEventDestroy(S->ev0);
+  if (S->ev1) (void)hipEventDestroy(S->ev1);
+  delete S;
+}
+
+// add_all_energy_sources (solvers/energy/add_all_energy_sources.f90:21-80)
+int nrg_sources(moons_nrg_s *S) {
+  int st;
+  if ((st = moons_field_copy(S->Fnm1, S->F))) return st;
+  if ((st = moons_field_fill(S->F, 0.0))) return st;
+  if (S->P.advection) {
+    // temp_F = cellCenter2Face(T); temp_F = temp_F * U; temp_CC = div(temp_F);
+    // temp_CC = -temp_CC; F = F + temp_CC * scale (the negation is exact, so
+    // it folds into the scale: (-t) s == t (-s) bit for bit)
+    for (int d = 0; d < 3; ++d) {
+      if ((st = launch_interp(S->tF[d], S->T, d))) return st;
+      hipLaunchKernelGGL(k_mul3, dim3(sblk(S->tF[d]->n)), dim3(kST), 0, stream(), S->tF[d]->d, S->tF[d]->d,
+                         S->U[d]->d, S->tF[d]->n);
+      MOONS_LAUNCH_CHECK();
+    }
+    if ((st = moons_div(S->tcc, S->tF[0], S->tF[1], S->tF[2]))) return st;
+    hipLaunchKernelGGL(k_axpy, dim3(sblk(S->F->n)), dim3(kST), 0, stream(), S->F->d, S->tcc->d, -S->P.advection_scale,
+                       S->F->n);
+    MOONS_LAUNCH_CHECK();
+  }
+  if (S->P.diffusion) {
+    // lapUniformCoeff_SF (ops/ops_discrete.f90:101-110): collocated second
+    // derivatives along x (assign), y and z (add)
+    if ((st = launch_del(S->tcc, S->T, 2, 0, 0, 1))) return st;
+    if ((st = launch_del(S->tcc, S->T, 2, 1, 0, 2))) return st;
+    if ((st = launch_del(S->tcc, S->T, 2, 2, 0, 2))) return st;
+    hipLaunchKernelGGL(k_axpy, dim3(sblk(S->F->n)), dim3(kST), 0, stream(), S->F->d, S->tcc->d, S->P.diffusion_scale,
+                       S->F->n);
+    MOONS_LAUNCH_CHECK();
+  }
+  return MOONS_OK;
+}
+
+// solve_energy (governing_equations/energy_extend.f90:236-259), methods 1-3.
+// A cell-centred field has no wall planes, so assign_wall_Dirichlet is a no-op.
+int nrg_march(moons_nrg_s *S) {
+  const moons_field_s *T = S->T;
+  const unsigned nb = sblk(T->n);
+  if (S->P.method == 1) { // T = T + F*dt
+    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(kST), 0, stream(), S->T->d, S->F->d, S->P.dt, T->n);
+    MOONS_LAUNCH_CHECK();
+    return MOONS_OK;
+  }
+  if (S->P.method == 2) { // T = ((F*1.5 + Fnm1*(-0.5)) * dt) + T
+    hipLaunchKernelGGL(k_ab2, dim3(nb), dim3(kST), 0, stream(), S->T->d, S->F->d, S->Fnm1->d, (const double *)nullptr,
+                       S->T->d, S->P.dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
+    MOONS_LAUNCH_CHECK();
+    return MOONS_OK;
+  }
+  hipLaunchKernelGGL(k_ab2, dim3(nb), dim3(kST), 0, stream(), S->rhs->d, S->F->d, S->Fnm1->d, (const double *)nullptr,
+                     S->T->d, S->P.dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
+  MOONS_LAUNCH_CHECK();
+  int st;
+  if ((st = moons_field_copy(S->Tnm1, S->T))) return st;
+  if ((st = moons_pcg_sf_update_mfp(S->pcg, S->P.dt * 1.0 * S->P.coeff_implicit, 1))) return st;
+  moons_pcg_info info;
+  if ((st = moons_pcg_sf_solve(S->pcg, S->T, S->rhs, &info))) return st;
+  S->iters = info.iterations;
+  return MOONS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int moons_nrg_create(moons_mesh_t m, const moons_nrg_params *P, moons_field_t T, moons_field_t U[3],
+                     moons_field_t k[3], moons_nrg_t *out) {
+  if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
+  if (!m || !P || !T || !out) return fail(MOONS_ERR_ARG, "moons_nrg_create: null argument");
+  if (P->method < 1 || P->method > 3 || !(P->dt > 0.0))
+    return fail(MOONS_ERR_ARG, "moons_nrg_create: need dt > 0 and method 1, 2 or 3");
+  if (T->dl != MOONS_DL_CC || T->m != m) return fail(MOONS_ERR_ARG, "moons_nrg_create: T must be cell-centred on the mesh");
+  if (P->advection)
+    for (int d = 0; d < 3; ++d)
+      if (!U || !U[d] || U[d]->dl != (1 << d) || U[d]->m != m)
+        return fail(MOONS_ERR_ARG, "moons_nrg_create: advection needs U as the three face components on the mesh");
+  if (P->method == 3 && !k) return fail(MOONS_ERR_ARG, "moons_nrg_create: method 3 needs k");
+  moons_nrg_s *S = new moons_nrg_s();
+  S->m = m;
+  S->P = *P;
+  S->T = T;
+  for (int d = 0; d < 3; ++d) S->U[d] = U ? U[d] : nullptr;
+  int st = MOONS_OK;
+  moons_field_s **cc[] = {&S->F, &S->Fnm1, &S->Tnm1, &S->rhs, &S->tcc};
+  for (auto f : cc)
+    if (!st) st = fcreate(m, MOONS_DL_CC, f);
+  for (int d = 0; d < 3 && !st; ++d) st = fcreate(m, 1 << d, &S->tF[d]);
+  if (!st) st = moons_field_fill(S->F, 0.0);
+  if (!st) st = moons_field_copy(S->Tnm1, T);
+  if (!st && P->method == 3) {
+    st = moons_pcg_sf_create(m, MOONS_OP_NRG_DIFFUSION, MOONS_PREC_LAP_SF, T, &S->pcg);
+    if (!st) st = moons_pcg_set_isp(S->pcg, P->iter_max, P->tol_rel, P->tol_abs, P->n_skip);
+    if (!st) st = moons_pcg_sf_set_k(S->pcg, k);
+  }
+  if (!st && (hipEventCreate(&S->ev0) != hipSuccess || hipEventCreate(&S->ev1) != hipSuccess))
+    st = fail(MOONS_ERR_HIP, "moons_nrg_create: event creation failed");
+  if (st) {
+    nrg_free(S);
+    return st;
+  }
+  *out = S;
+  return MOONS_OK;
+}
+
+int moons_nrg_destroy(moons_nrg_t S) {
+  if (!S) return MOONS_OK;
+  if (stream()) (void)hipStreamSynchronize(stream());
+  nrg_free(S);
+  return MOONS_OK;
+}
+
+int moons_nrg_step(moons_nrg_t S, int n_steps, moons_nrg_info *info) {
+  if (!S || n_steps < 0) return fail(MOONS_ERR_ARG, "moons_nrg_step: bad argument");
+  MOONS_HIP(hipEventRecord(S->ev0, stream()));
+  int st;
+  for (int n = 0; n < n_steps; ++n) {
+    if ((st = nrg_sources(S))) return st;
+    if ((st = nrg_march(S))) return st;
+  }
+  MOONS_HIP(hipEventRecord(S->ev1, stream()));
+  MOONS_HIP(hipEventSynchronize(S->ev1));
+  if (info) {
+    info->iterations = S->iters;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, S->ev0, S->ev1);
+    info->ms = ms;
+  }
+  return MOONS_OK;
+}
+
+int moons_nrg_field(moons_nrg_t S, int which, moons_field_t *out) {
+  if (!S || !out) return fail(MOONS_ERR_ARG, "moons_nrg_field: null argument");
+  switch (which) {
+    case MOONS_NRG_F: *out = S->F; return MOONS_OK;
+    case MOONS_NRG_FNM1: *out = S->Fnm1; return MOONS_OK;
+    case MOONS_NRG_TNM1: *out = S->Tnm1; return MOONS_OK;
+    default: return fail(MOONS_ERR_ARG, "moons_nrg_field: unknown field");
+  }
+}
+
+} // extern "C"

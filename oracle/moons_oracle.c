@@ -794,12 +794,72 @@ static void lap_explicit(double *Ax, const double *x, int dl, const orc_grid *g)
   orc_lap_centered_sf(Ax, x, dl, g, NULL);
 }
 
+/* nrg_diffusion_explicit (matrix_free_operators.f90:148-161):
+   grad(tempk%x, x); multiply(tempk%x, k%x); div(Ax, tempk%x); multiply(Ax, c);
+   add(Ax, x).  k holds three face arrays (x-, y-, z-faces). */
+static void nrg_explicit(double *Ax, const double *x, double *const k[3], double c, const orc_grid *g) {
+  double *t[3];
+  size_t nf[3];
+  for (int d = 0; d < 3; ++d) {
+    nf[d] = orc_field_size(g, 1 << d);
+    t[d] = dalloc(nf[d]);
+  }
+  orc_grad(t[0], t[1], t[2], x, ORC_DL_CC, g);
+  for (int d = 0; d < 3; ++d)
+    for (size_t q = 0; q < nf[d]; ++q) t[d][q] = t[d][q] * k[d][q];
+  orc_div(Ax, ORC_DL_CC, t[0], t[1], t[2], ORC_DL_FX, g, NULL);
+  size_t n = orc_field_size(g, ORC_DL_CC);
+  for (size_t q = 0; q < n; ++q) Ax[q] = Ax[q] * c;
+  for (size_t q = 0; q < n; ++q) Ax[q] = Ax[q] + x[q];
+  for (int d = 0; d < 3; ++d) free(t[d]);
+}
+
+/* nrg_diffusion (matrix_free_operators.f90:163-174): implicit BCs on x first */
+void orc_nrg_op(double *Ax, double *x, double *const k[3], double c, const orc_grid *g, const orc_bcs *bc) {
+  orc_apply_bcs(x, ORC_DL_CC, g, bc, 1);
+  nrg_explicit(Ax, x, k, c, g);
+}
+
+/* the operator of an SF PCG: Lap_uniform_SF_explicit (kind 0) or
+   nrg_diffusion_explicit with k and c (kind 1) */
+typedef struct {
+  int kind;
+  double *const *k;
+  double c;
+} sf_op;
+
+static void op_explicit(const sf_op *op, double *Ax, const double *x, int dl, const orc_grid *g) {
+  if (op->kind == 1) nrg_explicit(Ax, x, op->k, op->c, g);
+  else lap_explicit(Ax, x, dl, g);
+}
+
 /* ------------------------------------------------------------------------- */
 /* PCG: hw/solvers/PCG/PCG_solver_algorithm.f90:37-107, PCG_aux.f90:30-123    */
 /* ------------------------------------------------------------------------- */
+static int pcg_sf(const sf_op *op, const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann, double *x,
+                  const double *b, int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+                  double *res_hist);
+
 int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann, double *x,
                    const double *b, int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
                    double *res_hist) {
+  sf_op op = {0, NULL, 0.0};
+  return pcg_sf(&op, g, dl, bc, all_neumann, x, b, iter_max, tol_rel, tol_abs, n_skip, out, res_hist);
+}
+
+/* The energy equation's solver (energy_extend.f90:129-130): nrg_diffusion with
+   prec_lap_SF, c = coeff_implicit_time_split set by update_MFP
+   (PCG_solver_extend.f90:269-279). */
+int orc_pcg_nrg_sf(const orc_grid *g, const orc_bcs *bc, int all_neumann, double *const k[3], double c, double *x,
+                   const double *b, int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+                   double *res_hist) {
+  sf_op op = {1, k, c};
+  return pcg_sf(&op, g, ORC_DL_CC, bc, all_neumann, x, b, iter_max, tol_rel, tol_abs, n_skip, out, res_hist);
+}
+
+static int pcg_sf(const sf_op *op, const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann, double *x,
+                  const double *b, int iter_max, double tol_rel, double tol_abs, int n_skip, double *out,
+                  double *res_hist) {
   if (n_skip < 1) return -1;
   int s[3];
   orc_field_shape(g, dl, s);
@@ -828,12 +888,12 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
   memset(tmp, 0, sizeof(double) * n);
   orc_apply_bcs(tmp, dl, g, bc, 0);
   assign_ghost_n_xperiodic(tmp, dl, g, bc, 0.0);
-  lap_explicit(Ax, tmp, dl, g);
+  op_explicit(op, Ax, tmp, dl, g);
   orc_assign_ghost_xperiodic(Ax, dl, g, bc, 0.0);
   for (size_t q = 0; q < n; ++q) r[q] = r[q] - Ax[q];
-  /* operator(Ax,x): Lap_uniform_SF applies the implicit BCs to x first */
+  /* operator(Ax,x): the operator applies the implicit BCs to x first */
   orc_apply_bcs(x, dl, g, bc, 1);
-  lap_explicit(Ax, x, dl, g);
+  op_explicit(op, Ax, x, dl, g);
   for (size_t q = 0; q < n; ++q) r[q] = r[q] - Ax[q];
   for (size_t q = 0; q < n; ++q) r[q] = r[q] * vol[q];
   if (dl != ORC_DL_CC) {
@@ -859,7 +919,7 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
   if (!e2) {
     for (int it = 1; it <= iter_max; ++it) {
       orc_apply_bcs(p, dl, g, bc, 1);
-      lap_explicit(Ax, p, dl, g);
+      op_explicit(op, Ax, p, dl, g);
       orc_multiply_wall_neumann(Ax, dl, g, bc, 0.5);
       orc_assign_wall_dirichlet(Ax, dl, g, bc, 0.0);
       assign_wall_periodic_single(Ax, dl, g, bc, 0.0);

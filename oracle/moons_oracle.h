@@ -132,6 +132,15 @@ int orc_pcg_lap_sf(const orc_grid *g, int dl, const orc_bcs *bc, int all_neumann
                    double *x, const double *b, int iter_max, double tol_rel, double tol_abs,
                    int n_skip_check_res, double *out, double *res_hist);
 
+/* nrg_diffusion (matrix_free_operators.f90:148-174): Ax = c div(k grad x) + x
+   on a cell-centred x (implicit BCs applied to x first); k: x-, y-, z-face arrays. */
+void orc_nrg_op(double *Ax, double *x, double *const k[3], double c, const orc_grid *g, const orc_bcs *bc);
+/* The same PCG with op = nrg_diffusion and prec = prec_lap_SF (the energy
+   equation's solver, energy_extend.f90:129-130). */
+int orc_pcg_nrg_sf(const orc_grid *g, const orc_bcs *bc, int all_neumann, double *const k[3], double c,
+                   double *x, const double *b, int iter_max, double tol_rel, double tol_abs,
+                   int n_skip_check_res, double *out, double *res_hist);
+
 /* ---- vector-field PCG (momentum and induction implicit diffusion) ----
    Restates solve_PCG_VF (hw/solvers/PCG/PCG_solver_algorithm.f90:109-179) and
    modify_RHS_VF (PCG_aux.f90:56-79) with

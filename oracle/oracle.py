@@ -67,6 +67,10 @@ def lib():
         L.orc_pcg_lap_sf.argtypes = [C.c_void_p, C.c_int, C.POINTER(BCs), C.c_int, _dp, _dp, C.c_int, C.c_double,
                                      C.c_double, C.c_int, _dp, _dp]
         L.orc_splitmix_fill.argtypes = [_dp, C.c_size_t, C.c_ulonglong, C.c_double]
+        L.orc_nrg_op.argtypes = [_dp, _dp, _dp * 3, C.c_double, C.c_void_p, C.POINTER(BCs)]
+        L.orc_pcg_nrg_sf.restype = C.c_int
+        L.orc_pcg_nrg_sf.argtypes = [C.c_void_p, C.POINTER(BCs), C.c_int, _dp * 3, C.c_double, _dp, _dp, C.c_int,
+                                     C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_interp.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_void_p, C.c_int]
         L.orc_edge_cross_face_E.argtypes = [_dp, _dp, _dp, _dp * 6, _dp * 3, C.c_void_p]
         L.orc_advect_B.argtypes = [_dp * 3, _dp * 6, _dp * 3, C.c_void_p]
@@ -225,6 +229,24 @@ def pcg_vf(g: Grid, kind, c, bcs3, x, b, iter_max, tol_rel, tol_abs, n_skip=1, k
                           n_skip, p(out), p(hist))
     if st != 0:
         raise RuntimeError("oracle pcg_vf failed")
+    return int(out[0]), out[1], out[2], hist[: int(out[3])].copy()
+
+
+def nrg_op(g: Grid, x, k, c, bcs: BCs):
+    """Ax of nrg_diffusion (c div(k grad x) + x); x gets its implicit BCs."""
+    Ax = g.zeros(0)
+    lib().orc_nrg_op(_F(Ax), _F(x), _p3(k), c, g.h, C.byref(bcs))
+    return Ax
+
+
+def pcg_nrg(g: Grid, bcs: BCs, all_neumann, k, c, x, b, iter_max, tol_rel, tol_abs, n_skip=1):
+    """Energy PCG (nrg_diffusion, prec_lap_SF); x updated in place."""
+    out = np.zeros(4)
+    hist = np.zeros(iter_max + 2)
+    st = lib().orc_pcg_nrg_sf(g.h, C.byref(bcs), 1 if all_neumann else 0, _p3(k), c, _F(x), _F(b), iter_max,
+                              tol_rel, tol_abs, n_skip, p(out), p(hist))
+    if st != 0:
+        raise RuntimeError("oracle pcg_nrg failed")
     return int(out[0]), out[1], out[2], hist[: int(out[3])].copy()
 
 

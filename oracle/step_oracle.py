@@ -243,3 +243,73 @@ class MHDState:
         if self.induction:
             self.induction_sources()
             self.solve_induction()
+
+
+class EnergyState:
+    """The energy equation of one step (solve_energy,
+    governing_equations/energy_extend.f90:236-262, after
+    add_all_energy_sources, solvers/energy/add_all_energy_sources.f90:21-80):
+
+      Fnm1 = F; F = 0
+      advection: F += scale * (-div(cellCenter2Face(T) * U))   add_advection (energy_sources.f90:26-38)
+      diffusion: F += scale * lap(T), lapUniformCoeff_SF       add_diffusion (energy_sources.f90:40-48)
+      method 1: T = T + F dt             Euler_time_no_diff_Euler_sources_SF (time_marching_methods_SF.f90:154-170)
+      method 2: T = T + dt AB2(F, Fnm1)  Euler_time_no_diff_AB2_sources_SF (:135-152)
+      method 3: Tnm1 = T; solve (I + c div(k grad)) T = T + dt AB2(F, Fnm1), c = dt coeff_implicit
+                                         Euler_time_AB2_sources_SF (:71-98)
+
+    U is the face velocity the step reads (embedFace is the identity on one
+    domain).  k are the face arrays bound to the T solver's operator.  The
+    Dirichlet wall planes of assign_wall_Dirichlet do not exist on a
+    cell-centred field, so that call changes nothing here.
+    """
+
+    def __init__(self, g, T, Tbcs, k, dt, method=3, coeff_implicit=-1.0, advection=(True, -1.0),
+                 diffusion=(False, 1.0), isp=(5, 1e-6, 1e-13, 1), all_neumann=False):
+        self.g = g
+        self.T = np.array(T, dtype=np.float64, order="F")  # a copy: the step advances it in place
+        self.Tbcs, self.k = Tbcs, [np.asfortranarray(a) for a in k]
+        self.dt, self.method, self.coeff_implicit = dt, method, coeff_implicit
+        self.advection, self.diffusion, self.isp, self.all_neumann = advection, diffusion, isp, all_neumann
+        self.F = g.zeros(0)
+        self.Fnm1 = g.zeros(0)
+        self.Tnm1 = self.T.copy(order="F")
+        self.iters = 0
+
+    def sources(self, U):
+        g = self.g
+        self.Fnm1 = self.F.copy(order="F")
+        F = g.zeros(0)
+        add, scale = self.advection
+        if add:
+            tF = [np.asfortranarray(O.interp(g, self.T, 0, 1 << d, d) * U[d]) for d in range(3)]
+            t = O.div(g, tF[0], tF[1], tF[2], 0)
+            t = t * (-1.0)
+            F = F + t * scale
+        add, scale = self.diffusion
+        if add:
+            t = g.zeros(0)
+            O.del_op(g, t, 0, self.T, 0, 2, 0, 0, 1)
+            O.del_op(g, t, 0, self.T, 0, 2, 1, 0, 2)
+            O.del_op(g, t, 0, self.T, 0, 2, 2, 0, 2)
+            F = F + t * scale
+        self.F = np.asfortranarray(F)
+
+    def solve(self):
+        if self.method == 1:
+            self.T = np.asfortranarray(self.T + self.F * self.dt)
+            return
+        t = self.F * 1.5
+        t = t + self.Fnm1 * (-0.5)
+        t = t * self.dt
+        if self.method == 2:
+            self.T = np.asfortranarray(self.T + t)
+            return
+        rhs = np.asfortranarray(t + self.T)
+        self.Tnm1 = self.T.copy(order="F")
+        c = self.dt * 1.0 * self.coeff_implicit
+        self.iters, _, _, _ = O.pcg_nrg(self.g, self.Tbcs, self.all_neumann, self.k, c, self.T, rhs, *self.isp)
+
+    def step(self, U):
+        self.sources(U)
+        self.solve()
