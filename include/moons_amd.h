@@ -93,6 +93,7 @@ typedef struct moons_mhd_s *moons_mhd_t;
 typedef struct moons_unet_s *moons_unet_t;
 typedef struct moons_fft_s *moons_fft_t;
 typedef struct moons_nrg_s *moons_nrg_t;
+typedef struct moons_domain_s *moons_domain_t;
 
 /* Exit information of one solve, the reference's ISP and norms after
  * solve_PCG_SF (code/handwritten/solvers/PCG/PCG_solver_algorithm.f90:37-107):
@@ -133,6 +134,12 @@ int moons_nodes_uniform(double hmin, double hmax, int N, double *hn);
  * code/handwritten/grid/grid_init.f90:60-68.  hn has N+3 entries. */
 int moons_nodes_roberts_both(double hmin, double hmax, int N, double beta, double *hn);
 
+/* ext_uniform_IO (code/handwritten/grid/grid_continue.f90:41-47,94-107,197-222):
+ * hn (with ghosts, sn >= 4 entries) extended by N uniform cells at both ends,
+ * each with its end's ghost spacing, continued from hmin down and hmax up;
+ * out gets sn + 2N nodes, with ghosts. */
+int moons_nodes_ext_uniform(const double *hn, int sn, int N, double *out);
+
 /* ---------------------------------------------------------------- mesh */
 /* init_c per direction (code/handwritten/grid/coordinates_extend.f90:78-173):
  * node coordinates including the two ghost nodes, sn = N+3 >= 5 entries. */
@@ -160,6 +167,31 @@ int moons_field_copy(moons_field_t dst, moons_field_t src);
 /* Device address of the field's array, for callers that share memory with
  * the library on its stream (e.g. torch tensors over the same buffers). */
 int moons_field_device_ptr(moons_field_t f, void **dptr);
+
+/* ---------------------------------------------------- multi-domain meshes */
+/* mesh_domain (init_MD, code/handwritten/mesh_domain/mesh_domain_extend.f90:31-38):
+ * where the interior mesh (e.g. the fluid) lies in the total mesh (e.g. the
+ * induction mesh with its walls and vacuum), per direction the range of
+ * coinciding interior cell centres and nodes (physical_sub_domain with
+ * padding 1 and tolerance 1e-12,
+ * code/handwritten/physical_domain/physical_sub_domain_extend.f90:36-45,
+ * code/handwritten/sub_domain/overlap_extend.f90:204-278).  MOONS_ERR_ARG
+ * when a direction has no common range. */
+int moons_domain_create(moons_mesh_t interior, moons_mesh_t total, moons_domain_t *out);
+int moons_domain_destroy(moons_domain_t d);
+/* r = interior lo, interior hi, total lo, total hi (0-based, inclusive) of
+ * direction dir (0..2): node = 0 the cell centres, 1 the nodes. */
+int moons_domain_overlap(moons_domain_t d, int dir, int node, int r[4]);
+/* embed / extract (code/handwritten/ops/ops_embedExtract.f90:169-278,
+ * code/handwritten/grid_field/GF_embed_extract/GF_embed_extract.f90:14-60):
+ * copy the overlap of two fields of one location from the interior field
+ * into the total one, or back; every other value is left alone. */
+int moons_embed(moons_field_t total, moons_field_t interior, moons_domain_t d);
+int moons_extract(moons_field_t interior, moons_field_t total, moons_domain_t d);
+/* set_sigma_inv_SF/VF (code/handwritten/solvers/induction/induction_aux.f90:162-194)
+ * on one field of the total mesh: 1/sig_local_over_sig_f everywhere, 1 on
+ * the overlap of the conducting mesh d->interior. */
+int moons_set_sigma_inv(moons_field_t sig, moons_domain_t d, double sig_local_over_sig_f);
 
 /* ------------------------------------------ restart primitives, raw export */
 /* Text files in the reference's list-directed layout (reals with the fewest
@@ -387,6 +419,21 @@ typedef struct {
 int moons_mhd_create(moons_mesh_t m, const moons_mhd_params *params, moons_field_t U[3], moons_field_t p,
                      moons_field_t B[3], moons_field_t phi, moons_mhd_t *out);
 int moons_mhd_destroy(moons_mhd_t s);
+/* The same step with the induction on its own mesh (the fluid mesh m
+ * extended by walls and vacuum, MOONS' multi-domain MHD): B, phi and
+ * sigma_inv live on m_ind, d is the fluid mesh inside m_ind
+ * (moons_domain_create(m, m_ind)).  Each step embeds the fluid's edge
+ * velocity into m_ind before the CT source (embedVelocity_E,
+ * code/handwritten/solvers/induction/add_all_induction_sources.f90:35-36,
+ * code/handwritten/solvers/induction/induction_aux.f90:135-143; zero outside
+ * the fluid) and extracts J x B to the fluid faces (extract_add_JCrossB,
+ * code/handwritten/solvers/momentum/momentum_sources.f90:111-121).
+ * sigma_inv: the three edge fields of 1/sigma on m_ind (copied; NULL: 1),
+ * e.g. from moons_set_sigma_inv.  m_ind == m with d == NULL is
+ * moons_mhd_create. */
+int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t m_ind, moons_domain_t d, const moons_mhd_params *params,
+                        moons_field_t U[3], moons_field_t p, moons_field_t B[3], moons_field_t phi,
+                        moons_field_t sigma_inv[3], moons_mhd_t *out);
 /* Advances n_steps time steps; info (may be NULL) gets the last step's solver
  * iterations and the device time of the call. */
 int moons_mhd_step(moons_mhd_t s, int n_steps, moons_mhd_info *info);

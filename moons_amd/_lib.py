@@ -119,6 +119,16 @@ SIGNATURES = {
     "moons_pcg_vf_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(PcgInfo)]),
     "moons_mhd_create": (C.c_int, [C.c_void_p, C.POINTER(MhdParams), C.POINTER(C.c_void_p), C.c_void_p,
                                    C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_mhd_create_md": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MhdParams), C.POINTER(C.c_void_p),
+                                      C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p)]),
+    "moons_nodes_ext_uniform": (C.c_int, [c_double_p, C.c_int, C.c_int, c_double_p]),
+    "moons_domain_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "moons_domain_destroy": (C.c_int, [C.c_void_p]),
+    "moons_domain_overlap": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p]),
+    "moons_embed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_extract": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "moons_set_sigma_inv": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "moons_mhd_destroy": (C.c_int, [C.c_void_p]),
     "moons_mhd_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MhdInfo)]),
     "moons_mhd_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),

@@ -53,6 +53,15 @@ def nodes_roberts_both(hmin: float, hmax: float, N: int, beta: float) -> np.ndar
     return hn
 
 
+def nodes_ext_uniform(hn, N: int) -> np.ndarray:
+    """ext_uniform_IO: the node array (with ghosts) extended by N uniform
+    cells at both ends (code/handwritten/grid/grid_continue.f90:41-47)."""
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    out = np.zeros(hn.size + 2 * N)
+    check(load().moons_nodes_ext_uniform(_dp(hn), hn.size, N, _dp(out)))
+    return out
+
+
 class mesh:
     """Single-block mesh from three node-coordinate arrays (ghosts included)."""
 
@@ -161,6 +170,50 @@ class SF:
                 self.handle = None
         except Exception:
             pass
+
+
+# ------------------------------------------------------ multi-domain meshes
+class mesh_domain:
+    """Where the interior mesh lies in the total mesh (init_MD,
+    code/handwritten/mesh_domain/mesh_domain_extend.f90:31-38)."""
+
+    def __init__(self, m_interior: "mesh", m_total: "mesh"):
+        h = C.c_void_p()
+        check(load().moons_domain_create(m_interior.handle, m_total.handle, C.byref(h)))
+        self.handle = h
+        self._keep = (m_interior, m_total)
+
+    def overlap(self, dir: int, node: bool):
+        """(interior lo, interior hi, total lo, total hi), 0-based inclusive."""
+        r = (C.c_int * 4)()
+        check(load().moons_domain_overlap(self.handle, dir, 1 if node else 0, r))
+        return tuple(r)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().moons_domain_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def embed(total, interior, MD: mesh_domain) -> None:
+    """embed (code/handwritten/ops/ops_embedExtract.f90:169-278): an SF or lists of SFs."""
+    for t, i in zip(_raw_fields(total), _raw_fields(interior)):
+        check(load().moons_embed(t.handle, i.handle, MD.handle))
+
+
+def extract(interior, total, MD: mesh_domain) -> None:
+    """extract (code/handwritten/ops/ops_embedExtract.f90:49-165): an SF or lists of SFs."""
+    for i, t in zip(_raw_fields(interior), _raw_fields(total)):
+        check(load().moons_extract(i.handle, t.handle, MD.handle))
+
+
+def set_sigma_inv(sigma_inv, MD_sigma: mesh_domain, sig_local_over_sig_f: float) -> None:
+    """set_sigma_inv_SF/VF (code/handwritten/solvers/induction/induction_aux.f90:162-194)."""
+    for f in _raw_fields(sigma_inv):
+        check(load().moons_set_sigma_inv(f.handle, MD_sigma.handle, sig_local_over_sig_f))
 
 
 # ------------------------------------------ restart primitives, raw export
@@ -482,7 +535,8 @@ class MHD_step:
 
     def __init__(self, m: mesh, U, p: SF, B=None, phi: SF | None = None, Re=400.0, Ha=0.0, Rem=1.0, dt=1e-3,
                  alpha_U=1.0, B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
-                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), prec_p="Lap_SF"):
+                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), prec_p="Lap_SF", m_ind: mesh | None = None,
+                 domain: "mesh_domain | None" = None, sigma_inv=None):
         P = MhdParams()
         P.Re, P.Ha, P.Rem, P.dt, P.alpha_U = Re, Ha, Rem, dt, alpha_U
         P.induction = 1 if B is not None else 0
@@ -492,11 +546,15 @@ class MHD_step:
         for q, isp in enumerate((isp_U, isp_p, isp_B, isp_phi)):
             P.iter_max[q], P.tol_rel[q], P.tol_abs[q], P.n_skip[q] = isp
         h = C.c_void_p()
-        check(load().moons_mhd_create(m.handle, C.byref(P), _arr(U), p.handle, _arr(B) if B is not None else None,
-                                      phi.handle if phi is not None else None, C.byref(h)))
+        # m_ind, domain: the induction on its own mesh (B, phi, sigma_inv there), the fluid mesh inside it
+        check(load().moons_mhd_create_md(m.handle, m_ind.handle if m_ind is not None else None,
+                                         domain.handle if domain is not None else None, C.byref(P), _arr(U), p.handle,
+                                         _arr(B) if B is not None else None, phi.handle if phi is not None else None,
+                                         _arr(sigma_inv) if sigma_inv is not None else None, C.byref(h)))
         self.handle = h
         self.m = m
-        self._keep = (U, p, B, phi)
+        self.m_ind = m_ind if m_ind is not None else m
+        self._keep = (U, p, B, phi, m_ind, domain, sigma_inv)
 
     def step(self, n: int = 1):
         info = MhdInfo()
@@ -515,7 +573,8 @@ class MHD_step:
         out = (C.c_void_p * 6)()
         check(load().moons_mhd_field(self.handle, self.FIELDS[which], out))
         n = 6 if which == "U_E" else (1 if which == "ppe_rhs" else 3)
-        return [SF.borrow(self.m, out[q]) for q in range(n)]
+        mm = self.m_ind if which in ("J", "jCrossB", "FB") else self.m
+        return [SF.borrow(mm, out[q]) for q in range(n)]
 
     def __del__(self):
         try:

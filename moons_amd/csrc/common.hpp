@@ -149,7 +149,18 @@ struct moons_field_s {
   double bval[6] = {0, 0, 0, 0, 0, 0};
 };
 
+// Where an interior mesh lies in a total mesh: per direction the ranges of
+// coinciding cell centres (C) and nodes (N), 0-based inclusive:
+// interior lo, interior hi, total lo, total hi.
+struct moons_domain_s {
+  moons_mesh_s *mi = nullptr, *mt = nullptr;
+  int C[3][4], N[3][4];
+};
+
 namespace moons {
+// subarray copies over a domain's ranges (the field's location picks C or N per direction)
+int launch_embed(moons_field_s *total, const moons_field_s *interior, const moons_domain_s *d);
+int launch_extract(moons_field_s *interior, const moons_field_s *total, const moons_domain_s *d);
 BCDev make_bcdev(const moons_field_s *f);
 int launch_apply_bcs(moons_field_s *u, bool implicit_);
 int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_); // three fields (e.g. a VF)

@@ -77,12 +77,18 @@ static inline unsigned sblk(size_t n) { return (unsigned)((n + kST - 1) / kST); 
 
 struct moons_mhd_s {
   moons_mesh_s *m = nullptr;
+  moons_mesh_s *mi = nullptr;          // mesh of the induction fields (m, or the extended mesh)
+  const moons_domain_s *dom = nullptr; // the fluid mesh inside mi (multi-domain only)
   moons_mhd_params P{};
   moons_field_s *U[3] = {}, *p = nullptr, *B[3] = {}, *phi = nullptr; // caller's
   moons_field_s *Ustar[3] = {}, *F[3] = {}, *Fnm1[3] = {}, *L[3] = {}, *T1[3] = {}, *T2[3] = {}, *T3[3] = {};
   moons_field_s *UE[6] = {}, *cc = nullptr, *tcc = nullptr;
   moons_field_s *Bstar[3] = {}, *FB[3] = {}, *FBnm1[3] = {}, *LB[3] = {}, *B0[3] = {}, *J[3] = {}, *JxB[3] = {};
   moons_field_s *sig[3] = {}, *E[3] = {}, *ETF[6] = {}, *A6[6] = {}, *B6[6] = {};
+  // induction-mesh temporaries: U_E embedded (multi-domain only; else UE),
+  // face temporaries (multi-domain only; else T1, T2) and the phi right-hand side
+  moons_field_s *UEi[6] = {}, *T1i[3] = {}, *T2i[3] = {}, *cci = nullptr, *tcci = nullptr;
+  bool own_i = false; // UEi, T1i, T2i, cci are fields of their own
   moons_pcg_t pcg_p = nullptr, pcg_phi = nullptr;
   moons_pcgvf_t pcg_U = nullptr, pcg_B = nullptr;
   int iters[4] = {0, 0, 0, 0};
@@ -117,6 +123,16 @@ void mhd_free(moons_mhd_s *S) {
   for (auto g : groups6)
     for (int q = 0; q < 6; ++q)
       if (g[q]) moons_field_destroy(g[q]);
+  if (S->own_i) {
+    for (int q = 0; q < 6; ++q)
+      if (S->UEi[q]) moons_field_destroy(S->UEi[q]);
+    for (int d = 0; d < 3; ++d) {
+      if (S->T1i[d]) moons_field_destroy(S->T1i[d]);
+      if (S->T2i[d]) moons_field_destroy(S->T2i[d]);
+    }
+    if (S->cci) moons_field_destroy(S->cci);
+  }
+  if (S->tcci) moons_field_destroy(S->tcci);
   if (S->cc) moons_field_destroy(S->cc);
   if (S->tcc) moons_field_destroy(S->tcc);
   if (S->pcg_p) moons_pcg_destroy(S->pcg_p);
@@ -149,10 +165,11 @@ int addmul(moons_field_s *y, const moons_field_s *a, const moons_field_s *b) {
 }
 
 // face i -> CC (along i) -> face j (along j)
-int face2face(moons_mhd_s *S, moons_field_s *out, const moons_field_s *u, int i, int j) {
+// face i -> CC (cc, a cell-centred temporary of u's mesh) -> face j
+int face2face(moons_field_s *cc, moons_field_s *out, const moons_field_s *u, int i, int j) {
   int st;
-  if ((st = launch_interp(S->cc, u, i))) return st;
-  return launch_interp(out, S->cc, j);
+  if ((st = launch_interp(cc, u, i))) return st;
+  return launch_interp(out, cc, j);
 }
 
 // U_E = face2Edge_no_diag(U) (ops/ops_interp.f90:531-544)
@@ -179,7 +196,7 @@ int convection(moons_mhd_s *S, moons_field_s *div[3]) {
     for (int t = 0; t < 2; ++t) {
       const int q = terms[d][t][0], ddir = terms[d][t][1], vel = terms[d][t][2];
       if ((st = launch_del(S->T2[d], S->UE[q], 1, ddir, 0, 1))) return st;
-      if ((st = face2face(S, S->T3[d], S->U[vel], vel, d))) return st;
+      if ((st = face2face(S->cc, S->T3[d], S->U[vel], vel, d))) return st;
       if ((st = addmul(div[d], S->T2[d], S->T3[d]))) return st;
     }
   return MOONS_OK;
@@ -212,7 +229,7 @@ int post_induction(moons_mhd_s *S) {
   for (int q = 0; q < 6; ++q) {
     const int i = kNoDiag[q][0], j = kNoDiag[q][1];
     if ((st = launch_interp(S->A6[q], S->J[i], 3 - i - j))) return st;  // edge2Face
-    if ((st = face2face(S, S->B6[q], S->B0[i], i, j))) return st;       // face2Face(Btot)
+    if ((st = face2face(S->cci, S->B6[q], S->B0[i], i, j))) return st;     // face2Face(Btot)
   }
   // (J x B)_x = J_y Bz - J_z By on x-faces; y negated form; z = J_x By - J_y Bx
   if ((st = launch_cross(S->JxB[0], S->A6[0], S->B6[1], S->A6[1], S->B6[0], 1))) return st;
@@ -244,7 +261,10 @@ int momentum(moons_mhd_s *S) {
   }
   if (S->P.induction)
     for (int d = 0; d < 3; ++d) {
-      if ((st = moons_field_copy(S->T1[d], S->JxB[d]))) return st;
+      // extract_add_JCrossB: the overlap of the induction mesh's J x B onto the
+      // fluid faces (every fluid face but the ghosts), ghosts zeroed
+      if ((st = S->dom ? launch_extract(S->T1[d], S->JxB[d], S->dom) : moons_field_copy(S->T1[d], S->JxB[d])))
+        return st;
       if ((st = launch_assign_ghost_xperiodic(S->T1[d], 0.0))) return st;
       if ((st = axpy(S->F[d], S->T1[d], 1.0))) return st;
     }
@@ -274,22 +294,25 @@ int induction(moons_mhd_s *S) {
     if ((st = moons_field_fill(S->LB[d], 0.0))) return st;
     if ((st = moons_field_fill(S->FB[d], 0.0))) return st;
   }
-  if ((st = moons_advect_B(S->T1, S->UE, S->B0, S->ETF, S->E))) return st;
+  if (S->dom) // embedVelocity_E: the fluid's edge velocity into the induction mesh
+    for (int q = 0; q < 6; ++q)
+      if ((st = launch_embed(S->UEi[q], S->UE[q], S->dom))) return st;
+  if ((st = moons_advect_B(S->T1i, S->UEi, S->B0, S->ETF, S->E))) return st;
   for (int d = 0; d < 3; ++d)
-    if ((st = axpy(S->FB[d], S->T1[d], 1.0))) return st;
+    if ((st = axpy(S->FB[d], S->T1i[d], 1.0))) return st;
   for (int d = 0; d < 3; ++d)
     if ((st = mul3(S->E[d], S->J[d], S->sig[d]))) return st;
   for (int d = 0; d < 3; ++d) {
-    if ((st = moons_curl(S->T2[d], S->E[0], S->E[1], S->E[2], d))) return st;
-    if ((st = axpy(S->FB[d], S->T2[d], -0.0))) return st; // -beta, beta = 1 - alpha = 0
+    if ((st = moons_curl(S->T2i[d], S->E[0], S->E[1], S->E[2], d))) return st;
+    if ((st = axpy(S->FB[d], S->T2i[d], -0.0))) return st; // -beta, beta = 1 - alpha = 0
   }
-  if ((st = ab2_rhs(S->T1, S->FB, S->FBnm1, S->LB, S->B, S->P.dt, true))) return st;
+  if ((st = ab2_rhs(S->T1i, S->FB, S->FBnm1, S->LB, S->B, S->P.dt, true))) return st;
   for (int d = 0; d < 3; ++d)
     if ((st = moons_field_copy(S->Bstar[d], S->B[d]))) return st;
   if ((st = moons_pcg_vf_update_mfp(S->pcg_B, S->P.dt * 1.0 * (1.0 / S->P.Rem), 1))) return st;
-  if ((st = moons_pcg_vf_solve(S->pcg_B, S->Bstar, S->T1, &info))) return st;
+  if ((st = moons_pcg_vf_solve(S->pcg_B, S->Bstar, S->T1i, &info))) return st;
   S->iters[2] = info.iterations;
-  if ((st = moons_clean_div(S->pcg_phi, S->B, S->Bstar, S->phi, 1.0 / S->P.dt, S->T2, S->tcc, &info))) return st;
+  if ((st = moons_clean_div(S->pcg_phi, S->B, S->Bstar, S->phi, 1.0 / S->P.dt, S->T2i, S->tcci, &info))) return st;
   S->iters[3] = info.iterations;
   return post_induction(S);
 }
@@ -300,19 +323,37 @@ extern "C" {
 
 int moons_mhd_create(moons_mesh_t m, const moons_mhd_params *P, moons_field_t U[3], moons_field_t p,
                      moons_field_t B[3], moons_field_t phi, moons_mhd_t *out) {
+  return moons_mhd_create_md(m, m, nullptr, P, U, p, B, phi, nullptr, out);
+}
+
+int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, const moons_mhd_params *P,
+                        moons_field_t U[3], moons_field_t p, moons_field_t B[3], moons_field_t phi,
+                        moons_field_t sigma_inv[3], moons_mhd_t *out) {
   if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
   if (!m || !P || !U || !p || !out) return fail(MOONS_ERR_ARG, "moons_mhd_create: null argument");
+  if (!mi) mi = m;
   if (!(P->dt > 0.0) || !(P->Re > 0.0) || (P->alpha_U != 0.0 && P->alpha_U != 1.0))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: need dt > 0, Re > 0, alpha_U 0 (method 4) or 1 (method 6)");
   if (P->induction && (!B || !phi || !(P->Rem > 0.0)))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: induction needs B, phi and Rem > 0");
+  if (mi != m && (!dom || dom->mi != m || dom->mt != mi || !P->induction))
+    return fail(MOONS_ERR_ARG, "moons_mhd_create_md: a separate induction mesh needs induction and the domain of "
+                               "the fluid mesh inside it");
+  if (mi == m && dom) return fail(MOONS_ERR_ARG, "moons_mhd_create_md: a domain needs a separate induction mesh");
   for (int d = 0; d < 3; ++d)
-    if (!U[d] || U[d]->dl != (1 << d) || U[d]->m != m || (P->induction && (!B[d] || B[d]->dl != (1 << d))))
-      return fail(MOONS_ERR_ARG, "moons_mhd_create: U and B must be face components on the mesh");
-  if (p->dl != MOONS_DL_CC || (P->induction && phi->dl != MOONS_DL_CC))
-    return fail(MOONS_ERR_ARG, "moons_mhd_create: p and phi must be cell-centred");
+    if (!U[d] || U[d]->dl != (1 << d) || U[d]->m != m ||
+        (P->induction && (!B[d] || B[d]->dl != (1 << d) || B[d]->m != mi)))
+      return fail(MOONS_ERR_ARG, "moons_mhd_create: U and B must be face components on the fluid and induction meshes");
+  if (p->dl != MOONS_DL_CC || (P->induction && (phi->dl != MOONS_DL_CC || phi->m != mi)))
+    return fail(MOONS_ERR_ARG, "moons_mhd_create: p and phi must be cell-centred on their meshes");
+  if (sigma_inv)
+    for (int d = 0; d < 3; ++d)
+      if (!sigma_inv[d] || sigma_inv[d]->m != mi || sigma_inv[d]->dl != kEdgeDL[d])
+        return fail(MOONS_ERR_ARG, "moons_mhd_create_md: sigma_inv must be the three edge fields of the induction mesh");
   moons_mhd_s *S = new moons_mhd_s();
   S->m = m;
+  S->mi = mi;
+  S->dom = dom;
   S->P = *P;
   S->p = p;
   S->phi = phi;
@@ -341,25 +382,42 @@ int moons_mhd_create(moons_mesh_t m, const moons_mhd_params *P, moons_field_t U[
     for (int d = 0; d < 3 && !st; ++d) {
       moons_field_s **vecs[] = {S->Bstar, S->FB, S->FBnm1, S->LB, S->B0, S->JxB};
       for (auto v : vecs)
-        if (!st) st = fcreate(m, 1 << d, &v[d]);
+        if (!st) st = fcreate(mi, 1 << d, &v[d]);
       if (!st) copy_bcs_of(S->Bstar[d], B[d]);
       if (!st) st = moons_field_fill(S->B0[d], P->B0[d]);
       if (!st) st = moons_field_fill(S->FB[d], 0.0);
-      if (!st) st = fcreate(m, kEdgeDL[d], &S->J[d]);
-      if (!st) st = fcreate(m, kEdgeDL[d], &S->sig[d]);
-      if (!st) st = moons_field_fill(S->sig[d], 1.0);
-      if (!st) st = fcreate(m, kEdgeDL[d], &S->E[d]);
+      if (!st) st = fcreate(mi, kEdgeDL[d], &S->J[d]);
+      if (!st) st = fcreate(mi, kEdgeDL[d], &S->sig[d]);
+      if (!st) st = sigma_inv ? moons_field_copy(S->sig[d], sigma_inv[d]) : moons_field_fill(S->sig[d], 1.0);
+      if (!st) st = fcreate(mi, kEdgeDL[d], &S->E[d]);
     }
     // temp_E_TF of advect_B: Bx@y, Bx@z, By@x, By@z, Bz@x, Bz@y edges
     const int etf_edge[6] = {1, 2, 0, 2, 0, 1};
-    for (int q = 0; q < 6 && !st; ++q) st = fcreate(m, kEdgeDL[etf_edge[q]], &S->ETF[q]);
+    for (int q = 0; q < 6 && !st; ++q) st = fcreate(mi, kEdgeDL[etf_edge[q]], &S->ETF[q]);
     for (int q = 0; q < 6 && !st; ++q) {
-      st = fcreate(m, 1 << kNoDiag[q][1], &S->A6[q]);
-      if (!st) st = fcreate(m, 1 << kNoDiag[q][1], &S->B6[q]);
+      st = fcreate(mi, 1 << kNoDiag[q][1], &S->A6[q]);
+      if (!st) st = fcreate(mi, 1 << kNoDiag[q][1], &S->B6[q]);
     }
-    if (!st) st = moons_pcg_vf_create(m, MOONS_OP_IND_DIFFUSION, S->Bstar, S->sig, &S->pcg_B);
+    if (!st) st = fcreate(mi, MOONS_DL_CC, &S->tcci);
+    S->own_i = mi != m;
+    for (int q = 0; q < 6 && !st; ++q) {
+      if (!S->own_i) S->UEi[q] = S->UE[q];
+      else if (!(st = fcreate(mi, kEdgeDL[kNoDiag[q][1]], &S->UEi[q]))) st = moons_field_fill(S->UEi[q], 0.0);
+    }
+    if (!S->own_i) S->cci = S->cc;
+    else if (!st) st = fcreate(mi, MOONS_DL_CC, &S->cci);
+    for (int d = 0; d < 3 && !st; ++d) {
+      if (!S->own_i) {
+        S->T1i[d] = S->T1[d];
+        S->T2i[d] = S->T2[d];
+      } else {
+        st = fcreate(mi, 1 << d, &S->T1i[d]);
+        if (!st) st = fcreate(mi, 1 << d, &S->T2i[d]);
+      }
+    }
+    if (!st) st = moons_pcg_vf_create(mi, MOONS_OP_IND_DIFFUSION, S->Bstar, S->sig, &S->pcg_B);
     if (!st) st = moons_pcg_vf_set_isp(S->pcg_B, P->iter_max[2], P->tol_rel[2], P->tol_abs[2], P->n_skip[2]);
-    if (!st) st = moons_pcg_sf_create(m, MOONS_OP_LAP_UNIFORM_SF, MOONS_PREC_LAP_SF, phi, &S->pcg_phi);
+    if (!st) st = moons_pcg_sf_create(mi, MOONS_OP_LAP_UNIFORM_SF, MOONS_PREC_LAP_SF, phi, &S->pcg_phi);
     if (!st) st = moons_pcg_set_isp(S->pcg_phi, P->iter_max[3], P->tol_rel[3], P->tol_abs[3], P->n_skip[3]);
     if (!st) st = post_induction(S);
   }

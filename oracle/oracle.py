@@ -310,6 +310,8 @@ def ref():
                C.c_char_p, C.c_int, C.c_int]
         R.ref_export_raw.argtypes = raw
         R.ref_import_raw.argtypes = raw
+        R.ref_ext_uniform.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        R.ref_overlap.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _ip]
         _ref = R
     return _ref
 
@@ -392,3 +394,20 @@ def ref_import_raw(hn, dl, f, directory, name, pad) -> np.ndarray:
     args, fl = _raw_args(hn, dl, f, directory, name, pad)
     ref().ref_import_raw(*args)
     return fl.reshape(np.shape(f), order="F")
+
+
+def ref_ext_uniform(hn, N: int) -> np.ndarray:
+    """The reference's ext_uniform_IO of the node array hn (with ghosts)."""
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    out = np.zeros(hn.size + 2 * N)
+    ref().ref_ext_uniform(p(hn), hn.size, N, p(out))
+    return out
+
+
+def ref_overlap(kind: str, hn1, hn2, pad: int = 1):
+    """The reference's get_C_overlap / get_N_overlap: (i1 lo, i1 hi, i2 lo, i2 hi), 1-based."""
+    a = np.ascontiguousarray(hn1, dtype=np.float64)
+    b = np.ascontiguousarray(hn2, dtype=np.float64)
+    r = (C.c_int * 4)()
+    ref().ref_overlap(0 if kind == "C" else 1, p(a), a.size, p(b), b.size, pad, r)
+    return tuple(r)

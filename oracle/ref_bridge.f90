@@ -29,8 +29,45 @@ module moons_ref_bridge
   use construct_suffix_mod, only: construct_suffix
   use GF_export_mod, only: exp_3D_1C_GF
   use GF_import_mod, only: imp_3D_1C_GF
+  use grid_continue_mod, only: ext_uniform_IO
+  use coordinates_mod, only: coordinates
+  use coordinates_extend_mod, only: coord_init => init
+  use overlap_mod, only: overlap
+  use overlap_extend_mod, only: get_C_overlap, get_N_overlap
   implicit none
 contains
+
+  ! ext_uniform_IO(g, N, dir) (handwritten/grid/grid_continue.f90:41-47) on a
+  ! grid whose three directions all have the node array hn; out gets the
+  ! extended nodes of direction 1 (sn + 2N entries)
+  subroutine ref_ext_uniform(hn, sn, N, out) bind(C, name="ref_ext_uniform")
+    integer(c_int), value :: sn, N
+    real(c_double), intent(in) :: hn(sn)
+    real(c_double), intent(out) :: out(sn + 2 * N)
+    type(grid) :: g
+    call init(g, hn, hn, hn)
+    call ext_uniform_IO(g, int(N), 1)
+    out = g%c(1)%hn%f
+  end subroutine
+
+  ! get_C_overlap (kind 0) / get_N_overlap (kind 1) of the coordinates hn1
+  ! (R1) and hn2 (R2), tol 1e-12, padding p (handwritten/sub_domain/overlap_extend.f90:204-278);
+  ! r = (i1(1), i1(2), i2(1), i2(2)), 1-based, 0 when none
+  subroutine ref_overlap(kind, hn1, sn1, hn2, sn2, p, r) bind(C, name="ref_overlap")
+    integer(c_int), value :: kind, sn1, sn2, p
+    real(c_double), intent(in) :: hn1(sn1), hn2(sn2)
+    integer(c_int), intent(out) :: r(4)
+    type(coordinates), dimension(2) :: c
+    type(overlap) :: OL
+    call coord_init(c(1), hn1, sn1)
+    call coord_init(c(2), hn2, sn2)
+    if (kind == 0) then
+      OL = get_C_overlap(c, 1.0d-12, int(p))
+    else
+      OL = get_N_overlap(c, 1.0d-12, int(p))
+    end if
+    r = (/ OL%i1(1), OL%i1(2), OL%i2(1), OL%i2(2) /)
+  end subroutine
 
   function c2f(c, n) result(f)
     integer(c_int), intent(in) :: n

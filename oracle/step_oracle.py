@@ -21,8 +21,12 @@ Sequence (code/handwritten/MOONS/MOONS_solver.f90:47-144, one RK stage):
              solve_induction, method 6 (induction_extend.f90:475-525)
              J = curl(B)/Rem; Btot = B0 (finite_Rem = F); jCrossB = N edge2Face(J) x face2Face(Btot)
                                           (induction_sources.f90:70-84)
-Embedding/extraction between the fluid and induction meshes (embedVelocity_E,
-extractFace) is the identity here: one mesh, one domain.
+With one mesh, embedding and extraction between the fluid and induction
+meshes (embedVelocity_E, extractFace) are the identity.  With an induction
+grid gi and the fluid's Domain in it (oracle/domain_oracle.py), B, phi, J and
+sigma_inv live on gi, U_E is embedded into gi each step (zero outside the
+fluid; add_all_induction_sources.f90:35-36) and J x B is extracted to the fluid
+faces (extract_add_JCrossB, momentum_sources.f90:111-121).
 Element-wise arithmetic follows the reference's order (multiply then add,
 AB2 = 1.5 Fn then + (-0.5) Fnm1).
 """
@@ -30,6 +34,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import domain_oracle as D
 from . import oracle as O
 
 EDGE_DL = (6, 5, 3)
@@ -121,8 +126,10 @@ class MHDState:
 
     def __init__(self, g, U, Ubcs, pbcs, Re, dt, alpha_U=1.0, B=None, Bbcs=None, phibcs=None, Rem=1.0, Ha=0.0,
                  B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
-                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1)):
+                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), gi=None, dom=None, sigInv=None):
         self.g = g
+        self.gi = gi if gi is not None else g  # the induction grid
+        self.dom = dom
         self.Re, self.dt, self.alpha_U = Re, dt, alpha_U
         self.coeff_explicit_U = (1.0 / Re) * (1.0 - alpha_U)
         self.coeff_implicit_U = -(1.0 / Re) * alpha_U
@@ -140,14 +147,17 @@ class MHDState:
         if self.induction:
             self.Rem, self.N = Rem, Ha * Ha / Re
             self.coeff_implicit_B = 1.0 / Rem  # -coeff_natural * alpha, coeff_natural = -1/Rem, alpha = 1
+            gi = self.gi
             self.B = [np.asfortranarray(b) for b in B]
             self.Bbcs, self.phibcs = Bbcs, phibcs
-            self.phi = g.zeros(0)
-            self.B0 = [np.asfortranarray(np.full(g.shape(1 << d), B0[d])) for d in range(3)]
-            self.FB = [g.zeros(1 << d) for d in range(3)]
-            self.FBnm1 = [g.zeros(1 << d) for d in range(3)]
-            self.LB = [g.zeros(1 << d) for d in range(3)]
-            self.sigInv = [np.asfortranarray(np.ones(g.shape(dl))) for dl in EDGE_DL]
+            self.phi = gi.zeros(0)
+            self.B0 = [np.asfortranarray(np.full(gi.shape(1 << d), B0[d])) for d in range(3)]
+            self.FB = [gi.zeros(1 << d) for d in range(3)]
+            self.FBnm1 = [gi.zeros(1 << d) for d in range(3)]
+            self.LB = [gi.zeros(1 << d) for d in range(3)]
+            self.sigInv = ([np.asfortranarray(s) for s in sigInv] if sigInv is not None
+                           else [np.asfortranarray(np.ones(gi.shape(dl))) for dl in EDGE_DL])
+            self.U_Ei = {k: gi.zeros(EDGE_DL[k[1]]) for k in NO_DIAG}  # embedded U_E, zero outside the fluid
             self.post_induction()
 
     # ---------------------------------------------------------------- momentum
@@ -161,18 +171,23 @@ class MHDState:
         lap = lap_collocated(g, self.U)
         F = [F[d] + lap[d] * self.coeff_explicit_U for d in range(3)]
         if self.induction:
-            jcb = [zero_ghosts(g, self.jCrossB[d], 1 << d) for d in range(3)]
+            if self.dom is None:
+                jcb = [zero_ghosts(g, self.jCrossB[d], 1 << d) for d in range(3)]
+            else:
+                jcb = [zero_ghosts(g, D.extract(g.zeros(1 << d), self.jCrossB[d], 1 << d, self.dom), 1 << d)
+                       for d in range(3)]
             F = [F[d] + jcb[d] * 1.0 for d in range(3)]
         self.F = [np.asfortranarray(f) for f in F]
 
-    def _ab2_rhs(self, X, Xbcs, F, Fnm1, L):
+    def _ab2_rhs(self, X, Xbcs, F, Fnm1, L, g=None):
+        g = self.g if g is None else g
         out = []
         for d in range(3):
             t = F[d] * 1.5
             t = t + Fnm1[d] * (-0.5)
             t = t + L[d]
             t = t * self.dt
-            t = assign_wall_dirichlet(self.g, np.asfortranarray(t), 1 << d, Xbcs[d])
+            t = assign_wall_dirichlet(g, np.asfortranarray(t), 1 << d, Xbcs[d])
             t = t + X[d]
             out.append(np.asfortranarray(t))
         return out
@@ -201,11 +216,16 @@ class MHDState:
 
     # --------------------------------------------------------------- induction
     def induction_sources(self):
-        g = self.g
+        g = self.gi
         self.FBnm1 = [f.copy(order="F") for f in self.FB]
         self.LB = [g.zeros(1 << d) for d in range(3)]
         F = [g.zeros(1 << d) for d in range(3)]
-        U_E = [self.U_E[k] for k in NO_DIAG]
+        if self.dom is None:
+            U_E = [self.U_E[k] for k in NO_DIAG]
+        else:
+            for k in NO_DIAG:
+                self.U_Ei[k] = D.embed(self.U_Ei[k], self.U_E[k], EDGE_DL[k[1]], self.dom)
+            U_E = [self.U_Ei[k] for k in NO_DIAG]
         adv = O.advect_B(g, U_E, self.Btot)
         F = [F[d] + adv[d] * 1.0 for d in range(3)]
         E = [np.asfortranarray(self.J[d] * self.sigInv[d]) for d in range(3)]
@@ -215,8 +235,8 @@ class MHDState:
         self.FB = [np.asfortranarray(f) for f in F]
 
     def solve_induction(self):
-        g = self.g
-        rhs = self._ab2_rhs(self.B, self.Bbcs, self.FB, self.FBnm1, self.LB)
+        g = self.gi
+        rhs = self._ab2_rhs(self.B, self.Bbcs, self.FB, self.FBnm1, self.LB, g)
         Bstar = [b.copy(order="F") for b in self.B]
         c = self.dt * 1.0 * self.coeff_implicit_B
         it, _, _, _ = O.pcg_vf(g, O.IND_DIFFUSION, c, self.Bbcs, Bstar, rhs, *self.isp_B, k=self.sigInv)
@@ -226,7 +246,7 @@ class MHDState:
         self.post_induction()
 
     def post_induction(self):
-        g = self.g
+        g = self.gi
         self.J = [np.asfortranarray(O.curl(g, self.B[0], self.B[1], self.B[2], EDGE_DL[d], d) * (1.0 / self.Rem))
                   for d in range(3)]
         self.Btot = [b.copy(order="F") for b in self.B0]

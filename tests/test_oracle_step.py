@@ -60,7 +60,7 @@ def test_jcrossb_sign_and_pairing():
     """B = (0, 0, y): J = curl(B) = (1, 0, 0); J x B0 with B0 = (0, 1, 0) is (0, 0, 1)."""
     g = uniform_grid()
     st = S.MHDState.__new__(S.MHDState)
-    st.g, st.Rem, st.N = g, 1.0, 1.0
+    st.g, st.gi, st.Rem, st.N = g, g, 1.0, 1.0
     st.B0 = [np.asfortranarray(np.full(g.shape(1 << d), (0.0, 1.0, 0.0)[d])) for d in range(3)]
     X, Y, Z = coords(g, 4)
     st.B = [g.zeros(1), g.zeros(2), np.asfortranarray(Y.copy())]
