@@ -314,8 +314,8 @@ def main():
         Pk = M.PCG_solver_SF(m, xk, M.iter_solver_params(2, 0.0, 0.0, 1), prec="MG_FP32")
         Pk.solve(xk, b)
         mk = Pk.time_MG(50)
-        names = ["half_sweep_128", "restrict_128_to_64", "prolong_64_to_128", "tail", "vcycle", "fused_sweep_128",
-                 "half_sweep_64", "fused_sweep_64", "half_sweep_32", "fused_sweep_32"]
+        names = ["line_half_128", "restrict_128_to_64", "prolong_64_to_128", "tail", "vcycle", "fused_sweep_128",
+                 "line_half_64", "fused_sweep_64", "half_sweep_32", "fused_sweep_32"]
         print(json.dumps({n + "_us": round(1e3 * v, 2) for n, v in zip(names, mk)}), flush=True)
         # calibration for the PMC byte counters: 20 device-to-device copies of
         # one 130^3 FP64 field (17,576,000 bytes read and written per copy)
@@ -404,19 +404,20 @@ def main():
                      "per_kernel_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
                      "per_kernel_GBs": {n: round(alg[n] / (t * 1e-3) / 1e9, 1) for n, t in zip(names, kms)},
                      "iteration_GBs": round(88 * ncell / (iter_us * 1e-6) / 1e9, 1)}
-    # the mixed solver's dominant kernel: the finest-level red-black
-    # half-sweep (2 (nu1 + nu2) launches per V-cycle), 6 B per cell: the
-    # other colour's x (2 B), this colour's f (2 B) and x written (2 B), FP32
+    # the mixed solver's dominant kernel: the finest level's zebra line
+    # half-sweep (y- and z-lines, 4 (nu1 + nu2) launches per V-cycle; timed:
+    # the y-lines of one colour), 6 B per cell of the level: the other
+    # colour's x (2 B), this colour's f (2 B) and x written (2 B), FP32
     mk = Pm.time_MG(50)
     sm_bytes = 6 * ncell
     sm_gbs = sm_bytes / (mk[0] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "mg_smooth (128^3 half-sweep)", "achieved": round(sm_gbs, 1),
+    roofline = {"bound": "hbm", "kernel": "mg_line (128^3 zebra y-line half-sweep)", "achieved": round(sm_gbs, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sm_gbs / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic("mg_smooth_128_traffic.txt"),
+                "traffic": load_traffic("mg_line_128_traffic.txt"),
                 "algorithmic_bytes": sm_bytes,
-                "per_kernel_us": {"mg_smooth_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
+                "per_kernel_us": {"mg_line_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
                                   "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2),
-                                  "mg_fused_sweep_64": round(1e3 * mk[7], 2), "mg_fused_sweep_32": round(1e3 * mk[9], 2)},
+                                  "mg_line_64": round(1e3 * mk[6], 2), "mg_fused_sweep_32": round(1e3 * mk[9], 2)},
                 "vcycle_us": round(1e3 * mk[4], 2),
                 "mixed_iteration_us": round(1e3 * mg_ms / max(mg_info.iterations, 1), 1)}
 

@@ -321,10 +321,13 @@ int moons_clean_div_fft(moons_fft_t fft, moons_field_t X[3], moons_field_t Xstar
                         double scale, moons_field_t temp_F[3], moons_field_t temp_CC);
 
 /* Multigrid parameters of a MOONS_PREC_MG_FP32 solver (rebuilds the level
- * hierarchy): at most max_levels levels, nu1/nu2 red-black Gauss-Seidel sweeps
- * before/after the coarse correction, coarse_sweeps sweeps on the coarsest
- * level.  Defaults 16, 3, 3, 16.  No reference counterpart (see
- * MOONS_PREC_MG_FP32). */
+ * hierarchy): at most max_levels levels, nu1/nu2 smoothing sweeps before/after
+ * the coarse correction, coarse_sweeps sweeps on the coarsest level.  A sweep
+ * is red-black Gauss-Seidel, or on a level of more than 32^3 cells whose
+ * non-periodic y or z cell widths vary by more than 1.5x (lines of at most 128
+ * cells), zebra line Gauss-Seidel along those directions: y-lines of both
+ * colours, then z-lines (reversed after the correction).  Defaults 16, 2, 2,
+ * 16.  No reference counterpart (see MOONS_PREC_MG_FP32). */
 int moons_pcg_set_mg(moons_pcg_t pcg, int max_levels, int nu1, int nu2, int coarse_sweeps);
 /* Number of multigrid levels and the coarsest level's cells per direction. */
 int moons_pcg_mg_info(moons_pcg_t pcg, int *levels, int n_coarsest[3]);
@@ -347,8 +350,10 @@ int moons_pcg_time_kernels(moons_pcg_t pcg, int reps, double ms[3]);
  * residual, ms[2] prolongation into it, ms[3] the coarse end, ms[4] a whole
  * V-cycle, ms[5] one fused whole sweep (both colours, one launch) of the
  * finest level, ms[6]/ms[7] a half-sweep / fused sweep of level 1 and
- * ms[8]/ms[9] of level 2 (0 where the level does not exist).  Overwrites the
- * multigrid arrays.  No reference counterpart (profiling). */
+ * ms[8]/ms[9] of level 2 (0 where the level does not exist).  On a level
+ * smoothed by zebra lines the half-sweep entry is one line half-sweep (one
+ * colour of the first line direction) and the fused-sweep entry is 0.
+ * Overwrites the multigrid arrays.  No reference counterpart (profiling). */
 int moons_pcg_time_mg(moons_pcg_t pcg, int reps, double ms[10]);
 
 /* --------------------------------------------- vector (VF) Jacobi-PCG */

@@ -384,7 +384,7 @@ class PCG_solver_SF:
     def update_MFP(self, coeff_implicit_time_split: float, update_preconditioner: bool = True) -> None:
         check(load().moons_pcg_sf_update_mfp(self.handle, coeff_implicit_time_split, 1 if update_preconditioner else 0))
 
-    def set_MG(self, max_levels: int = 16, nu1: int = 3, nu2: int = 3, coarse_sweeps: int = 16) -> None:
+    def set_MG(self, max_levels: int = 16, nu1: int = 2, nu2: int = 2, coarse_sweeps: int = 16) -> None:
         check(load().moons_pcg_set_mg(self.handle, max_levels, nu1, nu2, coarse_sweeps))
 
     def mg_info(self):

@@ -30,6 +30,7 @@ struct LevelDev {
   int n0, n1, n2;
   int h0, half;                // cells per colour and row ((n0+1)/2), per colour (h0*n1*n2)
   int per0, per1, per2;
+  int line1, line2;            // smoothed by zebra lines along y / z (stretched, non-periodic directions)
   float *x, *f;
   float *xalt;                 // the other buffer of x for the fused sweeps (swapped with x on the host)
   const float *wm[3], *wp[3], *dg[3];
@@ -191,6 +192,120 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
     s += wmj * xjm + wpj * xjp;
     s += L.wm[2][k] * xkm + L.wp[2][k] * xkp;
     if (!skip) xc[c] = (fv - s) / d;
+  }
+}
+
+// Zebra line Gauss-Seidel along direction D (1: y, 2: z) on the lines of
+// colour c -- lines (i, k) with (i + k) % 2 == c for y-lines, (i, j) with
+// (i + j) % 2 == c for z-lines -- which couple only to lines of the other
+// colour.  Each line's tridiagonal system (the weights of D along the line,
+// diagonal dg_x + dg_y + dg_z, the off-line neighbours moved to the
+// right-hand side; with zero_guess they are zero) is solved exactly: one
+// odd-even reduction step, then parallel cyclic reduction.  A thread owns
+// the cells 2u and 2u + 1 of a line (NP >= the line length, a power of two;
+// NP / 2 threads per line, 2048 / NP lines per block).  It eliminates its
+// odd cell and its predecessor's from its even row, the even rows are
+// reduced by log2(NP / 2) PCR steps (rows (a, c, d) normalised to b = 1,
+// exchanged through two LDS buffers, one barrier per step), and the odd cell
+// follows from its row and the two solved even neighbours.  Cells past the
+// line's end hold the identity equation.  Threads of a block take
+// consecutive lines fastest, contiguous in the colour-split layout, so the
+// gather and the store are coalesced across lines.  A block of TB threads
+// holds 2 TB / NP lines; blocks are numbered so that each XCD (the hardware
+// deals consecutive block ids round-robin over the 8 XCDs) sweeps a
+// contiguous slab of planes q, whose neighbouring blocks share its L2: the
+// other colour's x read by the planes q - 1, q, q + 1 and both halves of a
+// 128-byte row are then fetched from HBM once.
+constexpr int kLineMax = 128; // longest line (at least 16 lines per block)
+constexpr long long kLineMinCells = 32LL * 32 * 32; // levels this small smooth by points
+
+// Row a x[t-1] + x[t] + c x[t+1] = d of cell t (0-based) of a line (normalised).
+template <int D>
+__device__ __forceinline__ float3 line_row(const LevelDev &L, int i, int m, int q, int t, int nl, bool ok,
+                                           bool zero_guess, size_t &at) {
+  const int j = D == 1 ? t + 1 : q, k = D == 1 ? q : t + 1;
+  const int h0 = L.h0, n1 = L.n1;
+  const int col = (i + j + k) & 1;
+  const int row = h0 * ((j - 1) + n1 * (k - 1));
+  at = (size_t)col * L.half + row + m;
+  if (!ok || t >= nl) return make_float3(0.f, 0.f, 0.f);
+  const float *xo = L.x + (size_t)(1 - col) * L.half;
+  const int im = (i == 1) ? (L.per0 ? L.n0 : 1) : i - 1;
+  const int ip = (i == L.n0) ? (L.per0 ? 1 : L.n0) : i + 1;
+  float s = L.wm[0][i] * xo[row + ((im - 1) >> 1)] + L.wp[0][i] * xo[row + ((ip - 1) >> 1)];
+  if (D == 1) { // off-line neighbours along z
+    const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1, kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
+    s += L.wm[2][k] * xo[h0 * ((j - 1) + n1 * (km - 1)) + m] + L.wp[2][k] * xo[h0 * ((j - 1) + n1 * (kp - 1)) + m];
+  } else {      // off-line neighbours along y
+    const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1, jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
+    s += L.wm[1][j] * xo[h0 * ((jm - 1) + n1 * (k - 1)) + m] + L.wp[1][j] * xo[h0 * ((jp - 1) + n1 * (k - 1)) + m];
+  }
+  const float d = L.f[at] - (zero_guess ? 0.f : s);
+  const float r = __builtin_amdgcn_rcpf(L.dg[0][i] + L.dg[1][j] + L.dg[2][k]);
+  return make_float3(L.wm[D][t + 1] * r, L.wp[D][t + 1] * r, d * r);
+}
+
+template <int D, int NP, int TB>
+__global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_guess, const int *done, int gx) {
+  constexpr int NT = NP / 2;             // threads per line
+  constexpr int LPB = TB / NT;           // lines per block
+  __shared__ float4 rows[2 * TB];        // two buffers of one row per thread
+  const int nl = D == 1 ? L.n1 : L.n2;   // line length, <= NP
+  // XCD-contiguous numbering: the first 8 * (nb / 8) ids are dealt as slabs
+  // of nb / 8 consecutive logical blocks per XCD, the rest keep their id
+  const int nb = (int)gridDim.x, per = nb >> 3, b = (int)blockIdx.x;
+  const int lb = b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+  const int bx = lb % gx;
+  const int q = lb / gx + 1;             // the other line index (k for y-lines, j for z-lines)
+  const int p = (c + 1 + q) & 1;         // i-parity of the lines of colour c at this q
+  const int l = threadIdx.x % LPB, u = threadIdx.x / LPB; // line in the block, cell pair
+  const int m = bx * LPB + l;
+  const int i = 2 * m + 1 + p;
+  const bool ok = i <= L.n0;
+  const int me = threadIdx.x;
+  size_t at0, at1;
+  const float3 e = line_row<D>(L, i, m, q, 2 * u, nl, ok, zero_guess, at0);     // even cell 2u
+  const float3 o = line_row<D>(L, i, m, q, 2 * u + 1, nl, ok, zero_guess, at1); // odd cell 2u+1
+  // odd-even reduction: the even row without x[2u-1] (the predecessor's odd row) and x[2u+1]
+  rows[me] = make_float4(o.x, o.y, o.z, 0.f);
+  __syncthreads();
+  const float4 op = u > 0 ? rows[me - LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float a, cc, d;
+  {
+    const float bn = 1.f - e.x * op.y - e.y * o.x;
+    const float r = __builtin_amdgcn_rcpf(bn);
+    a = -e.x * op.x * r;   // couples to x[2u-2]
+    cc = -e.y * o.y * r;   // couples to x[2u+2]
+    d = (e.z - e.x * op.z - e.y * o.z) * r;
+  }
+  int buf = 1;
+#pragma unroll
+  for (int st = 1; st < NT; st <<= 1) {
+    // ping-pong buffers: a buffer is rewritten two steps later, after the
+    // barrier that follows every read of it
+    float4 *S = rows + buf * TB;
+    S[me] = make_float4(a, cc, d, 0.f);
+    __syncthreads();
+    const float4 mm = u - st >= 0 ? S[me - st * LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 pp = u + st < NT ? S[me + st * LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float bn = 1.f - a * mm.y - cc * pp.x;
+    const float r = __builtin_amdgcn_rcpf(bn);
+    const float an = -a * mm.x * r, cn = -cc * pp.y * r;
+    d = (d - a * mm.z - cc * pp.z) * r;
+    a = an;
+    cc = cn;
+    buf ^= 1;
+  }
+  // d is x[2u]; the odd cell from its row and x[2u], x[2u+2]
+  float *xe = reinterpret_cast<float *>(rows + buf * TB);
+  xe[me] = d;
+  __syncthreads();
+  const float xnext = u + 1 < NT ? xe[me + LPB] : 0.f;
+  const float xodd = o.z - o.x * d - o.y * xnext;
+  const bool skip = done && *done; // stores skipped once the solver has exited
+  if (ok && !skip) {
+    if (2 * u < nl) L.x[at0] = d;
+    if (2 * u + 1 < nl) L.x[at1] = xodd;
   }
 }
 
@@ -766,6 +881,7 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     L.per0 = bct[0] == MOONS_BC_PERIODIC;
     L.per1 = bct[2] == MOONS_BC_PERIODIC;
     L.per2 = bct[4] == MOONS_BC_PERIODIC;
+    L.line1 = L.line2 = 0;
     const size_t ncell = (size_t)2 * ((L.n0 + 1) / 2) * L.n1 * L.n2;
     if (ncell >= ((size_t)1 << 31)) {
       if (err) *err = "multigrid: a level of 2^31 or more cells is not supported";
@@ -789,6 +905,21 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
         hc[i] = 0.5 * (hn[i + 1] + hn[i]);
       }
       for (int i = 0; i < s - 1; ++i) dhc[i] = hc[i + 1] - hc[i];
+      if (d > 0) {
+        // zebra lines along a stretched (cell widths varying by more than 1.5x),
+        // non-periodic y or z direction of a level of more than 32^3 cells
+        // (below that the point sweeps' single fused launches cost less and
+        // the iteration count does not change) whose lines fit one block
+        double dmin = dhn[1], dmax = dhn[1];
+        for (int i = 1; i <= n; ++i) {
+          dmin = std::min(dmin, dhn[i]);
+          dmax = std::max(dmax, dhn[i]);
+        }
+        const long long cells = (long long)hl.n[0] * hl.n[1] * hl.n[2];
+        const bool per = bct[2 * d] == MOONS_BC_PERIODIC;
+        const int ln = !per && n >= 4 && n <= kLineMax && cells > kLineMinCells && dmax > 1.5 * dmin;
+        (d == 1 ? L.line1 : L.line2) = ln;
+      }
       std::vector<float> wm(s, 0.f), wp(s, 0.f), dg(s, 0.f), rf(s, 0.f), pa(s, 0.f), pb(s, 0.f);
       std::vector<int> pI(s, 1), pI2(s, 1);
       const int bmin = bct[2 * d], bmax = bct[2 * d + 1];
@@ -887,6 +1018,7 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
     h->tail0 = l;
   }
   h->tail_lds = floats * sizeof(float);
+  for (int l = h->tail0; l < nl; ++l) h->L[l].line1 = h->L[l].line2 = 0; // the tail smooths by points
   {
     TailDesc &T = h->tail;
     T.nt = nl - h->tail0;
@@ -1058,8 +1190,52 @@ static void sweep2(LevelDev &L, int c_first, bool zero_first, const int *done) {
   std::swap(L.x, L.xalt);
 }
 
-// One red-black sweep of level L, colour c_first first.
+// Zebra line half-sweep of colour c along direction D of level L.
+template <int D, int NP, int TB>
+static void line_launch(const LevelDev &L, int c, int zg, const int *done, int o) {
+  const int lpb = TB / (NP / 2);
+  const int gx = (L.h0 + lpb - 1) / lpb;
+  hipLaunchKernelGGL((k_mg_line<D, NP, TB>), dim3((unsigned)(gx * o)), dim3(TB), 0, stream(), L, c, zg, done, gx);
+}
+
+template <int D>
+static void line_half_d(const LevelDev &L, int c, bool zero_guess, const int *done) {
+  const int nl = D == 1 ? L.n1 : L.n2, o = D == 1 ? L.n2 : L.n1;
+  const int zg = zero_guess ? 1 : 0;
+  if (nl <= 16) line_launch<D, 16, 256>(L, c, zg, done, o);
+  else if (nl <= 32) line_launch<D, 32, 256>(L, c, zg, done, o);
+  else if (nl <= 64) line_launch<D, 64, 256>(L, c, zg, done, o);
+  else line_launch<D, kLineMax, 1024>(L, c, zg, done, o);
+}
+
+static void line_half(const LevelDev &L, int D, int c, bool zero_guess, const int *done) {
+  if (D == 1) line_half_d<1>(L, c, zero_guess, done);
+  else line_half_d<2>(L, c, zero_guess, done);
+}
+
+// One line sweep of level L: pre-smoothing (forward) y-lines colour 0, 1,
+// then z-lines colour 0, 1; post-smoothing (backward) the reverse order, so
+// the V-cycle stays a symmetric preconditioner.
+static void line_sweep(LevelDev &L, bool forward, bool zero_first, const int *done) {
+  int ds[2], n = 0;
+  if (L.line1) ds[n++] = 1;
+  if (L.line2) ds[n++] = 2;
+  if (forward) {
+    for (int a = 0; a < n; ++a)
+      for (int c = 0; c < 2; ++c) line_half(L, ds[a], c, zero_first && a == 0 && c == 0, done);
+  } else {
+    for (int a = n - 1; a >= 0; --a)
+      for (int c = 1; c >= 0; --c) line_half(L, ds[a], c, false, done);
+  }
+}
+
+// One red-black sweep of level L, colour c_first first (lines, where the
+// level has line directions: c_first 0 is the forward order).
 static void sweep(LevelDev &L, int c_first, bool zero_first, const int *done) {
+  if (L.line1 || L.line2) {
+    line_sweep(L, c_first == 0, zero_first, done);
+    return;
+  }
   if ((long long)L.n0 * L.n1 * L.n2 <= kFusedSweepMaxCells) {
     sweep2(L, c_first, zero_first, done);
   } else {
@@ -1129,13 +1305,24 @@ int mg_time_kernels(MGHier *h, int reps, double ms[10]) {
     const int lw = w == 5 ? 0 : (w == 6 || w == 7) ? 1 : 2;
     const int g = h->tail0 < nl ? h->tail0 : nl - 1; // grid-launched levels
     if (w >= 5 && lw >= g) continue;
+    // levels smoothed by lines: their half-sweep entry times a line half-sweep
+    // (the first line direction), and they have no fused sweeps
+    auto lines = [&](int l) { return l < g && (h->L[l].line1 || h->L[l].line2); };
+    auto ldir = [&](int l) { return h->L[l].line1 ? 1 : 2; };
+    if ((w == 5 || w == 7 || w == 9) && lines(lw)) continue;
     auto launch = [&]() -> int {
       switch (w) {
-        case 0: smooth(h->L[0], 0, false, nullptr); return MOONS_OK;
+        case 0:
+          if (lines(0)) line_half(h->L[0], ldir(0), 0, false, nullptr);
+          else smooth(h->L[0], 0, false, nullptr);
+          return MOONS_OK;
         case 1: restrict_to(h->L[0], h->L[1], nullptr); return MOONS_OK;
         case 2: prolong_from(h->L[0], h->L[1], nullptr); return MOONS_OK;
         case 5: case 7: case 9: sweep2(h->L[lw], 0, false, nullptr); return MOONS_OK;
-        case 6: case 8: smooth(h->L[lw], 0, false, nullptr); return MOONS_OK;
+        case 6: case 8:
+          if (lines(lw)) line_half(h->L[lw], ldir(lw), 0, false, nullptr);
+          else smooth(h->L[lw], 0, false, nullptr);
+          return MOONS_OK;
         case 3:
           if (h->tail0 < nl)
             hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail,

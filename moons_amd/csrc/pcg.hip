@@ -578,7 +578,7 @@ struct moons_pcg_s {
   int mg_net_level = -1;
   double *xbak = nullptr;                     // x at entry of a learned solve (for the fallback)
   float *uf0 = nullptr, *uz0 = nullptr;       // its FP32 input r/vol and output z (padded)
-  int mg_max_levels = 16, mg_nu1 = 3, mg_nu2 = 3, mg_coarse = 16;
+  int mg_max_levels = 16, mg_nu1 = 2, mg_nu2 = 2, mg_coarse = 16;
   double *kf[3] = {nullptr, nullptr, nullptr}; // nrg_diffusion: the solver's copy of k (PCG%k)
   double cn = 0.0;                             // nrg_diffusion: coeff_implicit_time_split
   PcgCtrl *ctrl = nullptr;

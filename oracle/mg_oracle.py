@@ -6,8 +6,10 @@ residual, restrict, recurse, prolongate, correct, smooth), so this file
 restates the build's own V-cycle step for step, from the same mathematics:
 node-coarsened levels (code/handwritten/grid/coordinates_extend.f90:336-355),
 the Laplacian with homogeneous BCs folded into per-direction weights,
-red-black Gauss-Seidel, volume-weighted restriction of residuals, trilinear
-prolongation in true coordinates.  Pinned by: the level operator equals the
+red-black Gauss-Seidel -- or, on grid levels stretched along a non-periodic
+y or z direction, zebra line Gauss-Seidel along those directions --,
+volume-weighted restriction of residuals, trilinear prolongation in true
+coordinates.  Pinned by: the level operator equals the
 oracle's vol-weighted Laplacian divided by vol (assembled densely), the V-cycle
 is a convergent stationary iteration, and MG-preconditioned CG reaches the
 dense solution.
@@ -61,6 +63,15 @@ class Level:
             self.dg.append(dg)
             self.hc.append(hc)
             self.dhn.append(dhn)
+        # zebra lines along y / z where the cell widths vary by more than 1.5x,
+        # the direction is not periodic, and the level has more than 32^3 cells
+        cells = self.n[0] * self.n[1] * self.n[2]
+        self.line_dirs = []
+        for d in (1, 2):
+            w = self.dhn[d][1:self.n[d] + 1]
+            if (bct[2 * d] != PERIODIC and 4 <= self.n[d] <= LINE_MAX and cells > LINE_MIN_CELLS
+                    and w.max() > 1.5 * w.min()):
+                self.line_dirs.append(d)
 
     def shape(self):
         return tuple(n + 2 for n in self.n)
@@ -111,11 +122,68 @@ class Level:
         xi = self.interior(x)
         xi[m] = new[m]
 
+    def line_smooth(self, x, f, d, color, zero_guess):
+        """Zebra line Gauss-Seidel along d (1 or 2) on the lines of colour c:
+        (i + k) % 2 == c for y-lines, (i + j) % 2 == c for z-lines; each line's
+        tridiagonal system solved exactly (Thomas), the off-line neighbours on
+        the right-hand side (zero with zero_guess)."""
+        n = self.n
+        fi = self.interior(f)
+        s = 0.0
+        if not zero_guess:
+            for e in range(3):
+                if e != d:
+                    s = s + self._w(self.wm[e], e) * self._nb(x, e, -1) + self._w(self.wp[e], e) * self._nb(x, e, +1)
+        rhs = np.moveaxis(fi - s, d, 0)
+        dg = np.moveaxis(self.diag() * np.ones(tuple(n)), d, 0)
+        a, c = self.wm[d][1:n[d] + 1], self.wp[d][1:n[d] + 1]
+        m = n[d]
+        cp = np.zeros_like(rhs)
+        gp = np.zeros_like(rhs)
+        cprev = np.zeros(rhs.shape[1:])
+        gprev = np.zeros(rhs.shape[1:])
+        for t in range(m):
+            beta = dg[t] - a[t] * cprev
+            gp[t] = (rhs[t] - a[t] * gprev) / beta
+            cp[t] = c[t] / beta
+            cprev, gprev = cp[t], gp[t]
+        sol = np.zeros_like(rhs)
+        xn = np.zeros(rhs.shape[1:])
+        for t in range(m - 1, -1, -1):
+            xn = gp[t] - cp[t] * xn
+            sol[t] = xn
+        sol = np.moveaxis(sol, 0, d)
+        others = [e for e in range(3) if e != d]
+        idx = np.meshgrid(*[np.arange(1, k + 1) for k in n], indexing="ij")
+        mask = ((idx[others[0]] + idx[others[1]]) % 2) == color
+        xi = self.interior(x)
+        xi[mask] = sol[mask]
+
+    def sweep(self, x, f, forward, zero_first):
+        """One smoothing sweep: red-black points (forward: colour 0 first), or
+        where the level has line directions, y-lines colour 0, 1 then z-lines
+        colour 0, 1 (forward) or the reverse order (backward)."""
+        if not self.line_dirs:
+            c0 = 0 if forward else 1
+            self.smooth(x, f, c0, zero_first)
+            self.smooth(x, f, 1 - c0, False)
+            return
+        if forward:
+            for a, d in enumerate(self.line_dirs):
+                for c in (0, 1):
+                    self.line_smooth(x, f, d, c, zero_first and a == 0 and c == 0)
+        else:
+            for d in self.line_dirs[::-1]:
+                for c in (1, 0):
+                    self.line_smooth(x, f, d, c, False)
+
     def residual(self, x, f):
         return self.interior(f) - (self.offdiag(x) + self.diag() * self.interior(x))
 
 
 COARSE_CELLS = 4  # a level of at most this many cells per direction is the coarsest
+LINE_MIN_CELLS = 32 * 32 * 32  # levels of at most this many cells are smoothed by points
+LINE_MAX = 128  # longest line the device holds in LDS
 
 
 def build(hn3, bct, max_levels=16, coarse_cells=COARSE_CELLS):
@@ -179,7 +247,7 @@ def prolong(F: Level, C: Level, xc, bct):
     return e
 
 
-def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=16, x0=None):
+def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=16, x0=None):
     """x0 = V f0 (padded arrays; f0 = r/vol on the fine level).  With x0 given,
     the fine level's pre-smoothing starts from it (the learned initial guess)
     instead of from zero."""
@@ -198,8 +266,7 @@ def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=16, x0=None):
                 L.smooth(x, f, 1, False)
             return
         for s in range(nu1):
-            L.smooth(x, f, 0, s == 0 and not (l == 0 and x0 is not None))
-            L.smooth(x, f, 1, False)
+            L.sweep(x, f, True, s == 0 and not (l == 0 and x0 is not None))
         C = levels[l + 1]
         fc = np.zeros(C.shape())
         fc[1:-1, 1:-1, 1:-1] = restrict(L, C, L.residual(x, f))
@@ -207,8 +274,7 @@ def vcycle(levels, f0, bct, nu1=3, nu2=3, coarse_sweeps=16, x0=None):
         cyc(l + 1)
         L.interior(x)[...] += prolong(L, C, xs[l + 1], bct)
         for s in range(nu2):
-            L.smooth(x, f, 1, False)
-            L.smooth(x, f, 0, False)
+            L.sweep(x, f, False, False)
 
     cyc(0)
     return xs[0]

@@ -17,6 +17,10 @@ CASES = {
     "cube16_neumann": (16, None, [2] * 6),
     "duct24_roberts": (24, 1.05, [3, 3, 2, 2, 2, 2]),
     "dirichlet_mix20": (20, 1.2, [1, 2, 1, 1, 2, 1]),
+    # zebra y- and z-lines on the finest level (24^3 smooths by points)
+    "duct48_lines": (48, 1.05, [3, 3, 2, 2, 2, 2]),
+    # z periodic: y-lines only
+    "ylines48_zperiodic": (48, 1.1, [2, 1, 1, 2, 3, 3]),
 }
 
 
@@ -45,6 +49,7 @@ def test_vcycle_matches_restatement(gpu, case):
     P.apply_prec(R, Z)
     z = Z.download()
     levels = MGO.build(hs, bct)
+    print(case, "line directions per level", [L.line_dirs for L in levels], flush=True)
     vol = O.volume(g, 0)
     f = np.zeros(g.shape(0))
     f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
@@ -57,10 +62,11 @@ def test_vcycle_matches_restatement(gpu, case):
 
 @pytest.mark.parametrize("bct", [[3, 3, 2, 2, 2, 2], [2, 1, 3, 3, 2, 2]])
 def test_vcycle_half_sweep_levels_match_restatement(gpu, bct):
-    """A mesh whose finest level (80 x 64 x 64 cells, above the fused-sweep
-    size) is smoothed by the in-place half-sweeps and whose next level by the
-    fused whole sweeps: one V-cycle against the restatement, periodic along x
-    or along y."""
+    """A mesh whose finest level has 80 x 64 x 64 cells (above the fused-sweep
+    size): one V-cycle against the restatement.  Periodic along x with y and z
+    stretched, the two finest levels are smoothed by zebra y- and z-lines;
+    periodic along z with y uniform, the finest level by the in-place
+    red-black half-sweeps and the next by the fused whole sweeps."""
     print(f"vcycle 80x64x64 {bct}", flush=True)
     hx = O.uniform_nodes(0.0, 1.25, 80)
     hy = O.roberts_both_nodes(0.0, 1.0, 64, 1.05) if bct[2] != 3 else O.uniform_nodes(0.0, 1.0, 64)
@@ -120,6 +126,7 @@ def test_mixed_pcg_128_reaches_1e8_true_residual(gpu, beta):
     X = gpu.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
     P = mg_solver(gpu, m, X, tol=1e-8)
     info = P.solve(X, gpu.SF(m, "CC").upload(b))
+    print("128^3 beta", beta, "mixed PCG iterations", info.iterations, flush=True)
     assert info.res_norm / info.res_norm0 < 1e-8
     assert info.iterations < 60
     x = X.download()
