@@ -200,22 +200,24 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
 // (i + j) % 2 == c for z-lines -- which couple only to lines of the other
 // colour.  Each line's tridiagonal system (the weights of D along the line,
 // diagonal dg_x + dg_y + dg_z, the off-line neighbours moved to the
-// right-hand side; with zero_guess they are zero) is solved exactly: one
-// odd-even reduction step, then parallel cyclic reduction.  A thread owns
-// the cells 2u and 2u + 1 of a line (NP >= the line length, a power of two;
-// NP / 2 threads per line, 2048 / NP lines per block).  It eliminates its
-// odd cell and its predecessor's from its even row, the even rows are
-// reduced by log2(NP / 2) PCR steps (rows (a, c, d) normalised to b = 1,
-// exchanged through two LDS buffers, one barrier per step), and the odd cell
-// follows from its row and the two solved even neighbours.  Cells past the
-// line's end hold the identity equation.  Threads of a block take
-// consecutive lines fastest, contiguous in the colour-split layout, so the
-// gather and the store are coalesced across lines.  A block of TB threads
-// holds 2 TB / NP lines; blocks are numbered so that each XCD (the hardware
-// deals consecutive block ids round-robin over the 8 XCDs) sweeps a
-// contiguous slab of planes q, whose neighbouring blocks share its L2: the
-// other colour's x read by the planes q - 1, q, q + 1 and both halves of a
-// 128-byte row are then fetched from HBM once.
+// right-hand side; with zero_guess they are zero) is solved exactly.  A
+// thread owns the cells 4u .. 4u + 3 of a line (NP >= the line length, a
+// power of two; NP / 4 threads per line) with their rows normalised to a unit
+// diagonal.  Two odd-even reduction levels leave one row per thread, coupling
+// x[4u] to x[4u - 4] and x[4u + 4]: row 4u + 2 drops x[4u + 1] and x[4u + 3]
+// (its own cells), then row 4u drops x[4u - 1] and x[4u - 2] with the
+// predecessor's row 4u - 1 and reduced row 4u - 2 (one LDS exchange).  Those
+// rows are solved by log2(NP / 4) parallel cyclic reduction steps (rows
+// (a, c, d) in two struct-of-arrays LDS buffers, one barrier per step), and
+// x[4u + 2], x[4u + 1], x[4u + 3] follow from their rows with the
+// successor's x[4u + 4].  Cells past the line's end hold the identity
+// equation.  Threads of a block take consecutive lines fastest, contiguous in
+// the colour-split layout, so the gather and the store are coalesced across
+// lines.  A block of TB threads holds 4 TB / NP lines; blocks are numbered
+// so that each XCD (the hardware deals consecutive block ids round-robin over
+// the 8 XCDs) sweeps a contiguous slab of planes q, whose neighbouring blocks
+// share its L2: the other colour's x read by the planes q - 1, q, q + 1 and
+// both halves of a 128-byte row are then fetched from HBM once.
 constexpr int kLineMax = 128; // longest line (at least 16 lines per block)
 constexpr long long kLineMinCells = 32LL * 32 * 32; // levels this small smooth by points
 
@@ -247,9 +249,10 @@ __device__ __forceinline__ float3 line_row(const LevelDev &L, int i, int m, int 
 
 template <int D, int NP, int TB>
 __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_guess, const int *done, int gx) {
-  constexpr int NT = NP / 2;             // threads per line
-  constexpr int LPB = TB / NT;           // lines per block
-  __shared__ float4 rows[2 * TB];        // two buffers of one row per thread
+  constexpr int NQ = NP / 4;             // threads per line
+  constexpr int LPB = TB / NQ;           // lines per block
+  __shared__ float E[6][TB];             // rows 4u + 3 and reduced 4u + 2, read by the successor
+  __shared__ float S[2][3][TB];          // PCR rows (a, c, d), two buffers
   const int nl = D == 1 ? L.n1 : L.n2;   // line length, <= NP
   // XCD-contiguous numbering: the first 8 * (nb / 8) ids are dealt as slabs
   // of nb / 8 consecutive logical blocks per XCD, the rest keep their id
@@ -258,54 +261,95 @@ __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_gues
   const int bx = lb % gx;
   const int q = lb / gx + 1;             // the other line index (k for y-lines, j for z-lines)
   const int p = (c + 1 + q) & 1;         // i-parity of the lines of colour c at this q
-  const int l = threadIdx.x % LPB, u = threadIdx.x / LPB; // line in the block, cell pair
+  const int l = threadIdx.x % LPB, u = threadIdx.x / LPB; // line in the block, cell quad
   const int m = bx * LPB + l;
   const int i = 2 * m + 1 + p;
   const bool ok = i <= L.n0;
   const int me = threadIdx.x;
-  size_t at0, at1;
-  const float3 e = line_row<D>(L, i, m, q, 2 * u, nl, ok, zero_guess, at0);     // even cell 2u
-  const float3 o = line_row<D>(L, i, m, q, 2 * u + 1, nl, ok, zero_guess, at1); // odd cell 2u+1
-  // odd-even reduction: the even row without x[2u-1] (the predecessor's odd row) and x[2u+1]
-  rows[me] = make_float4(o.x, o.y, o.z, 0.f);
+  size_t at[4];
+  float3 r[4];
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4) r[s4] = line_row<D>(L, i, m, q, 4 * u + s4, nl, ok, zero_guess, at[s4]);
+  // row 4u + 2 without x[4u + 1], x[4u + 3]: A2 x[4u] + x[4u + 2] + C2 x[4u + 4] = D2
+  float A2, C2, D2;
+  {
+    const float rr = __builtin_amdgcn_rcpf(1.f - r[2].x * r[1].y - r[2].y * r[3].x);
+    A2 = -r[2].x * r[1].x * rr;
+    C2 = -r[2].y * r[3].y * rr;
+    D2 = (r[2].z - r[2].x * r[1].z - r[2].y * r[3].z) * rr;
+  }
+  E[0][me] = r[3].x;
+  E[1][me] = r[3].y;
+  E[2][me] = r[3].z;
+  E[3][me] = A2;
+  E[4][me] = C2;
+  E[5][me] = D2;
   __syncthreads();
-  const float4 op = u > 0 ? rows[me - LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float a3p = 0.f, c3p = 0.f, d3p = 0.f, a2p = 0.f, c2p = 0.f, d2p = 0.f; // the predecessor's
+  if (u > 0) {
+    const int o = me - LPB;
+    a3p = E[0][o];
+    c3p = E[1][o];
+    d3p = E[2][o];
+    a2p = E[3][o];
+    c2p = E[4][o];
+    d2p = E[5][o];
+  }
   float a, cc, d;
   {
-    const float bn = 1.f - e.x * op.y - e.y * o.x;
-    const float r = __builtin_amdgcn_rcpf(bn);
-    a = -e.x * op.x * r;   // couples to x[2u-2]
-    cc = -e.y * o.y * r;   // couples to x[2u+2]
-    d = (e.z - e.x * op.z - e.y * o.z) * r;
+    // row 4u without x[4u - 1], x[4u + 1]: couples to x[4u - 2], x[4u + 2]
+    const float rr = __builtin_amdgcn_rcpf(1.f - r[0].x * c3p - r[0].y * r[1].x);
+    const float A0 = -r[0].x * a3p * rr, C0 = -r[0].y * r[1].y * rr;
+    const float D0 = (r[0].z - r[0].x * d3p - r[0].y * r[1].z) * rr;
+    // then without x[4u - 2], x[4u + 2]: couples to x[4u - 4], x[4u + 4]
+    const float r2 = __builtin_amdgcn_rcpf(1.f - A0 * c2p - C0 * A2);
+    a = -A0 * a2p * r2;
+    cc = -C0 * C2 * r2;
+    d = (D0 - A0 * d2p - C0 * D2) * r2;
   }
-  int buf = 1;
+  int buf = 0;
 #pragma unroll
-  for (int st = 1; st < NT; st <<= 1) {
+  for (int st = 1; st < NQ; st <<= 1) {
     // ping-pong buffers: a buffer is rewritten two steps later, after the
     // barrier that follows every read of it
-    float4 *S = rows + buf * TB;
-    S[me] = make_float4(a, cc, d, 0.f);
+    S[buf][0][me] = a;
+    S[buf][1][me] = cc;
+    S[buf][2][me] = d;
     __syncthreads();
-    const float4 mm = u - st >= 0 ? S[me - st * LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 pp = u + st < NT ? S[me + st * LPB] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float bn = 1.f - a * mm.y - cc * pp.x;
-    const float r = __builtin_amdgcn_rcpf(bn);
-    const float an = -a * mm.x * r, cn = -cc * pp.y * r;
-    d = (d - a * mm.z - cc * pp.z) * r;
+    float am = 0.f, cm = 0.f, dm = 0.f, ap = 0.f, cp = 0.f, dp = 0.f;
+    if (u - st >= 0) {
+      const int o = me - st * LPB;
+      am = S[buf][0][o];
+      cm = S[buf][1][o];
+      dm = S[buf][2][o];
+    }
+    if (u + st < NQ) {
+      const int o = me + st * LPB;
+      ap = S[buf][0][o];
+      cp = S[buf][1][o];
+      dp = S[buf][2][o];
+    }
+    const float rr = __builtin_amdgcn_rcpf(1.f - a * cm - cc * ap);
+    const float an = -a * am * rr, cn = -cc * cp * rr;
+    d = (d - a * dm - cc * dp) * rr;
     a = an;
     cc = cn;
     buf ^= 1;
   }
-  // d is x[2u]; the odd cell from its row and x[2u], x[2u+2]
-  float *xe = reinterpret_cast<float *>(rows + buf * TB);
-  xe[me] = d;
+  // d is x[4u]; x[4u + 4] from the successor, then the eliminated cells
+  S[buf][0][me] = d;
   __syncthreads();
-  const float xnext = u + 1 < NT ? xe[me + LPB] : 0.f;
-  const float xodd = o.z - o.x * d - o.y * xnext;
+  const float x4 = u + 1 < NQ ? S[buf][0][me + LPB] : 0.f;
+  const float x2 = D2 - A2 * d - C2 * x4;
+  const float x1 = r[1].z - r[1].x * d - r[1].y * x2;
+  const float x3 = r[3].z - r[3].x * x2 - r[3].y * x4;
   const bool skip = done && *done; // stores skipped once the solver has exited
   if (ok && !skip) {
-    if (2 * u < nl) L.x[at0] = d;
-    if (2 * u + 1 < nl) L.x[at1] = xodd;
+    const int t0 = 4 * u;
+    if (t0 < nl) L.x[at[0]] = d;
+    if (t0 + 1 < nl) L.x[at[1]] = x1;
+    if (t0 + 2 < nl) L.x[at[2]] = x2;
+    if (t0 + 3 < nl) L.x[at[3]] = x3;
   }
 }
 
@@ -1193,7 +1237,7 @@ static void sweep2(LevelDev &L, int c_first, bool zero_first, const int *done) {
 // Zebra line half-sweep of colour c along direction D of level L.
 template <int D, int NP, int TB>
 static void line_launch(const LevelDev &L, int c, int zg, const int *done, int o) {
-  const int lpb = TB / (NP / 2);
+  const int lpb = TB / (NP / 4);
   const int gx = (L.h0 + lpb - 1) / lpb;
   hipLaunchKernelGGL((k_mg_line<D, NP, TB>), dim3((unsigned)(gx * o)), dim3(TB), 0, stream(), L, c, zg, done, gx);
 }
@@ -1202,10 +1246,10 @@ template <int D>
 static void line_half_d(const LevelDev &L, int c, bool zero_guess, const int *done) {
   const int nl = D == 1 ? L.n1 : L.n2, o = D == 1 ? L.n2 : L.n1;
   const int zg = zero_guess ? 1 : 0;
-  if (nl <= 16) line_launch<D, 16, 256>(L, c, zg, done, o);
-  else if (nl <= 32) line_launch<D, 32, 256>(L, c, zg, done, o);
-  else if (nl <= 64) line_launch<D, 64, 256>(L, c, zg, done, o);
-  else line_launch<D, kLineMax, 1024>(L, c, zg, done, o);
+  if (nl <= 16) line_launch<D, 16, 64>(L, c, zg, done, o);
+  else if (nl <= 32) line_launch<D, 32, 128>(L, c, zg, done, o);
+  else if (nl <= 64) line_launch<D, 64, 128>(L, c, zg, done, o);
+  else line_launch<D, kLineMax, 512>(L, c, zg, done, o);
 }
 
 static void line_half(const LevelDev &L, int D, int c, bool zero_guess, const int *done) {
