@@ -121,6 +121,82 @@ __global__ void __launch_bounds__(kUT) k_conv_mfma(const bf16 *__restrict__ X, i
     }
 }
 
+// The same convolution with its 27 taps split into groups, for layers whose
+// few voxels give too few waves to hide the load latency of the tap loop:
+// blockIdx.y = group g takes taps [g T, min(27, g T + T)) and writes its FP32
+// partial sums P[(g nvox + v) Cout + co]; k_conv_reduce adds the groups in
+// order, then the bias and the activation.
+template <int NB>
+__global__ void __launch_bounds__(kUT) k_conv_mfma_split(const bf16 *__restrict__ X, int Cin,
+                                                        const s16x4 *__restrict__ WB, float *__restrict__ P, int n0,
+                                                        int n1, int n2, int T, const int *done) {
+  if (gated(done)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nvox = n0 * n1 * n2;
+  const int base = (blockIdx.x * (kUT / 64) + wave) * 32;
+  if (base >= nvox) return;
+  const int g = blockIdx.y;
+  const int t0 = g * T, t1 = min(27, t0 + T);
+  const int s0 = n0 + 2, s1 = n1 + 2;
+  int vb[2];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    const int v = base + mt * 16 + (lane & 15);
+    vb[mt] = pvox(v < nvox ? v : 0, n0, n1);
+  }
+  const int kq = (lane >> 4) * 4;
+  f32x4 acc[2][NB];
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int KB = Cin >> 4;
+  for (int tap = t0; tap < t1; ++tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int off = (dz * s1 + dy) * s0 + dx;
+    const bf16 *x0 = X + (size_t)(vb[0] + off) * Cin + kq;
+    const bf16 *x1 = X + (size_t)(vb[1] + off) * Cin + kq;
+    const s16x4 *wt = WB + (size_t)tap * KB * NB * 64 + lane;
+    for (int kb = 0; kb < KB; ++kb) {
+      const s16x4 a0 = *reinterpret_cast<const s16x4 *>(x0 + kb * 16);
+      const s16x4 a1 = *reinterpret_cast<const s16x4 *>(x1 + kb * 16);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const s16x4 b = wt[(kb * NB + nb) * 64];
+        acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b, acc[0][nb], 0, 0, 0);
+        acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, acc[1][nb], 0, 0, 0);
+      }
+    }
+  }
+  const int Cout = NB * 16;
+  float *Pg = P + (size_t)g * nvox * Cout;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int v = base + mt * 16 + (lane >> 4) * 4 + j;
+      if (v >= nvox) continue;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) Pg[(size_t)v * Cout + nb * 16 + (lane & 15)] = acc[mt][nb][j];
+    }
+}
+
+__global__ void __launch_bounds__(kUT) k_conv_reduce(const float *__restrict__ P, int S, int nvox, int Cout,
+                                                    const float *__restrict__ bias, int relu, bf16 *__restrict__ Y,
+                                                    int n0, int n1, const int *done) {
+  if (gated(done)) return;
+  const size_t n = (size_t)nvox * Cout;
+  const size_t t = (size_t)blockIdx.x * kUT + threadIdx.x;
+  if (t >= n) return;
+  float o = 0.f;
+#pragma unroll 9
+  for (int g = 0; g < S; ++g) o += P[(size_t)g * n + t];
+  const int co = (int)(t % Cout), v = (int)(t / Cout);
+  if (bias) o += bias[co];
+  if (relu) o = fmaxf(o, 0.f);
+  Y[(size_t)pvox(v, n0, n1) * Cout + co] = f2bf(o);
+}
+
 // dW partials: part[chunk][tap][ci][co] = sum over the chunk's voxels of
 // X[v + tap][ci] * dY[v][co] (MFMA with M = ci, N = co, K = voxels).
 template <int NB>
@@ -196,27 +272,32 @@ __global__ void __launch_bounds__(kUT) k_dbias(const bf16 *dY, int C, int n0, in
 __global__ void __launch_bounds__(kUT) k_conv_in(const bf16 *__restrict__ X, const float *__restrict__ W,
                                                 const float *__restrict__ bias, bf16 *__restrict__ Y, int n0, int n1,
                                                 int n2, const int *done) {
+  // a thread per voxel and group of 4 output channels (the group fastest, so
+  // the 8-byte stores of a wave are contiguous)
   if (gated(done)) return;
-  const int v = blockIdx.x * kUT + threadIdx.x;
+  const int t = blockIdx.x * kUT + threadIdx.x;
+  const int v = t >> 2, cg = (t & 3) * 4;
   if (v >= n0 * n1 * n2) return;
   const int s0 = n0 + 2, s1 = n1 + 2;
   const int pv = pvox(v, n0, n1);
-  float acc[16];
+  float acc[4];
 #pragma unroll
-  for (int co = 0; co < 16; ++co) acc[co] = bias[co];
+  for (int c = 0; c < 4; ++c) acc[c] = bias[cg + c];
   for (int tap = 0; tap < 27; ++tap) {
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
     const bf16 *x = X + (size_t)(pv + (dz * s1 + dy) * s0 + dx) * 4;
 #pragma unroll
     for (int ci = 0; ci < 4; ++ci) {
       const float xv = bf2f(x[ci]);
-      const float *w = W + (tap * 4 + ci) * 16;
+      const float *w = W + (tap * 4 + ci) * 16 + cg;
 #pragma unroll
-      for (int co = 0; co < 16; ++co) acc[co] += xv * w[co];
+      for (int c = 0; c < 4; ++c) acc[c] += xv * w[c];
     }
   }
+  bf16 o[4];
 #pragma unroll
-  for (int co = 0; co < 16; ++co) Y[(size_t)pv * 16 + co] = f2bf(fmaxf(acc[co], 0.f));
+  for (int c = 0; c < 4; ++c) o[c] = f2bf(fmaxf(acc[c], 0.f));
+  *reinterpret_cast<uint2 *>(Y + (size_t)pv * 16 + cg) = *reinterpret_cast<const uint2 *>(o);
 }
 
 // last layer: out[pv] = scale * (b + sum_tap,ci W[tap][ci] X[v+tap][ci]), Cin = 16, FP32 padded output
@@ -652,10 +733,42 @@ int pack_weights(moons_unet_s *U) {
   return MOONS_OK;
 }
 
-void conv_mfma(const s16x4 *WB, const float *bias, const Buf &X, Buf &Y, int n0, int n1, int n2, int relu,
-               const int *done) {
+// Waves below which a convolution splits its taps (about two per SIMD).
+constexpr int kSplitWaves = 512;
+
+template <int NB>
+static void conv_split(const moons_unet_s *U, const s16x4 *WB, const float *bias, const Buf &X, Buf &Y, int n0, int n1,
+                       int n2, int relu, const int *done, int S, int T) {
+  const int nvox = n0 * n1 * n2;
+  const dim3 grid((unsigned)((nvox + 127) / 128), (unsigned)S);
+  hipLaunchKernelGGL(k_conv_mfma_split<NB>, grid, dim3(kUT), 0, stream(), X.d, X.C, WB, U->part, n0, n1, n2, T, done);
+  hipLaunchKernelGGL(k_conv_reduce, dim3(ublk((size_t)nvox * Y.C)), dim3(kUT), 0, stream(), U->part, S, nvox, Y.C, bias,
+                     relu, Y.d, n0, n1, done);
+}
+
+void conv_mfma(const moons_unet_s *U, const s16x4 *WB, const float *bias, const Buf &X, Buf &Y, int n0, int n1, int n2,
+               int relu, const int *done) {
   const int nvox = n0 * n1 * n2;
   const unsigned grid = (unsigned)((nvox + 127) / 128);
+  const int waves = (nvox + 31) / 32;
+  if (waves < kSplitWaves) {
+    // groups of T taps so that the launch has about kSplitWaves * 4 waves,
+    // within the partial-sum buffer
+    int S = std::min(27, (4 * kSplitWaves + waves - 1) / waves);
+    while (S > 1 && (size_t)S * nvox * Y.C > U->part_n) --S;
+    const int T = (27 + S - 1) / S;
+    S = (27 + T - 1) / T;
+    if (S > 1) {
+      switch (Y.C / 16) {
+        case 1: conv_split<1>(U, WB, bias, X, Y, n0, n1, n2, relu, done, S, T); break;
+        case 2: conv_split<2>(U, WB, bias, X, Y, n0, n1, n2, relu, done, S, T); break;
+        case 3: conv_split<3>(U, WB, bias, X, Y, n0, n1, n2, relu, done, S, T); break;
+        case 4: conv_split<4>(U, WB, bias, X, Y, n0, n1, n2, relu, done, S, T); break;
+        default: conv_split<6>(U, WB, bias, X, Y, n0, n1, n2, relu, done, S, T); break;
+      }
+      return;
+    }
+  }
   switch (Y.C / 16) {
     case 1: hipLaunchKernelGGL(k_conv_mfma<1>, dim3(grid), dim3(kUT), 0, stream(), X.d, X.C, WB, bias, Y.d, n0, n1, n2, relu, done); break;
     case 2: hipLaunchKernelGGL(k_conv_mfma<2>, dim3(grid), dim3(kUT), 0, stream(), X.d, X.C, WB, bias, Y.d, n0, n1, n2, relu, done); break;
@@ -670,21 +783,21 @@ int forward(moons_unet_s *U, float *out, const float *scale, const int *done) {
   const int n0 = U->n[0], n1 = U->n[1], n2 = U->n[2];
   const float *W = U->Wr;
   auto nv = [&](int lev) { return (size_t)(n0 >> lev) * (n1 >> lev) * (n2 >> lev); };
-  hipLaunchKernelGGL(k_conv_in, dim3(ublk(nv(0))), dim3(kUT), 0, stream(), U->in.d, W + U->L[0].w_off,
+  hipLaunchKernelGGL(k_conv_in, dim3(ublk(4 * nv(0))), dim3(kUT), 0, stream(), U->in.d, W + U->L[0].w_off,
                      U->W + U->L[0].b_off, U->a0.d, n0, n1, n2, done);
-  conv_mfma(U->WB[1], U->W + U->L[1].b_off, U->a0, U->e1, n0, n1, n2, 1, done);
+  conv_mfma(U, U->WB[1], U->W + U->L[1].b_off, U->a0, U->e1, n0, n1, n2, 1, done);
   hipLaunchKernelGGL(k_pool, dim3(ublk(nv(1) * 16)), dim3(kUT), 0, stream(), U->e1.d, U->p1.d, 16, n0 / 2, n1 / 2, n2 / 2, done);
-  conv_mfma(U->WB[2], U->W + U->L[2].b_off, U->p1, U->a2, n0 / 2, n1 / 2, n2 / 2, 1, done);
-  conv_mfma(U->WB[3], U->W + U->L[3].b_off, U->a2, U->e2, n0 / 2, n1 / 2, n2 / 2, 1, done);
+  conv_mfma(U, U->WB[2], U->W + U->L[2].b_off, U->p1, U->a2, n0 / 2, n1 / 2, n2 / 2, 1, done);
+  conv_mfma(U, U->WB[3], U->W + U->L[3].b_off, U->a2, U->e2, n0 / 2, n1 / 2, n2 / 2, 1, done);
   hipLaunchKernelGGL(k_pool, dim3(ublk(nv(2) * 32)), dim3(kUT), 0, stream(), U->e2.d, U->p2.d, 32, n0 / 4, n1 / 4, n2 / 4, done);
-  conv_mfma(U->WB[4], U->W + U->L[4].b_off, U->p2, U->a4, n0 / 4, n1 / 4, n2 / 4, 1, done);
-  conv_mfma(U->WB[5], U->W + U->L[5].b_off, U->a4, U->b, n0 / 4, n1 / 4, n2 / 4, 1, done);
+  conv_mfma(U, U->WB[4], U->W + U->L[4].b_off, U->p2, U->a4, n0 / 4, n1 / 4, n2 / 4, 1, done);
+  conv_mfma(U, U->WB[5], U->W + U->L[5].b_off, U->a4, U->b, n0 / 4, n1 / 4, n2 / 4, 1, done);
   hipLaunchKernelGGL(k_upcat, dim3(ublk(nv(1) * 96)), dim3(kUT), 0, stream(), U->b.d, 64, U->e2.d, 32, U->c2.d, n0 / 2, n1 / 2, n2 / 2, done);
-  conv_mfma(U->WB[6], U->W + U->L[6].b_off, U->c2, U->a6, n0 / 2, n1 / 2, n2 / 2, 1, done);
-  conv_mfma(U->WB[7], U->W + U->L[7].b_off, U->a6, U->d2, n0 / 2, n1 / 2, n2 / 2, 1, done);
+  conv_mfma(U, U->WB[6], U->W + U->L[6].b_off, U->c2, U->a6, n0 / 2, n1 / 2, n2 / 2, 1, done);
+  conv_mfma(U, U->WB[7], U->W + U->L[7].b_off, U->a6, U->d2, n0 / 2, n1 / 2, n2 / 2, 1, done);
   hipLaunchKernelGGL(k_upcat, dim3(ublk(nv(0) * 48)), dim3(kUT), 0, stream(), U->d2.d, 32, U->e1.d, 16, U->c1.d, n0, n1, n2, done);
-  conv_mfma(U->WB[8], U->W + U->L[8].b_off, U->c1, U->a8, n0, n1, n2, 1, done);
-  conv_mfma(U->WB[9], U->W + U->L[9].b_off, U->a8, U->d1, n0, n1, n2, 1, done);
+  conv_mfma(U, U->WB[8], U->W + U->L[8].b_off, U->c1, U->a8, n0, n1, n2, 1, done);
+  conv_mfma(U, U->WB[9], U->W + U->L[9].b_off, U->a8, U->d1, n0, n1, n2, 1, done);
   hipLaunchKernelGGL(k_conv_out, dim3(ublk(nv(0))), dim3(kUT), 0, stream(), U->d1.d, W + U->L[10].w_off,
                      U->W + U->L[10].b_off, out, n0, n1, n2, scale, done);
   MOONS_LAUNCH_CHECK();
@@ -734,7 +847,7 @@ int backward(moons_unet_s *U, int acc) {
   };
   auto dgrad = [&](int l, const Buf &dY, Buf &dX, int a0, int a1, int a2) {
     // dX = conv(dY; W flipped/transposed): Cin' = Cout, Cout' = Cin
-    conv_mfma(U->WT[l], nullptr, dY, dX, a0, a1, a2, 0, nullptr);
+    conv_mfma(U, U->WT[l], nullptr, dY, dX, a0, a1, a2, 0, nullptr);
   };
   relu_bwd(U->g_d1, U->d1, 0);
   grad_mfma(U, 9, U->a8, U->g_d1, n0, n1, n2, acc);
