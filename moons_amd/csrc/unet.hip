@@ -733,7 +733,8 @@ int pack_weights(moons_unet_s *U) {
   return MOONS_OK;
 }
 
-// Waves below which a convolution splits its taps (about two per SIMD).
+// Waves below which a convolution splits its taps (about two per SIMD; the
+// 32^3 layers, 1024 waves, measured no faster split).
 constexpr int kSplitWaves = 512;
 
 template <int NB>
