@@ -93,6 +93,46 @@ def test_vcycle_half_sweep_levels_match_restatement(gpu, bct):
     assert err < 2e-5, err
 
 
+RAGGED = {
+    # y-lines of 100 cells in 128-cell slots, z-lines of 36 in 64; z Dirichlet at one wall
+    "x40_y100_z36": ((40, 100, 36), (1.05, 1.1), [3, 3, 2, 2, 1, 2]),
+    # y- and z-lines of 24 cells (32-cell slots)
+    "x64_y24_z24": ((64, 24, 24), (1.02, 1.05), [2, 2, 2, 1, 2, 2]),
+    # y-lines of 40 (64-cell slots), z-lines of 12 (16-cell slots)
+    "x128_y40_z12": ((128, 40, 12), (1.05, 1.02), [3, 3, 1, 2, 2, 2]),
+}
+
+
+@pytest.mark.parametrize("case", list(RAGGED))
+def test_vcycle_ragged_lines_match_restatement(gpu, case):
+    """Zebra lines whose length is not the slot size: every line-kernel
+    instantiation (16, 32, 64 and 128-cell slots), padded rows, and y- and
+    z-lines of different lengths on one level, against the restatement."""
+    (nx, ny, nz), (by, bz), bct = RAGGED[case]
+    print(f"vcycle ragged lines {case}", flush=True)
+    hs = [O.uniform_nodes(0.0, 1.0, nx), O.roberts_both_nodes(0.0, 1.0, ny, by),
+          O.roberts_both_nodes(0.0, 1.0, nz, bz)]
+    g, m = O.Grid(*hs), gpu.mesh(*hs)
+    levels = MGO.build(hs, bct)
+    assert levels[0].line_dirs == [1, 2]
+    x = gpu.SF(m, "CC").set_BCs([NAMES[t] for t in bct], None, all_Neumann=(1 not in bct))
+    P = mg_solver(gpu, m, x)
+    rng = np.random.default_rng(11)
+    r = np.zeros(g.shape(0), order="F")
+    r[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(n - 2 for n in g.shape(0)))
+    Z = gpu.SF(m, "CC")
+    P.apply_prec(gpu.SF(m, "CC").upload(r), Z)
+    z = Z.download()
+    vol = O.volume(g, 0)
+    f = np.zeros(g.shape(0))
+    f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
+    zr = MGO.vcycle(levels, f, bct)
+    zi, zri = z[1:-1, 1:-1, 1:-1], zr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(zi - zri) / np.linalg.norm(zri)
+    assert err < 2e-5, err
+    assert P.mg_info()[0] == len(levels)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_mixed_pcg_matches_fp64_oracle(gpu, case):
     N, beta, bct = CASES[case]
