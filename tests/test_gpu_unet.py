@@ -34,7 +34,9 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (8, 12, 4)])
+# 8^3 and 8x12x4 run every layer through the tap-split convolution; at
+# 32x32x16 the first level (16,384 voxels) runs the whole-tap kernel
+@pytest.mark.parametrize("shape", [(8, 8, 8), (8, 12, 4), (32, 32, 16)])
 def test_forward_matches_restatement(gpu, shape):
     print(f"unet forward {shape}", flush=True)
     net = gpu.UNet(seed=3)
@@ -44,9 +46,9 @@ def test_forward_matches_restatement(gpu, shape):
     assert rel(out, ref) < 1e-2, rel(out, ref)
 
 
-def test_gradients_match_restatement(gpu):
-    print("unet gradients", flush=True)
-    shape = (8, 8, 8)
+@pytest.mark.parametrize("shape", [(8, 8, 8), (32, 32, 16)])
+def test_gradients_match_restatement(gpu, shape):
+    print(f"unet gradients {shape}", flush=True)
     net = gpu.UNet(seed=5)
     x = sample_input(shape, 2)
     t = np.zeros(tuple(s + 2 for s in shape), dtype=np.float32)
@@ -55,15 +57,24 @@ def test_gradients_match_restatement(gpu):
     ref = UO.Net(net.get_weights(), bf=True)
     rloss, rg, _ = ref.loss_grad(UO.to_internal(x), np.transpose(t[1:-1, 1:-1, 1:-1], (2, 1, 0)).astype(np.float64))
     assert abs(loss - rloss) <= 1e-3 * rloss
+    # The two differ by FP32-vs-float64 sums, whose BF16 roundings then flip
+    # now and then, and the flips (and the ReLU masks they move) cascade down
+    # the backward pass.  The restatement's own gradients move by 2-9% per
+    # layer between bf16=True and float64 at both sizes, so the per-layer bound
+    # is 3e-2 at 8^3 and 6e-2 at 32x32x16 (over 30x more voxels, more flips).
+    bound = 3e-2 if np.prod(shape) <= 512 else 6e-2
     off = 0
+    errs = []
     for l, (ci, co) in enumerate(UO.LAYERS):
         nw = 27 * ci * co
         gw, rw = g[off:off + nw], rg[off:off + nw]
         gb, rb = g[off + nw:off + nw + co], rg[off + nw:off + nw + co]
         off += nw + co
-        assert rel(gw, rw) < 3e-2, (l, rel(gw, rw))
-        assert rel(gb, rb) < 3e-2, (l, rel(gb, rb))
+        errs.append((l, round(rel(gw, rw), 4), round(rel(gb, rb), 4)))
     cos = float(g @ rg / (np.linalg.norm(g) * np.linalg.norm(rg)))
+    print("per-layer relative errors (layer, W, b):", errs, "cosine:", cos, flush=True)
+    for l, ew, eb in errs:
+        assert ew < bound and eb < bound, (l, ew, eb)
     assert cos > 0.999, cos
 
 
