@@ -279,8 +279,8 @@ def main():
                          "(no early-exited launches; for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding the solve over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import moons_amd as M
