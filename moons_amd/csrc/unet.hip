@@ -752,10 +752,13 @@ void conv_mfma(const moons_unet_s *U, const s16x4 *WB, const float *bias, const 
   const int nvox = n0 * n1 * n2;
   const unsigned grid = (unsigned)((nvox + 127) / 128);
   const int waves = (nvox + 31) / 32;
-  if (waves < kSplitWaves) {
-    // groups of T taps so that the launch has about kSplitWaves * 4 waves,
-    // within the partial-sum buffer
-    int S = std::min(27, (4 * kSplitWaves + waves - 1) / waves);
+  // a wide input (48 channels or more: 81+ MFMA steps a wave) splits up to
+  // four times as many waves
+  const int target = X.C >= 48 ? 16 * kSplitWaves : 4 * kSplitWaves;
+  if (waves < kSplitWaves || (X.C >= 48 && waves < 4 * kSplitWaves)) {
+    // groups of T taps so that the launch has about target waves, within the
+    // partial-sum buffer
+    int S = std::min(27, (target + waves - 1) / waves);
     while (S > 1 && (size_t)S * nvox * Y.C > U->part_n) --S;
     const int T = (27 + S - 1) / S;
     S = (27 + T - 1) / T;
