@@ -59,6 +59,12 @@ extern "C" {
 #define MOONS_BC_PERIODIC 3
 #define MOONS_BC_SYMMETRIC 4
 #define MOONS_BC_ANTISYMMETRIC 5
+/* u + c_w du/dh = 0 with the face's c_w (moons_field_set_bc_props): the
+ * thin-wall conducting BC, apply_Robin_C/N
+ * (code/handwritten/BCs/direct/apply_BCs_faces_raw.f90:236-282) and
+ * apply_Robin_C/N_implicit
+ * (code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90:213-257). */
+#define MOONS_BC_ROBIN 6
 
 /* ----------------------------------------------------------- operators */
 #define MOONS_OP_LAP_UNIFORM_SF 1 /* code/handwritten/solvers/matrix_free_operators.f90:35-62 */
@@ -159,6 +165,11 @@ int moons_field_shape(moons_field_t f, int s[3]);
  * uniform value; all_neumann is the reference's SF%all_Neumann flag, which the
  * caller sets (code/handwritten/solvers/momentum/init_P_BCs.f90:26). */
 int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6], int all_neumann);
+/* init_BC_props (code/handwritten/block_field/block_field_extend.f90:395-400,
+ * code/handwritten/boundary_conditions/boundary_conditions_extend.f90:504-511):
+ * the wall conductance ratio c_w of each face, which Robin faces use, and the
+ * Robin coefficients kept beside it (robin_coeff may be NULL). */
+int moons_field_set_bc_props(moons_field_t f, const double c_w[6], const double robin_coeff[6]);
 /* Host <-> device copies of the whole ghost-padded array (n = element count). */
 int moons_field_upload(moons_field_t f, const double *host, size_t n);
 int moons_field_download(moons_field_t f, double *host, size_t n);
@@ -384,13 +395,21 @@ int moons_pcg_vf_solve(moons_pcgvf_t pcg, moons_field_t x[3], moons_field_t b[3]
  *   alpha_U = 0: Euler_time_no_diff_AB2_sources (method 4), alpha_U = 1:
  *     Euler_time_AB2_sources (method 6) with A = I - dt/Re lap
  *     (solvers/time_marching_methods.f90:117-150,266-296);
- *   induction (when induction != 0): low-Rem (Btot = B0), CT source
- *     curl(U_E x Btot), backward-Euler curl-curl diffusion with sigma = 1,
+ *   induction (when induction != 0): CT source curl(U_E x Btot),
+ *     backward-Euler curl-curl diffusion with sigma = 1,
  *     phi cleaning; J = curl(B)/Rem, J x B scaled by N = Ha^2/Re
  *     (induction_extend.f90:475-536, induction_sources.f90:28-84).
  * Solver settings index 0..3 = U, p, B, phi (iter_max, tol_rel, tol_abs,
  * n_skip_check_res); prec_p selects the PPE preconditioner
- * (MOONS_PREC_LAP_SF or MOONS_PREC_MG_FP32). */
+ * (MOONS_PREC_LAP_SF or MOONS_PREC_MG_FP32).
+ * mpg_dir 1..3 (0: off) drives the flow by a mean pressure gradient: before
+ * the sources, every value of U's mpg_dir component gains mpg_scale*dt, then
+ * U's BCs are applied (compute_add_MPG,
+ * solvers/momentum/momentum_sources.f90:97-108, called at
+ * add_all_momentum_sources.f90:31-33).
+ * finite_Rem != 0: Btot = B0 + B after each induction solve, else Btot = B0
+ * (compute_Btot, governing_equations/induction_extend.f90:527-536); Btot
+ * enters the CT source and J x B. */
 typedef struct {
   double Re, Ha, Rem, dt;
   double alpha_U;
@@ -401,6 +420,9 @@ typedef struct {
   double tol_rel[4];
   double tol_abs[4];
   int n_skip[4];
+  int mpg_dir;
+  double mpg_scale;
+  int finite_Rem;
 } moons_mhd_params;
 
 typedef struct {

@@ -31,7 +31,8 @@ class MhdParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("Ha", C.c_double), ("Rem", C.c_double), ("dt", C.c_double),
                 ("alpha_U", C.c_double), ("induction", C.c_int), ("prec_p", C.c_int), ("B0", C.c_double * 3),
                 ("iter_max", C.c_int * 4), ("tol_rel", C.c_double * 4), ("tol_abs", C.c_double * 4),
-                ("n_skip", C.c_int * 4)]
+                ("n_skip", C.c_int * 4), ("mpg_dir", C.c_int), ("mpg_scale", C.c_double),
+                ("finite_Rem", C.c_int)]
 
 
 class MhdInfo(C.Structure):
@@ -63,6 +64,7 @@ SIGNATURES = {
     "moons_field_destroy": (C.c_int, [C.c_void_p]),
     "moons_field_shape": (C.c_int, [C.c_void_p, c_int_p]),
     "moons_field_set_bcs": (C.c_int, [C.c_void_p, c_int_p, c_double_p, C.c_int]),
+    "moons_field_set_bc_props": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "moons_field_upload": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
     "moons_field_download": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
     "moons_field_fill": (C.c_int, [C.c_void_p, C.c_double]),

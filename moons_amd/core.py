@@ -19,7 +19,7 @@ from . import _lib
 from ._lib import MhdInfo, MhdParams, NrgInfo, NrgParams, PcgInfo, check, load
 
 DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
-BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
+BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5, "Robin": 6}
 
 _initialised = False
 
@@ -112,10 +112,21 @@ class SF:
     def size(self) -> int:
         return int(np.prod(self.shape))
 
-    def set_BCs(self, types, vals=None, all_Neumann: bool = False) -> "SF":
+    def set_BCs(self, types, vals=None, all_Neumann: bool = False, c_w=None) -> "SF":
+        """Per-face BC types and values; c_w: the wall conductance ratio of
+        each face, which Robin faces use (init_BC_props)."""
         t = (C.c_int * 6)(*[BC[x] if isinstance(x, str) else int(x) for x in types])
         v = (C.c_double * 6)(*(vals if vals is not None else [0.0] * 6))
         check(load().moons_field_set_bcs(self.handle, t, v, 1 if all_Neumann else 0))
+        if c_w is not None:
+            self.set_BC_props(c_w)
+        return self
+
+    def set_BC_props(self, c_w, Robin_coeff=None) -> "SF":
+        """init_BC_props: c_w per face, and the Robin coefficients kept beside it."""
+        cw = (C.c_double * 6)(*[float(x) for x in c_w])
+        rc = (C.c_double * 6)(*[float(x) for x in (Robin_coeff if Robin_coeff is not None else [0.0] * 6)])
+        check(load().moons_field_set_bc_props(self.handle, cw, rc))
         return self
 
     def upload(self, a: np.ndarray) -> "SF":
@@ -529,16 +540,21 @@ class MHD_step:
     Euler), pressure projection; with B, low-Rem induction with the CT source,
     implicit diffusion, phi cleaning, J and J x B.  U, p, B, phi are the
     caller's fields (with their BCs) and are advanced in place.
-    ISPs are (iter_max, tol_rel, tol_abs, n_skip_check_res)."""
+    ISPs are (iter_max, tol_rel, tol_abs, n_skip_check_res); mpg_dir 1..3
+    drives the flow by a mean pressure gradient of mpg_scale."""
 
     FIELDS = {"F": 1, "Fnm1": 2, "Ustar": 3, "U_E": 4, "J": 5, "jCrossB": 6, "FB": 7, "ppe_rhs": 8}
 
     def __init__(self, m: mesh, U, p: SF, B=None, phi: SF | None = None, Re=400.0, Ha=0.0, Rem=1.0, dt=1e-3,
                  alpha_U=1.0, B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
                  isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), prec_p="Lap_SF", m_ind: mesh | None = None,
-                 domain: "mesh_domain | None" = None, sigma_inv=None):
+                 domain: "mesh_domain | None" = None, sigma_inv=None, mpg_dir: int = 0, mpg_scale: float = 1.0,
+                 finite_Rem: bool = False):
         P = MhdParams()
         P.Re, P.Ha, P.Rem, P.dt, P.alpha_U = Re, Ha, Rem, dt, alpha_U
+        # mean pressure gradient forcing along mpg_dir (1..3, 0 off): compute_add_MPG
+        P.mpg_dir, P.mpg_scale = int(mpg_dir), float(mpg_scale)
+        P.finite_Rem = 1 if finite_Rem else 0  # Btot = B0 + B (else B0)
         P.induction = 1 if B is not None else 0
         P.prec_p = PCG_solver_SF.PRECS[prec_p]
         for d in range(3):

@@ -24,6 +24,7 @@ struct CCGeomT {
   int bct[6];
   double bval[6];
   double dh_nhat[6];
+  double robc[6]; // Robin ghost factor of a cell-centred field (direct and implicit)
 };
 
 using Coef1 = Coef1T<double>;
@@ -51,15 +52,17 @@ __device__ __forceinline__ T lap7(T c, T xm, T xp, T ym, T yp, T zm, T zp, const
 // Face-ghost value of a cell-centred field (never an edge or corner ghost).
 // implicit_: homogeneous rules; else the BC value rule.  vI: first interior
 // value behind the ghost; vIo: first interior value behind the opposite face
-// (periodic); vmem: the value stored in the ghost (faces without BCs).
-__device__ __forceinline__ double ghost_cc(int type, bool implicit_, double bval, double dh_nhat, double vI,
-                                           double vIo, double vmem) {
+// (periodic); vmem: the value stored in the ghost (faces without BCs); robc:
+// the face's Robin factor (the direct rule's value term has factor 0).
+__device__ __forceinline__ double ghost_cc(int type, bool implicit_, double bval, double dh_nhat, double robc,
+                                           double vI, double vIo, double vmem) {
   switch (type) {
     case MOONS_BC_DIRICHLET: return implicit_ ? -vI : 2.0 * bval - vI;
     case MOONS_BC_NEUMANN: return implicit_ ? vI : vI + bval * dh_nhat;
     case MOONS_BC_PERIODIC: return vIo;
     case MOONS_BC_SYMMETRIC: return vI;
     case MOONS_BC_ANTISYMMETRIC: return -vI;
+    case MOONS_BC_ROBIN: return implicit_ ? vI * robc : vI * robc + bval * 0.0;
     default: return vmem;
   }
 }
@@ -105,6 +108,7 @@ __device__ __forceinline__ void fill_ghosts_implicit(T *p, const G_ &G, int i, i
     if (t == MOONS_BC_NONE) return;
     if (t == MOONS_BC_PERIODIC) p[at_opp] = v;
     else if (t == MOONS_BC_DIRICHLET || t == MOONS_BC_ANTISYMMETRIC) p[at] = -v;
+    else if (t == MOONS_BC_ROBIN) p[at] = v * (T)G.robc[face];
     else p[at] = v;
   };
   const size_t base = (size_t)j * sx + (size_t)k * sxy;

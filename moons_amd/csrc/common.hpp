@@ -68,6 +68,11 @@ struct BCDev {
   int type[6];
   double val[6];
   double dh_nhat[6]; // dh at the boundary times the outward normal sign
+  // Robin faces (u + c_w du/dh = 0), by the field's location along the face
+  // normal: direct rule ug = ui*rob1 + bval*rob2 (cell-centred) or
+  // ug = ui + ub*rob1 + bval*rob2 (node-centred); the implicit rule has robi
+  // in place of rob1 and no value term.
+  double rob1[6], rob2[6], robi[6];
 };
 
 // ------------------------------------------------------ reductions
@@ -147,7 +152,22 @@ struct moons_field_s {
   bool all_neumann = false;
   int bct[6] = {0, 0, 0, 0, 0, 0};
   double bval[6] = {0, 0, 0, 0, 0, 0};
+  double c_w[6] = {0, 0, 0, 0, 0, 0};         // wall conductance ratio per face (Robin faces)
+  double robin_coeff[6] = {0, 0, 0, 0, 0, 0}; // kept with the BC properties
 };
+
+// Copies the BCs of src (types, values, all_Neumann and the BC properties
+// c_w and Robin coefficients) to dst, a field of the same location.
+inline void copy_bcs_of(moons_field_s *dst, const moons_field_s *src) {
+  dst->has_bcs = src->has_bcs;
+  dst->all_neumann = src->all_neumann;
+  for (int f = 0; f < 6; ++f) {
+    dst->bct[f] = src->bct[f];
+    dst->bval[f] = src->bval[f];
+    dst->c_w[f] = src->c_w[f];
+    dst->robin_coeff[f] = src->robin_coeff[f];
+  }
+}
 
 // Where an interior mesh lies in a total mesh: per direction the ranges of
 // coinciding cell centres (C) and nodes (N), 0-based inclusive:

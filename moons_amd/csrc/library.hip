@@ -155,6 +155,13 @@ void coords_init(HostCoords &c, const double *hn, int sn) {
   }
 }
 
+// dh of a face's sub-domain (FSD%dh, code/handwritten/sub_domain/face_SD_extend.f90:63):
+// the spacing between the ghost and boundary node at a min face, the last
+// interior cell at a max face (get_dh_boundary over the face grid get_face_GI
+// keeps, code/handwritten/sub_domain/sub_domain_extend.f90:125-134,
+// code/handwritten/grid/grid_extend.f90:260-268).
+static double face_dh(const HostCoords &c, int is_max) { return is_max ? c.dhn[c.sn - 3] : c.dhn[0]; }
+
 BCDev make_bcdev(const moons_field_s *f) {
   BCDev b;
   for (int face = 0; face < 6; ++face) {
@@ -162,9 +169,24 @@ BCDev make_bcdev(const moons_field_s *f) {
     b.val[face] = f->bval[face];
     int d = face / 2;
     const HostCoords &c = f->m->hc[d];
-    double dh = (face % 2) ? c.dhn[c.sn - 2] : c.dhn[0];
-    double nhat = (face % 2) ? 1.0 : -1.0;
+    const double dh = face_dh(c, face % 2);
+    const double nhat = (face % 2) ? 1.0 : -1.0;
     b.dh_nhat[face] = dh * nhat;
+    // Robin coefficients in the reference's operation order: apply_Robin_C/N
+    // (code/handwritten/BCs/direct/apply_BCs_faces_raw.f90:236-282, whose bvals
+    // coefficient of the C rule is overwritten by 0) and the implicit twins
+    // (code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90:213-257).
+    const double cw = f->c_w[face];
+    if ((f->dl >> d) & 1) {
+      b.rob1[face] = -(2.0 * dh / cw * nhat);
+      b.rob2[face] = 2.0 * dh / cw * nhat;
+      b.robi[face] = -(2.0 * dh * nhat / cw);
+    } else {
+      const double c1 = (2.0 * cw * nhat / dh - 1.0) / (2.0 * cw * nhat / dh + 1.0);
+      b.rob1[face] = c1;
+      b.rob2[face] = 0.0;
+      b.robi[face] = c1;
+    }
   }
   return b;
 }
@@ -188,12 +210,13 @@ struct BcMulti {
   Shape3 S[3];
   int nalong[3];
   int type[3][2];
-  double val[3][2], dh[3][2];
+  double val[3][2], dh[3][2], r1[3][2], r2[3][2];
 };
 
 // and code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90.
+// r1, r2: the face's Robin coefficients for the rule applied (implicit: robi, 0).
 __device__ __forceinline__ void apply_bc_point(double *u, const Shape3 &S, int face, int type, int nalong, double bval,
-                                               double dh_nhat, int implicit_, int ja, int jb) {
+                                               double dh_nhat, double r1, double r2, int implicit_, int ja, int jb) {
   const int d = face / 2, is_max = face % 2;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
   const int s = S.s[d];
@@ -232,19 +255,34 @@ __device__ __forceinline__ void apply_bc_point(double *u, const Shape3 &S, int f
     } break;
     case MOONS_BC_SYMMETRIC: u[ig] = u[iin]; break;
     case MOONS_BC_ANTISYMMETRIC: u[ig] = -u[iin]; break;
+    case MOONS_BC_ROBIN:
+      if (!nalong) {
+        u[ig] = implicit_ ? u[iin] * r1 : u[iin] * r1 + bval * r2;
+      } else {
+        ii[d] = B; const size_t ib = idx3(S.s, ii[0], ii[1], ii[2]);
+        u[ig] = implicit_ ? u[iin] + u[ib] * r1 : u[iin] + u[ib] * r1 + bval * r2;
+      }
+      break;
     default: break;
   }
 }
 
+// Robin coefficients of a face for the direct or implicit rule.
+static inline void robin_pair(const BCDev &b, int face, bool implicit_, double &r1, double &r2) {
+  r1 = implicit_ ? b.robi[face] : b.rob1[face];
+  r2 = implicit_ ? 0.0 : b.rob2[face];
+}
+
 __global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int face, int type, int nalong,
-                                                       double bval, double dh_nhat, int implicit_, const int *gate) {
+                                                       double bval, double dh_nhat, double r1, double r2,
+                                                       int implicit_, const int *gate) {
   if (gate && *gate) return;
   const int d = face / 2;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
   const int na = S.s[a], nb = S.s[b];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= na * nb) return;
-  apply_bc_point(u, S, face, type, nalong, bval, dh_nhat, implicit_, t % na, t / na);
+  apply_bc_point(u, S, face, type, nalong, bval, dh_nhat, r1, r2, implicit_, t % na, t / na);
 }
 
 // Both faces of direction d in one launch: the first na*nb threads the min
@@ -253,6 +291,7 @@ __global__ void __launch_bounds__(256) k_apply_bc_face(double *u, Shape3 S, int 
 // so the result equals the min-then-max sequence.
 __global__ void __launch_bounds__(256) k_apply_bc_pair(double *u, Shape3 S, int d, int type0, int type1, int nalong,
                                                        double bval0, double bval1, double dh0, double dh1,
+                                                       double r10, double r11, double r20, double r21,
                                                        int implicit_, const int *gate) {
   if (gate && *gate) return;
   const int a = (d + 1) % 3, b = (d + 2) % 3;
@@ -262,8 +301,8 @@ __global__ void __launch_bounds__(256) k_apply_bc_pair(double *u, Shape3 S, int 
   if (t >= 2 * np) return;
   const int side = t >= np ? 1 : 0;
   const int q = t - side * np;
-  apply_bc_point(u, S, 2 * d + side, side ? type1 : type0, nalong, side ? bval1 : bval0, side ? dh1 : dh0, implicit_,
-                 q % na, q / na);
+  apply_bc_point(u, S, 2 * d + side, side ? type1 : type0, nalong, side ? bval1 : bval0, side ? dh1 : dh0,
+                 side ? r11 : r10, side ? r21 : r20, implicit_, q % na, q / na);
 }
 
 // Assigns val to one whole plane idx along direction d.
@@ -500,17 +539,21 @@ int launch_apply_bcs(moons_field_s *u, bool implicit_) {
     const size_t np = (size_t)u->s[a] * u->s[bb];
     const int nalong = (u->dl >> d) & 1;
     const int f0 = 2 * d, f1 = 2 * d + 1;
+    double r1[2], r2[2];
+    robin_pair(b, f0, implicit_, r1[0], r2[0]);
+    robin_pair(b, f1, implicit_, r1[1], r2[1]);
     if (b.type[f0] != MOONS_BC_NONE && b.type[f1] != MOONS_BC_NONE && u->s[d] >= 6) {
       hipLaunchKernelGGL(k_apply_bc_pair, dim3(nblk(2 * np)), dim3(256), 0, g_stream, u->d, S, d, b.type[f0],
-                         b.type[f1], nalong, b.val[f0], b.val[f1], b.dh_nhat[f0], b.dh_nhat[f1], implicit_ ? 1 : 0,
-                         g_gate);
+                         b.type[f1], nalong, b.val[f0], b.val[f1], b.dh_nhat[f0], b.dh_nhat[f1], r1[0], r1[1], r2[0],
+                         r2[1], implicit_ ? 1 : 0, g_gate);
       MOONS_LAUNCH_CHECK();
       continue;
     }
     for (int face : {f0, f1}) {
       if (b.type[face] == MOONS_BC_NONE) continue;
       hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, u->d, S, face, b.type[face],
-                         nalong, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
+                         nalong, b.val[face], b.dh_nhat[face], r1[face - f0], r2[face - f0], implicit_ ? 1 : 0,
+                         g_gate);
       MOONS_LAUNCH_CHECK();
     }
   }
@@ -532,8 +575,8 @@ __global__ void __launch_bounds__(256) k_apply_bc_multi(BcMulti M, int d, int im
   if (t >= 2 * np) return;
   const int side = t >= np ? 1 : 0;
   const int q = t - side * np;
-  apply_bc_point(M.u[f], S, 2 * d + side, M.type[f][side], M.nalong[f], M.val[f][side], M.dh[f][side], implicit_,
-                 q % na, q / na);
+  apply_bc_point(M.u[f], S, 2 * d + side, M.type[f][side], M.nalong[f], M.val[f][side], M.dh[f][side],
+                 M.r1[f][side], M.r2[f][side], implicit_, q % na, q / na);
 }
 
 int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_) {
@@ -556,14 +599,17 @@ int launch_apply_bcs3(moons_field_s *const u[3], bool implicit_) {
           M.type[f][side] = b.type[2 * d + side];
           M.val[f][side] = b.val[2 * d + side];
           M.dh[f][side] = b.dh_nhat[2 * d + side];
+          robin_pair(b, 2 * d + side, implicit_, M.r1[f][side], M.r2[f][side]);
         }
         npmax = std::max(npmax, np);
       } else {
         Shape3 S{{x->s[0], x->s[1], x->s[2]}};
         for (int face : {f0, f1}) {
           if (b.type[face] == MOONS_BC_NONE) continue;
+          double r1, r2;
+          robin_pair(b, face, implicit_, r1, r2);
           hipLaunchKernelGGL(k_apply_bc_face, dim3(nblk(np)), dim3(256), 0, g_stream, x->d, S, face, b.type[face],
-                             (x->dl >> d) & 1, b.val[face], b.dh_nhat[face], implicit_ ? 1 : 0, g_gate);
+                             (x->dl >> d) & 1, b.val[face], b.dh_nhat[face], r1, r2, implicit_ ? 1 : 0, g_gate);
           MOONS_LAUNCH_CHECK();
         }
       }
@@ -860,7 +906,7 @@ int moons_field_shape(moons_field_t f, int s[3]) {
 int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6], int all_neumann) {
   if (!f || !type) return fail(MOONS_ERR_ARG, "moons_field_set_bcs: null argument");
   for (int face = 0; face < 6; ++face) {
-    if (type[face] < MOONS_BC_NONE || type[face] > MOONS_BC_ANTISYMMETRIC)
+    if (type[face] < MOONS_BC_NONE || type[face] > MOONS_BC_ROBIN)
       return fail(MOONS_ERR_ARG, "moons_field_set_bcs: unknown BC type");
   }
   for (int d = 0; d < 3; ++d) {
@@ -872,6 +918,15 @@ int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6],
   for (int face = 0; face < 6; ++face) {
     f->bct[face] = type[face];
     f->bval[face] = val ? val[face] : 0.0;
+  }
+  return MOONS_OK;
+}
+
+int moons_field_set_bc_props(moons_field_t f, const double c_w[6], const double robin_coeff[6]) {
+  if (!f || !c_w) return fail(MOONS_ERR_ARG, "moons_field_set_bc_props: null argument");
+  for (int face = 0; face < 6; ++face) {
+    f->c_w[face] = c_w[face];
+    f->robin_coeff[face] = robin_coeff ? robin_coeff[face] : 0.0;
   }
   return MOONS_OK;
 }

@@ -867,6 +867,12 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
   }
   int bct[6];
   for (int f = 0; f < 6; ++f) bct[f] = x->has_bcs ? x->bct[f] : MOONS_BC_NEUMANN;
+  for (int f = 0; f < 6; ++f)
+    if (bct[f] == MOONS_BC_ROBIN) {
+      // the Robin ghost factor depends on the boundary cell width of each level
+      if (err) *err = "multigrid: Robin BCs are not supported (use the FP64 Jacobi-PCG)";
+      return nullptr;
+    }
   std::vector<HostLevel> H;
   {
     HostLevel l0;

@@ -135,8 +135,9 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
           const int ty = G.bct[f];
           const double vmem = x[off[f]];
           const double vio = (ty == MOONS_BC_PERIODIC) ? x[opp[f]] : 0.0;
-          nb[f] = ghost_cc(ty, true, 0.0, 0.0, c, vio, vmem);
-          nbc[f] = (ty == MOONS_BC_NONE) ? 0.0 : ghost_cc(ty, false, G.bval[f], G.dh_nhat[f], 0.0, 0.0, 0.0);
+          nb[f] = ghost_cc(ty, true, 0.0, 0.0, G.robc[f], c, vio, vmem);
+          nbc[f] = (ty == MOONS_BC_NONE) ? 0.0
+                                         : ghost_cc(ty, false, G.bval[f], G.dh_nhat[f], G.robc[f], 0.0, 0.0, 0.0);
         }
       }
       const double axbc =
@@ -540,6 +541,7 @@ CCGeom make_geom(const moons_field_s *x, int kz) {
     G.bct[f] = b.type[f];
     G.bval[f] = b.val[f];
     G.dh_nhat[f] = b.dh_nhat[f];
+    G.robc[f] = b.robi[f];
   }
   return G;
 }

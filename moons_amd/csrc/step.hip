@@ -36,6 +36,12 @@ __global__ void __launch_bounds__(kST) k_axpy(double *y, const double *x, double
   if (t < n) y[t] = y[t] + x[t] * s;
 }
 
+// y = y + s over every value, ghosts included (add of a scalar to a grid field)
+__global__ void __launch_bounds__(kST) k_adds(double *y, double s, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t < n) y[t] = y[t] + s;
+}
+
 // y = a*b
 __global__ void __launch_bounds__(kST) k_mul3(double *y, const double *a, const double *b, size_t n) {
   const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
@@ -84,6 +90,7 @@ struct moons_mhd_s {
   moons_field_s *Ustar[3] = {}, *F[3] = {}, *Fnm1[3] = {}, *L[3] = {}, *T1[3] = {}, *T2[3] = {}, *T3[3] = {};
   moons_field_s *UE[6] = {}, *cc = nullptr, *tcc = nullptr;
   moons_field_s *Bstar[3] = {}, *FB[3] = {}, *FBnm1[3] = {}, *LB[3] = {}, *B0[3] = {}, *J[3] = {}, *JxB[3] = {};
+  moons_field_s *Btot[3] = {}; // B0 (low Rem) or a field of its own, B0 + B (finite Rem)
   moons_field_s *sig[3] = {}, *E[3] = {}, *ETF[6] = {}, *A6[6] = {}, *B6[6] = {};
   // induction-mesh temporaries: U_E embedded (multi-domain only; else UE),
   // face temporaries (multi-domain only; else T1, T2) and the phi right-hand side
@@ -106,14 +113,10 @@ const int kNoDiag[6][2] = {{1, 0}, {2, 0}, {0, 1}, {2, 1}, {0, 2}, {1, 2}};
 
 int fcreate(moons_mesh_s *m, int dl, moons_field_s **f) { return moons_field_create(m, dl, f); }
 
-void copy_bcs_of(moons_field_s *dst, const moons_field_s *src) {
-  dst->has_bcs = src->has_bcs;
-  dst->all_neumann = src->all_neumann;
-  std::memcpy(dst->bct, src->bct, sizeof dst->bct);
-  std::memcpy(dst->bval, src->bval, sizeof dst->bval);
-}
-
 void mhd_free(moons_mhd_s *S) {
+  if (S->P.finite_Rem)
+    for (int d = 0; d < 3; ++d)
+      if (S->Btot[d]) moons_field_destroy(S->Btot[d]);
   moons_field_s **groups3[] = {S->Ustar, S->F, S->Fnm1, S->L, S->T1, S->T2, S->T3, S->Bstar, S->FB,
                                S->FBnm1, S->LB, S->B0, S->J, S->JxB, S->sig, S->E};
   for (auto g : groups3)
@@ -219,17 +222,22 @@ int ab2_rhs(moons_field_s *out[3], moons_field_s *F[3], moons_field_s *Fnm1[3], 
   return MOONS_OK;
 }
 
-// J = curl(B)/Rem; Btot = B0; JxB = N edge2Face(J) x face2Face(Btot)
+// J = curl(B)/Rem; Btot = B0 (+ B at finite Rem); JxB = N edge2Face(J) x face2Face(Btot)
 int post_induction(moons_mhd_s *S) {
   int st;
   for (int d = 0; d < 3; ++d) {
     if ((st = moons_curl(S->J[d], S->B[0], S->B[1], S->B[2], d))) return st;
     if ((st = launch_scale(S->J[d], 1.0 / S->P.Rem))) return st;
   }
+  if (S->P.finite_Rem)
+    for (int d = 0; d < 3; ++d) {
+      if ((st = moons_field_copy(S->Btot[d], S->B0[d]))) return st;
+      if ((st = axpy(S->Btot[d], S->B[d], 1.0))) return st; // B0 + B*1 = B0 + B exactly
+    }
   for (int q = 0; q < 6; ++q) {
     const int i = kNoDiag[q][0], j = kNoDiag[q][1];
     if ((st = launch_interp(S->A6[q], S->J[i], 3 - i - j))) return st;  // edge2Face
-    if ((st = face2face(S->cci, S->B6[q], S->B0[i], i, j))) return st;     // face2Face(Btot)
+    if ((st = face2face(S->cci, S->B6[q], S->Btot[i], i, j))) return st;   // face2Face(Btot)
   }
   // (J x B)_x = J_y Bz - J_z By on x-faces; y negated form; z = J_x By - J_y Bx
   if ((st = launch_cross(S->JxB[0], S->A6[0], S->B6[1], S->A6[1], S->B6[0], 1))) return st;
@@ -244,6 +252,14 @@ int post_induction(moons_mhd_s *S) {
 int momentum(moons_mhd_s *S) {
   int st;
   moons_pcg_info info;
+  if (S->P.mpg_dir) {
+    // compute_add_MPG (solvers/momentum/momentum_sources.f90:97-108): the
+    // component gains scale*dt everywhere, then apply_BCs(U)
+    moons_field_s *u = S->U[S->P.mpg_dir - 1];
+    hipLaunchKernelGGL(k_adds, dim3(sblk(u->n)), dim3(kST), 0, stream(), u->d, S->P.mpg_scale * S->P.dt, u->n);
+    MOONS_LAUNCH_CHECK();
+    if ((st = launch_apply_bcs3(S->U, false))) return st;
+  }
   for (int d = 0; d < 3; ++d) {
     if ((st = moons_field_copy(S->Fnm1[d], S->F[d]))) return st;
     if ((st = moons_field_fill(S->L[d], 0.0))) return st;
@@ -297,7 +313,7 @@ int induction(moons_mhd_s *S) {
   if (S->dom) // embedVelocity_E: the fluid's edge velocity into the induction mesh
     for (int q = 0; q < 6; ++q)
       if ((st = launch_embed(S->UEi[q], S->UE[q], S->dom))) return st;
-  if ((st = moons_advect_B(S->T1i, S->UEi, S->B0, S->ETF, S->E))) return st;
+  if ((st = moons_advect_B(S->T1i, S->UEi, S->Btot, S->ETF, S->E))) return st;
   for (int d = 0; d < 3; ++d)
     if ((st = axpy(S->FB[d], S->T1i[d], 1.0))) return st;
   for (int d = 0; d < 3; ++d)
@@ -334,6 +350,7 @@ int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, con
   if (!mi) mi = m;
   if (!(P->dt > 0.0) || !(P->Re > 0.0) || (P->alpha_U != 0.0 && P->alpha_U != 1.0))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: need dt > 0, Re > 0, alpha_U 0 (method 4) or 1 (method 6)");
+  if (P->mpg_dir < 0 || P->mpg_dir > 3) return fail(MOONS_ERR_ARG, "moons_mhd_create: mpg_dir must be 0..3");
   if (P->induction && (!B || !phi || !(P->Rem > 0.0)))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: induction needs B, phi and Rem > 0");
   if (mi != m && (!dom || dom->mi != m || dom->mt != mi || !P->induction))
@@ -385,6 +402,11 @@ int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, con
         if (!st) st = fcreate(mi, 1 << d, &v[d]);
       if (!st) copy_bcs_of(S->Bstar[d], B[d]);
       if (!st) st = moons_field_fill(S->B0[d], P->B0[d]);
+      if (P->finite_Rem) {
+        if (!st) st = fcreate(mi, 1 << d, &S->Btot[d]);
+      } else {
+        S->Btot[d] = S->B0[d];
+      }
       if (!st) st = moons_field_fill(S->FB[d], 0.0);
       if (!st) st = fcreate(mi, kEdgeDL[d], &S->J[d]);
       if (!st) st = fcreate(mi, kEdgeDL[d], &S->sig[d]);

@@ -369,14 +369,7 @@ static void vf_free(moons_pcgvf_s *S) {
 }
 
 // Fields of the solver carry x's BCs (p is x's copy in the reference).
-static void copy_bcs(moons_field_s *dst, const moons_field_s &src) {
-  dst->has_bcs = src.has_bcs;
-  dst->all_neumann = src.all_neumann;
-  for (int f = 0; f < 6; ++f) {
-    dst->bct[f] = src.bct[f];
-    dst->bval[f] = src.bval[f];
-  }
-}
+static void copy_bcs(moons_field_s *dst, const moons_field_s &src) { copy_bcs_of(dst, &src); }
 
 // Minv of prec_mom_VF / prec_ind_VF computed on the host in the reference's
 // operation order (laplacian_diagonal / curl_curl_diagonal, *c, +1, *vol, 1/x,

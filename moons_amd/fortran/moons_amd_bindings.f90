@@ -11,7 +11,8 @@ module moons_amd_bindings
   integer(c_int), parameter, public :: MOONS_DL_CC = 0
   integer(c_int), parameter, public :: MOONS_DL_FACE_X = 1, MOONS_DL_FACE_Y = 2, MOONS_DL_FACE_Z = 4
   integer(c_int), parameter, public :: MOONS_BC_NONE = 0, MOONS_BC_DIRICHLET = 1, MOONS_BC_NEUMANN = 2
-  integer(c_int), parameter, public :: MOONS_BC_PERIODIC = 3
+  integer(c_int), parameter, public :: MOONS_BC_PERIODIC = 3, MOONS_BC_SYMMETRIC = 4, MOONS_BC_ANTISYMMETRIC = 5
+  integer(c_int), parameter, public :: MOONS_BC_ROBIN = 6
   integer(c_int), parameter, public :: MOONS_OP_LAP_UNIFORM_SF = 1
   integer(c_int), parameter, public :: MOONS_PREC_LAP_SF = 1, MOONS_PREC_MG_FP32 = 2
   integer(c_int), parameter, public :: MOONS_OP_MOM_DIFFUSION = 1, MOONS_OP_IND_DIFFUSION = 2
@@ -28,6 +29,9 @@ module moons_amd_bindings
     real(c_double) :: tol_rel(4)
     real(c_double) :: tol_abs(4)
     integer(c_int) :: n_skip(4)
+    integer(c_int) :: mpg_dir
+    real(c_double) :: mpg_scale
+    integer(c_int) :: finite_Rem
   end type
 
   type, bind(C), public :: moons_mhd_info
@@ -47,7 +51,7 @@ module moons_amd_bindings
   public :: moons_init, moons_finalize, moons_synchronize
   public :: moons_nodes_uniform, moons_nodes_roberts_both
   public :: moons_mesh_create, moons_mesh_destroy
-  public :: moons_field_create, moons_field_destroy, moons_field_set_bcs
+  public :: moons_field_create, moons_field_destroy, moons_field_set_bcs, moons_field_set_bc_props
   public :: moons_field_upload, moons_field_download, moons_field_fill
   public :: moons_apply_bcs, moons_grad, moons_div
   public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
@@ -111,6 +115,11 @@ module moons_amd_bindings
       integer(c_int), intent(in) :: types(6)
       real(c_double), intent(in) :: vals(6)
       integer(c_int), value :: all_neumann
+    end function
+    integer(c_int) function moons_field_set_bc_props(f, c_w, robin_coeff) bind(C, name="moons_field_set_bc_props")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      real(c_double), intent(in) :: c_w(6), robin_coeff(6)
     end function
     integer(c_int) function moons_field_upload(f, host, n) bind(C, name="moons_field_upload")
       import c_int, c_double, c_ptr, c_size_t

@@ -535,6 +535,17 @@ static void face_planes(int nalong, int s, int is_max, int *G, int *B, int *I, i
   }
 }
 
+/* dh of a face's sub-domain (FSD%dh, hw/sub_domain/face_SD_extend.f90:63):
+   get_dh_boundary (hw/sub_domain/sub_domain_extend.f90:125-134) returns dhn(1)
+   of the three-node face grid get_face_GI keeps (hw/grid/grid_extend.f90:260-268,
+   hw/grid/coordinates_extend.f90:472-485): ghost and boundary node at the min
+   face, the last interior cell at the max face. */
+double orc_face_dh(const orc_coords *c, int is_max) {
+  return is_max ? c->dhn[c->sn - 3] : c->dhn[0];
+}
+
+double orc_grid_face_dh(const orc_grid *g, int d, int is_max) { return orc_face_dh(&g->c[d], is_max); }
+
 /* apply_BCs_faces (hw/BCs/apply/apply_BCs.f90:42-64) over raw kernels
    hw/BCs/direct/apply_BCs_faces_raw.f90:29-283 and
    hw/BCs/implicit/apply_BCs_faces_raw_implicit.f90:27-257.  Faces are applied
@@ -552,7 +563,7 @@ void orc_apply_bcs(double *u, int dl, const orc_grid *g, const orc_bcs *bc, int 
     face_planes(nal, s[d], is_max, &G, &B, &I, &Io);
     double nhat = is_max ? 1.0 : -1.0;
     const orc_coords *c = &g->c[d];
-    double dh = is_max ? c->dhn[c->sn - 2] : c->dhn[0];
+    double dh = orc_face_dh(c, is_max);
     double bv = implicit_ ? 0.0 : bc->val[face];
     int Gp = G, Iop = Io;
     if (nal && t == ORC_BC_PERIODIC && is_max && s[d] > 4) { Gp = B; Iop = 1; }
@@ -585,6 +596,26 @@ void orc_apply_bcs(double *u, int dl, const orc_grid *g, const orc_bcs *bc, int 
           } break;
           case ORC_BC_SYMMETRIC: u[ig] = u[iin]; break;
           case ORC_BC_ANTISYMMETRIC: u[ig] = -u[iin]; break;
+          case ORC_BC_ROBIN: {
+            /* u + c_w du/dh = 0: apply_Robin_C/N (hw/BCs/direct/apply_BCs_faces_raw.f90:236-282)
+               and apply_Robin_C/N_implicit (hw/BCs/implicit/apply_BCs_faces_raw_implicit.f90:213-257),
+               with the face sub-domain's dh and nhat. */
+            double dr = dh, cw = bc->cw[face];
+            if (!nal) {
+              double c1 = (2.0 * cw * nhat / dr - 1.0) / (2.0 * cw * nhat / dr + 1.0);
+              /* the direct rule's bvals coefficient is assigned 0 before use (:244-245) */
+              u[ig] = implicit_ ? u[iin] * c1 : u[iin] * c1 + bv * 0.0;
+            } else {
+              ii[d] = B; size_t ib = IDX(s, ii[0], ii[1], ii[2]);
+              if (implicit_) {
+                double c1 = -(2.0 * dr * nhat / cw);
+                u[ig] = u[iin] + u[ib] * c1;
+              } else {
+                double c1 = -(2.0 * dr / cw * nhat), c2 = 2.0 * dr / cw * nhat;
+                u[ig] = u[iin] + u[ib] * c1 + bv * c2;
+              }
+            }
+          } break;
           default: break;
         }
       }

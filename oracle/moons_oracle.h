@@ -42,6 +42,7 @@ extern "C" {
 #define ORC_BC_PERIODIC 3
 #define ORC_BC_SYMMETRIC 4
 #define ORC_BC_ANTISYMMETRIC 5
+#define ORC_BC_ROBIN 6
 
 typedef struct {
   int sn, sc;
@@ -60,6 +61,7 @@ typedef struct {
 typedef struct {
   int type[6];   /* faces xmin,xmax,ymin,ymax,zmin,zmax */
   double val[6]; /* uniform BC value per face */
+  double cw[6];  /* wall conductance ratio c_w per face (Robin faces only) */
 } orc_bcs;
 
 /* ---- 1D grids ---- */
@@ -109,6 +111,9 @@ void orc_advect_B(double *adv[3], double *const U_E[6], double *const B[3], cons
 
 /* ---- BCs (reference hw/BCs/direct and hw/BCs/implicit raw kernels) ---- */
 void orc_apply_bcs(double *u, int dl, const orc_grid *g, const orc_bcs *bc, int implicit_);
+/* dh of the face sub-domain at the min (is_max 0) or max face of a direction */
+double orc_face_dh(const orc_coords *c, int is_max);
+double orc_grid_face_dh(const orc_grid *g, int d, int is_max);
 void orc_assign_ghost_xperiodic(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val);
 void orc_assign_wall_dirichlet(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val);
 void orc_multiply_wall_neumann(double *u, int dl, const orc_grid *g, const orc_bcs *bc, double val);

@@ -16,21 +16,24 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmoons_ref.so")
 
 DL = {"CC": 0, "Fx": 1, "Fy": 2, "Fz": 4, "Ex": 6, "Ey": 5, "Ez": 3, "N": 7}
-BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5}
+BC = {"none": 0, "Dirichlet": 1, "Neumann": 2, "Periodic": 3, "Symmetric": 4, "Antisymmetric": 5, "Robin": 6}
 
 _dp = C.POINTER(C.c_double)
 
 
 class BCs(C.Structure):
-    _fields_ = [("type", C.c_int * 6), ("val", C.c_double * 6)]
+    _fields_ = [("type", C.c_int * 6), ("val", C.c_double * 6), ("cw", C.c_double * 6)]
 
     @classmethod
-    def make(cls, types, vals=None):
+    def make(cls, types, vals=None, c_w=None):
+        """Per-face types and values; c_w: the wall conductance ratio of each
+        face, used by Robin faces (SP%DP%c_w, init_BC_props)."""
         b = cls()
         for f in range(6):
             t = types[f]
             b.type[f] = BC[t] if isinstance(t, str) else int(t)
             b.val[f] = 0.0 if vals is None else float(vals[f])
+            b.cw[f] = 0.0 if c_w is None else float(c_w[f])
         return b
 
 
@@ -61,6 +64,8 @@ def lib():
         L.orc_curl_component.argtypes = [_dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_void_p, C.c_int]
         L.orc_lap_centered_sf.argtypes = [_dp, _dp, C.c_int, C.c_void_p, C.POINTER(BCs)]
         L.orc_apply_bcs.argtypes = [_dp, C.c_int, C.c_void_p, C.POINTER(BCs), C.c_int]
+        L.orc_grid_face_dh.restype = C.c_double
+        L.orc_grid_face_dh.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_volume.argtypes = [_dp, C.c_int, C.c_void_p]
         L.orc_lap_diagonal.argtypes = [_dp, C.c_int, C.c_void_p]
         L.orc_pcg_lap_sf.restype = C.c_int
@@ -137,6 +142,11 @@ class Grid:
         out = np.zeros(n)
         lib().orc_coord_array(self.h, d, which, p(out))
         return out
+
+
+def face_dh(g: Grid, d: int, is_max: int) -> float:
+    """dh of the face sub-domain that the Neumann and Robin rules use."""
+    return lib().orc_grid_face_dh(g.h, d, is_max)
 
 
 def _F(a):
@@ -312,6 +322,9 @@ def ref():
         R.ref_import_raw.argtypes = raw
         R.ref_ext_uniform.argtypes = [_dp, C.c_int, C.c_int, _dp]
         R.ref_overlap.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _ip]
+        R.ref_robin_plane.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int]
+        R.ref_face_dh.argtypes = [_dp, C.c_int, C.c_int, _dp]
         _ref = R
     return _ref
 
@@ -357,6 +370,26 @@ def ref_bc_plane(kind, ug, ub, ui, bvals, dh_nhat):
     x, y = ug.shape
     ref().ref_bc_plane(kind, p(ug), p(ub), p(ui), p(bvals), dh_nhat, x, y)
     return ug, ub
+
+
+def ref_robin_plane(kind, ug, ub, ui, bvals, dh, nhat, c_w):
+    """The reference's raw Robin kernels (kind 0 Robin_C, 1 Robin_N,
+    2 Robin_C_implicit, 3 Robin_N_implicit) on one ghost plane."""
+    ug = np.array(ug, dtype=np.float64, order="F")
+    ub = np.asfortranarray(ub, dtype=np.float64)
+    ui = np.asfortranarray(ui, dtype=np.float64)
+    bvals = np.asfortranarray(bvals, dtype=np.float64)
+    x, y = ug.shape
+    ref().ref_robin_plane(kind, p(ug), p(ub), p(ui), p(bvals), float(dh), float(nhat), float(c_w), x, y)
+    return ug
+
+
+def ref_face_dh(hn, face: int) -> float:
+    """FSD%dh of face 1..6 on a grid with node array hn in every direction."""
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    out = C.c_double(0.0)
+    ref().ref_face_dh(p(hn), len(hn), int(face), C.byref(out))
+    return out.value
 
 
 def ref_gf_export(path: str, f: np.ndarray) -> None:

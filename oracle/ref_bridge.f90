@@ -21,7 +21,7 @@ module moons_ref_bridge
   use string_mod, only: string, str, delete
   use IO_tools_mod, only: new_and_open
   use grid_mod, only: grid
-  use grid_extend_mod, only: init, get_shape
+  use grid_extend_mod, only: init, get_shape, get_face_GI
   use data_location_mod, only: data_location
   use data_location_extend_mod, only: init_CC, init_Node, init_Face, init_Edge
   use grid_field_mod, only: grid_field, export, import
@@ -274,5 +274,35 @@ contains
     case (6); call apply_Dirichlet_N_implicit(ug, ub, ui, x, y, 0)
     case (7); call apply_Neumann_C_implicit(ug, ui, x, y, 0)
     end select
+  end subroutine
+
+  ! Raw Robin kernels on an x-by-y plane, p = 0 (u + c_w du/dh = 0).
+  ! kind: 0 Robin_C, 1 Robin_N, 2 Robin_C_implicit, 3 Robin_N_implicit
+  subroutine ref_robin_plane(kind, ug, ub, ui, bvals, dh, nhat, c_w, x, y) bind(C, name="ref_robin_plane")
+    integer(c_int), value :: kind, x, y
+    real(c_double), intent(inout) :: ug(x, y)
+    real(c_double), intent(in) :: ub(x, y), ui(x, y), bvals(x, y)
+    real(c_double), value :: dh, nhat, c_w
+    select case (kind)
+    case (0); call apply_Robin_C(ug, ui, bvals, dh, nhat, c_w, x, y, 0)
+    case (1); call apply_Robin_N(ug, ui, ub, bvals, dh, nhat, c_w, x, y, 0)
+    case (2); call apply_Robin_C_implicit(ug, ui, dh, nhat, c_w, x, y, 0)
+    case (3); call apply_Robin_N_implicit(ug, ub, ui, dh, nhat, c_w, x, y, 0)
+    end select
+  end subroutine
+
+  ! dh of face (1..6) on a grid whose three directions have the node array
+  ! hn (ghosts included): dhn(1) of the face grid get_face_GI keeps
+  ! (handwritten/grid/grid_extend.f90:260-268), which get_dh_boundary returns
+  ! for a grid of two or more cells (handwritten/sub_domain/sub_domain_extend.f90:125-134)
+  ! and face_SD stores as FSD%dh (handwritten/sub_domain/face_SD_extend.f90:63).
+  subroutine ref_face_dh(hn, sn, face, dh) bind(C, name="ref_face_dh")
+    integer(c_int), value :: sn, face
+    real(c_double), intent(in) :: hn(sn)
+    real(c_double), intent(out) :: dh
+    type(grid) :: g, gf
+    call init(g, hn, hn, hn)
+    call get_face_GI(gf, g, int(face))
+    dh = gf%c((face + 1) / 2)%dhn%f(1)
   end subroutine
 end module

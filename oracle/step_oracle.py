@@ -124,10 +124,17 @@ def clean_div(g, X, Xstar, xbcs, phi, phibcs, all_neumann, scale, isp):
 class MHDState:
     """State and parameters of the coupled step; arrays are F-ordered FP64."""
 
+    finite_Rem = False
+    mpg_dir, mpg_scale = 0, 1.0
+
     def __init__(self, g, U, Ubcs, pbcs, Re, dt, alpha_U=1.0, B=None, Bbcs=None, phibcs=None, Rem=1.0, Ha=0.0,
                  B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
-                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), gi=None, dom=None, sigInv=None):
+                 isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), gi=None, dom=None, sigInv=None,
+                 mpg_dir=0, mpg_scale=1.0, finite_Rem=False):
         self.g = g
+        # mean pressure gradient forcing (SP%MT%mean_pressure_grad, SP%SCP%mpg_dir; 0 = off)
+        self.mpg_dir, self.mpg_scale = mpg_dir, mpg_scale
+        self.finite_Rem = finite_Rem  # Btot = B0 + B (SP%SCP%finite_Rem), else B0
         self.gi = gi if gi is not None else g  # the induction grid
         self.dom = dom
         self.Re, self.dt, self.alpha_U = Re, dt, alpha_U
@@ -163,6 +170,14 @@ class MHDState:
     # ---------------------------------------------------------------- momentum
     def momentum_sources(self):
         g = self.g
+        if self.mpg_dir:
+            # compute_add_MPG (hw/solvers/momentum/momentum_sources.f90:97-108), first in
+            # add_all_momentum_sources (add_all_momentum_sources.f90:31-33): every value of
+            # the component, ghosts included, += scale*dt; then apply_BCs(U)
+            d = self.mpg_dir - 1
+            self.U[d] = np.asfortranarray(self.U[d] + self.mpg_scale * self.dt)
+            for e in range(3):
+                O.apply_bcs(g, self.U[e], 1 << e, self.Ubcs[e])
         self.Fnm1 = [f.copy(order="F") for f in self.F]
         self.L = [g.zeros(1 << d) for d in range(3)]
         F = [g.zeros(1 << d) for d in range(3)]
@@ -249,7 +264,11 @@ class MHDState:
         g = self.gi
         self.J = [np.asfortranarray(O.curl(g, self.B[0], self.B[1], self.B[2], EDGE_DL[d], d) * (1.0 / self.Rem))
                   for d in range(3)]
-        self.Btot = [b.copy(order="F") for b in self.B0]
+        # compute_Btot (hw/governing_equations/induction_extend.f90:527-536)
+        if self.finite_Rem:
+            self.Btot = [np.asfortranarray(self.B0[d] + self.B[d]) for d in range(3)]
+        else:
+            self.Btot = [b.copy(order="F") for b in self.B0]
         A = {(i, j): edge2face(g, self.J[i], i, j) for (i, j) in NO_DIAG}
         Bf = {(i, j): face2face(g, self.Btot[i], i, j) for (i, j) in NO_DIAG}
         jx = A[(1, 0)] * Bf[(2, 0)] - A[(2, 0)] * Bf[(1, 0)]

@@ -112,6 +112,29 @@ def test_apply_bcs_bitexact(grids, gpu, bc, implicit):
         np.testing.assert_array_equal(U.download(), ref, err_msg=f"dl={dl}")
 
 
+ROBIN_SETS = [
+    (["Robin"] * 6, [0.3, -0.2, 1.0, 0.0, 0.5, -1.5], [0.01, 0.02, 0.5, 1.0, 3.0, 0.07]),
+    (["Periodic", "Periodic", "Robin", "Neumann", "Dirichlet", "Robin"], [0, 0, 0.2, 0.7, -0.1, 0.4],
+     [0, 0, 0.01, 0, 0, 0.01]),
+]
+
+
+@pytest.mark.parametrize("bc", range(len(ROBIN_SETS)))
+@pytest.mark.parametrize("implicit", [False, True])
+def test_apply_robin_bcs_bitexact(grids, gpu, bc, implicit):
+    """Thin-wall (Robin) ghost rules with each face's c_w, every location."""
+    g, m = grids
+    types, vals, cw = ROBIN_SETS[bc]
+    for dl in (0, 1, 2, 4, 3, 5, 6, 7):
+        u = rand(g.shape(dl), 60 + dl)
+        ref = u.copy(order="F")
+        O.apply_bcs(g, ref, dl, O.BCs.make(types, vals, cw), implicit)
+        U = gpu.SF(m, dl).upload(u).set_BCs(types, vals, c_w=cw)
+        (gpu.apply_BCs_implicit if implicit else gpu.apply_BCs)(U)
+        np.testing.assert_array_equal(U.download(), ref, err_msg=f"dl={dl}")
+        assert not np.array_equal(ref, u)
+
+
 @pytest.mark.parametrize("bc", [0, 1, 2])
 def test_lap_uniform_sf_bitexact(grids, gpu, bc):
     """Lap_uniform_SF: implicit BCs on x, then div(grad x) -- fused kernel for CC."""

@@ -23,6 +23,10 @@ CASES = {
     "dirichlet_inhomog": ([("u", 10), ("r", 9, 1.2), ("u", 11)], ["Dirichlet", "Neumann", "Dirichlet", "Dirichlet",
                                                                     "Neumann", "Dirichlet"],
                           [0.5, 0.1, -0.3, 1.0, 0.0, 0.2], False),
+    # thin conducting walls on y and z (c_w 0.01 and 0.5), x periodic
+    "robin_thin_wall": ([("u", 12), ("r", 10, 1.05), ("r", 11, 1.1)],
+                        ["Periodic", "Periodic", "Robin", "Robin", "Robin", "Neumann"], [0.0, 0.0, 0.1, 0.0, -0.2, 0.0],
+                        False, [0.0, 0.0, 0.01, 0.01, 0.5, 0.0]),
 }
 
 
@@ -31,12 +35,25 @@ def rhs(shape, seed):
     return np.asfortranarray(b)
 
 
+class _Types(list):
+    """BC types that carry the faces' c_w into set_BCs."""
+
+
 def setup(gpu, case):
-    specs, types, vals, alln = CASES[case]
+    specs, types, vals, alln = CASES[case][:4]
+    cw = CASES[case][4] if len(CASES[case]) > 4 else None
     hs = [nodes(s) for s in specs]
     g = O.Grid(*hs)
     m = gpu.mesh(*hs)
-    return g, m, O.BCs.make(types, vals), types, vals, alln
+    if cw is not None:
+        types = _Types(types)
+        types.c_w = cw
+    return g, m, O.BCs.make(types, vals, cw), types, vals, alln
+
+
+def set_bcs(X, types, vals, alln):
+    """set_BCs with the case's c_w (Robin faces)."""
+    return X.set_BCs(types, vals, all_Neumann=alln, c_w=getattr(types, "c_w", None))
 
 
 def rel_l2(a, b):
@@ -51,7 +68,7 @@ def test_pcg_fixed_iterations_match_oracle(gpu, case, iters):
     x0 = rhs(g.shape(0), 4) * 0.1
     xr = x0.copy(order="F")
     it_r, res0_r, res_r, _ = O.pcg_lap(g, 0, bcs, alln, xr, b, iters, 0.0, 0.0)
-    X = gpu.SF(m, 0).upload(x0).set_BCs(types, vals, all_Neumann=alln)
+    X = set_bcs(gpu.SF(m, 0).upload(x0), types, vals, alln)
     B = gpu.SF(m, 0).upload(b)
     P = gpu.PCG_solver_SF(m, X, gpu.iter_solver_params(iters, 0.0, 0.0, 1))
     info = P.solve(X, B)
@@ -67,7 +84,7 @@ def test_pcg_converged_matches_oracle(gpu, case):
     b = rhs(g.shape(0), 9)
     xr = np.zeros(g.shape(0), order="F")
     it_r, res0_r, res_r, hist = O.pcg_lap(g, 0, bcs, alln, xr, b, 2000, 1e-8, 1e-300)
-    X = gpu.SF(m, 0).set_BCs(types, vals, all_Neumann=alln)
+    X = set_bcs(gpu.SF(m, 0), types, vals, alln)
     B = gpu.SF(m, 0).upload(b)
     P = gpu.PCG_solver_SF(m, X, gpu.iter_solver_params(2000, 1e-8, 1e-300, 1))
     info = P.solve(X, B)
@@ -83,7 +100,7 @@ def test_pcg_n_skip_and_iter_max_exits(gpu):
     for n_skip, it_max, tol in ((5, 1000, 1e-6), (1, 13, 1e-12), (4, 30, 1e-12)):
         xr = np.zeros(g.shape(0), order="F")
         it_r, _, res_r, _ = O.pcg_lap(g, 0, bcs, alln, xr, b, it_max, tol, 1e-300, n_skip)
-        X = gpu.SF(m, 0).set_BCs(types, vals, all_Neumann=alln)
+        X = set_bcs(gpu.SF(m, 0), types, vals, alln)
         P = gpu.PCG_solver_SF(m, X, gpu.iter_solver_params(it_max, tol, 1e-300, n_skip))
         info = P.solve(X, gpu.SF(m, 0).upload(b))
         assert info.iterations == it_r

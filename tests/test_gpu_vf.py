@@ -19,6 +19,18 @@ PSEUDO_VACUUM = [
 ]
 
 
+# thin conducting walls (thin_wall, code/handwritten/solvers/induction/init_B_BCs.f90:151-175):
+# pseudo-vacuum normal components, Robin tangential ones with c_w = 0.01
+CW = [0.01] * 6
+THIN_WALL = [
+    (["Neumann", "Neumann"] + ["Robin"] * 4, None, CW),
+    (["Robin"] * 2 + ["Neumann", "Neumann"] + ["Robin"] * 2, None, CW),
+    (["Robin"] * 4 + ["Neumann", "Neumann"], None, CW),
+]
+# the periodic duct with thin walls (periodic_duct_thin_wall, init_B_BCs.f90:124-134 with x periodic)
+DUCT_THIN_WALL = [(["Periodic"] * 2 + t[2:], None, CW) for t, _, _ in THIN_WALL]
+
+
 def rand(shape, seed, amp=1.0):
     return np.asfortranarray(amp * np.random.default_rng(seed).standard_normal(shape))
 
@@ -26,10 +38,11 @@ def rand(shape, seed, amp=1.0):
 def setup(gpu, N, bcspec, seed):
     hs = [O.roberts_both_nodes(0, 1, N[0], 1.1), O.uniform_nodes(0, 2, N[1]), O.roberts_both_nodes(-1, 1, N[2], 1.05)]
     g, m = O.Grid(*hs), gpu.mesh(*hs)
-    obcs = [O.BCs.make(t, v) for t, v in bcspec]
+    obcs = [O.BCs.make(*bc) for bc in bcspec]
     x0 = [rand(g.shape(1 << d), seed + d, 0.1) for d in range(3)]
     b = [rand(g.shape(1 << d), seed + 10 + d) for d in range(3)]
-    X = [gpu.SF(m, 1 << d).upload(x0[d]).set_BCs(t, v) for d, (t, v) in enumerate(bcspec)]
+    X = [gpu.SF(m, 1 << d).upload(x0[d]).set_BCs(bc[0], bc[1], c_w=bc[2] if len(bc) > 2 else None)
+         for d, bc in enumerate(bcspec)]
     B = [gpu.SF(m, 1 << d).upload(b[d]) for d in range(3)]
     return g, m, obcs, x0, b, X, B
 
@@ -66,10 +79,13 @@ def sigma_inv(g, seed=0):
         ("mom_diffusion", "duct", (16, 9, 11), -0.05),
         ("ind_diffusion", "pv", (10, 12, 9), 0.03),
         ("ind_diffusion", "duct", (8, 10, 12), 0.01),
+        ("ind_diffusion", "thin_wall", (10, 12, 9), 0.03),
+        ("ind_diffusion", "duct_thin_wall", (12, 10, 14), 0.02),
     ],
 )
 def test_vf_pcg_matches_oracle(gpu, op, bcname, N, c):
-    spec = {"ldc": LDC, "duct": DUCT, "pv": PSEUDO_VACUUM}[bcname]
+    spec = {"ldc": LDC, "duct": DUCT, "pv": PSEUDO_VACUUM, "thin_wall": THIN_WALL,
+            "duct_thin_wall": DUCT_THIN_WALL}[bcname]
     print(f"vf pcg {op} {bcname} N={N}", flush=True)
     g, m, obcs, x0, b, X, B = setup(gpu, N, spec, 7)
     k = sigma_inv(g) if op == "ind_diffusion" else None

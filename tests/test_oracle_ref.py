@@ -118,8 +118,7 @@ def test_bc_rules_match_reference_kernels():
                 B = (s - 2 if is_max else 1) if nal else None
                 I = (s - 3 if is_max else 2) if nal else (s - 2 if is_max else 1)
                 um = np.moveaxis(u, d, 0)
-                dhn = g.coord(d, 2)
-                dh = dhn[-1] if is_max else dhn[0]
+                dh = O.ref_face_dh(g.coord(d, 0), face + 1)
                 nhat = 1.0 if is_max else -1.0
                 bv = np.full(um.shape[1:], 0.37)
                 ub = um[B] if B is not None else np.zeros(um.shape[1:])
@@ -129,3 +128,54 @@ def test_bc_rules_match_reference_kernels():
                 if B is not None and t == "Dirichlet":
                     exp[B] = ubn
                 np.testing.assert_array_equal(np.moveaxis(ours, d, 0), exp)
+
+
+def test_face_dh_matches_reference_face_grid():
+    """The dh every face rule uses is the reference's FSD%dh: dhn(1) of the
+    face grid get_face_GI keeps (the last interior cell at a max face, not
+    the ghost spacing)."""
+    for hn in (O.roberts_both_nodes(0, 1, 9, 1.05), O.uniform_nodes(0, 1, 7), O.roberts_both_nodes(-1, 2, 12, 1.2),
+               O.uniform_nodes(0, 3, 1)):
+        g = O.Grid(hn, hn, hn)
+        for face in range(6):
+            assert O.face_dh(g, face // 2, face % 2) == O.ref_face_dh(hn, face + 1)
+
+
+def test_robin_rules_match_reference_kernels():
+    """Robin ghost rules (u + c_w du/dh = 0), direct and implicit, C and N
+    located, equal the reference's raw kernels on every face, with the face
+    sub-domain's dh from the reference's own face grid."""
+    rng = np.random.default_rng(7)
+    h = O.roberts_both_nodes(0, 1, 6, 1.2)
+    g = O.Grid(h, O.uniform_nodes(0, 2, 5), O.roberts_both_nodes(-1, 1, 7, 1.05))
+    cws = [0.01, 0.3, 1.0, 2.5, 0.05, 7.0]
+    for dl in (0, 1, 2, 4, 3, 5, 6, 7):
+        for face in range(6):
+            d = face // 2
+            nal = (dl >> d) & 1
+            for implicit in (False, True):
+                types = ["none"] * 6
+                types[face] = "Robin"
+                vals = [0.0] * 6
+                vals[face] = 0.37
+                bcs = O.BCs.make(types, vals, cws)
+                u = np.asfortranarray(rng.standard_normal(g.shape(dl)))
+                ours = u.copy(order="F")
+                O.apply_bcs(g, ours, dl, bcs, implicit)
+                s = u.shape[d]
+                is_max = face % 2
+                G = s - 1 if is_max else 0
+                B = (s - 2 if is_max else 1) if nal else None
+                I = (s - 3 if is_max else 2) if nal else (s - 2 if is_max else 1)
+                um = np.moveaxis(u, d, 0)
+                dh = O.ref_face_dh(g.coord(d, 0), face + 1)
+                nhat = 1.0 if is_max else -1.0
+                bv = np.full(um.shape[1:], 0.37)
+                ub = um[B] if B is not None else np.zeros(um.shape[1:])
+                kind = (1 if nal else 0) + (2 if implicit else 0)
+                ug = O.ref_robin_plane(kind, um[G], ub, um[I], bv, dh, nhat, cws[face])
+                exp = um.copy()
+                exp[G] = ug
+                np.testing.assert_array_equal(np.moveaxis(ours, d, 0), exp)
+                # the rule is not the identity or a copy
+                assert not np.array_equal(ug, um[G]) and not np.array_equal(ug, um[I])

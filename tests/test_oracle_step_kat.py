@@ -251,3 +251,106 @@ def test_induction_backward_euler_decay(Rem):
         st.step()
         expect = expect / (1.0 + dt * mu / Rem)
         assert np.abs(st.B[0] - expect)[1:-1, 1:-1, 1:-1].max() > 1e-8 * eps
+
+
+def mpg_state(Re, dt, scale, N=6, NZ=12):
+    """Fluid at rest between walls at z = 0, 1 (x, y periodic), driven along x
+    by the mean pressure gradient, method 4."""
+    g = O.Grid(O.uniform_nodes(0.0, 1.0, N), O.uniform_nodes(0.0, 1.0, N), O.uniform_nodes(0.0, 1.0, NZ))
+    wall = O.BCs.make(["Periodic"] * 4 + ["Dirichlet"] * 2)
+    st = S.MHDState(g, [g.zeros(1 << d) for d in range(3)], [wall] * 3,
+                    O.BCs.make(["Periodic"] * 4 + ["Neumann"] * 2), Re=Re, dt=dt, alpha_U=0.0,
+                    isp_p=(200, 1e-12, 1e-300, 1), mpg_dir=1, mpg_scale=scale)
+    return g, st
+
+
+def test_mean_pressure_gradient_forcing():
+    """compute_add_MPG adds scale dt to u before the sources are computed:
+    away from the walls the flow is uniform along x (no convection, zero
+    Laplacian, divergence-free), so after n steps u = n scale dt exactly on the
+    rows the walls have not reached; on the wall rows the first step's source
+    is already the diffusion of the forced u (so the forcing precedes the
+    sources).  Planted mistakes (twice the scale, the forcing after the
+    sources) fail."""
+    Re, dt, s = 10.0, 1e-2, 3.0
+    g, st = mpg_state(Re, dt, s)
+    for n in range(1, 4):
+        st.step()
+        core = (slice(1, -1), slice(1, -1), slice(1 + n, -1 - n))
+        np.testing.assert_allclose(st.U[0][core], n * s * dt, rtol=1e-13)
+        if n == 1:
+            assert np.abs(st.F[0][1:-1, 1:-1, 1]).min() > 1e-3 * s * dt
+            assert np.abs(st.F[0][1:-1, 1:-1, 3:-3]).max() <= 1e-12
+        for d in (1, 2):
+            assert np.abs(st.U[d][1:-1, 1:-1, 1:-1]).max() <= 1e-14
+    g, st = mpg_state(Re, dt, s)
+    st.mpg_scale = 2 * s
+    st.step()
+    assert np.abs(st.U[0][1:-1, 1:-1, 3:-3] - s * dt).max() > 1e-3 * s * dt
+    g, st = mpg_state(Re, dt, s)
+    st.mpg_dir = 0  # forcing applied after the sources instead
+    st.momentum_sources()
+    st.U[0] = np.asfortranarray(st.U[0] + s * dt)
+    assert np.abs(st.F[0][1:-1, 1:-1, 1]).max() == 0.0
+
+
+def test_finite_rem_btot_enters_jcrossb():
+    """finite_Rem: Btot = B0 + B (compute_Btot).  B = (0, 0, y), B0 = (0, 1, 0):
+    J = (1/Rem, 0, 0) and J x Btot = N (0, -y/Rem, 1/Rem); with Btot = B0 the
+    y component vanishes."""
+    N = 8
+    g = O.Grid(*[O.uniform_nodes(0.0, 1.0, N)] * 3)
+    X, Y, Z = coords(g, 4)
+    Ubcs = [O.BCs.make(["Dirichlet"] * 6)] * 3
+    Rem, Ha, Re = 0.5, 3.0, 2.0
+    Nf = Ha * Ha / Re
+    for finite in (True, False):
+        B = [g.zeros(1), g.zeros(2), np.asfortranarray(Y.copy())]
+        st = S.MHDState(g, [g.zeros(1 << d) for d in range(3)], Ubcs, O.BCs.make(["Neumann"] * 6), Re=Re, dt=1e-3,
+                        B=B, Bbcs=[O.BCs.make(["Neumann"] * 6)] * 3, phibcs=O.BCs.make(["Dirichlet"] * 6),
+                        Rem=Rem, Ha=Ha, finite_Rem=finite)
+        _, Yy, _ = coords(g, 2)
+        jy = st.jCrossB[1][2:-2, 2:-2, 2:-2]
+        expect = -Nf * Yy[2:-2, 2:-2, 2:-2] / Rem if finite else 0.0 * Yy[2:-2, 2:-2, 2:-2]
+        np.testing.assert_allclose(jy, expect, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(st.jCrossB[2][2:-2, 2:-2, 2:-2], Nf / Rem, rtol=1e-12)
+
+
+def test_robin_rules_satisfy_u_plus_cw_dudx():
+    """The thin-wall rules impose u + c_w du/dx = 0 at every face, with du/dx
+    along the coordinate (not the outward normal): cell-centred data through
+    the wall value (ghost + interior)/2 and the one-sided difference over dh
+    (the value term is dropped, as in the reference), node-centred data at the
+    boundary node with the central difference over 2 dh, where the value b
+    gives u + c_w du/dx = b."""
+    rng = np.random.default_rng(11)
+    hs = [O.roberts_both_nodes(0, 1, 7, 1.1), O.uniform_nodes(-1, 1, 6), O.roberts_both_nodes(0, 2, 9, 1.05)]
+    g = O.Grid(*hs)
+    cw = [0.01, 0.3, 2.0, 0.05, 0.7, 0.02]
+    for dl in (0, 1, 2, 4, 7):
+        for implicit in (False, True):
+            vals = [0.4, -0.3, 0.2, 0.1, -0.6, 0.5]
+            u = np.asfortranarray(rng.standard_normal(g.shape(dl)))
+            O.apply_bcs(g, u, dl, O.BCs.make(["Robin"] * 6, vals, cw), implicit)
+            for face in range(6):
+                d, is_max = face // 2, face % 2
+                um = np.moveaxis(u, d, 0)
+                s = um.shape[0]
+                dh = O.face_dh(g, d, is_max)
+                core = (slice(2, -2), slice(2, -2))  # away from the edges other faces wrote
+                if not (dl >> d) & 1:
+                    G, I = (s - 1, s - 2) if is_max else (0, 1)
+                    uw = (um[G] + um[I]) / 2
+                    dudx = (um[G] - um[I]) / dh if is_max else (um[I] - um[G]) / dh
+                    target = 0.0
+                else:
+                    G, B, I = (s - 1, s - 2, s - 3) if is_max else (0, 1, 2)
+                    uw = um[B]
+                    dudx = (um[G] - um[I]) / (2 * dh) if is_max else (um[I] - um[G]) / (2 * dh)
+                    target = 0.0 if implicit else vals[face]
+                res = (uw + cw[face] * dudx - target)[core]
+                scale = max(1.0, np.abs(cw[face] * dudx[core]).max())
+                assert np.abs(res).max() <= 1e-13 * scale, (dl, face, implicit)
+                # the outward-normal reading u + c_w du/dn = 0 fails at the min faces
+                if not is_max:
+                    assert np.abs((uw - cw[face] * dudx - target)[core]).max() > 1e-6
