@@ -314,10 +314,17 @@ int moons_clean_div(moons_pcg_t pcg, moons_field_t X[3], moons_field_t Xstar[3],
  * (1, 2, 3) is an independent 2D problem in the transverse directions, solved
  * by staggered DCT-II transforms (code/handwritten/ops/ops_dct.f90:36-70).
  * The transverse spacings must be uniform (MOONS_ERR_ARG otherwise).  The
- * eigenvalues are those of the 5-point Neumann Laplacian for any transverse
- * spacings; the reference's factor equals them for equal spacings only, and
- * for dir = 1 it guards the zero mode with the wrong indices (see fft.hip). */
+ * mode factors are the reference's, 0.5 ha hb / (cos + cos - 2), with its
+ * dir = 1 guard of the zero mode by the indices (i, j) (the modes (0, q) of
+ * the first x-plane keep their value), unless moons_fft_set_coefficients
+ * selects the exact eigenvalues. */
 int moons_fft_create(moons_mesh_t m, int dir, moons_fft_t *fft);
+#define MOONS_FFT_COEF_REFERENCE 0 /* init_FFT's factors (FFT_solver_extend.f90:62-112) */
+#define MOONS_FFT_COEF_EXACT 1     /* 1/eigenvalue of the 5-point Neumann Laplacian for any spacings */
+/* Selects the mode factors (no reference counterpart: the reference has only
+ * its own; the exact eigenvalues differ from them for unequal transverse
+ * spacings and in the first x-plane for dir = 1). */
+int moons_fft_set_coefficients(moons_fft_t fft, int kind);
 int moons_fft_destroy(moons_fft_t fft);
 /* solve(FFT, u, f, m, compute_norms) (FFT_solver_extend.f90:128-137;
  * solve_FFT_SF_x/y/z, code/handwritten/FFT_solver/FFT_algorithm.f90:27-210):

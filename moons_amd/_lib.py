@@ -107,6 +107,7 @@ SIGNATURES = {
     "moons_pcg_time_mg": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "moons_fft_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "moons_fft_destroy": (C.c_int, [C.c_void_p]),
+    "moons_fft_set_coefficients": (C.c_int, [C.c_void_p, C.c_int]),
     "moons_fft_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "moons_clean_div_fft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                       C.c_void_p]),

@@ -465,13 +465,19 @@ class FFT_solver_SF:
     """The cosine-transform direct solver (init_FFT / solve,
     code/handwritten/FFT_solver/FFT_solver_extend.f90:43-137): every plane
     normal to `direction` (1, 2, 3) is an independent 2D Poisson problem solved
-    by DCT-II transforms; the transverse spacings must be uniform."""
+    by DCT-II transforms; the transverse spacings must be uniform.
+    coefficients: "reference" (init_FFT's mode factors, the default) or
+    "exact" (the 5-point Laplacian's eigenvalues for any spacings)."""
 
-    def __init__(self, m: mesh, direction: int):
+    COEFS = {"reference": 0, "exact": 1}
+
+    def __init__(self, m: mesh, direction: int, coefficients: str = "reference"):
         h = C.c_void_p()
         check(load().moons_fft_create(m.handle, int(direction), C.byref(h)))
         self.handle = h
         self.m = m
+        if coefficients != "reference":
+            check(load().moons_fft_set_coefficients(h, self.COEFS[coefficients]))
 
     def solve(self, u: SF, f: SF) -> None:
         check(load().moons_fft_solve(self.handle, u.handle, f.handle))

@@ -20,11 +20,17 @@
 // The reference computes the same transforms with a radix-2 FFT, so the two
 // agree to rounding.
 //
-// Where the reference deviates from the mathematics, this solver keeps the
-// mathematics: its factor 0.5 dy dz / (cos + cos - 2) equals 1/lambda only for
-// equal transverse spacings, and for dir = 1 it guards the zero mode with the
-// indices (i, j) instead of (j, k), so modes (0, q) of the first x-plane are
-// multiplied by 1 there.  Both agree wherever the reference is exact.
+// Mode factors.  By default they are the reference's (init_FFT,
+// FFT_solver_extend.f90:62-112): 0.5 ha hb / (cos(pi p/Na) + cos(pi q/Nb) - 2)
+// with ha, hb the first spacings of the transverse directions, which is
+// 1/lambda for equal spacings and 1/lambda of the 5-point Laplacian with both
+// spacings sqrt(ha hb) otherwise; for dir = 1 the reference guards the zero
+// mode with the indices (i, j) instead of (j, k), so the modes (0, q > 0) of
+// the first x-plane get the factor 1, which is reproduced.  Where the
+// reference's guard leaves 0.5 ha hb / 0 (the (0, 0) mode of every other
+// x-plane) this solver stores 0; the mode is set to 0 right after ("pin down
+// pressure"), so the result is the same.  MOONS_FFT_COEF_EXACT selects
+// 1/lambda for any spacings and no dir = 1 special case.
 #include <rocblas/rocblas.h>
 
 #include <cmath>
@@ -40,7 +46,8 @@ struct moons_fft_s {
   double *W = nullptr, *V = nullptr; // compact n0 n1 n2 work arrays
   double *C[3] = {nullptr, nullptr, nullptr};  // DCT-II matrices of the transverse directions
   double *Ci[3] = {nullptr, nullptr, nullptr}; // and their inverses
-  double *coef = nullptr;        // 1/lambda per mode (p, q), p fastest; 0 at (0, 0)
+  double *coef = nullptr;        // factor per mode (p, q), p fastest; 0 at (0, 0)
+  int kind = MOONS_FFT_COEF_REFERENCE;
   rocblas_handle blas = nullptr;
 };
 
@@ -68,15 +75,18 @@ __global__ void __launch_bounds__(kFT) k_fft_unpack(const double *__restrict__ w
   u[(size_t)(i + 1) + (size_t)(n0 + 2) * ((size_t)(j + 1) + (size_t)(n1 + 2) * (size_t)(k + 1))] = w[t];
 }
 
-// y *= 1/lambda of the cell's transverse mode (p along a, q along b)
+// y *= the factor of the cell's transverse mode (p along a, q along b);
+// x_quirk: the reference's dir = 1 guard (modes (0, q > 0) of the first
+// x-plane keep their value)
 __global__ void __launch_bounds__(kFT) k_fft_scale(double *__restrict__ y, const double *__restrict__ coef, int n0,
-                                                   int n1, int n2, int a, int b) {
+                                                   int n1, int n2, int a, int b, int x_quirk) {
   const long long t = (long long)blockIdx.x * kFT + threadIdx.x;
   const long long nc = (long long)n0 * n1 * n2;
   if (t >= nc) return;
   const int idx[3] = {(int)(t % n0), (int)((t / n0) % n1), (int)(t / ((long long)n0 * n1))};
   const int na = a == 0 ? n0 : (a == 1 ? n1 : n2);
-  y[t] = y[t] * coef[idx[a] + na * idx[b]];
+  const double c = (x_quirk && idx[0] == 0 && idx[a] == 0 && idx[b] > 0) ? 1.0 : coef[idx[a] + na * idx[b]];
+  y[t] = y[t] * c;
 }
 
 unsigned fblocks(long long n) { return (unsigned)((n + kFT - 1) / kFT); }
@@ -110,6 +120,32 @@ bool uniform(const std::vector<double> &dhn, int n) {
   for (int i = 1; i <= n; ++i)
     if (std::fabs(dhn[i] - h) > 1e-12 * std::fabs(h)) return false;
   return true;
+}
+
+// The mode factors of kind (MOONS_FFT_COEF_*) uploaded to F->coef.
+int fft_set_coefficients(moons_fft_s *F, int kind) {
+  const double pi = 3.14159265358979323846;
+  const int Na = F->n[F->a], Nb = F->n[F->b];
+  const HostCoords &ca = F->m->hc[F->a], &cb = F->m->hc[F->b];
+  std::vector<double> coef((size_t)Na * Nb);
+  for (int q = 0; q < Nb; ++q)
+    for (int p = 0; p < Na; ++p) {
+      double c;
+      if (kind == MOONS_FFT_COEF_EXACT) {
+        const double ha = ca.dhn[1], hb = cb.dhn[1];
+        const double lam = (2.0 * std::cos(pi * p / Na) - 2.0) / (ha * ha) + (2.0 * std::cos(pi * q / Nb) - 2.0) / (hb * hb);
+        c = 1.0 / lam;
+      } else {
+        // init_FFT (FFT_solver_extend.f90:62-112), the first spacing dhn(1) of each transverse direction
+        const double ha = ca.dhn[0], hb = cb.dhn[0];
+        const double cosa = std::cos(pi * (double)p / (double)Na), cosb = std::cos(pi * (double)q / (double)Nb);
+        c = 0.5 * ha * hb / (cosa + cosb - 2.0);
+      }
+      coef[(size_t)p + (size_t)Na * q] = (p == 0 && q == 0) ? 0.0 : c;
+    }
+  MOONS_HIP(hipMemcpy(F->coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+  F->kind = kind;
+  return MOONS_OK;
 }
 
 } // namespace
@@ -157,18 +193,11 @@ int moons_fft_create(moons_mesh_t m, int dir, moons_fft_t *out) {
         hipMemcpy(F->Ci[t], Ci.data(), Ci.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return bail(MOONS_ERR_HIP, "moons_fft_create: hipMemcpy failed");
   }
-  {
-    const int Na = F->n[F->a], Nb = F->n[F->b];
-    const double ha = m->hc[F->a].dhn[1], hb = m->hc[F->b].dhn[1];
-    std::vector<double> coef((size_t)Na * Nb);
-    for (int q = 0; q < Nb; ++q)
-      for (int p = 0; p < Na; ++p) {
-        const double lam = (2.0 * std::cos(pi * p / Na) - 2.0) / (ha * ha) + (2.0 * std::cos(pi * q / Nb) - 2.0) / (hb * hb);
-        coef[(size_t)p + (size_t)Na * q] = (p == 0 && q == 0) ? 0.0 : 1.0 / lam;
-      }
-    if (hipMalloc(&F->coef, coef.size() * sizeof(double)) != hipSuccess ||
-        hipMemcpy(F->coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-      return bail(MOONS_ERR_HIP, "moons_fft_create: allocation failed");
+  if (hipMalloc(&F->coef, (size_t)F->n[F->a] * F->n[F->b] * sizeof(double)) != hipSuccess)
+    return bail(MOONS_ERR_HIP, "moons_fft_create: allocation failed");
+  if (int st = fft_set_coefficients(F, MOONS_FFT_COEF_REFERENCE)) {
+    moons_fft_destroy(F);
+    return st;
   }
   if (rocblas_create_handle(&F->blas) != rocblas_status_success) {
     F->blas = nullptr;
@@ -176,6 +205,14 @@ int moons_fft_create(moons_mesh_t m, int dir, moons_fft_t *out) {
   }
   *out = F;
   return MOONS_OK;
+}
+
+int moons_fft_set_coefficients(moons_fft_t F, int kind) {
+  if (!F) return fail(MOONS_ERR_ARG, "moons_fft_set_coefficients: null solver");
+  if (kind != MOONS_FFT_COEF_REFERENCE && kind != MOONS_FFT_COEF_EXACT)
+    return fail(MOONS_ERR_ARG, "moons_fft_set_coefficients: kind must be MOONS_FFT_COEF_REFERENCE or _EXACT");
+  if (stream()) MOONS_HIP(hipStreamSynchronize(stream()));
+  return fft_set_coefficients(F, kind);
 }
 
 int moons_fft_destroy(moons_fft_t F) {
@@ -202,7 +239,9 @@ int moons_fft_solve(moons_fft_t F, moons_field_t u, moons_field_t f) {
   // forward: along a (W -> V), along b (V -> W)
   if ((st = transform(F, F->a, F->C[F->a], F->W, F->V))) return st;
   if ((st = transform(F, F->b, F->C[F->b], F->V, F->W))) return st;
-  hipLaunchKernelGGL(k_fft_scale, dim3(fblocks(nc)), dim3(kFT), 0, stream(), F->W, F->coef, n0, n1, n2, F->a, F->b);
+  const int x_quirk = F->kind == MOONS_FFT_COEF_REFERENCE && F->dir == 0;
+  hipLaunchKernelGGL(k_fft_scale, dim3(fblocks(nc)), dim3(kFT), 0, stream(), F->W, F->coef, n0, n1, n2, F->a, F->b,
+                     x_quirk);
   MOONS_LAUNCH_CHECK();
   // inverse: along a (W -> V), along b (V -> W)
   if ((st = transform(F, F->a, F->Ci[F->a], F->W, F->V))) return st;

@@ -84,6 +84,7 @@ class Duct:
     def __init__(self, n=(128, 128, 128), beta=1.05, Re=1000.0, Ha=100.0, Rem=1.0, c_w=0.01, dt=1e-3,
                  prec_p="MG_FP32", seed=3, isp_p=PPE_ISP, isp_phi=PHI_ISP, isp_U=PRESET_ISP, isp_B=PRESET_ISP):
         self.n, self.beta, self.Re, self.Ha, self.Rem, self.c_w, self.dt = tuple(n), beta, Re, Ha, Rem, c_w, dt
+        self.isp = {"U": isp_U, "p": isp_p, "B": isp_B, "phi": isp_phi}
         self.hs = duct_nodes(self.n, beta)
         self.m = M.mesh(*self.hs)
         Ub, pb, Bb, phib = bc_sets(c_w)

@@ -56,7 +56,7 @@ module moons_amd_bindings
   public :: moons_apply_bcs, moons_grad, moons_div
   public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
   public :: moons_clean_div
-  public :: moons_fft_create, moons_fft_destroy, moons_fft_solve, moons_clean_div_fft
+  public :: moons_fft_create, moons_fft_destroy, moons_fft_solve, moons_clean_div_fft, moons_fft_set_coefficients
   public :: moons_field_copy, moons_apply_bcs_implicit, moons_curl, moons_interp, moons_advect_B
   public :: moons_pcg_vf_create, moons_pcg_vf_destroy, moons_pcg_vf_set_isp, moons_pcg_vf_update_mfp
   public :: moons_pcg_vf_solve
@@ -206,6 +206,11 @@ module moons_amd_bindings
       type(c_ptr), value :: m
       integer(c_int), value :: dir
       type(c_ptr), intent(out) :: fft
+    end function
+    integer(c_int) function moons_fft_set_coefficients(fft, kind) bind(C, name="moons_fft_set_coefficients")
+      import c_int, c_ptr
+      type(c_ptr), value :: fft
+      integer(c_int), value :: kind
     end function
     integer(c_int) function moons_fft_destroy(fft) bind(C, name="moons_fft_destroy")
       import c_int, c_ptr

@@ -80,16 +80,43 @@ def inv_eigenvalues(n, h, a, b) -> np.ndarray:
     return out
 
 
-def solve_interior(f_int: np.ndarray, h, direction: int) -> np.ndarray:
-    """u's interior (n0, n1, n2) from f's interior for transverse spacings h."""
+def reference_factors(n, h_first, direction: int) -> np.ndarray:
+    """init_FFT's mode factors (code/handwritten/FFT_solver/FFT_solver_extend.f90:62-112)
+    on the full interior (n0, n1, n2): 0.5 ha hb / (cos(pi p/Na) + cos(pi q/Nb) - 2)
+    with ha, hb the first spacings dhn(1) of the transverse directions; for
+    direction 1 the guard tests (i, j), so the modes (0, q) of the first
+    x-plane get 1; the (0, 0) modes, set to zero right after ("pin down
+    pressure", FFT_algorithm.f90:68-71), are 0 here."""
+    d, (a, b) = transverse(direction)
+    Na, Nb = n[a], n[b]
+    ca = np.cos(np.pi * np.arange(Na) / Na)
+    cb = np.cos(np.pi * np.arange(Nb) / Nb)
+    with np.errstate(divide="ignore"):
+        c2 = 0.5 * h_first[a] * h_first[b] / (ca[:, None] + cb[None, :] - 2.0)
+    c2[0, 0] = 0.0
+    shape = [1, 1, 1]
+    shape[a], shape[b] = Na, Nb
+    c = np.broadcast_to(c2.reshape(shape), n).copy()
+    if direction == 1:
+        c[0, 0, 1:] = 1.0
+    return c
+
+
+def solve_interior(f_int: np.ndarray, h, direction: int, h_first=None) -> np.ndarray:
+    """u's interior (n0, n1, n2) from f's interior.  h_first given: the
+    reference's mode factors from the first spacings; else the exact
+    eigenvalues for the transverse spacings h."""
     n = f_int.shape
     _, (a, b) = transverse(direction)
     y = np.moveaxis(np.tensordot(dct_matrix(n[a]), f_int, axes=([1], [a])), 0, a)
     y = np.moveaxis(np.tensordot(dct_matrix(n[b]), y, axes=([1], [b])), 0, b)
-    c = inv_eigenvalues(n, h, a, b)
-    shape = [1, 1, 1]
-    shape[a], shape[b] = n[a], n[b]
-    y = y * c.reshape(shape)
+    if h_first is not None:
+        y = y * reference_factors(n, h_first, direction)
+    else:
+        c = inv_eigenvalues(n, h, a, b)
+        shape = [1, 1, 1]
+        shape[a], shape[b] = n[a], n[b]
+        y = y * c.reshape(shape)
     y = np.moveaxis(np.tensordot(idct_matrix(n[a]), y, axes=([1], [a])), 0, a)
     y = np.moveaxis(np.tensordot(idct_matrix(n[b]), y, axes=([1], [b])), 0, b)
     return y

@@ -1,6 +1,7 @@
 """GPU parity of the cosine-transform direct solver and clean_div_FFT with
 the restatement (oracle/fft_oracle.py: transforms as cosine matrices, pinned
-against the reference's FFT-based DCT), and at 128^3 the size-independent
+against the reference's FFT-based DCT; the reference's mode factors by
+default, the exact eigenvalues as the opt-in), and at 128^3 the size-independent
 property: the solution satisfies the 5-point Neumann Laplacian of every plane
 with right-hand side f minus its plane mean.  The GPU transforms are FP64
 GEMMs, the restatement sums in another order, so fields are compared within
@@ -28,10 +29,17 @@ BCS = {
 }
 
 
+def first_spacings(hs):
+    return [float(hn[1] - hn[0]) for hn in hs]
+
+
+@pytest.mark.parametrize("coefficients", ["reference", "exact"])
 @pytest.mark.parametrize("direction", [1, 2, 3])
 @pytest.mark.parametrize("bc", list(BCS))
-def test_fft_solve_matches_restatement(gpu, direction, bc):
-    print(f"fft solve dir {direction} {bc}", flush=True)
+def test_fft_solve_matches_restatement(gpu, direction, bc, coefficients):
+    """Unequal transverse spacings, where the reference's mode factors and the
+    exact eigenvalues differ; both modes against their restatement."""
+    print(f"fft solve dir {direction} {bc} {coefficients}", flush=True)
     hs = grid((16, 12, 8), (1.0, 0.75, 1.25))
     g = O.Grid(*hs)
     m = gpu.mesh(*hs)
@@ -40,10 +48,11 @@ def test_fft_solve_matches_restatement(gpu, direction, bc):
     f = np.asfortranarray(O.splitmix(int(np.prod(g.shape(0))), 11 + direction).reshape(g.shape(0), order="F"))
     U = gpu.SF(m, 0).fill(0.0).set_BCs(types, vals, all_Neumann=bc == "neumann")
     F = gpu.SF(m, 0).upload(f)
-    S = gpu.FFT_solver_SF(m, direction)
+    S = gpu.FFT_solver_SF(m, direction, coefficients=coefficients)
     S.solve(U, F)
     ref = np.zeros(g.shape(0), order="F")
-    ref[1:-1, 1:-1, 1:-1] = FO.solve_interior(f[1:-1, 1:-1, 1:-1], h, direction)
+    h0 = first_spacings(hs) if coefficients == "reference" else None
+    ref[1:-1, 1:-1, 1:-1] = FO.solve_interior(f[1:-1, 1:-1, 1:-1], h, direction, h_first=h0)
     O.apply_bcs(g, ref, 0, O.BCs.make(types, vals))
     assert rel_l2(U.download(), ref) < 1e-12
 
@@ -68,10 +77,10 @@ def test_clean_div_fft_matches_restatement(gpu):
     ptypes = ["Neumann"] * 6
     scale = 1.0 / 1e-3
     direction = 3
-    # restated sequence (clean_divergence.f90:61-83)
+    # restated sequence (clean_divergence.f90:61-83), the reference's mode factors
     dv = O.div(g, us[0], us[1], us[2], 0) * scale
     phi_r = np.zeros(g.shape(0), order="F")
-    phi_r[1:-1, 1:-1, 1:-1] = FO.solve_interior(dv[1:-1, 1:-1, 1:-1], h, direction)
+    phi_r[1:-1, 1:-1, 1:-1] = FO.solve_interior(dv[1:-1, 1:-1, 1:-1], h, direction, h_first=first_spacings(hs))
     O.apply_bcs(g, phi_r, 0, O.BCs.make(ptypes))
     gr = O.grad(g, phi_r, 0)
     Xr = []
@@ -91,18 +100,21 @@ def test_clean_div_fft_matches_restatement(gpu):
         assert rel_l2(X[d].download(), Xr[d]) < 1e-12
 
 
-def test_fft_128_plane_laplacian(gpu):
-    """128^3, direction 1: the GPU solution satisfies the Neumann Laplacian of
-    every x-plane (the oracle's staggered derivatives) with f minus its plane
-    mean, and every plane of u has zero mean."""
-    print("fft 128", flush=True)
+@pytest.mark.parametrize("coefficients", ["reference", "exact"])
+def test_fft_128_plane_laplacian(gpu, coefficients):
+    """128^3, direction 1, equal transverse spacings: the GPU solution
+    satisfies the Neumann Laplacian of every x-plane (the oracle's staggered
+    derivatives) with f minus its plane mean, and every plane of u has zero
+    mean -- with the reference's factors on every plane but the first, whose
+    modes (0, q) the reference's guard leaves untouched."""
+    print(f"fft 128 {coefficients}", flush=True)
     N = 128
     hs = grid((N, N, N), (2.0, 1.0, 1.0))
     g = O.Grid(*hs)
     m = gpu.mesh(*hs)
     f = np.asfortranarray(O.splitmix(int(np.prod(g.shape(0))), 21).reshape(g.shape(0), order="F"))
     U = gpu.SF(m, 0).fill(0.0).set_BCs(["Neumann"] * 6, None, all_Neumann=True)
-    S = gpu.FFT_solver_SF(m, 1)
+    S = gpu.FFT_solver_SF(m, 1, coefficients=coefficients)
     S.solve(U, gpu.SF(m, 0).upload(f))
     u = np.asfortranarray(U.download())
     out = g.zeros(0)
@@ -112,5 +124,6 @@ def test_fft_128_plane_laplacian(gpu):
         O.del_op(g, out, 0, t, 1 << e, 1, e, 0, 1 if q == 0 else 2)
     fi = f[1:-1, 1:-1, 1:-1]
     expect = fi - fi.mean(axis=(1, 2), keepdims=True)
-    assert np.abs(out[1:-1, 1:-1, 1:-1] - expect).max() < 1e-9 * np.abs(fi).max()
+    first = 1 if coefficients == "reference" else 0
+    assert np.abs(out[1:-1, 1:-1, 1:-1][first:] - expect[first:]).max() < 1e-9 * np.abs(fi).max()
     assert np.abs(u[1:-1, 1:-1, 1:-1].mean(axis=(1, 2))).max() < 1e-12 * np.abs(u).max()

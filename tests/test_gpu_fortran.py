@@ -49,7 +49,8 @@ def test_fortran_driver_ppe(tmp_path, prec, tol, bar):
 
 def test_fortran_driver_fft(tmp_path):
     """init_FFT / solve from Fortran host arrays (ppe_driver prec 0): the x-plane
-    cosine-transform solve equals the restatement."""
+    cosine-transform solve, with the reference's mode factors, equals the
+    restatement."""
     from oracle import fft_oracle as FO
 
     if not os.path.exists(DRIVER):
@@ -68,7 +69,8 @@ def test_fortran_driver_fft(tmp_path):
     x = np.fromfile(xpath, dtype=np.float64).reshape(shape, order="F")
     h = float(np.diff(hx)[1])
     xr = np.zeros(shape, order="F")
-    xr[1:-1, 1:-1, 1:-1] = FO.solve_interior(b[1:-1, 1:-1, 1:-1], [h, h, h], 1)
+    h0 = float(hx[1] - hx[0])
+    xr[1:-1, 1:-1, 1:-1] = FO.solve_interior(b[1:-1, 1:-1, 1:-1], [h, h, h], 1, h_first=[h0, h0, h0])
     O.apply_bcs(g, xr, 0, O.BCs.make(["Neumann"] * 6))
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
 
