@@ -298,6 +298,19 @@ int moons_pcg_sf_update_mfp(moons_pcg_t pcg, double coeff_implicit_time_split, i
  * b the right-hand side, both device fields of the solver's location.  The
  * whole loop runs on the device; info may be NULL. */
 int moons_pcg_sf_solve(moons_pcg_t pcg, moons_field_t x, moons_field_t b, moons_pcg_info *info);
+/* The PPE solve of moons_pcg_sf_solve (Lap_uniform_SF with prec_Lap_SF, FP64
+ * Jacobi-PCG, ISP as moons_pcg_set_isp) without a solver object and without
+ * allocating: no counterpart in the reference; it is for callers that cannot
+ * allocate device memory (tight time-step loops, harnesses that forbid
+ * hipMalloc).  Its work vectors, block
+ * partial sums and control block live in one static device array of about
+ * 15.7 MB per GPU (held until the process ends): x may have at most 2^19
+ * padded cells (e.g. 78^3 interior), else MOONS_ERR_ARG.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (several host
+ * threads, or streams other than the library's) race; each call clears what
+ * it reads before use.  Not inside a graph capture (MOONS_ERR_STATE). */
+int moons_pcg_sf_solve_noalloc(moons_field_t x, moons_field_t b, int iter_max, double tol_rel, double tol_abs,
+                               int n_skip_check_res, moons_pcg_info *info);
 /* clean_div_PCG (code/handwritten/solvers/clean_divergence.f90:37-59):
  *   temp_CC = scale*div(Xstar); solve(PCG, phi, temp_CC);
  *   X = Xstar - grad(phi)/scale; apply_BCs(X).

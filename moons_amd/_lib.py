@@ -101,6 +101,8 @@ SIGNATURES = {
     "moons_nrg_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NrgInfo)]),
     "moons_nrg_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "moons_pcg_sf_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgInfo)]),
+    "moons_pcg_sf_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int,
+                                             C.POINTER(PcgInfo)]),
     "moons_clean_div": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p, C.c_double,
                                   C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(PcgInfo)]),
     "moons_pcg_time_kernels": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),

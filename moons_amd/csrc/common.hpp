@@ -193,6 +193,9 @@ int launch_interp(moons_field_s *out, const moons_field_s *in, int dir);
 void set_gate(const int *gate);
 const int *current_gate();
 int launch_plane_set(moons_field_s *u, int d, int idx, double val);
+// The device address of the allocation-free solve's scratch (pcg.hip), taken
+// once per device with that device current.
+int pcg_noalloc_setup(int device);
 int launch_plane_scale(moons_field_s *u, int d, int idx, double val);
 int launch_cross(moons_field_s *out, const moons_field_s *a, const moons_field_s *b, const moons_field_s *c,
                  const moons_field_s *d, int sign);

@@ -724,6 +724,7 @@ int moons_init(int device) {
   MOONS_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(MOONS_ERR_ARG, "moons_init: no such device");
   MOONS_HIP(hipSetDevice(device));
+  if (int st = pcg_noalloc_setup(device)) return st;
   if (g_stream && g_device == device) return MOONS_OK;
   if (g_stream) MOONS_HIP(hipStreamDestroy(g_stream));
   MOONS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));

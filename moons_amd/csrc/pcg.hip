@@ -29,9 +29,11 @@
 #include "unet.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 namespace moons {
 
@@ -587,6 +589,7 @@ struct moons_pcg_s {
   PcgCtrl *host_ctrl = nullptr; // pinned, read-backs
   PcgCtrl *host_init = nullptr; // pinned, initial control block (the source of every upload)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool noalloc = false;         // a one-shot solver over the static scratch (no host_init)
 };
 
 // Iterations to launch before the next read-back of the control block: the
@@ -607,9 +610,31 @@ static int next_chunk(const PcgCtrl &h, double tol_rel, int lo, int hi) {
   return std::max(lo, std::min(hi, chunk));
 }
 
+// The initial control block written by a kernel (the allocation-free solve).
+__global__ void k_pcg_ctrl_set(PcgCtrl *ctrl, int iter_max, int n_skip, double tol_rel, double tol_abs,
+                               double volsum) {
+  if (threadIdx.x != 0) return;
+  PcgCtrl h;
+  h.iter = h.done = h.exit_loop = h.pad_ = 0;
+  h.iter_max = iter_max;
+  h.n_skip = n_skip;
+  h.tol_rel = tol_rel;
+  h.tol_abs = tol_abs;
+  h.rho[0] = h.rho[1] = 0.0;
+  h.alpha = h.beta = h.res0 = h.res = h.mean = h.pq = h.rr = h.rz = 0.0;
+  h.volsum = volsum;
+  *ctrl = h;
+}
+
 // The initial control block lives in its own pinned buffer, written only
 // here, so a captured copy of it replays the same values.
 static int pcg_upload_ctrl(moons_pcg_s *S) {
+  if (S->noalloc) {
+    hipLaunchKernelGGL(k_pcg_ctrl_set, dim3(1), dim3(64), 0, stream(), S->ctrl, S->iter_max, S->n_skip, S->tol_rel,
+                       S->tol_abs, S->volsum);
+    MOONS_LAUNCH_CHECK();
+    return MOONS_OK;
+  }
   PcgCtrl h;
   std::memset(&h, 0, sizeof(PcgCtrl));
   h.iter_max = S->iter_max;
@@ -1121,6 +1146,121 @@ int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   }
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(e[i]);
   (void)hipFree(xs);
+  return MOONS_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// The allocation-free PPE solve: r, p, q, the block partial sums and the
+// control block of one Jacobi-PCG solve live in this file-scope array (about
+// 15.7 MB on every GPU the library runs on, held until the process ends).
+// Host code reaches it only through the per-device address taken at
+// moons_init (hipGetSymbolAddress); the array's name is not a device address
+// in host code.
+namespace {
+constexpr size_t kNaCells = size_t(1) << 19;   // padded cells of x at most (e.g. 78^3 interior)
+constexpr size_t kNaBlocks = size_t(1) << 16;  // tiles of the solve's launches at most
+constexpr size_t kNaCtrl = 32;                 // doubles reserved for the control block
+constexpr size_t kNaDoubles = kNaCtrl + 6 * kNaBlocks + 3 * kNaCells;
+constexpr int kNaMaxDevices = 64;
+static_assert(sizeof(PcgCtrl) <= kNaCtrl * sizeof(double), "control block does not fit its slot");
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+double *g_scratch_dev[kNaMaxDevices] = {};  // address of scratch on each device
+
+__global__ void __launch_bounds__(256) k_na_zero(double *a, size_t n) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) a[t] = 0.0;
+}
+} // namespace
+
+namespace moons {
+// Called by moons_init with the device current: the device address of scratch.
+int pcg_noalloc_setup(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return fail(MOONS_ERR_ARG, "moons_init: device index out of range");
+  if (g_scratch_dev[device]) return MOONS_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(MOONS_ERR_HIP, std::string("moons_init: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_dev[device] = static_cast<double *>(ptr);
+  return MOONS_OK;
+}
+} // namespace moons
+
+extern "C" {
+
+int moons_pcg_sf_solve_noalloc(moons_field_t x, moons_field_t b, int iter_max, double tol_rel, double tol_abs,
+                               int n_skip_check_res, moons_pcg_info *info) {
+  if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
+  if (!x || !b) return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve_noalloc: null argument");
+  if (sync_free()) return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve_noalloc: not inside a graph capture");
+  if (x->dl != MOONS_DL_CC || b->dl != MOONS_DL_CC || x->m != b->m)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve_noalloc: x and b must be cell-centred fields of one mesh");
+  if (iter_max < 1 || n_skip_check_res < 1)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve_noalloc: iter_max and n_skip must be >= 1");
+  if (x->n > kNaCells)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve_noalloc: the field has " + std::to_string(x->n) +
+                                   " padded cells, more than the scratch's " + std::to_string(kNaCells));
+  int dev = -1;
+  MOONS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices || !g_scratch_dev[dev])
+    return fail(MOONS_ERR_STATE, "moons_pcg_sf_solve_noalloc: no scratch on this device (call moons_init)");
+  moons_pcg_s S;
+  S.noalloc = true;
+  S.m = x->m;
+  S.xbc = *x;
+  S.xbc.d = nullptr;
+  S.op = MOONS_OP_LAP_UNIFORM_SF;
+  S.prec = MOONS_PREC_LAP_SF;
+  S.iter_max = iter_max;
+  S.tol_rel = tol_rel;
+  S.tol_abs = tol_abs;
+  S.n_skip = n_skip_check_res;
+  S.G = make_geom(x, pick_kz(x->s[2] - 2));
+  const size_t nb = (size_t)S.G.nblocks;
+  if (nb > kNaBlocks)
+    return fail(MOONS_ERR_ARG, "moons_pcg_sf_solve_noalloc: the field needs " + std::to_string(nb) +
+                                   " tiles, more than the scratch's " + std::to_string(kNaBlocks));
+  {
+    const HostCoords *c = x->m->hc;
+    double vs = 0.0;
+    for (int k = 1; k < x->s[2] - 1; ++k)
+      for (int j = 1; j < x->s[1] - 1; ++j)
+        for (int i = 1; i < x->s[0] - 1; ++i) vs += c[0].dhn[i] * c[1].dhn[j] * c[2].dhn[k];
+    S.volsum = vs;
+  }
+  double *base = g_scratch_dev[dev];
+  S.ctrl = reinterpret_cast<PcgCtrl *>(base);
+  S.partA = base + kNaCtrl;
+  S.partB = S.partA + kNaBlocks;
+  S.partM = S.partB + 2 * kNaBlocks;
+  S.partC = S.partM + kNaBlocks;
+  S.r = S.partC + 2 * kNaBlocks;
+  S.p = S.r + kNaCells;
+  S.q = S.p + kNaCells;
+  PcgCtrl hc;
+  std::memset(&hc, 0, sizeof hc);
+  S.host_ctrl = &hc;
+  // the previous call's vectors stay in the scratch: start from zero, as a new solver does
+  hipLaunchKernelGGL(k_na_zero, dim3((unsigned)((3 * kNaCells + 255) / 256)), dim3(256), 0, stream(), S.r,
+                     3 * kNaCells);
+  MOONS_LAUNCH_CHECK();
+  const auto t0 = std::chrono::steady_clock::now();
+  const int st = pcg_solve_impl(&S, x, b);
+  if (st) return st;
+  MOONS_HIP(hipStreamSynchronize(stream()));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (std::isnan(hc.res) || std::isnan(hc.res0))
+    return fail(MOONS_ERR_NAN, "Error: NaN in PCG residual (check_nans, PCG_solver_algorithm.f90:223-248)");
+  if (info) {
+    info->iterations = hc.iter;
+    info->exit_loop = hc.exit_loop;
+    info->res_norm0 = hc.res0;
+    info->res_norm = hc.res;
+    info->ms = ms;
+  }
   return MOONS_OK;
 }
 
