@@ -4,7 +4,9 @@ Workload `ppe_128_duct`: 128^3 cells, x periodic (uniform), y and z Roberts
 beta=1.05 walls (the rectangular-duct mesh), homogeneous Neumann walls, right-
 hand side (1/dt) div(U*) of a seeded 1e-3 SplitMix64 face-noise field, x0 = 0.
 One step = one PPE solve (x reset, modify_RHS, PCG loop, BCs on x), inputs
-resident in HBM.  Prints one JSON line.
+resident in HBM; `value` is the learned + mixed solver's time (the metric's
+method; --no-learned: the mixed solver's), beside the plain FP64 Jacobi-PCG
+and the mixed solver without the network.  Prints one JSON line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -75,41 +77,71 @@ def load_traffic(name):
     return None
 
 
+def host_cpu():
+    """(cores this process may run on, nproc, CPU model name)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, os.cpu_count() or cores, model
+
+
 def cpu_baseline(hs, bh, gpu_iters):
     """The oracle's FP64 Jacobi-PCG (the reference's operator sequence in C,
-    OpenMP over the host's cores, at most 16) on the same mesh and right-hand
-    side: 2 and 10 iterations timed, extrapolated linearly to the GPU's
-    iteration count; the same on one core beside it."""
+    OpenMP) on the same mesh and right-hand side, solved to the same 1e-8 on
+    every core this process may use; one core beside it on a bounded sample
+    (10 iterations, extrapolated to the solve's iteration count)."""
     from oracle import oracle as O
 
     hx, hy = hs
     g = O.Grid(hx, hy, hy)
     bcs = O.BCs.make(["Periodic", "Periodic", "Neumann", "Neumann", "Neumann", "Neumann"])
     b = np.asfortranarray(bh)
-    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1))
+    cores, nproc, model = host_cpu()
+    O.set_threads(cores)
+    x = np.zeros(g.shape(0), order="F")
+    t0 = time.perf_counter()
+    it, _, _, _ = O.pcg_lap(g, 0, bcs, True, x, b, 20000, TOL, 1e-300)
+    full_ms = 1e3 * (time.perf_counter() - t0)
+    O.set_threads(1)
+    ts = {}
+    for k in (2, 10):
+        x1 = np.zeros(g.shape(0), order="F")
+        t0 = time.perf_counter()
+        O.pcg_lap(g, 0, bcs, True, x1, b, k, 0.0, 0.0)
+        ts[k] = time.perf_counter() - t0
+    per_iter = (ts[10] - ts[2]) / 8.0
+    one_ms = 1e3 * (max(ts[2] - 2 * per_iter, 0.0) + per_iter * it)
+    return {"value": round(full_ms, 3), "unit": "ms", "cores": cores, "kind": "port",
+            "sample": f"the oracle's FP64 Jacobi-PCG solve of the same 128^3 PPE to 1e-8 ({it} iterations; the GPU's "
+                      f"FP64 solve took {gpu_iters}), OpenMP on the {cores} cores this process may use",
+            "iterations": it, "gpu_fp64_iterations": gpu_iters, "nproc": nproc, "cpu_model": model,
+            "omp_threads": cores,
+            "single_core": {"value": round(one_ms, 3), "unit": "ms", "cores": 1,
+                            "sample": f"2 and 10 iterations timed ({ts[2]:.2f} s, {ts[10]:.2f} s), extrapolated to "
+                                      f"{it} iterations; {1e3 * per_iter:.1f} ms per iteration"}}
 
-    def timed(nt):
-        O.set_threads(nt)
-        ts = {}
-        for it in (2, 10):
-            x = np.zeros(g.shape(0), order="F")
-            t0 = time.perf_counter()
-            O.pcg_lap(g, 0, bcs, True, x, b, it, 0.0, 0.0)
-            ts[it] = time.perf_counter() - t0
-        O.set_threads(1)
-        per_iter = (ts[10] - ts[2]) / 8.0
-        setup = max(ts[2] - 2 * per_iter, 0.0)
-        return 1e3 * (setup + per_iter * gpu_iters), ts, per_iter
 
-    ms1, ts1, pi1 = timed(1)
-    msn, tsn, pin = timed(threads)
-    return {"value": round(msn, 3), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 Jacobi-PCG (OpenMP, {threads} threads) on the same 128^3 mesh and RHS, 2 and 10 "
-                      f"iterations timed ({tsn[2]:.2f}s, {tsn[10]:.2f}s) and extrapolated to {gpu_iters} iterations; "
-                      f"{1e3 * pin:.1f} ms/iteration",
-            "single_core": {"value": round(ms1, 3), "unit": "ms", "cores": 1,
-                            "sample": f"2 and 10 iterations ({ts1[2]:.2f}s, {ts1[10]:.2f}s); "
-                                      f"{1e3 * pi1:.1f} ms/iteration"}}
+def load_rocprof_avg(name, kernel_prefix):
+    """Average duration (us) of a kernel in a committed rocprofv3 summary
+    (lines `kernel: <name> avg_us: <t>`), or None."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        for line in fh:
+            if line.startswith("kernel:") and kernel_prefix in line and "avg_us:" in line:
+                return float(line.rsplit("avg_us:", 1)[1].split()[0])
+    return None
 
 
 def run_ensemble(M, args):
@@ -129,6 +161,25 @@ def run_ensemble(M, args):
     sizes = {}
     for mb in ens.members:
         sizes[mb.N] = sizes.get(mb.N, 0) + 1
+    # per-member error against the plain FP64 path: the same members stepped
+    # eagerly (no graph, one stream) for the same number of steps
+    plain = Ensemble(n=64, sizes=(32, 48, 64, 96), seed=5, dt=5e-4, n_streams=1, capture=False)
+    for mb in plain.members:
+        mb.st.step(args.warmup + args.steps)
+    M.synchronize()
+
+    def rel(a, b):
+        den = float(np.linalg.norm(b))
+        return float(np.linalg.norm(a - b)) / (den if den > 0 else 1.0)
+
+    errs = []
+    for a, b in zip(ens.members, plain.members):
+        e = max([rel(a.U[d].download(), b.U[d].download()) for d in range(3)] +
+                [rel(a.B[d].download(), b.B[d].download()) for d in range(3)] +
+                [rel(a.p.download(), b.p.download())])
+        errs.append(e)
+    per_member = [{"N": mb.N, "Re": round(mb.Re, 1), "Ha": round(mb.Ha, 2), "rel_l2": e}
+                  for mb, e in zip(ens.members, errs)]
     out = {"metric": "ensemble MHD time steps (member-steps) per second", "value": round(64 * 1e3 / ms, 2),
            "unit": "member-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -136,7 +187,10 @@ def run_ensemble(M, args):
            "config": {"workload": "ensemble_64_mhd_cavity", "members": 64, "sizes": {str(k): v for k, v in sorted(sizes.items())},
                       "cells": ens.cells, "Re": [100, 1000], "Ha": [0, 50], "dt": 5e-4, "solver_iterations": 5,
                       "execution": "per-member HIP graph, 4 streams"},
-           "cell_steps_per_s": round(ens.cells * 1e3 / ms, 1), "setup_s": round(setup_s, 1)}
+           "cell_steps_per_s": round(ens.cells * 1e3 / ms, 1), "setup_s": round(setup_s, 1),
+           "per_member_error_vs_plain_fp64": {"max_rel_l2": max(errs), "fields": "U, B, p",
+                                              "plain": "the same members stepped eagerly, no graph",
+                                              "members": per_member}}
     print(json.dumps(out), flush=True)
 
 
@@ -214,6 +268,33 @@ def run_mhd_md(M, args):
                       "Re": 400, "Ha": 10, "Rem": 1, "dt": 5e-4, "sigma": "1 on fluid + 1-cell wall, 0.1 beyond",
                       "ppe": "mixed FP32 V-cycle in FP64 FPCG to 1e-8", "other_solves": "5 iterations",
                       "execution": "eager, one stream"},
+           "last_step_iterations": info["iterations"]}
+    print(json.dumps(out), flush=True)
+
+
+def run_duct(M, args):
+    """The duct MHD flow at N^3 (x periodic, Roberts beta 1.05 walls in y and
+    z, Re 1000, Ha 100, thin conducting walls c_w 0.01, mean pressure gradient,
+    finite Rem, dt 1e-3): the PPE by the mixed solver to 1e-8, phi to 1e-12,
+    U and B with the reference's 5-iteration settings.  One step = one time
+    step, eager on the library stream."""
+    from moons_amd.duct import Duct
+
+    N = args.duct
+    D = Duct(n=(N, N, N), prec_p="MG_FP32")
+    D.step(args.warmup)
+    M.synchronize()
+    t0 = time.perf_counter()
+    info = D.step(args.steps)
+    M.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    out = {"metric": "duct MHD time steps per second", "value": round(1e3 / ms, 2), "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"duct_mhd_{N}", "cells": f"{N}^3", "Re": 1000, "Ha": 100, "c_w": 0.01,
+                      "mesh": "x periodic uniform [-1,1]; y,z Roberts beta=1.05 on [-1,1]", "dt": 1e-3,
+                      "ppe": "mixed FP32 V-cycle in FP64 FPCG to 1e-8", "phi": "Jacobi-PCG to 1e-12",
+                      "U, B": "5 iterations", "execution": "eager, one stream"},
            "last_step_iterations": info["iterations"]}
     print(json.dumps(out), flush=True)
 
@@ -314,6 +395,8 @@ def main():
                     help="MHD time-step benchmark instead: one lid-driven MHD cavity at N^3, eager")
     ap.add_argument("--mhd-md", type=int, default=0, metavar="N",
                     help="the MHD benchmark with the induction on the N^3 fluid mesh extended by 11 cells a wall")
+    ap.add_argument("--duct", type=int, default=0, metavar="N",
+                    help="duct MHD time-step benchmark instead (thin conducting walls, Ha 100), at N^3")
     ap.add_argument("--fft", type=int, default=0, metavar="N",
                     help="cosine-transform direct PPE solver benchmark instead, at N^3")
     ap.add_argument("--energy", type=int, default=0, metavar="N",
@@ -340,6 +423,9 @@ def main():
         return
     if args.mhd_md:
         run_mhd_md(M, args)
+        return
+    if args.duct:
+        run_duct(M, args)
         return
     if args.fft:
         run_fft(M, args)
@@ -404,9 +490,12 @@ def main():
     Pc = M.PCG_solver_SF(m, xc, M.iter_solver_params(2000, TOL, 1e-300, 1), prec="MG_FP32")
     Pc.set_MG(4, 2, 2, 40)
     mg4_ms, mg4_info = timed(Pc, xc)
-    # learned + mixed: U-net (trained here on 32^3 solver-generated pairs) as
-    # the initial guess of the 32^3 level inside the FP32 V-cycle
+    # learned + mixed, the metric's method: the U-net (trained here, on the
+    # GPU, on pairs from random right-hand sides on the 32^3 level of this
+    # mesh and from cavity runs with Re in [100, 1000], Ha in [0, 100]) as the
+    # initial guess of the 32^3 level inside the FP32 V-cycle
     learned_out = None
+    Pl = xl = None
     if not args.no_learned:
         t_l0 = time.perf_counter()
         from moons_amd import learned
@@ -420,7 +509,7 @@ def main():
         for _ in range(2):
             lx, ly = coarsen(lx), coarsen(ly)
         pairs = learned.random_pairs([lx, ly, ly], types, 128, seed=2)
-        pairs += learned.run_pairs(32, 4, 12, seed=3)
+        pairs += learned.run_pairs(32, 4, 12, seed=3, Ha_max=100.0)
         net = M.UNet(seed=0)
         hist = net.train([p[0] for p in pairs], [p[1] for p in pairs], epochs=30, batch=4, lr=1e-3, seed=0)
         t_train = time.perf_counter() - t_l0
@@ -432,41 +521,83 @@ def main():
                        "final_rel_residual": l_info.res_norm / l_info.res_norm0,
                        "placement": "U-net initial guess of multigrid level 2 (32^3) inside the FP32 V-cycle",
                        "training": {"pairs": len(pairs), "grid": "32^3", "epochs": 30, "seconds": round(t_train, 1),
-                                    "loss_first": round(float(hist[0]), 4), "loss_last": round(float(hist[-1]), 4)}}
-    # agreement of the two solutions (free constant removed, interior cells)
+                                    "sources": "random right-hand sides on the 32^3 level of this mesh; MHD cavity "
+                                               "runs with Re in [100, 1000], Ha in [0, 100]",
+                                    "loss_first": round(float(hist[0]), 4), "loss_last": round(float(hist[-1]), 4)},
+                       "gain_over_mixed": {"ms_without_network": round(mg_ms, 4),
+                                           "iterations_without_network": mg_info.iterations,
+                                           "speedup": round(mg_ms / l_ms, 3), "pays": bool(l_ms < mg_ms)}}
+
+    # every solution against the FP64 Jacobi-PCG converged to 1e-11 (constant removed)
+    xr = M.SF(m, "CC").fill(0.0).set_BCs(types, None, all_Neumann=True)
+    Pr = M.PCG_solver_SF(m, xr, M.iter_solver_params(20000, 1e-11, 1e-300, 1))
+    conv_info = Pr.solve(xr, b)
+
+    def centered(f):
+        a = f.download()[1:-1, 1:-1, 1:-1]
+        return a - a.mean()
+
+    xc = centered(xr)
+
+    def vs_converged(f):
+        return float(np.linalg.norm(centered(f) - xc) / np.linalg.norm(xc))
+
+    agreement = {"reference": f"FP64 Jacobi-PCG to 1e-11 ({conv_info.iterations} iterations)",
+                 "plain_fp64": vs_converged(x), "mixed": vs_converged(xm)}
+    if Pl is not None:
+        agreement["learned_mixed"] = vs_converged(xl)
     xa, xb = x.download()[1:-1, 1:-1, 1:-1], xm.download()[1:-1, 1:-1, 1:-1]
     xa, xb = xa - xa.mean(), xb - xb.mean()
     mg_vs_fp64 = float(np.linalg.norm(xb - xa) / np.linalg.norm(xa))
 
-    # per-kernel durations (HIP events on the library stream)
-    kms = P.time_kernels(50)
+    # in-situ kernel durations: HIP events on the library stream around every
+    # launch of the dominant kernel, over one more pass of the measured
+    # method's timed solves (that pass is not in `value`)
+    Pmeas, xmeas = (Pl, xl) if Pl is not None else (Pm, xm)
+    nk = max(1, min(args.steps, 5))
+    with M.ktimer("mg_line_fine") as kt:
+        for _ in range(nk):
+            xmeas.fill(0.0)
+            Pmeas.solve(xmeas, b)
+    M.synchronize()
+    with M.ktimer("pcg_update") as kf:
+        x.fill(0.0)
+        P.solve(x, b)
+    M.synchronize()
+
     ncell = N ** 3
+    # FP64 PCG kernels, 50 back-to-back launches each (the dispatch gap included)
+    kms = P.time_kernels(50)
     alg = {"pcg_matvec": 16 * ncell, "pcg_update": 48 * ncell, "pcg_pupdate": 24 * ncell}
     names = list(alg)
-    dom = max(range(3), key=lambda i: kms[i])
-    dname = names[dom]
-    achieved = alg[dname] / (kms[dom] * 1e-3) / 1e9
+    upd_gbs = alg["pcg_update"] / (kf.avg_ms * 1e-3) / 1e9
     iter_us = 1e3 * sum(kms)
-    fp64_roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(f"{dname}_128_traffic.txt"),
-                     "per_kernel_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
-                     "per_kernel_GBs": {n: round(alg[n] / (t * 1e-3) / 1e9, 1) for n, t in zip(names, kms)},
+    upd_rp = load_rocprof_avg("pcg_update_128_trace.txt", "k_pcg_update")
+    fp64_roofline = {"bound": "hbm", "kernel": "pcg_update", "achieved": round(upd_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(upd_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic("pcg_update_128_traffic.txt"),
+                     "in_situ_us": round(1e3 * kf.avg_ms, 3), "in_situ_launches": kf.launches,
+                     "rocprof_avg_us": upd_rp,
+                     "frac_rocprof": round(alg["pcg_update"] / (upd_rp * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if upd_rp else None,
+                     "back_to_back_us": {n: round(1e3 * t, 2) for n, t in zip(names, kms)},
                      "iteration_GBs": round(88 * ncell / (iter_us * 1e-6) / 1e9, 1)}
     # the mixed solver's dominant kernel: the finest level's zebra line
-    # half-sweep (y- and z-lines, 4 (nu1 + nu2) launches per V-cycle; timed:
-    # the y-lines of one colour), 6 B per cell of the level: the other
-    # colour's x (2 B), this colour's f (2 B) and x written (2 B), FP32
+    # half-sweep (y- and z-lines, 4 (nu1 + nu2) launches per V-cycle), 6 B per
+    # cell of the level: the other colour's x (2 B), this colour's f (2 B) and
+    # x written (2 B), FP32
     mk = Pm.time_MG(50)
     sm_bytes = 6 * ncell
-    sm_gbs = sm_bytes / (mk[0] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "mg_line (128^3 zebra y-line half-sweep)", "achieved": round(sm_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sm_gbs / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic("mg_line_128_traffic.txt"),
-                "algorithmic_bytes": sm_bytes,
-                "per_kernel_us": {"mg_line_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
-                                  "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2),
-                                  "mg_line_64": round(1e3 * mk[6], 2), "mg_fused_sweep_32": round(1e3 * mk[9], 2)},
+    sm_gbs = sm_bytes / (kt.avg_ms * 1e-3) / 1e9
+    line_rp = load_rocprof_avg("mg_line_128_trace.txt", "k_mg_line")
+    roofline = {"bound": "hbm", "kernel": "mg_line (128^3 zebra line half-sweep, y and z)",
+                "achieved": round(sm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(sm_gbs / HBM_PEAK_GBS, 4), "traffic": load_traffic("mg_line_128_traffic.txt"),
+                "algorithmic_bytes": sm_bytes, "in_situ_us": round(1e3 * kt.avg_ms, 3),
+                "in_situ_launches": kt.launches, "rocprof_avg_us": line_rp,
+                "frac_rocprof": round(sm_bytes / (line_rp * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if line_rp else None,
+                "back_to_back_us": {"mg_line_128": round(1e3 * mk[0], 2), "mg_restrict_128_to_64": round(1e3 * mk[1], 2),
+                                    "mg_prolong_64_to_128": round(1e3 * mk[2], 2), "mg_tail": round(1e3 * mk[3], 2),
+                                    "mg_line_64": round(1e3 * mk[6], 2), "mg_fused_sweep_32": round(1e3 * mk[9], 2)},
                 "vcycle_us": round(1e3 * mk[4], 2),
                 "mixed_iteration_us": round(1e3 * mg_ms / max(mg_info.iterations, 1), 1)}
 
@@ -482,11 +613,10 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(hs, bh, iters)
 
-    cands = {"fp64_jacobi_pcg": (ms_step, iters), "mixed_mg_fp32_in_fp64_fpcg": (mg_ms, mg_info.iterations)}
     if learned_out:
-        cands["learned_mixed_unet_mg_fp32_in_fp64_fpcg"] = (learned_out["ms"], learned_out["iterations"])
-    method = min(cands, key=lambda k: cands[k][0])
-    best = cands[method][0]
+        method, best, best_iters = "learned_mixed_unet_mg_fp32_in_fp64_fpcg", learned_out["ms"], learned_out["iterations"]
+    else:
+        method, best, best_iters = "mixed_mg_fp32_in_fp64_fpcg", mg_ms, mg_info.iterations
     out = {
         "metric": METRIC,
         "value": round(best, 4),
@@ -502,7 +632,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "ppe_128_duct", "cells": "128^3", "mesh": "x periodic uniform; y,z Roberts beta=1.05",
                    "bcs": "homogeneous Neumann walls, all_Neumann", "rhs": "(1/dt) div(U*), U* = 1e-3 SplitMix64 face noise",
-                   "tol_rel": TOL, "method": method, "iterations": cands[method][1]},
+                   "tol_rel": TOL, "method": method, "iterations": best_iters},
+        "speedup_vs_plain_fp64": round(ms_step / best, 3),
         "plain_fp64_ms": round(ms_step, 4),
         "plain_fp64_iterations": iters,
         "plain_fp64_final_rel_residual": rel,
@@ -512,6 +643,7 @@ def main():
         "mixed_mg_levels": mg_levels,
         "mixed_mg_coarsest_cells": list(mg_coarsest),
         "mixed_vs_fp64_solution_rel_l2": mg_vs_fp64,
+        "solution_vs_converged_fp64_rel_l2": agreement,
         "mixed_mg_4levels_2x2_ms": round(mg4_ms, 4),
         "mixed_mg_4levels_2x2_iterations": mg4_info.iterations,
         "speedup_mixed_vs_plain": round(ms_step / mg_ms, 3),

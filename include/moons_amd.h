@@ -122,6 +122,16 @@ typedef struct {
 int moons_init(int device);
 /* Extra HIP streams (default priority) for replaying captured steps of
  * independent states concurrently; synchronize(NULL) waits for the library stream. */
+/* In-situ kernel timer (no reference counterpart; instrumentation): while
+ * armed, an event pair is recorded on the library stream around each launch
+ * of the kernel class `which`, up to max_launches; stop returns the number of
+ * launches timed and their mean duration. */
+#define MOONS_KTIMER_MG_LINE_FINE 1 /* zebra line half-sweeps of the finest multigrid level */
+#define MOONS_KTIMER_PCG_MATVEC 2   /* FP64 PCG kernels of the PPE */
+#define MOONS_KTIMER_PCG_UPDATE 3
+#define MOONS_KTIMER_PCG_PUPDATE 4
+int moons_ktimer_start(int which, int max_launches);
+int moons_ktimer_stop(int *launches, double *avg_ms);
 int moons_stream_create(void **out);
 int moons_stream_destroy(void *hip_stream);
 int moons_stream_synchronize(void *hip_stream);

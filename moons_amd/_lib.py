@@ -53,6 +53,8 @@ SIGNATURES = {
     "moons_init": (C.c_int, [C.c_int]),
     "moons_finalize": (C.c_int, []),
     "moons_last_error": (C.c_char_p, []),
+    "moons_ktimer_start": (C.c_int, [C.c_int, C.c_int]),
+    "moons_ktimer_stop": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "moons_synchronize": (C.c_int, []),
     "moons_stream": (C.c_void_p, []),
     "moons_nodes_uniform": (C.c_int, [C.c_double, C.c_double, C.c_int, c_double_p]),

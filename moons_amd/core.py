@@ -35,6 +35,29 @@ def synchronize() -> None:
     check(load().moons_synchronize())
 
 
+KTIMERS = {"mg_line_fine": 1, "pcg_matvec": 2, "pcg_update": 3, "pcg_pupdate": 4}
+
+
+class ktimer:
+    """In-situ kernel timer: `with ktimer("mg_line_fine") as t: ...` times
+    every launch of that kernel class on the library stream inside the block;
+    afterwards t.launches and t.avg_ms hold the count and mean duration."""
+
+    def __init__(self, which: str, max_launches: int = 1 << 16):
+        self.which, self.max = KTIMERS[which], int(max_launches)
+        self.launches, self.avg_ms = 0, 0.0
+
+    def __enter__(self):
+        check(load().moons_ktimer_start(self.which, self.max))
+        return self
+
+    def __exit__(self, *exc):
+        n, a = C.c_int(0), C.c_double(0.0)
+        check(load().moons_ktimer_stop(C.byref(n), C.byref(a)))
+        self.launches, self.avg_ms = n.value, a.value
+        return False
+
+
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 

@@ -18,6 +18,9 @@ int fail(int code, const std::string &msg);
 hipStream_t stream();
 bool sync_free();
 void set_sync_free(bool on);
+// Records the next event of the armed kernel timer when `which` is the kernel
+// class it times (moons_ktimer_start); call before and after each launch.
+void ktimer_mark(int which);
 
 #define MOONS_HIP(call)                                                                  \
   do {                                                                                   \

@@ -32,6 +32,16 @@ static bool g_sync_free = false;
 bool sync_free() { return g_sync_free; }
 void set_sync_free(bool on) { g_sync_free = on; }
 
+// In-situ kernel timer: event pairs recorded on the library stream around
+// every launch of one kernel class while it is armed.
+static int g_kt_which = 0;
+static std::vector<hipEvent_t> g_kt_ev;
+static size_t g_kt_used = 0;
+void ktimer_mark(int which) {
+  if (which == 0 || which != g_kt_which || g_kt_used >= g_kt_ev.size() || g_sync_free) return;
+  (void)hipEventRecord(g_kt_ev[g_kt_used++], g_stream);
+}
+
 // ------------------------------------------------------------------ 1D grids
 static void add_ghost_nodes(const std::vector<double> &h, double *out) {
   size_t np = h.size();
@@ -729,6 +739,47 @@ int moons_init(int device) {
   if (g_stream) MOONS_HIP(hipStreamDestroy(g_stream));
   MOONS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
+  return MOONS_OK;
+}
+
+int moons_ktimer_start(int which, int max_launches) {
+  if (which < 1 || max_launches < 1) return fail(MOONS_ERR_ARG, "moons_ktimer_start: bad argument");
+  if (g_kt_which) return fail(MOONS_ERR_STATE, "moons_ktimer_start: a timer is already armed");
+  g_kt_ev.assign((size_t)2 * max_launches, nullptr);
+  for (auto &e : g_kt_ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto &f : g_kt_ev)
+        if (f) (void)hipEventDestroy(f);
+      g_kt_ev.clear();
+      return fail(MOONS_ERR_HIP, "moons_ktimer_start: hipEventCreate failed");
+    }
+  g_kt_used = 0;
+  g_kt_which = which;
+  return MOONS_OK;
+}
+
+int moons_ktimer_stop(int *launches, double *avg_ms) {
+  if (!g_kt_which) return fail(MOONS_ERR_STATE, "moons_ktimer_stop: no timer armed");
+  double sum = 0.0;
+  const size_t pairs = g_kt_used / 2;
+  int st = MOONS_OK;
+  if (pairs > 0 && hipEventSynchronize(g_kt_ev[2 * pairs - 1]) != hipSuccess)
+    st = fail(MOONS_ERR_HIP, "moons_ktimer_stop: hipEventSynchronize failed");
+  for (size_t q = 0; q < pairs && !st; ++q) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_kt_ev[2 * q], g_kt_ev[2 * q + 1]) != hipSuccess) {
+      st = fail(MOONS_ERR_HIP, "moons_ktimer_stop: hipEventElapsedTime failed");
+      break;
+    }
+    sum += ms;
+  }
+  for (auto &e : g_kt_ev) (void)hipEventDestroy(e);
+  g_kt_ev.clear();
+  g_kt_which = 0;
+  g_kt_used = 0;
+  if (st) return st;
+  if (launches) *launches = (int)pairs;
+  if (avg_ms) *avg_ms = pairs ? sum / (double)pairs : 0.0;
   return MOONS_OK;
 }
 

@@ -1266,24 +1266,30 @@ static void line_half(const LevelDev &L, int D, int c, bool zero_guess, const in
 // One line sweep of level L: pre-smoothing (forward) y-lines colour 0, 1,
 // then z-lines colour 0, 1; post-smoothing (backward) the reverse order, so
 // the V-cycle stays a symmetric preconditioner.
-static void line_sweep(LevelDev &L, bool forward, bool zero_first, const int *done) {
+// timer: the kernel-timer class of these launches (0: none).
+static void line_sweep(LevelDev &L, bool forward, bool zero_first, const int *done, int timer) {
   int ds[2], n = 0;
   if (L.line1) ds[n++] = 1;
   if (L.line2) ds[n++] = 2;
+  auto half = [&](int D, int c, bool zg) {
+    ktimer_mark(timer);
+    line_half(L, D, c, zg, done);
+    ktimer_mark(timer);
+  };
   if (forward) {
     for (int a = 0; a < n; ++a)
-      for (int c = 0; c < 2; ++c) line_half(L, ds[a], c, zero_first && a == 0 && c == 0, done);
+      for (int c = 0; c < 2; ++c) half(ds[a], c, zero_first && a == 0 && c == 0);
   } else {
     for (int a = n - 1; a >= 0; --a)
-      for (int c = 1; c >= 0; --c) line_half(L, ds[a], c, false, done);
+      for (int c = 1; c >= 0; --c) half(ds[a], c, false);
   }
 }
 
 // One red-black sweep of level L, colour c_first first (lines, where the
 // level has line directions: c_first 0 is the forward order).
-static void sweep(LevelDev &L, int c_first, bool zero_first, const int *done) {
+static void sweep(LevelDev &L, int c_first, bool zero_first, const int *done, int timer = 0) {
   if (L.line1 || L.line2) {
-    line_sweep(L, c_first == 0, zero_first, done);
+    line_sweep(L, c_first == 0, zero_first, done, timer);
     return;
   }
   if ((long long)L.n0 * L.n1 * L.n2 <= kFusedSweepMaxCells) {
@@ -1323,7 +1329,7 @@ int mg_vcycle(MGHier *h, const int *done) {
       hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
       hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
     }
-    for (int s = 0; s < h->nu1; ++s) sweep(L, 0, s == 0 && !learned, done);
+    for (int s = 0; s < h->nu1; ++s) sweep(L, 0, s == 0 && !learned, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
     restrict_to(L, h->L[l + 1], done);
   }
   if (tail)
@@ -1335,7 +1341,7 @@ int mg_vcycle(MGHier *h, const int *done) {
   for (int l = g - 1; l >= 0; --l) {
     LevelDev &L = h->L[l];
     prolong_from(L, h->L[l + 1], done);
-    for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done);
+    for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));

@@ -685,21 +685,31 @@ static void pcg_launch_kz(moons_pcg_s *S, double *x, int it, int which) {
   float *f0 = mg ? flex_f0(S) : nullptr;
   const F32Map M = flex_map(S);
   const NrgK K = nrg_k(S);
+  // the kernel timer times the plain (Jacobi) PPE's kernels only
+  const int tm = (!mg && S->op == MOONS_OP_LAP_UNIFORM_SF) ? 1 : 0;
   if (which & 1) {
+    ktimer_mark(tm * MOONS_KTIMER_PCG_MATVEC);
     if (S->op == MOONS_OP_NRG_DIFFUSION)
       hipLaunchKernelGGL((k_pcg_matvec<KZ, true>), grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA, K);
     else
       hipLaunchKernelGGL((k_pcg_matvec<KZ, false>), grid, block, 0, stream(), S->p, S->q, S->G, S->ctrl, S->partA, K);
+    ktimer_mark(tm * MOONS_KTIMER_PCG_MATVEC);
   }
   if (which & 2) {
+    ktimer_mark(tm * MOONS_KTIMER_PCG_UPDATE);
     if (mg)
       hipLaunchKernelGGL((k_pcg_update<KZ, true>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
                          S->partA, S->partB, it, f0, M);
     else
       hipLaunchKernelGGL((k_pcg_update<KZ, false>), grid, block, 0, stream(), x, S->p, S->r, S->q, S->G, S->ctrl,
                          S->partA, S->partB, it, f0, M);
+    ktimer_mark(tm * MOONS_KTIMER_PCG_UPDATE);
   }
-  if (which & 4) hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
+  if (which & 4) {
+    ktimer_mark(tm * MOONS_KTIMER_PCG_PUPDATE);
+    hipLaunchKernelGGL(k_pcg_pupdate<KZ>, grid, block, 0, stream(), S->p, S->r, S->G, S->ctrl, S->partB, it);
+    ktimer_mark(tm * MOONS_KTIMER_PCG_PUPDATE);
+  }
   if (which & 8) hipLaunchKernelGGL(k_mg_zdots<KZ>, grid, block, 0, stream(), flex_z0(S), S->r, S->q, S->G, S->ctrl,
                                     S->partC, it > 0 ? 1 : 0, M);
   if (which & 16) hipLaunchKernelGGL(k_mg_pupdate<KZ>, grid, block, 0, stream(), S->p, flex_z0(S), S->G, S->ctrl,

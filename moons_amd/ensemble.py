@@ -49,11 +49,15 @@ class Member:
 
 class Ensemble:
     def __init__(self, n=64, sizes=(32, 48, 64, 96), Re=(100.0, 1000.0), Ha=(0.0, 50.0), seed=5, dt=5e-4,
-                 isp=(5, 1e-6, 1e-13, 1), n_streams=4, capture=True):
+                 isp=(5, 1e-6, 1e-13, 1), n_streams=4, capture=True, member_sizes=None):
+        """n members with N drawn uniformly from sizes (or exactly member_sizes),
+        Re and Ha uniform in their ranges, all seeded by seed."""
         rng = np.random.default_rng(seed)
         self.members = []
+        if member_sizes is not None:
+            n = len(member_sizes)
         for q in range(n):
-            N = int(rng.choice(sizes))
+            N = int(member_sizes[q]) if member_sizes is not None else int(rng.choice(sizes))
             self.members.append(Member(N, float(rng.uniform(*Re)), float(rng.uniform(*Ha)), seed * 1000 + q, dt, isp))
         self.streams = [M.Stream() for _ in range(n_streams)]
         if capture:
