@@ -123,9 +123,11 @@ int moons_init(int device);
 /* Extra HIP streams (default priority) for replaying captured steps of
  * independent states concurrently; synchronize(NULL) waits for the library stream. */
 /* In-situ kernel timer (no reference counterpart; instrumentation): while
- * armed, an event pair is recorded on the library stream around each launch
- * of the kernel class `which`, up to max_launches; stop returns the number of
- * launches timed and their mean duration. */
+ * armed, event pairs are recorded on the library stream around the launches
+ * of the kernel class `which` (each launch of the FP64 PCG kernels; each
+ * group of consecutive launches of the line half-sweeps, one line sweep), up
+ * to max_launches pairs; stop returns the number of launches timed and their
+ * mean duration (the group's time over its launches). */
 #define MOONS_KTIMER_MG_LINE_FINE 1 /* zebra line half-sweeps of the finest multigrid level */
 #define MOONS_KTIMER_PCG_MATVEC 2   /* FP64 PCG kernels of the PPE */
 #define MOONS_KTIMER_PCG_UPDATE 3
@@ -595,6 +597,9 @@ int moons_mfma_check(const float *A, const float *B, float *D);
  * that level's pre-smoothing x = c N(f), c = (N, f) / (N, L N) (0 when not
  * positive).  net = NULL removes it.  No reference counterpart. */
 int moons_pcg_set_mg_unet(moons_pcg_t pcg, moons_unet_t net, int level);
+/* The learned initial guess in the first `cycles` V-cycles of each solve only
+ * (0, the default: in every V-cycle).  No reference counterpart. */
+int moons_pcg_set_mg_unet_cycles(moons_pcg_t pcg, int cycles);
 /* Enables (default) or disables the Jacobi redo of failed MOONS_PREC_UNET solves. */
 int moons_pcg_set_fallback(moons_pcg_t pcg, int enable);
 /* Binds the network to a MOONS_PREC_UNET solver (required before its first solve). */

@@ -427,9 +427,11 @@ class PCG_solver_SF:
         check(load().moons_pcg_mg_info(self.handle, C.byref(lv), n))
         return lv.value, (n[0], n[1], n[2])
 
-    def set_MG_UNet(self, net: "UNet | None", level: int = 2) -> None:
-        """Learned initial guess of multigrid level `level` inside the V-cycle (prec="MG_FP32")."""
+    def set_MG_UNet(self, net: "UNet | None", level: int = 2, cycles: int = 0) -> None:
+        """Learned initial guess of multigrid level `level` inside the V-cycle
+        (prec="MG_FP32"), in the first `cycles` V-cycles of each solve (0: every one)."""
         check(load().moons_pcg_set_mg_unet(self.handle, net.handle if net is not None else None, level))
+        check(load().moons_pcg_set_mg_unet_cycles(self.handle, int(cycles)))
         self._mg_net = net
 
     def set_fallback(self, enable: bool) -> None:

@@ -19,8 +19,9 @@ hipStream_t stream();
 bool sync_free();
 void set_sync_free(bool on);
 // Records the next event of the armed kernel timer when `which` is the kernel
-// class it times (moons_ktimer_start); call before and after each launch.
-void ktimer_mark(int which);
+// class it times (moons_ktimer_start): call before and after a group of
+// launches of that class, the closing call with the group's launch count.
+void ktimer_mark(int which, int launches = 1);
 
 #define MOONS_HIP(call)                                                                  \
   do {                                                                                   \

@@ -37,8 +37,10 @@ void set_sync_free(bool on) { g_sync_free = on; }
 static int g_kt_which = 0;
 static std::vector<hipEvent_t> g_kt_ev;
 static size_t g_kt_used = 0;
-void ktimer_mark(int which) {
+static long long g_kt_launches = 0; // launches inside the recorded pairs
+void ktimer_mark(int which, int launches) {
   if (which == 0 || which != g_kt_which || g_kt_used >= g_kt_ev.size() || g_sync_free) return;
+  if (g_kt_used % 2 == 1) g_kt_launches += launches; // closing a pair
   (void)hipEventRecord(g_kt_ev[g_kt_used++], g_stream);
 }
 
@@ -754,6 +756,7 @@ int moons_ktimer_start(int which, int max_launches) {
       return fail(MOONS_ERR_HIP, "moons_ktimer_start: hipEventCreate failed");
     }
   g_kt_used = 0;
+  g_kt_launches = 0;
   g_kt_which = which;
   return MOONS_OK;
 }
@@ -778,8 +781,8 @@ int moons_ktimer_stop(int *launches, double *avg_ms) {
   g_kt_which = 0;
   g_kt_used = 0;
   if (st) return st;
-  if (launches) *launches = (int)pairs;
-  if (avg_ms) *avg_ms = pairs ? sum / (double)pairs : 0.0;
+  if (launches) *launches = (int)g_kt_launches;
+  if (avg_ms) *avg_ms = g_kt_launches ? sum / (double)g_kt_launches : 0.0;
   return MOONS_OK;
 }
 

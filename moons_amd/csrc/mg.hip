@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -74,6 +75,8 @@ struct MGHier {
   std::vector<std::vector<double>> hn; // node arrays (ghosts included) of every level, 3 per level
   UnetPrec *learn = nullptr;   // learned initial guess of one grid level (or none)
   int learn_level = -1;
+  int learn_cycles = 0;        // the first learn_cycles V-cycles of a solve use it (0: every one)
+  int cycle = 0;               // V-cycles since the solve began
   float *lpf = nullptr, *lpx = nullptr; // its padded (n+2)^3 input and output images
   float *gpart = nullptr;      // its Galerkin partial sums
 };
@@ -220,6 +223,11 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
 // both halves of a 128-byte row are then fetched from HBM once.
 constexpr int kLineMax = 128; // longest line (at least 16 lines per block)
 constexpr long long kLineMinCells = 32LL * 32 * 32; // levels this small smooth by points
+// MOONS_MG_LINE_MIN_CELLS overrides kLineMinCells, for tuning runs.
+static long long line_min_cells() {
+  const char *e = getenv("MOONS_MG_LINE_MIN_CELLS");
+  return e ? atoll(e) : kLineMinCells;
+}
 
 // Row a x[t-1] + x[t] + c x[t+1] = d of cell t (0-based) of a line (normalised).
 template <int D>
@@ -967,7 +975,7 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
         }
         const long long cells = (long long)hl.n[0] * hl.n[1] * hl.n[2];
         const bool per = bct[2 * d] == MOONS_BC_PERIODIC;
-        const int ln = !per && n >= 4 && n <= kLineMax && cells > kLineMinCells && dmax > 1.5 * dmin;
+        const int ln = !per && n >= 4 && n <= kLineMax && cells > line_min_cells() && dmax > 1.5 * dmin;
         (d == 1 ? L.line1 : L.line2) = ln;
       }
       std::vector<float> wm(s, 0.f), wp(s, 0.f), dg(s, 0.f), rf(s, 0.f), pa(s, 0.f), pb(s, 0.f);
@@ -1271,18 +1279,16 @@ static void line_sweep(LevelDev &L, bool forward, bool zero_first, const int *do
   int ds[2], n = 0;
   if (L.line1) ds[n++] = 1;
   if (L.line2) ds[n++] = 2;
-  auto half = [&](int D, int c, bool zg) {
-    ktimer_mark(timer);
-    line_half(L, D, c, zg, done);
-    ktimer_mark(timer);
-  };
+  // one timer pair around the whole sweep: the event overhead spreads over its launches
+  ktimer_mark(timer);
   if (forward) {
     for (int a = 0; a < n; ++a)
-      for (int c = 0; c < 2; ++c) half(ds[a], c, zero_first && a == 0 && c == 0);
+      for (int c = 0; c < 2; ++c) line_half(L, ds[a], c, zero_first && a == 0 && c == 0, done);
   } else {
     for (int a = n - 1; a >= 0; --a)
-      for (int c = 1; c >= 0; --c) half(ds[a], c, false);
+      for (int c = 1; c >= 0; --c) line_half(L, ds[a], c, false, done);
   }
+  ktimer_mark(timer, 2 * n);
 }
 
 // One red-black sweep of level L, colour c_first first (lines, where the
@@ -1317,7 +1323,7 @@ int mg_vcycle(MGHier *h, const int *done) {
   // descend
   for (int l = 0; l < g; ++l) {
     LevelDev &L = h->L[l];
-    const bool learned = h->learn && l == h->learn_level;
+    const bool learned = h->learn && l == h->learn_level && (h->learn_cycles == 0 || h->cycle < h->learn_cycles);
     if (learned) {
       // learned initial guess x = c N(f), then smoothing from it
       const int nc = L.n0 * L.n1 * L.n2;
@@ -1343,10 +1349,14 @@ int mg_vcycle(MGHier *h, const int *done) {
     prolong_from(L, h->L[l + 1], done);
     for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
   }
+  ++h->cycle;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));
   return MOONS_OK;
 }
+
+void mg_begin_solve(MGHier *h) { h->cycle = 0; }
+void mg_set_learned_cycles(MGHier *h, int cycles) { h->learn_cycles = cycles; }
 
 int mg_time_kernels(MGHier *h, int reps, double ms[10]) {
   const int nl = (int)h->L.size();

@@ -27,6 +27,10 @@ void mg_f32_map(const MGHier *h, int map[3]);
 // One V-cycle x0 = V f0 on the library stream; every kernel returns at once
 // when *done (device) is non-zero.
 int mg_vcycle(MGHier *h, const int *done);
+// Start of a solve (the learned level's cycle count restarts).
+void mg_begin_solve(MGHier *h);
+// The learned initial guess in the first `cycles` V-cycles of each solve (0: every cycle).
+void mg_set_learned_cycles(MGHier *h, int cycles);
 void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
 // Average duration (ms) of one launch, reps launches back to back on the
 // library stream: [0] a half-sweep of level 0, [1] restriction 0 -> 1,

@@ -827,6 +827,13 @@ int moons_pcg_set_mg_unet(moons_pcg_t S, moons_unet_t net, int level) {
   return MOONS_OK;
 }
 
+int moons_pcg_set_mg_unet_cycles(moons_pcg_t S, int cycles) {
+  if (!S || cycles < 0) return fail(MOONS_ERR_ARG, "moons_pcg_set_mg_unet_cycles: bad argument");
+  if (!S->mg) return fail(MOONS_ERR_STATE, "moons_pcg_set_mg_unet_cycles: solver was not created with MOONS_PREC_MG_FP32");
+  mg_set_learned_cycles(S->mg, cycles);
+  return MOONS_OK;
+}
+
 int moons_pcg_set_fallback(moons_pcg_t S, int enable) {
   if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_set_fallback: null solver");
   S->fallback = enable != 0;
@@ -903,6 +910,7 @@ int moons_pcg_set_isp(moons_pcg_t S, int iter_max, double tol_rel, double tol_ab
 static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
   int st = pcg_upload_ctrl(S);
   if (st) return st;
+  if (S->mg) mg_begin_solve(S->mg);
   const dim3 grid(S->G.nblocks), block(kNT);
   const int alln = S->xbc.all_neumann ? 1 : 0;
   const bool mg = flex(S);
