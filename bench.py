@@ -131,17 +131,59 @@ def cpu_baseline(hs, bh, gpu_iters):
                                       f"{it} iterations; {1e3 * per_iter:.1f} ms per iteration"}}
 
 
-def load_rocprof_avg(name, kernel_prefix):
-    """Average duration (us) of a kernel in a committed rocprofv3 summary
-    (lines `kernel: <name> avg_us: <t>`), or None."""
+def load_rocprof_avg(name, kernel_prefix, grid=None):
+    """Average duration (us) of a kernel in a committed rocprofv3 summary, or
+    None.  Summary lines read `kernel: <name> grid: <g> calls: <c> avg_us: <t>`
+    (the --rocprof-summary output); every line whose name starts with
+    kernel_prefix (a string or a tuple of them) and whose grid (work-items) is
+    `grid`, when given, counts, weighted by its calls."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
+    tot = n = 0.0
     with open(p) as fh:
         for line in fh:
-            if line.startswith("kernel:") and kernel_prefix in line and "avg_us:" in line:
-                return float(line.rsplit("avg_us:", 1)[1].split()[0])
-    return None
+            if not line.startswith("kernel:") or "avg_us:" not in line:
+                continue
+            kname = line[len("kernel:"):].split(" grid:")[0].strip()
+            kname = kname[5:] if kname.startswith("void ") else kname  # the return type and namespace
+            kname = kname[7:] if kname.startswith("moons::") else kname
+            if not kname.startswith(tuple(kernel_prefix) if isinstance(kernel_prefix, (list, tuple)) else kernel_prefix):
+                continue
+            f = line.split()
+            g = f[f.index("grid:") + 1] if "grid:" in f else None
+            if grid is not None and g != str(grid):
+                continue
+            c = float(f[f.index("calls:") + 1])
+            tot += c * float(f[f.index("avg_us:") + 1])
+            n += c
+    return round(tot / n, 3) if n else None
+
+
+def rocprof_summary(d, top=60):
+    """Per (kernel, grid size) dispatch count and duration statistics of the
+    rocprofv3 kernel-trace CSVs under directory d, longest total first."""
+    import csv
+    import glob
+
+    files = sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True))
+    groups = {}
+    for fpath in files:
+        with open(fpath) as fh:
+            for r in csv.DictReader(fh):
+                if "Grid_Size_X" in r:
+                    grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+                else:
+                    grid = int(r.get("Grid_Size", 0))
+                dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+                groups.setdefault((r["Kernel_Name"], grid), []).append(dur)
+    rows = sorted(groups.items(), key=lambda kv: -sum(kv[1]))
+    total = sum(sum(v) for v in groups.values())
+    print(f"trace files: {len(files)}; {sum(len(v) for v in groups.values())} dispatches, "
+          f"{total / 1e3:.3f} ms of kernel time")
+    for (kname, grid), v in rows[:top]:
+        print(f"kernel: {kname[:120]} grid: {grid} calls: {len(v)} avg_us: {sum(v) / len(v):.3f} "
+              f"min_us: {min(v):.3f} max_us: {max(v):.3f} total_ms: {sum(v) / 1e3:.3f}")
 
 
 def run_ensemble(M, args):
@@ -406,7 +448,12 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="4 PCG iterations, then 50 timed launches of each iteration kernel "
                          "(no early-exited launches; for profiler runs)")
+    ap.add_argument("--rocprof-summary", metavar="DIR",
+                    help="print the per-kernel summary of the rocprofv3 kernel-trace CSVs under DIR and exit")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the solve over several GPUs is not implemented.", file=sys.stderr)
@@ -579,7 +626,8 @@ def main():
     names = list(alg)
     upd_gbs = alg["pcg_update"] / (kf.avg_ms * 1e-3) / 1e9
     iter_us = 1e3 * sum(kms)
-    upd_rp = load_rocprof_avg("pcg_update_128_trace.txt", "k_pcg_update")
+    # k_pcg_update handles 8 cells a work-item; the learned leg's 32^3 solves use it too
+    upd_rp = load_rocprof_avg("ppe_128_bench_trace.txt", "k_pcg_update", grid=ncell // 8)
     fp64_roofline = {"bound": "hbm", "kernel": "pcg_update", "achieved": round(upd_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(upd_gbs / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic("pcg_update_128_traffic.txt"),
@@ -595,7 +643,7 @@ def main():
     mk = Pm.time_MG(50)
     sm_bytes = 6 * ncell
     sm_gbs = sm_bytes / (kt.avg_ms * 1e-3) / 1e9
-    line_rp = load_rocprof_avg("mg_line_128_trace.txt", "k_mg_line")
+    line_rp = load_rocprof_avg("ppe_128_bench_trace.txt", ("k_mg_line<1, 128,", "k_mg_line<2, 128,"))
     roofline = {"bound": "hbm", "kernel": "mg_line (128^3 zebra line half-sweep, y and z)",
                 "achieved": round(sm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(sm_gbs / HBM_PEAK_GBS, 4), "traffic": load_traffic("mg_line_128_traffic.txt"),

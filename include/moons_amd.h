@@ -520,9 +520,16 @@ int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
  *     F += advection_scale * (-div(cellCenter2Face(T) * U));
  *   diffusion (add_diffusion, energy_sources.f90:40-48):
  *     F += diffusion_scale * lapUniformCoeff(T);
+ *   KE diffusion, viscous dissipation, Joule and volumetric heating (below);
  *   method 1: T += dt F; method 2: T += dt AB2(F, Fnm1); method 3: Tnm1 = T,
  *     solve (I + c div(k grad)) T = T + dt AB2(F, Fnm1) with c = dt coeff_implicit
- *     (code/handwritten/solvers/time_marching_methods_SF.f90:71-98,135-170).
+ *     (code/handwritten/solvers/time_marching_methods_SF.f90:71-98,135-170);
+ *   method 4 (O2_BDF_time_AB2_sources_SF, :42-69): rhs = 2/3 dt AB2(F, Fnm1)
+ *     + 4/3 T - 1/3 Tnm1, Tnm1 = T, c = dt 2/3 coeff_implicit;
+ *   method 5 (Euler_time_RK_sources_SF, :100-129): rhs = dt (gamma F + zeta Fnm1
+ *     + alpha L) + T, Tnm1 = T, c = dt beta coeff_implicit.
+ * U_CC = face2CellCenter(U) (momentum_extend.f90:370) is formed from U at
+ * each step's sources.
  * coeff_implicit is the MFP's (-alpha/Pe; code/handwritten/var_set/var_set_extend.f90:82).
  * The reference's presets scale advection by -1 and diffusion by 1/Pe
  * (e.g. code/handwritten/sim_params/sim_params_default_MHD_Shercliff/sim_params_init_flow_config.f90:71-72). */
@@ -537,16 +544,35 @@ typedef struct {
   int iter_max;
   double tol_rel, tol_abs;
   int n_skip;
+  /* add_KE_diffusion (energy_sources.f90:50-63): F += scale lap(|U_CC|^2) */
+  int KE_diffusion;
+  double KE_diffusion_scale;
+  /* add_viscous_dissipation (:65-76): F += scale grad(U_CC) : grad(U_CC) */
+  int viscous_dissipation;
+  double viscous_dissipation_scale;
+  /* add_joule_heating (:78-93): F += scale |edge2CellCenter(J)|^2 sigmaInv_CC
+   * (J and sigmaInv_CC bound by moons_nrg_set_joule) */
+  int joule_heating;
+  double joule_heating_scale;
+  /* add_volumetric_heating_Nuclear (:95-113): F += scale q(y), q = scale 2/(1 - e^-2) e^-(y_c + 1) */
+  int volumetric_heating;
+  double volumetric_heating_scale;
+  /* 1: RK inactive (one stage, coefficients one); 3: the 3-stage RK of init_RKP
+   * (code/handwritten/var_set/RK_params_extend.f90:29-62) is active: a step runs
+   * three stages, diffusion goes to L, and method 5 reads the stage's coefficients.
+   * 0 is taken as 1. */
+  int rk_stages;
 } moons_nrg_params;
 
 typedef struct {
-  int iterations; /* of the last step's T solve (method 3) */
+  int iterations; /* of the last step's T solve (methods 3-5) */
   double ms;      /* device time of the call */
 } moons_nrg_info;
 
 #define MOONS_NRG_F 1    /* energy sources F */
 #define MOONS_NRG_FNM1 2 /* previous step's F */
 #define MOONS_NRG_TNM1 3 /* T before the last implicit solve */
+#define MOONS_NRG_L 4    /* the RK stage's explicit diffusion */
 
 /* Creates the energy state over the caller's T (cell-centred, with BCs) and
  * face velocity U (read at each step: the momentum state's U); k are the
@@ -558,7 +584,13 @@ typedef struct {
 int moons_nrg_create(moons_mesh_t m, const moons_nrg_params *params, moons_field_t T, moons_field_t U[3],
                      moons_field_t k[3], moons_nrg_t *out);
 int moons_nrg_destroy(moons_nrg_t s);
-/* Advances n_steps energy steps with the current U. */
+/* Binds the induction's current density J (three edge fields: x edges,
+ * y edges, z edges) and sigmaInv_CC (cell-centred) that Joule heating reads
+ * (ind%J, ind%sigmaInv_CC of add_all_energy_sources.f90:59-62), on the
+ * energy mesh.  Read at each step. */
+int moons_nrg_set_joule(moons_nrg_t s, moons_field_t J[3], moons_field_t sigma_inv_cc);
+/* Advances n_steps energy steps with the current U (each step runs the RK
+ * stages when rk_stages = 3). */
 int moons_nrg_step(moons_nrg_t s, int n_steps, moons_nrg_info *info);
 /* Handle of an internal field (MOONS_NRG_*). */
 int moons_nrg_field(moons_nrg_t s, int which, moons_field_t *out);

@@ -42,7 +42,12 @@ class MhdInfo(C.Structure):
 class NrgParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("method", C.c_int), ("coeff_implicit", C.c_double), ("advection", C.c_int),
                 ("advection_scale", C.c_double), ("diffusion", C.c_int), ("diffusion_scale", C.c_double),
-                ("iter_max", C.c_int), ("tol_rel", C.c_double), ("tol_abs", C.c_double), ("n_skip", C.c_int)]
+                ("iter_max", C.c_int), ("tol_rel", C.c_double), ("tol_abs", C.c_double), ("n_skip", C.c_int),
+                ("KE_diffusion", C.c_int), ("KE_diffusion_scale", C.c_double),
+                ("viscous_dissipation", C.c_int), ("viscous_dissipation_scale", C.c_double),
+                ("joule_heating", C.c_int), ("joule_heating_scale", C.c_double),
+                ("volumetric_heating", C.c_int), ("volumetric_heating_scale", C.c_double),
+                ("rk_stages", C.c_int)]
 
 
 class NrgInfo(C.Structure):
@@ -101,6 +106,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "moons_nrg_destroy": (C.c_int, [C.c_void_p]),
     "moons_nrg_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(NrgInfo)]),
+    "moons_nrg_set_joule": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p]),
     "moons_nrg_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "moons_pcg_sf_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PcgInfo)]),
     "moons_pcg_sf_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int,

@@ -635,17 +635,24 @@ class MHD_step:
 class Energy_step:
     """The energy equation's step (solve_energy after add_all_energy_sources,
     code/handwritten/governing_equations/energy_extend.f90:236-262): sources
-    advection (-div(face(T) U) times advection_scale) and diffusion
-    (lapUniformCoeff(T) times diffusion_scale), then method 1 (Euler), 2
-    (explicit AB2) or 3 (AB2 with backward-Euler I + c div(k grad),
-    c = dt coeff_implicit).  T (with BCs) is advanced in place; U is read at
-    each step.  k: three face SFs bound to the implicit operator; None gives
-    zeros, as the reference's init binds (its operator is then I)."""
+    advection (-div(face(T) U) times advection_scale), diffusion
+    (lapUniformCoeff(T) times diffusion_scale; into L when the RK is active),
+    KE diffusion, viscous dissipation, Joule heating and volumetric heating
+    (code/handwritten/solvers/energy/energy_sources.f90:50-113), each a
+    (flag, scale) pair, then method 1 (Euler), 2 (explicit AB2), 3 (AB2 with
+    backward-Euler I + c div(k grad), c = dt coeff_implicit), 4 (BDF2 with
+    AB2 sources) or 5 (the RK stage's Euler, rk_stages = 3 for the 3-stage
+    RK).  T (with BCs) is advanced in place; U is read at each step.  k: three
+    face SFs bound to the implicit operator; None gives zeros, as the
+    reference's init binds (its operator is then I).  J (x, y, z edge SFs) and
+    sigma_inv (cell-centred) feed Joule heating (set_joule)."""
 
-    FIELDS = {"F": 1, "Fnm1": 2, "Tnm1": 3}
+    FIELDS = {"F": 1, "Fnm1": 2, "Tnm1": 3, "L": 4}
 
     def __init__(self, m: mesh, T: SF, U=None, k=None, dt=1e-3, method=3, Pe=100.0, coeff_implicit=None,
-                 advection=(True, -1.0), diffusion=(False, None), isp_T=(5, 1e-6, 1e-13, 1)):
+                 advection=(True, -1.0), diffusion=(False, None), isp_T=(5, 1e-6, 1e-13, 1),
+                 KE_diffusion=(False, 1.0), viscous_dissipation=(False, 1.0), joule_heating=(False, 1.0),
+                 volumetric_heating=(False, 1.0), rk_stages=1, J=None, sigma_inv=None):
         P = NrgParams()
         P.dt, P.method = dt, method
         P.coeff_implicit = -1.0 / Pe if coeff_implicit is None else coeff_implicit
@@ -653,6 +660,13 @@ class Energy_step:
         P.diffusion = 1 if diffusion[0] else 0
         P.diffusion_scale = (1.0 / Pe) if diffusion[1] is None else diffusion[1]
         P.iter_max, P.tol_rel, P.tol_abs, P.n_skip = isp_T
+        P.KE_diffusion, P.KE_diffusion_scale = (1 if KE_diffusion[0] else 0), KE_diffusion[1]
+        P.viscous_dissipation = 1 if viscous_dissipation[0] else 0
+        P.viscous_dissipation_scale = viscous_dissipation[1]
+        P.joule_heating, P.joule_heating_scale = (1 if joule_heating[0] else 0), joule_heating[1]
+        P.volumetric_heating = 1 if volumetric_heating[0] else 0
+        P.volumetric_heating_scale = volumetric_heating[1]
+        P.rk_stages = rk_stages
         if k is None:
             k = [SF(m, 1 << d).fill(0.0) for d in range(3)]
         h = C.c_void_p()
@@ -661,6 +675,15 @@ class Energy_step:
         self.handle = h
         self.m = m
         self._keep = (T, U, k)
+        self._joule = None
+        if J is not None:
+            self.set_joule(J, sigma_inv)
+
+    def set_joule(self, J, sigma_inv: SF):
+        """Binds the current density (x, y, z edge SFs) and sigmaInv_CC read by Joule heating."""
+        check(load().moons_nrg_set_joule(self.handle, _arr(J), sigma_inv.handle))
+        self._joule = (J, sigma_inv)
+        return self
 
     def step(self, n: int = 1):
         info = NrgInfo()

@@ -19,8 +19,10 @@
 // each value is rounded exactly as in the sequential passes.  The fluid and
 // induction meshes are the same single block, so embedding and extraction
 // are identities.
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -74,6 +76,40 @@ __global__ void __launch_bounds__(kST) k_ab2(double *out, const double *F, const
   const int ii = d == 0 ? i : (d == 1 ? j : k);
   const int sd = d == 0 ? s0 : (d == 1 ? s1 : s2);
   if (((wall_mask & 1) && ii == 1) || ((wall_mask & 2) && ii == sd - 2)) v = 0.0;
+  out[t] = v + X[t];
+}
+
+// acc = x*x (first) or acc + x*x: a left-to-right sum of squares over
+// successive launches, as square() then add() of the reference's fields
+__global__ void __launch_bounds__(kST) k_sqacc(double *acc, const double *x, int first, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t >= n) return;
+  const double q = x[t] * x[t];
+  acc[t] = first ? q : acc[t] + q;
+}
+
+// O2_BDF_time_AB2_sources_SF's right-hand side (time_marching_methods_SF.f90:61-65):
+// ((F*1.5 + Fnm1*(-0.5)) * (2/3 dt) + X*(4/3)) + Xnm1*(-1/3)
+__global__ void __launch_bounds__(kST) k_bdf2_rhs(double *out, const double *F, const double *Fnm1, const double *X,
+                                                 const double *Xnm1, double c23dt, double c43, double cm13, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t >= n) return;
+  double v = F[t] * 1.5;
+  v = v + Fnm1[t] * (-0.5);
+  v = v * c23dt;
+  v = v + X[t] * c43;
+  out[t] = v + Xnm1[t] * cm13;
+}
+
+// Euler_time_RK_sources_SF's right-hand side (time_marching_methods_SF.f90:121-125):
+// ((F*(dt gamma) + Fnm1*(dt zeta)) + L*(dt alpha)) + X
+__global__ void __launch_bounds__(kST) k_rk_rhs(double *out, const double *F, const double *Fnm1, const double *L,
+                                               const double *X, double cg, double cz, double ca, size_t n) {
+  const size_t t = (size_t)blockIdx.x * kST + threadIdx.x;
+  if (t >= n) return;
+  double v = F[t] * cg;
+  v = v + Fnm1[t] * cz;
+  v = v + L[t] * ca;
   out[t] = v + X[t];
 }
 
@@ -549,16 +585,28 @@ struct moons_nrg_s {
   moons_mesh_s *m = nullptr;
   moons_nrg_params P{};
   moons_field_s *T = nullptr, *U[3] = {}; // caller's
-  moons_field_s *F = nullptr, *Fnm1 = nullptr, *Tnm1 = nullptr, *rhs = nullptr, *tcc = nullptr, *tF[3] = {};
+  moons_field_s *J[3] = {}, *sig = nullptr; // caller's (Joule heating)
+  moons_field_s *F = nullptr, *Fnm1 = nullptr, *L = nullptr, *Tnm1 = nullptr, *rhs = nullptr, *tcc = nullptr,
+                *tcc2 = nullptr, *tF[3] = {}, *UCC[3] = {}, *qv = nullptr;
   moons_pcg_t pcg = nullptr;
   int iters = 0;
+  int stage = 0; // RK stage (0-based) of the running step
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 namespace {
 
+// init_RKP (code/handwritten/var_set/RK_params_extend.f90:55-58): gamma, zeta,
+// alpha of the three stages; beta = alpha
+constexpr double kRKGamma[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+constexpr double kRKZeta[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+constexpr double kRKAlpha[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+
+bool rk_active(const moons_nrg_s *S) { return S->P.rk_stages == 3; }
+
 void nrg_free(moons_nrg_s *S) {
-  moons_field_s *fs[] = {S->F, S->Fnm1, S->Tnm1, S->rhs, S->tcc, S->tF[0], S->tF[1], S->tF[2]};
+  moons_field_s *fs[] = {S->F,     S->Fnm1,   S->L,      S->Tnm1,   S->rhs,    S->tcc,    S->tcc2,
+                         S->tF[0], S->tF[1],  S->tF[2],  S->UCC[0], S->UCC[1], S->UCC[2], S->qv};
   for (auto f : fs)
     if (f) moons_field_destroy(f);
   if (S->pcg) moons_pcg_destroy(S->pcg);
@@ -567,10 +615,26 @@ void nrg_free(moons_nrg_s *S) {
   delete S;
 }
 
+int sqacc(moons_field_s *acc, const moons_field_s *x, bool first) {
+  hipLaunchKernelGGL(k_sqacc, dim3(sblk(acc->n)), dim3(kST), 0, stream(), acc->d, x->d, first ? 1 : 0, acc->n);
+  MOONS_LAUNCH_CHECK();
+  return MOONS_OK;
+}
+
+// lapUniformCoeff_SF (ops/ops_discrete.f90:101-110): collocated second
+// derivatives along x (assign), y and z (add)
+int lap_uniform(moons_field_s *out, const moons_field_s *f) {
+  int st;
+  if ((st = launch_del(out, f, 2, 0, 0, 1))) return st;
+  if ((st = launch_del(out, f, 2, 1, 0, 2))) return st;
+  return launch_del(out, f, 2, 2, 0, 2);
+}
+
 // add_all_energy_sources (solvers/energy/add_all_energy_sources.f90:21-80)
 int nrg_sources(moons_nrg_s *S) {
   int st;
   if ((st = moons_field_copy(S->Fnm1, S->F))) return st;
+  if (rk_active(S) && (st = moons_field_fill(S->L, 0.0))) return st;
   if ((st = moons_field_fill(S->F, 0.0))) return st;
   if (S->P.advection) {
     // temp_F = cellCenter2Face(T); temp_F = temp_F * U; temp_CC = div(temp_F);
@@ -583,49 +647,99 @@ int nrg_sources(moons_nrg_s *S) {
       MOONS_LAUNCH_CHECK();
     }
     if ((st = moons_div(S->tcc, S->tF[0], S->tF[1], S->tF[2]))) return st;
-    hipLaunchKernelGGL(k_axpy, dim3(sblk(S->F->n)), dim3(kST), 0, stream(), S->F->d, S->tcc->d, -S->P.advection_scale,
-                       S->F->n);
-    MOONS_LAUNCH_CHECK();
+    if ((st = axpy(S->F, S->tcc, -S->P.advection_scale))) return st;
   }
   if (S->P.diffusion) {
-    // lapUniformCoeff_SF (ops/ops_discrete.f90:101-110): collocated second
-    // derivatives along x (assign), y and z (add)
-    if ((st = launch_del(S->tcc, S->T, 2, 0, 0, 1))) return st;
-    if ((st = launch_del(S->tcc, S->T, 2, 1, 0, 2))) return st;
-    if ((st = launch_del(S->tcc, S->T, 2, 2, 0, 2))) return st;
-    hipLaunchKernelGGL(k_axpy, dim3(sblk(S->F->n)), dim3(kST), 0, stream(), S->F->d, S->tcc->d, S->P.diffusion_scale,
-                       S->F->n);
-    MOONS_LAUNCH_CHECK();
+    if ((st = lap_uniform(S->tcc, S->T))) return st;
+    if ((st = axpy(rk_active(S) ? S->L : S->F, S->tcc, S->P.diffusion_scale))) return st;
   }
+  if (S->P.KE_diffusion || S->P.viscous_dissipation) // U_CC = face2CellCenter(U)
+    for (int d = 0; d < 3; ++d)
+      if ((st = launch_interp(S->UCC[d], S->U[d], d))) return st;
+  if (S->P.KE_diffusion) {
+    // temp_CC1 = Ux^2 + Uy^2 + Uz^2; temp_CC2 = lap(temp_CC1); F += temp_CC2 * scale
+    for (int d = 0; d < 3; ++d)
+      if ((st = sqacc(S->tcc, S->UCC[d], d == 0))) return st;
+    if ((st = lap_uniform(S->tcc2, S->tcc))) return st;
+    if ((st = axpy(S->F, S->tcc2, S->P.KE_diffusion_scale))) return st;
+  }
+  if (S->P.viscous_dissipation) {
+    // grad_TF (ops_discrete.f90:313-321): G[c][e] = d U_c / d x_e collocated
+    // on the cell centres; summed squared in add_SF_TF's order (TF_extend.f90:248-255):
+    // G[x][x], G[y][x], G[z][x], G[x][y], ..., G[z][z]
+    for (int e = 0; e < 3; ++e)
+      for (int c = 0; c < 3; ++c) {
+        if ((st = launch_del(S->tcc2, S->UCC[c], 1, e, 0, 1))) return st;
+        if ((st = sqacc(S->tcc, S->tcc2, e == 0 && c == 0))) return st;
+      }
+    if ((st = axpy(S->F, S->tcc, S->P.viscous_dissipation_scale))) return st;
+  }
+  if (S->P.joule_heating) {
+    if (!S->J[0] || !S->sig) return fail(MOONS_ERR_STATE, "moons_nrg_step: Joule heating needs moons_nrg_set_joule");
+    // edge2CellCenter_VF (ops_interp.f90:688-696): component d goes edge ->
+    // face fd = d+1 mod 3 along the third direction, then -> cell centre along fd
+    for (int d = 0; d < 3; ++d) {
+      const int fd = (d + 1) % 3, od = 3 - d - fd;
+      if ((st = launch_interp(S->tF[fd], S->J[d], od))) return st;
+      if ((st = launch_interp(S->tcc2, S->tF[fd], fd))) return st;
+      if ((st = sqacc(S->tcc, S->tcc2, d == 0))) return st;
+    }
+    if ((st = mul3(S->tcc, S->tcc, S->sig))) return st; // multiply(temp_CC, sigmaInv_CC)
+    if ((st = axpy(S->F, S->tcc, S->P.joule_heating_scale))) return st;
+  }
+  if (S->P.volumetric_heating && (st = axpy(S->F, S->qv, S->P.volumetric_heating_scale))) return st;
   return MOONS_OK;
 }
 
-// solve_energy (governing_equations/energy_extend.f90:236-259), methods 1-3.
+// solve_energy (governing_equations/energy_extend.f90:236-259), methods 1-5.
 // A cell-centred field has no wall planes, so assign_wall_Dirichlet is a no-op.
 int nrg_march(moons_nrg_s *S) {
   const moons_field_s *T = S->T;
   const unsigned nb = sblk(T->n);
+  const double dt = S->P.dt;
   if (S->P.method == 1) { // T = T + F*dt
-    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(kST), 0, stream(), S->T->d, S->F->d, S->P.dt, T->n);
+    hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(kST), 0, stream(), S->T->d, S->F->d, dt, T->n);
     MOONS_LAUNCH_CHECK();
     return MOONS_OK;
   }
   if (S->P.method == 2) { // T = ((F*1.5 + Fnm1*(-0.5)) * dt) + T
     hipLaunchKernelGGL(k_ab2, dim3(nb), dim3(kST), 0, stream(), S->T->d, S->F->d, S->Fnm1->d, (const double *)nullptr,
-                       S->T->d, S->P.dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
+                       S->T->d, dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
     MOONS_LAUNCH_CHECK();
     return MOONS_OK;
   }
-  hipLaunchKernelGGL(k_ab2, dim3(nb), dim3(kST), 0, stream(), S->rhs->d, S->F->d, S->Fnm1->d, (const double *)nullptr,
-                     S->T->d, S->P.dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
+  double c;
+  if (S->P.method == 3) {
+    hipLaunchKernelGGL(k_ab2, dim3(nb), dim3(kST), 0, stream(), S->rhs->d, S->F->d, S->Fnm1->d,
+                       (const double *)nullptr, S->T->d, dt, T->s[0], T->s[1], T->s[2], 0, 0, 0);
+    c = dt * 1.0 * S->P.coeff_implicit;
+  } else if (S->P.method == 4) {
+    const double two_thirds = 2.0 / 3.0, four_thirds = 4.0 / 3.0, neg_one_third = -1.0 / 3.0;
+    hipLaunchKernelGGL(k_bdf2_rhs, dim3(nb), dim3(kST), 0, stream(), S->rhs->d, S->F->d, S->Fnm1->d, S->T->d,
+                       S->Tnm1->d, two_thirds * dt, four_thirds, neg_one_third, T->n);
+    c = dt * two_thirds * S->P.coeff_implicit;
+  } else {
+    const bool rk = rk_active(S);
+    const double g = rk ? kRKGamma[S->stage] : 1.0, z = rk ? kRKZeta[S->stage] : 1.0,
+                 a = rk ? kRKAlpha[S->stage] : 1.0;
+    hipLaunchKernelGGL(k_rk_rhs, dim3(nb), dim3(kST), 0, stream(), S->rhs->d, S->F->d, S->Fnm1->d, S->L->d, S->T->d,
+                       dt * g, dt * z, dt * a, T->n);
+    c = dt * a * S->P.coeff_implicit; // beta = alpha
+  }
   MOONS_LAUNCH_CHECK();
   int st;
   if ((st = moons_field_copy(S->Tnm1, S->T))) return st;
-  if ((st = moons_pcg_sf_update_mfp(S->pcg, S->P.dt * 1.0 * S->P.coeff_implicit, 1))) return st;
+  if ((st = moons_pcg_sf_update_mfp(S->pcg, c, 1))) return st;
   moons_pcg_info info;
   if ((st = moons_pcg_sf_solve(S->pcg, S->T, S->rhs, &info))) return st;
   S->iters = info.iterations;
   return MOONS_OK;
+}
+
+bool is_face_set(moons_field_t U[3], const moons_mesh_s *m) {
+  for (int d = 0; d < 3; ++d)
+    if (!U || !U[d] || U[d]->dl != (1 << d) || U[d]->m != m) return false;
+  return true;
 }
 
 } // namespace
@@ -636,27 +750,48 @@ int moons_nrg_create(moons_mesh_t m, const moons_nrg_params *P, moons_field_t T,
                      moons_field_t k[3], moons_nrg_t *out) {
   if (!stream()) return fail(MOONS_ERR_STATE, "moons_init has not been called");
   if (!m || !P || !T || !out) return fail(MOONS_ERR_ARG, "moons_nrg_create: null argument");
-  if (P->method < 1 || P->method > 3 || !(P->dt > 0.0))
-    return fail(MOONS_ERR_ARG, "moons_nrg_create: need dt > 0 and method 1, 2 or 3");
+  if (P->method < 1 || P->method > 5 || !(P->dt > 0.0))
+    return fail(MOONS_ERR_ARG, "moons_nrg_create: need dt > 0 and method 1 to 5");
+  if (P->rk_stages != 0 && P->rk_stages != 1 && P->rk_stages != 3)
+    return fail(MOONS_ERR_ARG, "moons_nrg_create: rk_stages must be 1 or 3");
   if (T->dl != MOONS_DL_CC || T->m != m) return fail(MOONS_ERR_ARG, "moons_nrg_create: T must be cell-centred on the mesh");
-  if (P->advection)
-    for (int d = 0; d < 3; ++d)
-      if (!U || !U[d] || U[d]->dl != (1 << d) || U[d]->m != m)
-        return fail(MOONS_ERR_ARG, "moons_nrg_create: advection needs U as the three face components on the mesh");
-  if (P->method == 3 && !k) return fail(MOONS_ERR_ARG, "moons_nrg_create: method 3 needs k");
+  if ((P->advection || P->KE_diffusion || P->viscous_dissipation) && !is_face_set(U, m))
+    return fail(MOONS_ERR_ARG, "moons_nrg_create: advection, KE diffusion and viscous dissipation need U as the three "
+                               "face components on the mesh");
+  if (P->method >= 3 && !k) return fail(MOONS_ERR_ARG, "moons_nrg_create: methods 3-5 need k");
   moons_nrg_s *S = new moons_nrg_s();
   S->m = m;
   S->P = *P;
+  if (S->P.rk_stages == 0) S->P.rk_stages = 1;
   S->T = T;
   for (int d = 0; d < 3; ++d) S->U[d] = U ? U[d] : nullptr;
   int st = MOONS_OK;
-  moons_field_s **cc[] = {&S->F, &S->Fnm1, &S->Tnm1, &S->rhs, &S->tcc};
+  moons_field_s **cc[] = {&S->F, &S->Fnm1, &S->L, &S->Tnm1, &S->rhs, &S->tcc, &S->tcc2};
   for (auto f : cc)
     if (!st) st = fcreate(m, MOONS_DL_CC, f);
   for (int d = 0; d < 3 && !st; ++d) st = fcreate(m, 1 << d, &S->tF[d]);
+  if (P->KE_diffusion || P->viscous_dissipation)
+    for (int d = 0; d < 3 && !st; ++d) st = fcreate(m, MOONS_DL_CC, &S->UCC[d]);
   if (!st) st = moons_field_fill(S->F, 0.0);
+  if (!st) st = moons_field_fill(S->L, 0.0);
   if (!st) st = moons_field_copy(S->Tnm1, T);
-  if (!st && P->method == 3) {
+  if (!st && P->volumetric_heating) {
+    // add_volumetric_heating_Nuclear (energy_sources.f90:95-113): a = L = 1,
+    // m_ = a/L, F = scale 2 m_/(1 - exp(-2 m_)), q(j) = F exp(-m_ (y_c(j) + 1)),
+    // on every cell (ghosts included); formed once on the host
+    const double a = 1.0, Lh = 1.0, mm = a / Lh;
+    const double Fc = P->volumetric_heating_scale * 2.0 * mm / (1.0 - std::exp(-2.0 * mm));
+    const moons::HostCoords &cy = m->hc[1];
+    std::vector<double> q((size_t)T->n);
+    for (int kk = 0; kk < T->s[2]; ++kk)
+      for (int j = 0; j < T->s[1]; ++j) {
+        const double v = Fc * std::exp(-mm * (cy.hc[j] + 1.0));
+        for (int i = 0; i < T->s[0]; ++i) q[((size_t)kk * T->s[1] + j) * T->s[0] + i] = v;
+      }
+    st = fcreate(m, MOONS_DL_CC, &S->qv);
+    if (!st) st = moons_field_upload(S->qv, q.data(), q.size());
+  }
+  if (!st && P->method >= 3) {
     st = moons_pcg_sf_create(m, MOONS_OP_NRG_DIFFUSION, MOONS_PREC_LAP_SF, T, &S->pcg);
     if (!st) st = moons_pcg_set_isp(S->pcg, P->iter_max, P->tol_rel, P->tol_abs, P->n_skip);
     if (!st) st = moons_pcg_sf_set_k(S->pcg, k);
@@ -678,14 +813,31 @@ int moons_nrg_destroy(moons_nrg_t S) {
   return MOONS_OK;
 }
 
+int moons_nrg_set_joule(moons_nrg_t S, moons_field_t J[3], moons_field_t sigma_inv_cc) {
+  if (!S || !J || !sigma_inv_cc) return fail(MOONS_ERR_ARG, "moons_nrg_set_joule: null argument");
+  const int edl[3] = {6, 5, 3}; // x, y, z edges
+  for (int d = 0; d < 3; ++d)
+    if (!J[d] || J[d]->dl != edl[d] || J[d]->m != S->m)
+      return fail(MOONS_ERR_ARG, "moons_nrg_set_joule: J must be the x, y and z edge fields on the energy mesh");
+  if (sigma_inv_cc->dl != MOONS_DL_CC || sigma_inv_cc->m != S->m)
+    return fail(MOONS_ERR_ARG, "moons_nrg_set_joule: sigmaInv_CC must be cell-centred on the energy mesh");
+  for (int d = 0; d < 3; ++d) S->J[d] = J[d];
+  S->sig = sigma_inv_cc;
+  return MOONS_OK;
+}
+
 int moons_nrg_step(moons_nrg_t S, int n_steps, moons_nrg_info *info) {
   if (!S || n_steps < 0) return fail(MOONS_ERR_ARG, "moons_nrg_step: bad argument");
+  if (S->P.joule_heating && (!S->J[0] || !S->sig))
+    return fail(MOONS_ERR_STATE, "moons_nrg_step: Joule heating needs moons_nrg_set_joule");
   MOONS_HIP(hipEventRecord(S->ev0, stream()));
   int st;
-  for (int n = 0; n < n_steps; ++n) {
-    if ((st = nrg_sources(S))) return st;
-    if ((st = nrg_march(S))) return st;
-  }
+  for (int n = 0; n < n_steps; ++n)
+    for (S->stage = 0; S->stage < S->P.rk_stages; ++S->stage) {
+      if ((st = nrg_sources(S))) return st;
+      if ((st = nrg_march(S))) return st;
+    }
+  S->stage = 0;
   MOONS_HIP(hipEventRecord(S->ev1, stream()));
   MOONS_HIP(hipEventSynchronize(S->ev1));
   if (info) {
@@ -703,6 +855,7 @@ int moons_nrg_field(moons_nrg_t S, int which, moons_field_t *out) {
     case MOONS_NRG_F: *out = S->F; return MOONS_OK;
     case MOONS_NRG_FNM1: *out = S->Fnm1; return MOONS_OK;
     case MOONS_NRG_TNM1: *out = S->Tnm1; return MOONS_OK;
+    case MOONS_NRG_L: *out = S->L; return MOONS_OK;
     default: return fail(MOONS_ERR_ARG, "moons_nrg_field: unknown field");
   }
 }

@@ -284,40 +284,153 @@ class MHDState:
             self.solve_induction()
 
 
+# the 3-stage low-storage RK coefficients (init_RKP, code/handwritten/var_set/RK_params_extend.f90:29-62):
+# gamma, zeta, alpha per stage, beta = alpha; all ones when RK is inactive
+RK3 = {"gamma": (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0), "zeta": (0.0, -17.0 / 60.0, -5.0 / 12.0),
+       "alpha": (4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0)}
+TWO_THIRDS, FOUR_THIRDS, NEG_ONE_THIRD = 2.0 / 3.0, 4.0 / 3.0, -1.0 / 3.0  # time_marching_methods_SF.f90:35-37
+
+
+def cell_center_velocity(g, U):
+    """face2CellCenter of each component (ops_interp.f90:232-240): N -> CC along d."""
+    return [np.asfortranarray(O.interp(g, U[d], 1 << d, 0, d)) for d in range(3)]
+
+
+def edge2cellcenter(g, J):
+    """edge2CellCenter_VF (ops_interp.f90:688-696, edge2CellCenter_SF :415-431):
+    component d goes edge -> face (faceDir = d+1 mod 3, one interpolation
+    along the third direction) -> cell centre (along faceDir)."""
+    out = []
+    for d in range(3):
+        fd = (d + 1) % 3
+        od = 3 - d - fd
+        face = O.interp(g, J[d], EDGE_DL[d], 1 << fd, od)
+        out.append(np.asfortranarray(O.interp(g, face, 1 << fd, 0, fd)))
+    return out
+
+
+def lap_uniform_cc(g, f):
+    """lapUniformCoeff_SF (ops_discrete.f90:101-110) of a cell-centred field:
+    collocated second derivatives along x (assign), y and z (add)."""
+    t = g.zeros(0)
+    O.del_op(g, t, 0, f, 0, 2, 0, 0, 1)
+    O.del_op(g, t, 0, f, 0, 2, 1, 0, 2)
+    O.del_op(g, t, 0, f, 0, 2, 2, 0, 2)
+    return t
+
+
+def ke_diffusion_term(g, U_CC):
+    """add_KE_diffusion (energy_sources.f90:50-63) before the scaling:
+    lap(Ux^2 + Uy^2 + Uz^2); square(VF) then add(SF, VF) = x + y + z."""
+    sq = [u * u for u in U_CC]
+    t = np.asfortranarray(sq[0] + sq[1] + sq[2])
+    return lap_uniform_cc(g, t)
+
+
+def viscous_dissipation_term(g, U_CC):
+    """add_viscous_dissipation (energy_sources.f90:65-76) before the scaling:
+    grad_TF (ops_discrete.f90:313-321): G[c][e] = d U_c / d x_e, collocated
+    (del n = 1, pad 0) on the cell centres; square(TF); add(SF, TF)
+    (TF_extend.f90:248-255) sums G[x][x], G[y][x], G[z][x], G[x][y], ...,
+    G[z][z] left to right (GF_add.f90:137-171)."""
+    G = [[None] * 3 for _ in range(3)]
+    for c in range(3):
+        for e in range(3):
+            t = g.zeros(0)
+            O.del_op(g, t, 0, U_CC[c], 0, 1, e, 0, 1)
+            G[c][e] = t
+    acc = None
+    for e in range(3):
+        for c in range(3):
+            sq = G[c][e] * G[c][e]
+            acc = sq if acc is None else acc + sq
+    return np.asfortranarray(acc)
+
+
+def joule_heating_term(g, J, sigma_inv_cc):
+    """add_joule_heating (energy_sources.f90:78-93) before the scaling:
+    (Jx_cc^2 + Jy_cc^2 + Jz_cc^2) * sigmaInv_CC."""
+    jc = edge2cellcenter(g, J)
+    sq = [j * j for j in jc]
+    t = sq[0] + sq[1] + sq[2]
+    return np.asfortranarray(t * sigma_inv_cc)
+
+
+def volumetric_heating_profile(g, scale):
+    """add_volumetric_heating_Nuclear (energy_sources.f90:95-113) before its
+    final scaling: F exp(-m (y_c + 1)) with a = L = 1, m = a/L,
+    F = scale 2 m / (1 - exp(-2 m)) (scalar libm exp, as the library's host code)."""
+    import math
+
+    a, L = 1.0, 1.0
+    mm = a / L
+    Fc = scale * 2.0 * mm / (1.0 - math.exp(-2.0 * mm))
+    hc = g.coord(1, 1)
+    prof = np.array([Fc * math.exp(-mm * (hc[j] + 1.0)) for j in range(len(hc))])
+    s = g.shape(0)
+    return np.asfortranarray(np.broadcast_to(prof[None, :, None], s))
+
+
 class EnergyState:
     """The energy equation of one step (solve_energy,
     governing_equations/energy_extend.f90:236-262, after
     add_all_energy_sources, solvers/energy/add_all_energy_sources.f90:21-80):
 
-      Fnm1 = F; F = 0
+      Fnm1 = F; (RK active: L = 0); F = 0
       advection: F += scale * (-div(cellCenter2Face(T) * U))   add_advection (energy_sources.f90:26-38)
-      diffusion: F += scale * lap(T), lapUniformCoeff_SF       add_diffusion (energy_sources.f90:40-48)
+      diffusion: (RK active: L, else F) += scale * lap(T)      add_diffusion (energy_sources.f90:40-48)
+      KE diffusion: F += scale * lap(|U_CC|^2)                 add_KE_diffusion (:50-63)
+      viscous dissipation: F += scale * grad(U_CC):grad(U_CC)  add_viscous_dissipation (:65-76)
+      Joule heating: F += scale * |J_CC|^2 sigmaInv_CC         add_joule_heating (:78-93)
+      volumetric heating: F += scale * q(y)                    add_volumetric_heating_Nuclear (:95-113)
       method 1: T = T + F dt             Euler_time_no_diff_Euler_sources_SF (time_marching_methods_SF.f90:154-170)
       method 2: T = T + dt AB2(F, Fnm1)  Euler_time_no_diff_AB2_sources_SF (:135-152)
       method 3: Tnm1 = T; solve (I + c div(k grad)) T = T + dt AB2(F, Fnm1), c = dt coeff_implicit
                                          Euler_time_AB2_sources_SF (:71-98)
+      method 4: rhs = 2/3 dt AB2(F, Fnm1) + 4/3 T - 1/3 Tnm1; Tnm1 = T; c = dt 2/3 coeff_implicit
+                                         O2_BDF_time_AB2_sources_SF (:42-69)
+      method 5: rhs = F dt gamma + Fnm1 dt zeta + L dt alpha + T; Tnm1 = T; c = dt beta coeff_implicit
+                                         Euler_time_RK_sources_SF (:100-129)
 
-    U is the face velocity the step reads (embedFace is the identity on one
-    domain).  k are the face arrays bound to the T solver's operator.  The
-    Dirichlet wall planes of assign_wall_Dirichlet do not exist on a
-    cell-centred field, so that call changes nothing here.
+    With rk_stages = 3 one step runs the three RK stages (sources, then the
+    march with that stage's coefficients), as the stage loop of solve_MOONS
+    (MOONS/MOONS_solver.f90:70-90); with 1 the coefficients are all one.
+    U is the face velocity the step reads (embedFace and embedCC are the
+    identity on one domain; U_CC = face2CellCenter(U), which the momentum
+    solve leaves behind, momentum_extend.f90:370).  k are the face arrays bound
+    to the T solver's operator.  The Dirichlet wall planes of
+    assign_wall_Dirichlet do not exist on a cell-centred field, so that call
+    changes nothing here.
     """
 
     def __init__(self, g, T, Tbcs, k, dt, method=3, coeff_implicit=-1.0, advection=(True, -1.0),
-                 diffusion=(False, 1.0), isp=(5, 1e-6, 1e-13, 1), all_neumann=False):
+                 diffusion=(False, 1.0), isp=(5, 1e-6, 1e-13, 1), all_neumann=False, KE_diffusion=(False, 1.0),
+                 viscous_dissipation=(False, 1.0), joule_heating=(False, 1.0), volumetric_heating=(False, 1.0),
+                 rk_stages=1, J=None, sigma_inv_cc=None):
         self.g = g
         self.T = np.array(T, dtype=np.float64, order="F")  # a copy: the step advances it in place
         self.Tbcs, self.k = Tbcs, [np.asfortranarray(a) for a in k]
         self.dt, self.method, self.coeff_implicit = dt, method, coeff_implicit
         self.advection, self.diffusion, self.isp, self.all_neumann = advection, diffusion, isp, all_neumann
+        self.KE_diffusion, self.viscous_dissipation = KE_diffusion, viscous_dissipation
+        self.joule_heating, self.volumetric_heating = joule_heating, volumetric_heating
+        self.rk_stages, self.stage = rk_stages, 0
+        self.J, self.sigma_inv_cc = J, sigma_inv_cc
         self.F = g.zeros(0)
         self.Fnm1 = g.zeros(0)
+        self.L = g.zeros(0)
         self.Tnm1 = self.T.copy(order="F")
         self.iters = 0
 
+    def rk(self, name):
+        return RK3[name][self.stage] if self.rk_stages == 3 else 1.0
+
     def sources(self, U):
         g = self.g
+        rk_active = self.rk_stages == 3
         self.Fnm1 = self.F.copy(order="F")
+        if rk_active:
+            self.L = g.zeros(0)
         F = g.zeros(0)
         add, scale = self.advection
         if add:
@@ -327,28 +440,56 @@ class EnergyState:
             F = F + t * scale
         add, scale = self.diffusion
         if add:
-            t = g.zeros(0)
-            O.del_op(g, t, 0, self.T, 0, 2, 0, 0, 1)
-            O.del_op(g, t, 0, self.T, 0, 2, 1, 0, 2)
-            O.del_op(g, t, 0, self.T, 0, 2, 2, 0, 2)
-            F = F + t * scale
+            t = lap_uniform_cc(g, self.T)
+            if rk_active:
+                self.L = np.asfortranarray(self.L + t * scale)
+            else:
+                F = F + t * scale
+        U_CC = cell_center_velocity(g, U) if (self.KE_diffusion[0] or self.viscous_dissipation[0]) else None
+        add, scale = self.KE_diffusion
+        if add:
+            F = F + ke_diffusion_term(g, U_CC) * scale
+        add, scale = self.viscous_dissipation
+        if add:
+            F = F + viscous_dissipation_term(g, U_CC) * scale
+        add, scale = self.joule_heating
+        if add:
+            F = F + joule_heating_term(g, self.J, self.sigma_inv_cc) * scale
+        add, scale = self.volumetric_heating
+        if add:
+            F = F + volumetric_heating_profile(g, scale) * scale
         self.F = np.asfortranarray(F)
 
     def solve(self):
         if self.method == 1:
             self.T = np.asfortranarray(self.T + self.F * self.dt)
             return
-        t = self.F * 1.5
-        t = t + self.Fnm1 * (-0.5)
-        t = t * self.dt
-        if self.method == 2:
-            self.T = np.asfortranarray(self.T + t)
-            return
-        rhs = np.asfortranarray(t + self.T)
+        if self.method == 5:
+            t = self.F * (self.dt * self.rk("gamma"))
+            t = t + self.Fnm1 * (self.dt * self.rk("zeta"))
+            t = t + self.L * (self.dt * self.rk("alpha"))
+            rhs = np.asfortranarray(t + self.T)
+            c = self.dt * self.rk("alpha") * self.coeff_implicit  # beta = alpha
+        else:
+            t = self.F * 1.5
+            t = t + self.Fnm1 * (-0.5)
+            if self.method == 4:
+                t = t * (TWO_THIRDS * self.dt)
+                t = t + self.T * FOUR_THIRDS
+                rhs = np.asfortranarray(t + self.Tnm1 * NEG_ONE_THIRD)
+                c = self.dt * TWO_THIRDS * self.coeff_implicit
+            else:
+                t = t * self.dt
+                if self.method == 2:
+                    self.T = np.asfortranarray(self.T + t)
+                    return
+                rhs = np.asfortranarray(t + self.T)
+                c = self.dt * 1.0 * self.coeff_implicit
         self.Tnm1 = self.T.copy(order="F")
-        c = self.dt * 1.0 * self.coeff_implicit
         self.iters, _, _, _ = O.pcg_nrg(self.g, self.Tbcs, self.all_neumann, self.k, c, self.T, rhs, *self.isp)
 
     def step(self, U):
-        self.sources(U)
-        self.solve()
+        for self.stage in range(self.rk_stages if self.rk_stages == 3 else 1):
+            self.sources(U)
+            self.solve()
+        self.stage = 0

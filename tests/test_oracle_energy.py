@@ -2,8 +2,11 @@
 nrg_diffusion operator against the pinned Laplacian (k = 1) and the identity
 (k = 0), symmetry and positivity of its volume-weighted matrix, its PCG
 against a dense solve, the energy sources on fields they differentiate
-exactly, the explicit methods' closed form with the AB2 start-up (Fnm1 = 0),
-and backward Euler on a discrete eigenmode."""
+exactly (advection, diffusion, KE diffusion, viscous dissipation, Joule and
+volumetric heating), the explicit methods' closed form with the AB2 start-up
+(Fnm1 = 0), backward Euler on a discrete eigenmode, BDF2's and the 3-stage
+RK's exact integration of a constant source, and the RK's explicit diffusion
+weights."""
 import numpy as np
 import pytest
 
@@ -189,3 +192,134 @@ def test_backward_euler_on_a_discrete_eigenmode():
                         isp=(200, 1e-14, 1e-300, 1))
     st0.step(None)
     np.testing.assert_allclose(interior(st0.T), interior(T0), rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# KE diffusion, viscous dissipation, Joule and volumetric heating; methods 4, 5
+# ---------------------------------------------------------------------------
+
+def ugrid(N=(8, 7, 9)):
+    return O.Grid(*[O.uniform_nodes(0.0, 1.0, n) for n in N])
+
+
+def coords(g, dl):
+    """Per-direction coordinate arrays of a field at location dl (N: nodes, else centres)."""
+    return [g.coord(d, 0) if (dl >> d) & 1 else g.coord(d, 1) for d in range(3)]
+
+
+def field_of(g, dl, fn):
+    x, y, z = coords(g, dl)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    return np.asfortranarray(fn(X, Y, Z))
+
+
+def deep(a, w=2):
+    return a[w:-w, w:-w, w:-w]
+
+
+def test_cell_center_velocity_of_linear_faces_is_exact():
+    g = grid()
+    U = [field_of(g, 1 << d, lambda X, Y, Z, d=d: (1.0 + d) * (X, Y, Z)[d] + 0.5 * Z) for d in range(3)]
+    U_CC = S.cell_center_velocity(g, U)
+    for d in range(3):
+        ref = field_of(g, 0, lambda X, Y, Z, d=d: (1.0 + d) * (X, Y, Z)[d] + 0.5 * Z)
+        np.testing.assert_allclose(U_CC[d], ref, rtol=0, atol=1e-13)
+
+
+def test_ke_diffusion_of_linear_velocity_is_constant():
+    # U = (x, 2y, 3z) -> |U|^2 = x^2 + 4y^2 + 9z^2 -> lap = 2 + 8 + 18 = 28
+    g = ugrid()
+    U = [field_of(g, 1 << d, lambda X, Y, Z, d=d: (1.0 + d) * (X, Y, Z)[d]) for d in range(3)]
+    t = S.ke_diffusion_term(g, S.cell_center_velocity(g, U))
+    np.testing.assert_allclose(deep(t), 28.0, rtol=1e-9)
+
+
+def test_viscous_dissipation_is_the_full_gradient_norm():
+    # U_x = 3y + z, U_y = -2x + 5z, U_z = 4x: grad U : grad U = 9 + 1 + 4 + 25 + 16 = 55;
+    # a diagonal-only or transposed-square mistake gives 0 or another value
+    g = ugrid()
+    fns = [lambda X, Y, Z: 3.0 * Y + Z, lambda X, Y, Z: -2.0 * X + 5.0 * Z, lambda X, Y, Z: 4.0 * X]
+    U = [field_of(g, 1 << d, fns[d]) for d in range(3)]
+    t = S.viscous_dissipation_term(g, S.cell_center_velocity(g, U))
+    np.testing.assert_allclose(t[1:-1, 1:-1, 1:-1], 55.0, rtol=1e-9)
+    # stretched grid: the collocated first derivative is still exact for linear fields
+    g2 = grid(Ns=(7, 8, 9))
+    U2 = [field_of(g2, 1 << d, fns[d]) for d in range(3)]
+    t2 = S.viscous_dissipation_term(g2, S.cell_center_velocity(g2, U2))
+    np.testing.assert_allclose(deep(t2), 55.0, rtol=1e-9)
+
+
+def test_joule_heating_of_linear_edge_currents():
+    # J_x = y, J_y = z, J_z = x on their edges: at the cell centres (y_c, z_c, x_c),
+    # times sigmaInv_CC = 1 + x_c: (y_c^2 + z_c^2 + x_c^2)(1 + x_c)
+    g = grid()
+    fns = [lambda X, Y, Z: Y, lambda X, Y, Z: Z, lambda X, Y, Z: X]
+    J = [field_of(g, S.EDGE_DL[d], fns[d]) for d in range(3)]
+    sig = field_of(g, 0, lambda X, Y, Z: 1.0 + X)
+    t = S.joule_heating_term(g, J, sig)
+    ref = field_of(g, 0, lambda X, Y, Z: (Y * Y + Z * Z + X * X) * (1.0 + X))
+    np.testing.assert_allclose(t, ref, rtol=1e-13, atol=1e-14)
+    Jc = S.edge2cellcenter(g, J)
+    for d in range(3):
+        np.testing.assert_allclose(Jc[d], field_of(g, 0, fns[d]), rtol=0, atol=1e-14)
+
+
+def test_volumetric_heating_profile_closed_form():
+    g = grid()
+    q = S.volumetric_heating_profile(g, 0.7)
+    yc = g.coord(1, 1)
+    F = 0.7 * 2.0 / (1.0 - np.exp(-2.0))
+    for j, y in enumerate(yc):
+        np.testing.assert_allclose(q[:, j, :], F * np.exp(-(y + 1.0)), rtol=1e-15)
+
+
+def heated_state(g, method, rk_stages=1, diffusion=(False, 1.0), T0=None, qscale=0.7):
+    T = g.zeros(0) if T0 is None else T0
+    k0 = faces(g, lambda d, s: np.zeros(s))
+    return S.EnergyState(g, T, O.BCs.make(["Neumann"] * 6), k0, dt=1e-2, method=method, advection=(False, 0.0),
+                         diffusion=diffusion, isp=(50, 1e-14, 1e-300, 1), all_neumann=False,
+                         volumetric_heating=(True, qscale), rk_stages=rk_stages)
+
+
+def test_bdf2_grows_linearly_under_a_constant_source():
+    # k = 0 (identity operator), constant source q: the AB2 start-up (Fnm1 = 0)
+    # and BDF2's 4/3, -1/3 weights give T_n = n dt q exactly in closed form
+    g = ugrid((6, 5, 4))
+    st = heated_state(g, 4)
+    q = S.volumetric_heating_profile(g, 0.7) * 0.7
+    U = [g.zeros(1 << d) for d in range(3)]
+    for n in range(1, 5):
+        st.step(U)
+        np.testing.assert_allclose(interior(st.T), interior(n * 1e-2 * q), rtol=1e-12)
+
+
+def test_rk3_step_integrates_a_constant_source_exactly():
+    # gamma + zeta sum to one over the three stages: one step adds dt q
+    g = ugrid((6, 5, 4))
+    st = heated_state(g, 5, rk_stages=3)
+    q = S.volumetric_heating_profile(g, 0.7) * 0.7
+    U = [g.zeros(1 << d) for d in range(3)]
+    st.step(U)
+    np.testing.assert_allclose(interior(st.T), interior(1e-2 * q), rtol=1e-12)
+    st.step(U)
+    np.testing.assert_allclose(interior(st.T), interior(2e-2 * q), rtol=1e-12)
+    # inactive RK: all coefficients one, so F dt + Fnm1 dt: (1 + 0) then (1 + 1)
+    s1 = heated_state(g, 5, rk_stages=1)
+    s1.step(U)
+    np.testing.assert_allclose(interior(s1.T), interior(1e-2 * q), rtol=1e-12)
+    s1.step(U)
+    np.testing.assert_allclose(interior(s1.T), interior(3e-2 * q), rtol=1e-12)
+
+
+def test_rk3_explicit_diffusion_goes_through_L_with_alpha():
+    # T = x^2 on a uniform grid, diffusion scale s, k = 0: lap T = 2 away from
+    # the walls at every stage, so one step adds dt s 2 (alpha_1 + alpha_2 + alpha_3) = dt s
+    g = ugrid((16, 5, 5))
+    T0 = field_of(g, 0, lambda X, Y, Z: X * X)
+    st = heated_state(g, 5, rk_stages=3, diffusion=(True, 0.3), T0=T0, qscale=0.0)
+    U = [g.zeros(1 << d) for d in range(3)]
+    st.step(U)
+    inner = (slice(5, -5), slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose(st.T[inner] - T0[inner], 1e-2 * 0.3 * 2.0 * 0.5, rtol=1e-9)
+    # L holds the last stage's diffusion
+    np.testing.assert_allclose(st.L[inner], 0.3 * 2.0, rtol=1e-9)
