@@ -85,7 +85,7 @@ def build_fortran_shim() -> str | None:
         _run([FLANG, "-O2", "-fPIC", "-module-dir", FORTRAN_BUILD, "-c", s, "-o", o])
         objs.append(o)
     exe = None
-    for name in ("ppe_driver", "vf_driver"):
+    for name in ("ppe_driver", "vf_driver", "nrg_driver"):
         drv = os.path.join(FORTRAN_DIR, name + ".f90")
         out = os.path.join(FORTRAN_BUILD, name)
         if os.path.exists(drv):

@@ -8,6 +8,10 @@
 !   init(FFT, m, dir), solve(FFT, u, f), clean_div(FFT, ...)
 !                           init_FFT / solve (code/handwritten/FFT_solver/FFT_solver_extend.f90:43-137)
 !                           and clean_div_FFT (clean_divergence.f90:61-83)
+!   init(nrg, ...), set_joule(nrg, ...), solve(nrg, T, U, ...)
+!                           the energy equation's step: add_all_energy_sources then
+!                           solve_energy (code/handwritten/solvers/energy/add_all_energy_sources.f90:21-80,
+!                           code/handwritten/governing_equations/energy_extend.f90:236-262)
 !   export(f, path), import(f, path)
 !                           grid_field restart primitives of a device field
 !                           (code/generated_code/grid_field/grid_field.f90:108-156;
@@ -25,8 +29,8 @@ module moons_amd_PCG_mod
   implicit none
   private
 
-  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu, FFT_solver_SF_gpu
-  public :: init, solve, delete, clean_div, update_MFP
+  public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu, FFT_solver_SF_gpu, energy_gpu
+  public :: init, solve, delete, clean_div, update_MFP, set_joule
   public :: init_CC, init_Face, set_BCs
   public :: export, import, export_raw, import_raw
 
@@ -64,10 +68,21 @@ module moons_amd_PCG_mod
     type(SF_gpu) :: u, f
   end type
 
+  ! The energy step over host arrays: T (cell-centred, with its BCs) and the
+  ! momentum's face velocity U are uploaded at each solve, T downloaded after;
+  ! F, Fnm1, L and Tnm1 stay on the device between steps.
+  type energy_gpu
+    type(c_ptr) :: h = c_null_ptr
+    type(moons_nrg_info) :: info
+    type(SF_gpu) :: T, U(3), k(3), J(3), sig
+  end type
+
   interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu
-                    module procedure init_PCG_VF_gpu; module procedure init_FFT_gpu; end interface
+                    module procedure init_PCG_VF_gpu; module procedure init_FFT_gpu
+                    module procedure init_energy_gpu; end interface
   interface solve;  module procedure solve_PCG_SF_gpu; module procedure solve_PCG_VF_gpu
-                    module procedure solve_FFT_gpu; end interface
+                    module procedure solve_FFT_gpu; module procedure solve_energy_gpu; end interface
+  interface set_joule; module procedure set_joule_energy_gpu; end interface
   interface clean_div; module procedure clean_div_PCG_gpu; module procedure clean_div_FFT_gpu; end interface
   interface update_MFP; module procedure update_MFP_PCG_VF_gpu; end interface
   interface export; module procedure export_SF_gpu; end interface
@@ -76,7 +91,7 @@ module moons_amd_PCG_mod
   interface import_raw; module procedure import_raw_SF_gpu; end interface
   interface delete; module procedure delete_mesh_gpu; module procedure delete_SF_gpu;
                     module procedure delete_PCG_SF_gpu; module procedure delete_PCG_VF_gpu
-                    module procedure delete_FFT_gpu; end interface
+                    module procedure delete_FFT_gpu; module procedure delete_energy_gpu; end interface
 
 contains
 
@@ -368,6 +383,95 @@ contains
       call delete(PCG%x(d)); call delete(PCG%b(d))
       if (c_associated(PCG%k(d)%h)) call delete(PCG%k(d))
     enddo
+  end subroutine
+
+  ! init_energy: T's BCs (types, vals), the step's parameters (moons_nrg_params:
+  ! method, dt, coeff_implicit, the sources' flags and scales, the T solver's
+  ! ISP, rk_stages) and, for methods 3-5, the face arrays k1, k2, k3 bound to
+  ! the implicit operator (absent: zeros, as init_energy binds,
+  ! energy_extend.f90:103,129-130).
+  subroutine init_energy_gpu(nrg, m, T_types, T_vals, params, k1, k2, k3)
+    type(energy_gpu), intent(inout) :: nrg
+    type(mesh_gpu), intent(in) :: m
+    integer(c_int), intent(in) :: T_types(6)
+    real(c_double), intent(in) :: T_vals(6)
+    type(moons_nrg_params), intent(in) :: params
+    real(c_double), intent(in), optional, contiguous :: k1(:, :, :), k2(:, :, :), k3(:, :, :)
+    type(c_ptr) :: hU(3), hk(3)
+    integer :: d
+    call init_CC(nrg%T, m)
+    call set_BCs(nrg%T, T_types, T_vals, .false.)
+    call moons_check(moons_field_fill(nrg%T%h, 0.0_c_double), 'fill')
+    do d = 1, 3
+      call init_Face(nrg%U(d), m, d)
+      call init_Face(nrg%k(d), m, d)
+      call moons_check(moons_field_fill(nrg%U(d)%h, 0.0_c_double), 'fill')
+      call moons_check(moons_field_fill(nrg%k(d)%h, 0.0_c_double), 'fill')
+      hU(d) = nrg%U(d)%h
+      hk(d) = nrg%k(d)%h
+    enddo
+    if (present(k1) .and. present(k2) .and. present(k3)) then
+      call moons_check(moons_field_upload(nrg%k(1)%h, k1, int(size(k1), c_size_t)), 'upload k')
+      call moons_check(moons_field_upload(nrg%k(2)%h, k2, int(size(k2), c_size_t)), 'upload k')
+      call moons_check(moons_field_upload(nrg%k(3)%h, k3, int(size(k3), c_size_t)), 'upload k')
+    endif
+    call moons_check(moons_nrg_create(m%h, params, nrg%T%h, hU, hk, nrg%h), 'init_energy')
+  end subroutine
+
+  ! The induction's J (x, y, z edge arrays) and sigmaInv_CC that Joule heating
+  ! reads (ind%J, ind%sigmaInv_CC), copied to the device.
+  subroutine set_joule_energy_gpu(nrg, m, J1, J2, J3, sigmaInv_CC)
+    type(energy_gpu), intent(inout) :: nrg
+    type(mesh_gpu), intent(in) :: m
+    real(c_double), intent(in), contiguous :: J1(:, :, :), J2(:, :, :), J3(:, :, :), sigmaInv_CC(:, :, :)
+    type(c_ptr) :: hJ(3)
+    integer(c_int) :: edl(3)
+    integer :: d
+    edl = [MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z]
+    if (.not. c_associated(nrg%J(1)%h)) then
+      do d = 1, 3
+        call init_field(nrg%J(d), m, edl(d))
+      enddo
+      call init_CC(nrg%sig, m)
+    endif
+    call moons_check(moons_field_upload(nrg%J(1)%h, J1, int(size(J1), c_size_t)), 'upload J')
+    call moons_check(moons_field_upload(nrg%J(2)%h, J2, int(size(J2), c_size_t)), 'upload J')
+    call moons_check(moons_field_upload(nrg%J(3)%h, J3, int(size(J3), c_size_t)), 'upload J')
+    call moons_check(moons_field_upload(nrg%sig%h, sigmaInv_CC, int(size(sigmaInv_CC), c_size_t)), 'upload sigmaInv')
+    do d = 1, 3
+      hJ(d) = nrg%J(d)%h
+    enddo
+    call moons_check(moons_nrg_set_joule(nrg%h, hJ, nrg%sig%h), 'set_joule')
+  end subroutine
+
+  ! One energy step (the RK stages when rk_stages = 3) on host arrays:
+  ! T in/out (ghosts included), U1, U2, U3 the momentum's face velocity.
+  subroutine solve_energy_gpu(nrg, T, U1, U2, U3, compute_norms)
+    type(energy_gpu), intent(inout) :: nrg
+    real(c_double), intent(inout), contiguous :: T(:, :, :)
+    real(c_double), intent(in), contiguous :: U1(:, :, :), U2(:, :, :), U3(:, :, :)
+    logical, intent(in) :: compute_norms
+    if (any(shape(T) .ne. nrg%T%s)) error stop 'Error: T must have the cell-centred shape in solve_energy_gpu'
+    call moons_check(moons_field_upload(nrg%T%h, T, int(size(T), c_size_t)), 'upload T')
+    call moons_check(moons_field_upload(nrg%U(1)%h, U1, int(size(U1), c_size_t)), 'upload U')
+    call moons_check(moons_field_upload(nrg%U(2)%h, U2, int(size(U2), c_size_t)), 'upload U')
+    call moons_check(moons_field_upload(nrg%U(3)%h, U3, int(size(U3), c_size_t)), 'upload U')
+    call moons_check(moons_nrg_step(nrg%h, 1_c_int, nrg%info), 'solve_energy')
+    call moons_check(moons_field_download(nrg%T%h, T, int(size(T), c_size_t)), 'download T')
+    if (compute_norms) write(*, '(a,i8)') ' energy T-solve iterations = ', nrg%info%iterations
+  end subroutine
+
+  subroutine delete_energy_gpu(nrg)
+    type(energy_gpu), intent(inout) :: nrg
+    integer :: d
+    if (c_associated(nrg%h)) call moons_check(moons_nrg_destroy(nrg%h), 'delete energy')
+    nrg%h = c_null_ptr
+    call delete(nrg%T)
+    do d = 1, 3
+      call delete(nrg%U(d)); call delete(nrg%k(d))
+      if (c_associated(nrg%J(d)%h)) call delete(nrg%J(d))
+    enddo
+    if (c_associated(nrg%sig%h)) call delete(nrg%sig)
   end subroutine
 
   subroutine export_SF_gpu(f, path)

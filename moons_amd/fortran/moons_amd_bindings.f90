@@ -39,6 +39,36 @@ module moons_amd_bindings
     real(c_double) :: ms
   end type
 
+  ! moons_nrg_params / moons_nrg_info of include/moons_amd.h
+  integer(c_int), parameter, public :: MOONS_NRG_F = 1, MOONS_NRG_FNM1 = 2, MOONS_NRG_TNM1 = 3, MOONS_NRG_L = 4
+
+  type, bind(C), public :: moons_nrg_params
+    real(c_double) :: dt
+    integer(c_int) :: method
+    real(c_double) :: coeff_implicit
+    integer(c_int) :: advection
+    real(c_double) :: advection_scale
+    integer(c_int) :: diffusion
+    real(c_double) :: diffusion_scale
+    integer(c_int) :: iter_max
+    real(c_double) :: tol_rel, tol_abs
+    integer(c_int) :: n_skip
+    integer(c_int) :: KE_diffusion
+    real(c_double) :: KE_diffusion_scale
+    integer(c_int) :: viscous_dissipation
+    real(c_double) :: viscous_dissipation_scale
+    integer(c_int) :: joule_heating
+    real(c_double) :: joule_heating_scale
+    integer(c_int) :: volumetric_heating
+    real(c_double) :: volumetric_heating_scale
+    integer(c_int) :: rk_stages
+  end type
+
+  type, bind(C), public :: moons_nrg_info
+    integer(c_int) :: iterations
+    real(c_double) :: ms
+  end type
+
   type, bind(C), public :: moons_pcg_info
     integer(c_int) :: iterations
     integer(c_int) :: exit_loop
@@ -62,6 +92,7 @@ module moons_amd_bindings
   public :: moons_pcg_vf_solve
   public :: moons_mhd_create, moons_mhd_destroy, moons_mhd_step, moons_mhd_graph_capture, moons_mhd_graph_launch
   public :: moons_field_export, moons_field_import, moons_export_raw, moons_import_raw
+  public :: moons_nrg_create, moons_nrg_destroy, moons_nrg_step, moons_nrg_set_joule, moons_nrg_field
 
   interface
     integer(c_int) function moons_init(device) bind(C, name="moons_init")
@@ -304,6 +335,34 @@ module moons_amd_bindings
       import c_int, c_ptr
       type(c_ptr), value :: s, stream
       integer(c_int), value :: n
+    end function
+    integer(c_int) function moons_nrg_create(m, params, T, U, k, s) bind(C, name="moons_nrg_create")
+      import c_int, c_ptr, moons_nrg_params
+      type(c_ptr), value :: m, T
+      type(moons_nrg_params), intent(in) :: params
+      type(c_ptr), intent(in) :: U(3), k(3)
+      type(c_ptr), intent(out) :: s
+    end function
+    integer(c_int) function moons_nrg_destroy(s) bind(C, name="moons_nrg_destroy")
+      import c_int, c_ptr
+      type(c_ptr), value :: s
+    end function
+    integer(c_int) function moons_nrg_step(s, n, info) bind(C, name="moons_nrg_step")
+      import c_int, c_ptr, moons_nrg_info
+      type(c_ptr), value :: s
+      integer(c_int), value :: n
+      type(moons_nrg_info), intent(out) :: info
+    end function
+    integer(c_int) function moons_nrg_set_joule(s, J, sigma_inv_cc) bind(C, name="moons_nrg_set_joule")
+      import c_int, c_ptr
+      type(c_ptr), value :: s, sigma_inv_cc
+      type(c_ptr), intent(in) :: J(3)
+    end function
+    integer(c_int) function moons_nrg_field(s, which, f) bind(C, name="moons_nrg_field")
+      import c_int, c_ptr
+      type(c_ptr), value :: s
+      integer(c_int), value :: which
+      type(c_ptr), intent(out) :: f
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import c_size_t, c_ptr

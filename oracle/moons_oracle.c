@@ -1065,6 +1065,8 @@ static void curl_curl_diag(double *dg[3], double *const sig[3], const orc_grid *
             C[1]->n2cc_U[j] * sig[0][IDX(ss[0], i, j + 1, kk)] * C[1]->cc2n_D[j];
 }
 
+void orc_curl_curl_diagonal(double *dg[3], double *const sig[3], const orc_grid *g) { curl_curl_diag(dg, sig, g); }
+
 /* prec_mom_SF / prec_ind_VF: preconditioners.f90:86-129 */
 void orc_minv_vf(double *Minv[3], int kind, double c, double *const k[3], const orc_grid *g) {
   if (kind == ORC_OP_IND_DIFFUSION) curl_curl_diag(Minv, k, g);

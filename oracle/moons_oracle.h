@@ -121,6 +121,8 @@ void orc_multiply_wall_neumann(double *u, int dl, const orc_grid *g, const orc_b
 /* ---- auxiliaries ---- */
 void orc_volume(double *vol, int dl, const orc_grid *g);
 void orc_lap_diagonal(double *diag, int dl, const orc_grid *g);
+/* curl_curl_diagonal: hw/grid_field/GF_diagonals.f90:37-71; sig on the x, y, z edges, dg on the faces */
+void orc_curl_curl_diagonal(double *dg[3], double *const sig[3], const orc_grid *g);
 double orc_sum_pad(const double *u, const int s[3], int pad);
 /* Threads of the OpenMP loops (1, the default: sequential, the exact order). */
 void orc_set_threads(int n);

@@ -68,6 +68,7 @@ def lib():
         L.orc_grid_face_dh.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_volume.argtypes = [_dp, C.c_int, C.c_void_p]
         L.orc_lap_diagonal.argtypes = [_dp, C.c_int, C.c_void_p]
+        L.orc_curl_curl_diagonal.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_pcg_lap_sf.restype = C.c_int
         L.orc_pcg_lap_sf.argtypes = [C.c_void_p, C.c_int, C.POINTER(BCs), C.c_int, _dp, _dp, C.c_int, C.c_double,
                                      C.c_double, C.c_int, _dp, _dp]
@@ -270,6 +271,14 @@ def volume(g: Grid, dl):
     return out
 
 
+def curl_curl_diagonal(g: Grid, sig):
+    """The oracle's curl-curl diagonal: sig on the x, y, z edges -> x, y, z faces."""
+    sig = [np.asfortranarray(a, dtype=np.float64) for a in sig]
+    outs = [np.zeros(g.shape(1 << d), order="F") for d in range(3)]
+    lib().orc_curl_curl_diagonal(_p3(outs), _p3(sig), g.h)
+    return outs
+
+
 def lap_diagonal(g: Grid, dl):
     out = g.zeros(dl)
     lib().orc_lap_diagonal(_F(out), dl, g.h)
@@ -325,6 +334,10 @@ def ref():
         R.ref_robin_plane.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int]
         R.ref_face_dh.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        grid3 = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int]
+        R.ref_laplacian_diagonal.argtypes = grid3 + [C.c_int, _dp, C.c_int]
+        R.ref_curl_curl_diagonal.argtypes = grid3 + [_dp, C.c_int] * 6
+        R.ref_interp.argtypes = grid3 + [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int]
         _ref = R
     return _ref
 
@@ -390,6 +403,41 @@ def ref_face_dh(hn, face: int) -> float:
     out = C.c_double(0.0)
     ref().ref_face_dh(p(hn), len(hn), int(face), C.byref(out))
     return out.value
+
+
+def _grid_args(hs):
+    hs = [np.ascontiguousarray(h, dtype=np.float64) for h in hs]
+    return hs, [a for h in hs for a in (p(h), len(h))]
+
+
+def ref_laplacian_diagonal(g: Grid, dl: int) -> np.ndarray:
+    """laplacian_diagonal of the reference on g at data location dl."""
+    hs, ga = _grid_args(g.hn)
+    out = np.zeros(g.shape(dl), order="F")
+    ref().ref_laplacian_diagonal(*ga, dl, _F(out), out.size)
+    return out
+
+
+def ref_curl_curl_diagonal(g: Grid, sig):
+    """curl_curl_diagonal of the reference: sig on the x, y, z edges -> x, y, z faces."""
+    hs, ga = _grid_args(g.hn)
+    sig = [np.asfortranarray(a, dtype=np.float64) for a in sig]
+    outs = [np.zeros(g.shape(1 << d), order="F") for d in range(3)]
+    args = []
+    for a in sig + outs:
+        args += [_F(a), a.size]
+    ref().ref_curl_curl_diagonal(*ga, *args)
+    return outs
+
+
+def ref_interp_field(g: Grid, f, dl_in: int, dl_out: int, d: int) -> np.ndarray:
+    """interpO2_SF's per-block interpolation (C2N with end-plane extrapolation,
+    or N2C) of the reference along d (0-based)."""
+    hs, ga = _grid_args(g.hn)
+    f = np.asfortranarray(f, dtype=np.float64)
+    out = np.zeros(g.shape(dl_out), order="F")
+    ref().ref_interp(*ga, dl_out, dl_in, d + 1, _F(f), f.size, _F(out), out.size)
+    return out
 
 
 def ref_gf_export(path: str, f: np.ndarray) -> None:

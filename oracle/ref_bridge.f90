@@ -5,8 +5,10 @@
 ! oracle/moons_oracle.c against the reference's own compiled code for the
 ! parts that compile stand-alone: 1D grid distributions, derivative and
 ! interpolation stencil coefficients, the pencil kernels of stencils.f90,
-! the raw ghost-plane BC kernels, and the grid_field restart primitives and
-! Tecplot raw files written and read by the reference's own routines.
+! the raw ghost-plane BC kernels, the Laplacian and curl-curl diagonals of
+! the Jacobi preconditioners, the staggered interpolations with their ghost
+! extrapolation, and the grid_field restart primitives and Tecplot raw files
+! written and read by the reference's own routines.
 ! Nothing here is used by the product.
 module moons_ref_bridge
   use iso_c_binding
@@ -34,6 +36,8 @@ module moons_ref_bridge
   use coordinates_extend_mod, only: coord_init => init
   use overlap_mod, only: overlap
   use overlap_extend_mod, only: get_C_overlap, get_N_overlap
+  use GF_diagonals_mod, only: laplacian_diagonal, curl_curl_diagonal
+  use GF_interp_mod, only: interp_C2N, interp_N2C
   implicit none
 contains
 
@@ -304,5 +308,80 @@ contains
     call init(g, hn, hn, hn)
     call get_face_GI(gf, g, int(face))
     dh = gf%c((face + 1) / 2)%dhn%f(1)
+  end subroutine
+
+  ! A grid_field of the shape of DL (code) on g, zero-filled.
+  subroutine gf_alloc(GF, g, code)
+    type(grid_field), intent(inout) :: GF
+    type(grid), intent(in) :: g
+    integer(c_int), intent(in) :: code
+    type(data_location) :: DL
+    integer :: sh(3)
+    call dl_from_code(DL, code)
+    sh = get_shape(g, DL)
+    allocate(GF%f(sh(1), sh(2), sh(3)))
+    GF%f = 0.0_cp
+    GF%s = sh
+    GF%s_1D = sh(1) * sh(2) * sh(3)
+  end subroutine
+
+  ! laplacian_diagonal (handwritten/grid_field/GF_diagonals.f90:15-35) of a
+  ! field at data location code on the grid (hx, hy, hz); out gets n values.
+  subroutine ref_laplacian_diagonal(hx, sx, hy, sy, hz, sz, code, out, n) bind(C, name="ref_laplacian_diagonal")
+    integer(c_int), value :: sx, sy, sz, code, n
+    real(c_double), intent(in) :: hx(sx), hy(sy), hz(sz)
+    real(c_double), intent(out) :: out(n)
+    type(grid) :: g
+    type(data_location) :: DL
+    type(grid_field) :: GF
+    call init(g, hx, hy, hz)
+    call dl_from_code(DL, code)
+    call gf_alloc(GF, g, code)
+    call laplacian_diagonal(GF, g, DL)
+    out = reshape(GF%f, (/ n /))
+  end subroutine
+
+  ! curl_curl_diagonal (GF_diagonals.f90:37-71): sig_x, sig_y, sig_z on the
+  ! x, y, z edges (nsx, nsy, nsz values), dx, dy, dz on the x, y, z faces.
+  subroutine ref_curl_curl_diagonal(hx, sx, hy, sy, hz, sz, sgx, nsx, sgy, nsy, sgz, nsz, dx, ndx, dy, ndy, dz, ndz) &
+      bind(C, name="ref_curl_curl_diagonal")
+    integer(c_int), value :: sx, sy, sz, nsx, nsy, nsz, ndx, ndy, ndz
+    real(c_double), intent(in) :: hx(sx), hy(sy), hz(sz), sgx(nsx), sgy(nsy), sgz(nsz)
+    real(c_double), intent(out) :: dx(ndx), dy(ndy), dz(ndz)
+    type(grid) :: g
+    type(grid_field) :: S1, S2, S3, D1, D2, D3
+    call init(g, hx, hy, hz)
+    call gf_alloc(S1, g, 6_c_int); call gf_alloc(S2, g, 5_c_int); call gf_alloc(S3, g, 3_c_int)
+    call gf_alloc(D1, g, 1_c_int); call gf_alloc(D2, g, 2_c_int); call gf_alloc(D3, g, 4_c_int)
+    S1%f = reshape(sgx, S1%s); S2%f = reshape(sgy, S2%s); S3%f = reshape(sgz, S3%s)
+    call curl_curl_diagonal(D1, D2, D3, g, S1, S2, S3)
+    dx = reshape(D1%f, (/ ndx /)); dy = reshape(D2%f, (/ ndy /)); dz = reshape(D3%f, (/ ndz /))
+  end subroutine
+
+  ! interpO2_SF's per-block step (handwritten/ops/ops_interp.f90:190-219):
+  ! interp_C2N (with extrap_staggered of the two end planes) or interp_N2C
+  ! (handwritten/grid_field/GF_interp/GF_interp.f90:19-76,
+  ! GF_interp/GF_extrap.f90:17-44) along dir (1..3), from data location
+  ! code_in to code_out.
+  subroutine ref_interp(hx, sx, hy, sy, hz, sz, code_out, code_in, dir, fin, nin, fout, nout) &
+      bind(C, name="ref_interp")
+    integer(c_int), value :: sx, sy, sz, code_out, code_in, dir, nin, nout
+    real(c_double), intent(in) :: hx(sx), hy(sy), hz(sz), fin(nin)
+    real(c_double), intent(out) :: fout(nout)
+    type(grid) :: g
+    type(grid_field) :: GI, GO
+    integer :: e(3)
+    call init(g, hx, hy, hz)
+    call gf_alloc(GI, g, code_in)
+    call gf_alloc(GO, g, code_out)
+    GI%f = reshape(fin, GI%s)
+    e = 0
+    e(dir) = 1
+    if (btest(code_out, dir - 1)) then
+      call interp_C2N(GO, GI, g, int(dir), e(1), e(2), e(3))
+    else
+      call interp_N2C(GO, GI, e(1), e(2), e(3))
+    end if
+    fout = reshape(GO%f, (/ nout /))
   end subroutine
 end module

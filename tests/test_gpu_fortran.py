@@ -1,5 +1,6 @@
 """The Fortran host surface (moons_amd/fortran, flang-built) drives the GPU
-pressure-Poisson solve through the C ABI; its result matches the oracle."""
+pressure-Poisson solve, the face-centred implicit diffusion solves and the
+energy step through the C ABI; each result matches the oracle."""
 import os
 import subprocess
 
@@ -120,3 +121,46 @@ def test_fortran_driver_vf(tmp_path, op, c):
     num = sum(float(np.sum((p - q) ** 2)) for p, q in zip(xs, xr))
     den = sum(float(np.sum(q ** 2)) for q in xr)
     assert np.sqrt(num / den) <= 1e-10
+
+
+NRG_DRIVER = os.path.join(ROOT, "moons_amd", "fortran", "_build", "nrg_driver")
+
+
+@pytest.mark.parametrize("method,rk", [(3, 1), (5, 3)])
+def test_fortran_driver_energy(tmp_path, method, rk):
+    """init(nrg) / set_joule / solve(nrg, T, U) from Fortran host arrays, every
+    energy source on, 3 steps: T within 1e-10 of the oracle's step."""
+    from oracle import step_oracle as S
+
+    if not os.path.exists(NRG_DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    N, beta = 12, 1.05
+    hx = O.uniform_nodes(0.0, 1.0, N)
+    hy = O.roberts_both_nodes(0.0, 1.0, N, beta)
+    g = O.Grid(hx, hy, hy)
+    rng = np.random.default_rng(11)
+    tb = O.BCs.make(["Dirichlet", "Dirichlet"] + ["Neumann"] * 4, [1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    X = np.meshgrid(g.coord(0, 1), g.coord(1, 1), g.coord(2, 1), indexing="ij")[0]
+    T0 = np.asfortranarray(1.0 - X + 0.05 * np.sin(7.0 * X))
+    O.apply_bcs(g, T0, 0, tb)
+    U = [np.asfortranarray(0.3 * (2 * rng.random(g.shape(1 << d)) - 1)) for d in range(3)]
+    J = [np.asfortranarray(rng.standard_normal(g.shape(S.EDGE_DL[d]))) for d in range(3)]
+    sig = np.asfortranarray(0.5 + rng.random(g.shape(0)))
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([a.ravel(order="F") for a in [T0] + U + J + [sig]]).tofile(inp)
+    out = subprocess.run([NRG_DRIVER, str(N), str(beta), str(method), str(rk), "3", str(inp), str(outp)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "energy T-solve iterations" in out.stdout
+    res = np.fromfile(outp, dtype=np.float64)
+    n = int(np.prod(g.shape(0)))
+    T, F = res[:n].reshape(g.shape(0), order="F"), res[n:].reshape(g.shape(0), order="F")
+    k = [np.full(g.shape(1 << d), (1.0, 1.5, 2.0)[d], order="F") for d in range(3)]
+    ost = S.EnergyState(g, T0, tb, k, dt=2e-3, method=method, coeff_implicit=-1.0 / 50.0, advection=(True, -1.0),
+                        diffusion=(True, 1.0 / 50.0), isp=(50, 1e-12, 1e-15, 1), KE_diffusion=(True, 0.02),
+                        viscous_dissipation=(True, 0.5), joule_heating=(True, 0.25), volumetric_heating=(True, 0.7),
+                        rk_stages=rk, J=J, sigma_inv_cc=sig)
+    for _ in range(3):
+        ost.step(U)
+    assert np.linalg.norm(T - ost.T) / np.linalg.norm(ost.T) <= 1e-10
+    assert np.linalg.norm(F - ost.F) / np.linalg.norm(ost.F) <= 1e-10
