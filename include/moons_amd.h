@@ -158,6 +158,39 @@ int moons_nodes_roberts_both(double hmin, double hmax, int N, double beta, doubl
  * out gets sn + 2N nodes, with ghosts. */
 int moons_nodes_ext_uniform(const double *hn, int sn, int N, double *out);
 
+/* The grid_continue extensions (code/handwritten/grid/grid_continue.f90) the
+ * reference's mesh bank builds wall and vacuum meshes with, by kind:
+ *   ext_uniform_IO (:41-47), ext_Roberts_near_IO (:49-57: prep R + app L),
+ *   ext_Roberts_far_IO (:59-67: prep L + app R), ext_Roberts_B_IO (:69-77),
+ *   ext_prep_uniform_IO / _Roberts_L / _R / _B / _C2F_IO (:94-193),
+ *   ext_app_uniform_IO / _Roberts_L / _R / _B / _C2F_IO (:211-306).
+ * A Roberts segment of N cells and length L continues the mesh from hmin
+ * down (prep) or hmax up (app); its stretching beta is matched by Newton
+ * iteration (coordinate_stretch_param_match.f90:176-233) so that its cell at
+ * the matched end has the mesh's end spacing.  n_iter, max_stretch_ratio:
+ * the mesh-quality parameters N_iter and max_mesh_stretch_ratio (1 and any
+ * value for a manual mesh; with N_iter > 1 a segment whose neighbouring
+ * spacings differ by more than the ratio is redone with more cells).
+ * hn: nodes with ghosts, sn >= 4; out: room for cap nodes, *n_out gets the
+ * count.  A match that does not converge is MOONS_ERR_ARG (the reference
+ * stops at init on mesh_generation_error). */
+#define MOONS_EXT_UNIFORM 0
+#define MOONS_EXT_ROBERTS_NEAR 1
+#define MOONS_EXT_ROBERTS_FAR 2
+#define MOONS_EXT_ROBERTS_B 3
+#define MOONS_EXT_PREP_UNIFORM 4
+#define MOONS_EXT_PREP_ROBERTS_L 5
+#define MOONS_EXT_PREP_ROBERTS_R 6
+#define MOONS_EXT_PREP_ROBERTS_B 7
+#define MOONS_EXT_PREP_ROBERTS_C2F 8
+#define MOONS_EXT_APP_UNIFORM 9
+#define MOONS_EXT_APP_ROBERTS_L 10
+#define MOONS_EXT_APP_ROBERTS_R 11
+#define MOONS_EXT_APP_ROBERTS_B 12
+#define MOONS_EXT_APP_ROBERTS_C2F 13
+int moons_nodes_ext(const double *hn, int sn, int kind, double L, int N, int n_iter, double max_stretch_ratio,
+                    double *out, int cap, int *n_out);
+
 /* ---------------------------------------------------------------- mesh */
 /* init_c per direction (code/handwritten/grid/coordinates_extend.f90:78-173):
  * node coordinates including the two ghost nodes, sn = N+3 >= 5 entries. */

@@ -136,6 +136,8 @@ SIGNATURES = {
                                       C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p)]),
     "moons_nodes_ext_uniform": (C.c_int, [c_double_p, C.c_int, C.c_int, c_double_p]),
+    "moons_nodes_ext": (C.c_int, [c_double_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, c_double_p,
+                                  C.c_int, C.POINTER(C.c_int)]),
     "moons_domain_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "moons_domain_destroy": (C.c_int, [C.c_void_p]),
     "moons_domain_overlap": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_int_p]),

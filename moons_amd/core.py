@@ -85,6 +85,28 @@ def nodes_ext_uniform(hn, N: int) -> np.ndarray:
     return out
 
 
+EXT_KINDS = ("uniform", "near", "far", "B", "prep_uniform", "prep_L", "prep_R", "prep_B", "prep_C2F",
+             "app_uniform", "app_L", "app_R", "app_B", "app_C2F")
+
+
+def nodes_ext(hn, kind: str, L: float = 0.0, N: int = 1, n_iter: int = 1, max_stretch_ratio: float = 1.0) -> np.ndarray:
+    """A grid_continue extension of the node array hn (ghosts included), by
+    kind: "uniform" (ext_uniform_IO), "near" / "far" / "B"
+    (ext_Roberts_near/far/B_IO), "prep_uniform", "prep_L", "prep_R", "prep_B",
+    "prep_C2F" (ext_prep_*_IO) and "app_*" likewise
+    (code/handwritten/grid/grid_continue.f90): segments of N cells and length L
+    whose stretching is matched to the mesh's end spacing; n_iter and
+    max_stretch_ratio are the mesh-quality parameters N_iter and
+    max_mesh_stretch_ratio."""
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    cap = hn.size + 2 * (N + n_iter + 2)
+    out = np.zeros(cap)
+    n = C.c_int(0)
+    check(load().moons_nodes_ext(_dp(hn), hn.size, EXT_KINDS.index(kind), float(L), int(N), int(n_iter),
+                                 float(max_stretch_ratio), _dp(out), cap, C.byref(n)))
+    return out[: n.value].copy()
+
+
 class mesh:
     """Single-block mesh from three node-coordinate arrays (ghosts included)."""
 

@@ -4,6 +4,10 @@
 // total mesh.
 //
 //   ext_uniform_IO   code/handwritten/grid/grid_continue.f90:41-47,94-107,197-222
+//   ext_Roberts_*, ext_prep/app_*
+//                    code/handwritten/grid/grid_continue.f90:49-77,107-193,224-306 with the
+//                    stretch-parameter matching of coordinate_stretch_param_match.f90:47-95,176-233
+//                    and the Roberts transformations of coordinate_distribution_funcs.f90:145-248
 //   overlap          get_C_overlap_OL / get_N_overlap_OL with the physical
 //                    padding p = 1 and tol 1e-12,
 //                    code/handwritten/sub_domain/overlap_extend.f90:204-278,364-391,
@@ -105,11 +109,189 @@ static bool overlap(const std::vector<double> &a, const std::vector<double> &b, 
   return r[1] >= 0;
 }
 
+// ------------------------------------------------------------------
+// Roberts mesh extensions (host code).  Scalar libm arithmetic in the
+// reference's evaluation order; x**2 is x*x.
+// ------------------------------------------------------------------
+namespace ext {
+
+// T2_h_i: Roberts transformation 2 at node i of N, stretching alpha, beta
+double t2_h(double b, double a, int i, int N) {
+  const double g = (b + 1.0) / (b - 1.0);
+  const double t = ((double)i / (double)N - a) / (1.0 - a);
+  const double gt = std::pow(g, t);
+  return ((b + 2.0 * a) * gt - b + 2.0 * a) / ((2.0 * a + 1.0) * (1.0 + gt));
+}
+
+// d T2_h_i / d beta
+double t2_prime(double b, double a, int i, int N) {
+  const double g = (b + 1.0) / (b - 1.0);
+  const double t = ((double)i / (double)N - a) / (1.0 - a);
+  const double gt = std::pow(g, t), g2t = std::pow(g, 2.0 * t);
+  const double num = -(b * b) * g2t + b * b + 4.0 * b * gt * t + g2t - 1.0;
+  return -num / ((2.0 * a + 1.0) * (b - 1.0) * (b + 1.0) * ((gt + 1.0) * (gt + 1.0)));
+}
+
+// Newton for the beta whose end cell (near hmax, or near hmin) has width dh:
+// newtonT2 (tol 10 eps, giving up above 1000 or below 1 + 1e-10)
+bool match_beta(bool near_hmax, double a, double hmin, double hmax, int N, double dh, double *beta_out) {
+  const double tol = 10.0 * 2.220446049250313e-16;
+  const double beta_low = 1.0 + std::pow(10.0, -10.0);
+  double beta = 1.0 + std::pow(10.0, -10.0), fb = 0.0;
+  const int i1 = near_hmax ? N : 1, i0 = near_hmax ? N - 1 : 0;
+  for (int k = 0; k < 1000000; ++k) {
+    fb = t2_h(beta, a, i1, N) - t2_h(beta, a, i0, N) - dh / (hmax - hmin);
+    const double fp = t2_prime(beta, a, i1, N) - t2_prime(beta, a, i0, N);
+    if (std::fabs(fb) < tol) break;
+    if (beta > 1000.0) break;
+    if (beta < beta_low) {
+      beta = 1.1;
+      break;
+    }
+    beta = beta - fb / fp;
+  }
+  if (std::isnan(beta) || std::fabs(fb) > tol) return false;
+  *beta_out = beta;
+  return true;
+}
+
+std::vector<double> unit_nodes(double hmin, double hmax, int N) {
+  std::vector<double> x(N + 1);
+  const double dh = (hmax - hmin) / (double)N;
+  for (int i = 0; i <= N; ++i) x[i] = (hmin + (double)i * dh - hmin) / (hmax - hmin);
+  return x;
+}
+
+// every spacing ratio >= 1 (increasing) or <= 1 (decreasing)
+bool ratios_ok(const std::vector<double> &h, bool increasing) {
+  for (size_t i = 0; i + 2 < h.size(); ++i) {
+    const double R = (h[i + 2] - h[i + 1]) / (h[i + 1] - h[i]);
+    if ((increasing && R < 1.0) || (!increasing && R > 1.0)) return false;
+  }
+  return true;
+}
+
+// robertsLeft (transformation 1), robertsRight / robertsBoth (transformation 2, alpha 0 / 1/2)
+bool roberts(char kind, double hmin, double hmax, int N, double beta, std::vector<double> &out) {
+  const std::vector<double> x = unit_nodes(hmin, hmax, N);
+  std::vector<double> h(N + 1);
+  const double b = beta, g = (b + 1.0) / (b - 1.0);
+  if (kind == 'L') {
+    const double a = 1.0;
+    for (int i = 0; i <= N; ++i) {
+      const double e = std::pow(g, a - x[i]);
+      h[i] = ((b + a) - (b - a) * e) / (e + a);
+    }
+    if (!ratios_ok(h, true)) return false;
+  } else {
+    const double a = kind == 'R' ? 0.0 : 0.5;
+    for (int i = 0; i <= N; ++i) {
+      const double e = std::pow(g, (x[i] - a) / (1.0 - a));
+      h[i] = ((b + 2.0 * a) * e - b + 2.0 * a) / ((2.0 * a + 1.0) * (1.0 + e));
+    }
+    if (std::fabs(a) < std::pow(10.0, -10.0) && !ratios_ok(h, false)) return false;
+  }
+  out.resize(N + 1);
+  for (int i = 0; i <= N; ++i) out[i] = hmin + (hmax - hmin) * h[i];
+  return true;
+}
+
+// needs_more_points: some neighbouring spacing ratio above max_ratio
+bool needs_more_points(const std::vector<double> &b, double max_ratio) {
+  if (b.size() <= 2) return false;
+  double R = 0.0;
+  for (size_t i = 0; i + 2 < b.size(); ++i) {
+    const double d0 = b[i + 1] - b[i], d1 = b[i + 2] - b[i + 1];
+    R = std::max(R, std::max(d0 / d1, d1 / d0));
+  }
+  return R > max_ratio;
+}
+
+// One Roberts segment on [lo, hi]: the N_iter loop of ext_prep/app_Roberts_*
+// (N_final = the points of the last segment, one more when it fails the
+// quality check).  match: 's' dh near hmax alpha 0, 'g' near hmin alpha 0,
+// 'b' near hmax alpha 1/2.
+int segment(char match, char dist, double lo, double hi, int N, double dh, int n_iter, double max_ratio,
+            std::vector<double> &b) {
+  int N_final = N;
+  for (int it = 0; it < n_iter; ++it) {
+    double beta;
+    if (!match_beta(match != 'g', match == 'b' ? 0.5 : 0.0, lo, hi, N_final, dh, &beta))
+      return fail(MOONS_ERR_ARG, "moons_nodes_ext: the stretch parameter matching did not converge");
+    if (!roberts(dist, lo, hi, N_final, beta, b))
+      return fail(MOONS_ERR_ARG, "moons_nodes_ext: the Roberts segment fails its spacing-ratio check");
+    N_final = (int)b.size();
+    if (needs_more_points(b, max_ratio)) N_final = N_final + 1;
+    else break;
+  }
+  return MOONS_OK;
+}
+
+// prepend (kind 'U' uniform, 'L', 'R', 'B', 'C' C2F) of N cells, length L
+int prep(std::vector<double> &h, char kind, double L, int N, int n_iter, double max_ratio) {
+  const double hmin = h[1], dh = h[1] - h[0];
+  std::vector<double> b;
+  if (kind == 'U') {
+    for (int i = N; i >= 0; --i) b.push_back(hmin + (-1.0 * (double)i) * dh);
+  } else {
+    const char match = kind == 'B' ? 'b' : (kind == 'C' ? 'g' : 's');
+    const char dist = kind == 'C' ? 'L' : kind;
+    if (int st = segment(match, dist, hmin - L, hmin, N, dh, n_iter, max_ratio, b)) return st;
+  }
+  std::vector<double> a(b);
+  a.insert(a.end(), h.begin() + 2, h.end());
+  a.insert(a.begin(), a[0] - (a[1] - a[0])); // prepend_ghost_c
+  h.swap(a);
+  return MOONS_OK;
+}
+
+// append (kinds as prep) of N cells, length L
+int app(std::vector<double> &h, char kind, double L, int N, int n_iter, double max_ratio) {
+  const size_t n = h.size();
+  const double hmax = h[n - 2], dh = h[n - 1] - h[n - 2];
+  std::vector<double> b;
+  if (kind == 'U') {
+    for (int i = 0; i <= N; ++i) b.push_back(hmax + (1.0 * (double)i) * dh);
+  } else {
+    const char match = kind == 'B' ? 'b' : (kind == 'C' ? 'g' : 's');
+    const char dist = kind == 'C' ? 'R' : kind;
+    if (int st = segment(match, dist, hmax, hmax + L, N, dh, n_iter, max_ratio, b)) return st;
+  }
+  h.resize(n - 2);
+  h.insert(h.end(), b.begin(), b.end());
+  h.push_back(h.back() + (h.back() - h[h.size() - 2])); // append_ghost_c
+  return MOONS_OK;
+}
+
+} // namespace ext
+
 } // namespace moons
 
 using namespace moons;
 
 extern "C" {
+
+int moons_nodes_ext(const double *hn, int sn, int kind, double L, int N, int n_iter, double max_stretch_ratio,
+                    double *out, int cap, int *n_out) {
+  if (!hn || !out || !n_out || sn < 4 || N < 1 || n_iter < 1)
+    return fail(MOONS_ERR_ARG, "moons_nodes_ext: need sn >= 4, N >= 1 and n_iter >= 1");
+  if (kind < MOONS_EXT_UNIFORM || kind > MOONS_EXT_APP_ROBERTS_C2F)
+    return fail(MOONS_ERR_ARG, "moons_nodes_ext: unknown kind");
+  if (kind != MOONS_EXT_UNIFORM && kind != MOONS_EXT_PREP_UNIFORM && kind != MOONS_EXT_APP_UNIFORM && !(L > 0.0))
+    return fail(MOONS_ERR_ARG, "moons_nodes_ext: a Roberts extension needs L > 0");
+  std::vector<double> h(hn, hn + sn);
+  // prepend kind, append kind ('-': none)
+  static const char P[14] = {'U', 'R', 'L', 'B', 'U', 'L', 'R', 'B', 'C', '-', '-', '-', '-', '-'};
+  static const char A[14] = {'U', 'L', 'R', 'B', '-', '-', '-', '-', '-', 'U', 'L', 'R', 'B', 'C'};
+  int st;
+  if (P[kind] != '-' && (st = ext::prep(h, P[kind], L, N, n_iter, max_stretch_ratio))) return st;
+  if (A[kind] != '-' && (st = ext::app(h, A[kind], L, N, n_iter, max_stretch_ratio))) return st;
+  *n_out = (int)h.size();
+  if ((int)h.size() > cap) return fail(MOONS_ERR_ARG, "moons_nodes_ext: out has room for fewer nodes than the result");
+  for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  return MOONS_OK;
+}
+
 
 int moons_nodes_ext_uniform(const double *hn, int sn, int N, double *out) {
   if (!hn || !out || sn < 4 || N < 1) return fail(MOONS_ERR_ARG, "moons_nodes_ext_uniform: need sn >= 4 and N >= 1");

@@ -17,6 +17,14 @@ module moons_amd_bindings
   integer(c_int), parameter, public :: MOONS_PREC_LAP_SF = 1, MOONS_PREC_MG_FP32 = 2
   integer(c_int), parameter, public :: MOONS_OP_MOM_DIFFUSION = 1, MOONS_OP_IND_DIFFUSION = 2
   integer(c_int), parameter, public :: MOONS_DL_EDGE_X = 6, MOONS_DL_EDGE_Y = 5, MOONS_DL_EDGE_Z = 3
+  ! grid_continue extension kinds of moons_nodes_ext
+  integer(c_int), parameter, public :: MOONS_EXT_UNIFORM = 0, MOONS_EXT_ROBERTS_NEAR = 1, MOONS_EXT_ROBERTS_FAR = 2
+  integer(c_int), parameter, public :: MOONS_EXT_ROBERTS_B = 3, MOONS_EXT_PREP_UNIFORM = 4
+  integer(c_int), parameter, public :: MOONS_EXT_PREP_ROBERTS_L = 5, MOONS_EXT_PREP_ROBERTS_R = 6
+  integer(c_int), parameter, public :: MOONS_EXT_PREP_ROBERTS_B = 7, MOONS_EXT_PREP_ROBERTS_C2F = 8
+  integer(c_int), parameter, public :: MOONS_EXT_APP_UNIFORM = 9, MOONS_EXT_APP_ROBERTS_L = 10
+  integer(c_int), parameter, public :: MOONS_EXT_APP_ROBERTS_R = 11, MOONS_EXT_APP_ROBERTS_B = 12
+  integer(c_int), parameter, public :: MOONS_EXT_APP_ROBERTS_C2F = 13
 
   ! moons_mhd_params / moons_mhd_info of include/moons_amd.h
   type, bind(C), public :: moons_mhd_params
@@ -79,7 +87,7 @@ module moons_amd_bindings
 
   public :: moons_check
   public :: moons_init, moons_finalize, moons_synchronize
-  public :: moons_nodes_uniform, moons_nodes_roberts_both
+  public :: moons_nodes_uniform, moons_nodes_roberts_both, moons_nodes_ext
   public :: moons_mesh_create, moons_mesh_destroy
   public :: moons_field_create, moons_field_destroy, moons_field_set_bcs, moons_field_set_bc_props
   public :: moons_field_upload, moons_field_download, moons_field_fill
@@ -119,6 +127,15 @@ module moons_amd_bindings
       real(c_double), value :: hmin, hmax, beta
       integer(c_int), value :: N
       real(c_double), intent(out) :: hn(*)
+    end function
+    integer(c_int) function moons_nodes_ext(hn, sn, kind, L, N, n_iter, max_stretch_ratio, out, cap, n_out) &
+        bind(C, name="moons_nodes_ext")
+      import c_int, c_double
+      real(c_double), intent(in) :: hn(*)
+      integer(c_int), value :: sn, kind, N, n_iter, cap
+      real(c_double), value :: L, max_stretch_ratio
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: n_out
     end function
     integer(c_int) function moons_mesh_create(hx, sx, hy, sy, hz, sz, m) bind(C, name="moons_mesh_create")
       import c_int, c_double, c_ptr

@@ -330,6 +330,8 @@ def ref():
         R.ref_export_raw.argtypes = raw
         R.ref_import_raw.argtypes = raw
         R.ref_ext_uniform.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        R.ref_ext.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
+                              C.POINTER(C.c_int)]
         R.ref_overlap.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _ip]
         R.ref_robin_plane.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int]
@@ -475,6 +477,24 @@ def ref_import_raw(hn, dl, f, directory, name, pad) -> np.ndarray:
     args, fl = _raw_args(hn, dl, f, directory, name, pad)
     ref().ref_import_raw(*args)
     return fl.reshape(np.shape(f), order="F")
+
+
+REF_EXT_KINDS = ("uniform", "near", "far", "B", "prep_uniform", "prep_L", "prep_R", "prep_B", "prep_C2F",
+                 "app_uniform", "app_L", "app_R", "app_B", "app_C2F")
+
+
+def ref_ext(hn, kind: str, L: float, N: int, n_iter: int = 1, max_ratio: float = 1.0) -> np.ndarray:
+    """The reference's grid_continue extension `kind` of the node array hn
+    (ghosts included).  A stretch match that does not converge makes the
+    reference write the file mesh_generation_error in the working directory."""
+    hn = np.ascontiguousarray(hn, dtype=np.float64)
+    cap = hn.size + 8 * (N + n_iter + 4)
+    out = np.zeros(cap)
+    n = C.c_int(0)
+    ref().ref_ext(REF_EXT_KINDS.index(kind), p(hn), hn.size, float(L), int(N), int(n_iter), float(max_ratio), p(out),
+                  cap, C.byref(n))
+    assert n.value <= cap
+    return out[: n.value].copy()
 
 
 def ref_ext_uniform(hn, N: int) -> np.ndarray:

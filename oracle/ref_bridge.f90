@@ -31,7 +31,8 @@ module moons_ref_bridge
   use construct_suffix_mod, only: construct_suffix
   use GF_export_mod, only: exp_3D_1C_GF
   use GF_import_mod, only: imp_3D_1C_GF
-  use grid_continue_mod, only: ext_uniform_IO
+  use grid_continue_mod
+  use mesh_quality_params_mod, only: mesh_quality_params
   use coordinates_mod, only: coordinates
   use coordinates_extend_mod, only: coord_init => init
   use overlap_mod, only: overlap
@@ -52,6 +53,49 @@ contains
     call init(g, hn, hn, hn)
     call ext_uniform_IO(g, int(N), 1)
     out = g%c(1)%hn%f
+  end subroutine
+
+  ! The grid_continue extension of kind (0 ext_uniform_IO, 1 ext_Roberts_near_IO,
+  ! 2 ext_Roberts_far_IO, 3 ext_Roberts_B_IO, 4 ext_prep_uniform_IO,
+  ! 5 ext_prep_Roberts_L_IO, 6 _R, 7 _B, 8 ext_prep_Roberts_C2F_IO,
+  ! 9 ext_app_uniform_IO, 10 ext_app_Roberts_L_IO, 11 _R, 12 _B,
+  ! 13 ext_app_Roberts_C2F_IO; handwritten/grid/grid_continue.f90) of direction 1
+  ! of a grid with node array hn in every direction, segment length L, N cells,
+  ! mesh quality N_iter and max_mesh_stretch_ratio; out (cap entries) gets
+  ! the extended nodes, n_out their number.
+  subroutine ref_ext(kind, hn, sn, L, N, n_iter, max_ratio, out, cap, n_out) bind(C, name="ref_ext")
+    integer(c_int), value :: kind, sn, N, n_iter, cap
+    real(c_double), value :: L, max_ratio
+    real(c_double), intent(in) :: hn(sn)
+    real(c_double), intent(out) :: out(cap)
+    integer(c_int), intent(out) :: n_out
+    type(grid) :: g
+    type(mesh_quality_params) :: MQP
+    integer :: Ni
+    MQP%N_iter = n_iter
+    MQP%max_mesh_stretch_ratio = max_ratio
+    MQP%N_max_points_add = max(n_iter - 1, 0)
+    MQP%auto_find_N = n_iter > 1
+    Ni = int(N)
+    call init(g, hn, hn, hn)
+    select case (kind)
+    case (0); call ext_uniform_IO(g, Ni, 1)
+    case (1); call ext_Roberts_near_IO(g, L, Ni, 1, MQP)
+    case (2); call ext_Roberts_far_IO(g, L, Ni, 1, MQP)
+    case (3); call ext_Roberts_B_IO(g, L, Ni, 1, MQP)
+    case (4); call ext_prep_uniform_IO(g, Ni, 1)
+    case (5); call ext_prep_Roberts_L_IO(g, L, Ni, 1, MQP)
+    case (6); call ext_prep_Roberts_R_IO(g, L, Ni, 1, MQP)
+    case (7); call ext_prep_Roberts_B_IO(g, L, Ni, 1, MQP)
+    case (8); call ext_prep_Roberts_C2F_IO(g, L, Ni, 1, MQP)
+    case (9); call ext_app_uniform_IO(g, Ni, 1)
+    case (10); call ext_app_Roberts_L_IO(g, L, Ni, 1, MQP)
+    case (11); call ext_app_Roberts_R_IO(g, L, Ni, 1, MQP)
+    case (12); call ext_app_Roberts_B_IO(g, L, Ni, 1, MQP)
+    case (13); call ext_app_Roberts_C2F_IO(g, L, Ni, 1, MQP)
+    end select
+    n_out = int(g%c(1)%sn, c_int)
+    if (n_out <= cap) out(1:n_out) = g%c(1)%hn%f
   end subroutine
 
   ! get_C_overlap (kind 0) / get_N_overlap (kind 1) of the coordinates hn1

@@ -20,10 +20,21 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b)) / (den if den > 0 else 1.0)
 
 
-def meshes(gpu, N=(10, 9, 8), n_wall=1, n_ext=3):
+def meshes(gpu, N=(10, 9, 8), n_wall=1, n_ext=3, kind="uniform"):
+    """Fluid nodes, the wall (sigma) mesh and the induction mesh.  uniform:
+    ext_uniform_IO by n_wall and n_ext cells; roberts: the mesh bank's pattern
+    (code/handwritten/sim_params/mesh_bank.f90:222-229): ext_Roberts_B_IO of a
+    3-cell wall of thickness 0.5, then ext_Roberts_near_IO of 3 vacuum cells
+    of length 1, built by the library's own generator."""
     hn = [O.uniform_nodes(0, 1, N[0]), O.roberts_both_nodes(0, 1, N[1], 1.1), O.uniform_nodes(0, 1, N[2])]
-    wall = [D.ext_uniform(h, n_wall) for h in hn]
-    ind = [D.ext_uniform(h, n_ext) for h in hn]
+    if kind == "uniform":
+        wall = [D.ext_uniform(h, n_wall) for h in hn]
+        ind = [D.ext_uniform(h, n_ext) for h in hn]
+    else:
+        wall = [gpu.nodes_ext(h, "B", 0.5, 3) for h in hn]
+        ind = [gpu.nodes_ext(w, "near", 1.0, 3) for w in wall]
+        for w, i in zip(wall, ind):  # the library's generator equals the restatement
+            np.testing.assert_array_equal(i, D.ext(w, "near", 1.0, 3))
     return hn, wall, ind
 
 
@@ -56,13 +67,15 @@ def test_domain_embed_extract_and_sigma_match_the_restatement(gpu):
         gpu.mesh_domain(m, gpu.mesh(*[O.uniform_nodes(5.0, 6.0, 4)] * 3))
 
 
-def test_multi_domain_step_matches_oracle(gpu):
-    """Fluid 10 x 9 x 8 (Roberts in y) inside an induction mesh extended by
-    three uniform cells at every wall; sigma = 1 on the fluid and a one-cell
-    wall, 0.1 beyond; B = 0 Dirichlet at the far boundary, phi Dirichlet;
-    method 6 for U and B."""
-    print("multi-domain mhd step", flush=True)
-    hn, wall, ind = meshes(gpu)
+@pytest.mark.parametrize("kind", ["uniform", "roberts"])
+def test_multi_domain_step_matches_oracle(gpu, kind):
+    """Fluid 10 x 9 x 8 (Roberts in y) inside an induction mesh extended at
+    every wall (uniform: 3 cells, a one-cell wall; roberts: a Roberts-clustered
+    3-cell wall and 3 vacuum cells); sigma = 1 on the fluid and the wall, 0.1
+    beyond; B = 0 Dirichlet at the far boundary, phi Dirichlet; method 6 for U
+    and B."""
+    print(f"multi-domain mhd step ({kind} extension)", flush=True)
+    hn, wall, ind = meshes(gpu, kind=kind)
     g, gi = O.Grid(*hn), O.Grid(*ind)
     m, mw, mi = gpu.mesh(*hn), gpu.mesh(*wall), gpu.mesh(*ind)
     md, mdw = gpu.mesh_domain(m, mi), gpu.mesh_domain(mw, mi)
