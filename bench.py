@@ -581,6 +581,18 @@ def main():
                        "gain_over_mixed": {"ms_without_network": round(mg_ms, 4),
                                            "iterations_without_network": mg_info.iterations,
                                            "speedup": round(mg_ms / l_ms, 3), "pays": bool(l_ms < mg_ms)}}
+        # the ceiling of any coarse-level initial guess: the V-cycle's part from
+        # level l down cycled K times per visit makes that level's correction
+        # nearly exact, what a perfect network placed there would give
+        ceiling = {}
+        xg = M.SF(m, "CC").set_BCs(types, None, all_Neumann=True)
+        Pg = M.PCG_solver_SF(m, xg, M.iter_solver_params(2000, TOL, 1e-300, 1), prec="MG_FP32")
+        for lv, K in ((2, 4), (2, 16), (1, 4), (1, 16)):
+            Pg.set_MG_gamma(lv, K)
+            g_ms, g_info = timed(Pg, xg)
+            ceiling[f"level {lv} ({N >> lv}^3), {K} cycles"] = {"ms": round(g_ms, 4), "iterations": g_info.iterations}
+        Pg.set_MG_gamma(-1, 1)
+        learned_out["coarse_guess_ceiling"] = ceiling
 
     # every solution against the FP64 Jacobi-PCG converged to 1e-11 (constant removed)
     xr = M.SF(m, "CC").fill(0.0).set_BCs(types, None, all_Neumann=True)

@@ -665,6 +665,16 @@ int moons_pcg_set_mg_unet(moons_pcg_t pcg, moons_unet_t net, int level);
 /* The learned initial guess in the first `cycles` V-cycles of each solve only
  * (0, the default: in every V-cycle).  No reference counterpart. */
 int moons_pcg_set_mg_unet_cycles(moons_pcg_t pcg, int cycles);
+/* Inside a MOONS_PREC_MG_FP32 solver's V-cycle, the part from multigrid
+ * level `level` down (its smoothing, the coarser levels and the coarse end)
+ * is run `cycles` times per visit, each from the previous result: a
+ * gamma-cycle at one level.  Many cycles make that level's correction
+ * nearly exact, the best any initial guess of that level (such as the
+ * learned one) can give.  level < 0 or cycles = 1: the plain V-cycle.  The
+ * level must be launched over the grid (not in the single-workgroup coarse
+ * end; MOONS_ERR_ARG otherwise).  Holds until moons_pcg_set_mg rebuilds the
+ * hierarchy.  No reference counterpart. */
+int moons_pcg_set_mg_gamma(moons_pcg_t pcg, int level, int cycles);
 /* Enables (default) or disables the Jacobi redo of failed MOONS_PREC_UNET solves. */
 int moons_pcg_set_fallback(moons_pcg_t pcg, int enable);
 /* Binds the network to a MOONS_PREC_UNET solver (required before its first solve). */

@@ -166,6 +166,7 @@ SIGNATURES = {
     "moons_pcg_set_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_pcg_set_mg_unet": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "moons_pcg_set_mg_unet_cycles": (C.c_int, [C.c_void_p, C.c_int]),
+    "moons_pcg_set_mg_gamma": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "moons_pcg_set_fallback": (C.c_int, [C.c_void_p, C.c_int]),
 }
 

@@ -456,6 +456,11 @@ class PCG_solver_SF:
         check(load().moons_pcg_set_mg_unet_cycles(self.handle, int(cycles)))
         self._mg_net = net
 
+    def set_MG_gamma(self, level: int, cycles: int) -> None:
+        """Cycle the V-cycle's part from multigrid level `level` down `cycles`
+        times per visit (prec="MG_FP32"; level < 0 or cycles = 1: plain V-cycle)."""
+        check(load().moons_pcg_set_mg_gamma(self.handle, int(level), int(cycles)))
+
     def set_fallback(self, enable: bool) -> None:
         """Jacobi redo of failed learned solves (on by default)."""
         check(load().moons_pcg_set_fallback(self.handle, 1 if enable else 0))

@@ -76,6 +76,8 @@ struct MGHier {
   UnetPrec *learn = nullptr;   // learned initial guess of one grid level (or none)
   int learn_level = -1;
   int learn_cycles = 0;        // the first learn_cycles V-cycles of a solve use it (0: every one)
+  int gamma_level = -1;        // levels from this one down are cycled gamma_cycles times per visit
+  int gamma_cycles = 1;
   int cycle = 0;               // V-cycles since the solve began
   float *lpf = nullptr, *lpx = nullptr; // its padded (n+2)^3 input and output images
   float *gpart = nullptr;      // its Galerkin partial sums
@@ -1320,10 +1322,14 @@ int mg_vcycle(MGHier *h, const int *done) {
   const int nl = (int)h->L.size();
   const bool tail = h->tail0 < nl;
   const int g = tail ? h->tail0 : nl - 1; // levels [0, g) run as grid launches
-  // descend
-  for (int l = 0; l < g; ++l) {
+  // levels [c, g) and the coarse end are cycled K times per visit (a
+  // gamma-cycle from level c; K = 1, c = g: the plain V-cycle)
+  const int c = h->gamma_level >= 0 && h->gamma_level < g ? h->gamma_level : g;
+  const int K = c < g ? std::max(1, h->gamma_cycles) : 1;
+  auto descend = [&](int l, bool zero_first) -> int {
     LevelDev &L = h->L[l];
-    const bool learned = h->learn && l == h->learn_level && (h->learn_cycles == 0 || h->cycle < h->learn_cycles);
+    const bool learned = zero_first && h->learn && l == h->learn_level &&
+                         (h->learn_cycles == 0 || h->cycle < h->learn_cycles);
     if (learned) {
       // learned initial guess x = c N(f), then smoothing from it
       const int nc = L.n0 * L.n1 * L.n2;
@@ -1335,20 +1341,30 @@ int mg_vcycle(MGHier *h, const int *done) {
       hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
       hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
     }
-    for (int s = 0; s < h->nu1; ++s) sweep(L, 0, s == 0 && !learned, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
+    for (int s = 0; s < h->nu1; ++s)
+      sweep(L, 0, s == 0 && zero_first && !learned, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
     restrict_to(L, h->L[l + 1], done);
-  }
-  if (tail)
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail, h->nu1,
-                       h->nu2, h->coarse_sweeps, done);
-  else
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
-  // ascend
-  for (int l = g - 1; l >= 0; --l) {
+    return MOONS_OK;
+  };
+  auto ascend = [&](int l) {
     LevelDev &L = h->L[l];
     prolong_from(L, h->L[l + 1], done);
     for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
+  };
+  int st;
+  for (int l = 0; l < c; ++l)
+    if ((st = descend(l, true))) return st;
+  for (int k = 0; k < K; ++k) {
+    for (int l = c; l < g; ++l)
+      if ((st = descend(l, l > c || k == 0))) return st;
+    if (tail)
+      hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail, h->nu1,
+                         h->nu2, h->coarse_sweeps, done);
+    else
+      hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
+    for (int l = g - 1; l >= c; --l) ascend(l);
   }
+  for (int l = c - 1; l >= 0; --l) ascend(l);
   ++h->cycle;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MOONS_ERR_HIP, std::string("multigrid launch: ") + hipGetErrorString(e));
@@ -1357,6 +1373,14 @@ int mg_vcycle(MGHier *h, const int *done) {
 
 void mg_begin_solve(MGHier *h) { h->cycle = 0; }
 void mg_set_learned_cycles(MGHier *h, int cycles) { h->learn_cycles = cycles; }
+int mg_set_gamma(MGHier *h, int level, int cycles) {
+  const int nl = (int)h->L.size();
+  const int g = h->tail0 < nl ? h->tail0 : nl - 1; // grid-launched levels
+  if (level >= g) return MOONS_ERR_ARG;
+  h->gamma_level = level;
+  h->gamma_cycles = cycles;
+  return MOONS_OK;
+}
 
 int mg_time_kernels(MGHier *h, int reps, double ms[10]) {
   const int nl = (int)h->L.size();

@@ -31,6 +31,11 @@ int mg_vcycle(MGHier *h, const int *done);
 void mg_begin_solve(MGHier *h);
 // The learned initial guess in the first `cycles` V-cycles of each solve (0: every cycle).
 void mg_set_learned_cycles(MGHier *h, int cycles);
+// Levels from `level` down (and the coarse end) are cycled `cycles` times per
+// visit; level < 0 or cycles = 1: the plain V-cycle.  MOONS_ERR_ARG when
+// `level` is not a grid-launched level (the single-workgroup coarse end
+// starts from zero at each launch).
+int mg_set_gamma(MGHier *h, int level, int cycles);
 void mg_set_params(MGHier *h, int nu1, int nu2, int coarse_sweeps);
 // Average duration (ms) of one launch, reps launches back to back on the
 // library stream: [0] a half-sweep of level 0, [1] restriction 0 -> 1,

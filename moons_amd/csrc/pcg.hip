@@ -834,6 +834,14 @@ int moons_pcg_set_mg_unet_cycles(moons_pcg_t S, int cycles) {
   return MOONS_OK;
 }
 
+int moons_pcg_set_mg_gamma(moons_pcg_t S, int level, int cycles) {
+  if (!S || cycles < 1) return fail(MOONS_ERR_ARG, "moons_pcg_set_mg_gamma: bad argument");
+  if (!S->mg) return fail(MOONS_ERR_STATE, "moons_pcg_set_mg_gamma: solver was not created with MOONS_PREC_MG_FP32");
+  if (mg_set_gamma(S->mg, level, cycles))
+    return fail(MOONS_ERR_ARG, "moons_pcg_set_mg_gamma: level is not one of the grid-launched multigrid levels");
+  return MOONS_OK;
+}
+
 int moons_pcg_set_fallback(moons_pcg_t S, int enable) {
   if (!S) return fail(MOONS_ERR_ARG, "moons_pcg_set_fallback: null solver");
   S->fallback = enable != 0;

@@ -247,10 +247,12 @@ def prolong(F: Level, C: Level, xc, bct):
     return e
 
 
-def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=16, x0=None):
+def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=16, x0=None, gamma_level=-1, gamma_cycles=1):
     """x0 = V f0 (padded arrays; f0 = r/vol on the fine level).  With x0 given,
     the fine level's pre-smoothing starts from it (the learned initial guess)
-    instead of from zero."""
+    instead of from zero.  Level gamma_level's visit runs its smoothing,
+    restriction, coarser cycle, prolongation and smoothing gamma_cycles times,
+    each from the previous result (a gamma-cycle at that level)."""
     xs = [np.zeros(L.shape()) for L in levels]
     if x0 is not None:
         xs[0] = np.array(x0, dtype=float)
@@ -265,16 +267,17 @@ def vcycle(levels, f0, bct, nu1=2, nu2=2, coarse_sweeps=16, x0=None):
                 L.smooth(x, f, 0, s == 0)
                 L.smooth(x, f, 1, False)
             return
-        for s in range(nu1):
-            L.sweep(x, f, True, s == 0 and not (l == 0 and x0 is not None))
-        C = levels[l + 1]
-        fc = np.zeros(C.shape())
-        fc[1:-1, 1:-1, 1:-1] = restrict(L, C, L.residual(x, f))
-        fs[l + 1] = fc
-        cyc(l + 1)
-        L.interior(x)[...] += prolong(L, C, xs[l + 1], bct)
-        for s in range(nu2):
-            L.sweep(x, f, False, False)
+        for k in range(gamma_cycles if l == gamma_level else 1):
+            for s in range(nu1):
+                L.sweep(x, f, True, s == 0 and k == 0 and not (l == 0 and x0 is not None))
+            C = levels[l + 1]
+            fc = np.zeros(C.shape())
+            fc[1:-1, 1:-1, 1:-1] = restrict(L, C, L.residual(x, f))
+            fs[l + 1] = fc
+            cyc(l + 1)
+            L.interior(x)[...] += prolong(L, C, xs[l + 1], bct)
+            for s in range(nu2):
+                L.sweep(x, f, False, False)
 
     cyc(0)
     return xs[0]

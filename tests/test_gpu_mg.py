@@ -60,6 +60,43 @@ def test_vcycle_matches_restatement(gpu, case):
     assert P.mg_info()[0] == len(levels)
 
 
+@pytest.mark.parametrize("level,cycles", [(0, 2), (1, 3)])
+def test_gamma_cycle_matches_restatement(gpu, level, cycles):
+    """The V-cycle with the part from `level` down cycled `cycles` times per
+    visit, on the 48^3 stretched duct (zebra lines on the finest level),
+    against the restatement; cycles = 1 equals the plain V-cycle bit for bit."""
+    N, beta, bct = CASES["duct48_lines"]
+    print(f"gamma-cycle level {level} x{cycles}", flush=True)
+    hs, g, m = meshes(gpu, N, beta)
+    x = gpu.SF(m, "CC").set_BCs([NAMES[t] for t in bct], None, all_Neumann=(1 not in bct))
+    P = mg_solver(gpu, m, x)
+    rng = np.random.default_rng(4)
+    r = np.zeros(g.shape(0), order="F")
+    r[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(n - 2 for n in g.shape(0)))
+    R = gpu.SF(m, "CC").upload(r)
+    Z0, Z1, Z = gpu.SF(m, "CC"), gpu.SF(m, "CC"), gpu.SF(m, "CC")
+    P.apply_prec(R, Z0)
+    P.set_MG_gamma(level, 1)
+    P.apply_prec(R, Z1)
+    np.testing.assert_array_equal(Z1.download(), Z0.download())
+    P.set_MG_gamma(level, cycles)
+    P.apply_prec(R, Z)
+    z = Z.download()
+    levels = MGO.build(hs, bct)
+    vol = O.volume(g, 0)
+    f = np.zeros(g.shape(0))
+    f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
+    zr = MGO.vcycle(levels, f, bct, gamma_level=level, gamma_cycles=cycles)
+    zi, zri = z[1:-1, 1:-1, 1:-1], zr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(zi - zri) / np.linalg.norm(zri)
+    assert err < 2e-5, err
+    # the repeated coarse part changes the result
+    assert np.linalg.norm(zi - Z0.download()[1:-1, 1:-1, 1:-1]) > 1e-6 * np.linalg.norm(zri)
+    # a level of the single-workgroup coarse end is refused
+    with pytest.raises(gpu.MoonsError, match="grid-launched"):
+        P.set_MG_gamma(P.mg_info()[0] - 1, 2)
+
+
 @pytest.mark.parametrize("bct", [[3, 3, 2, 2, 2, 2], [2, 1, 3, 3, 2, 2]])
 def test_vcycle_half_sweep_levels_match_restatement(gpu, bct):
     """A mesh whose finest level has 80 x 64 x 64 cells (above the fused-sweep
