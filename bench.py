@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PPE time-to-1e-8 per timestep (learned+mixed vs plain FP64 PCG); HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 N = 128
 BETA = 1.05
 TOL = 1e-8
@@ -593,6 +594,38 @@ def main():
             ceiling[f"level {lv} ({N >> lv}^3), {K} cycles"] = {"ms": round(g_ms, 4), "iterations": g_info.iterations}
         Pg.set_MG_gamma(-1, 1)
         learned_out["coarse_guess_ceiling"] = ceiling
+        # the network's dominant kernel: the 16 -> 16 convolution at 32^3
+        # (k_conv_tile<1, 1>), 2 x 27 x 16 x 16 flops a voxel, timed in situ
+        # over the every-V-cycle solves
+        with M.ktimer("unet_conv16") as ku:
+            for _ in range(max(1, min(args.steps, 5))):
+                xe.fill(0.0)
+                Pe.solve(xe, b)
+        M.synchronize()
+        conv_flop = 2 * 27 * 16 * 16 * 32 ** 3
+        conv_tf = conv_flop / (ku.avg_ms * 1e-3) / 1e12 if ku.launches else 0.0
+        conv_rp = load_rocprof_avg("ppe_128_bench_trace.txt", "k_conv_tile<1, 1>")
+        learned_out["roofline"] = {
+            "bound": "mfma", "kernel": "U-net 16->16 3x3x3 convolution at 32^3 (k_conv_tile<1, 1>, BF16 MFMA 16x16x32)",
+            "achieved": round(conv_tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(conv_tf / BF16_PEAK_TFLOPS, 5), "flops": conv_flop,
+            "in_situ_us": round(1e3 * ku.avg_ms, 3), "in_situ_launches": ku.launches, "rocprof_avg_us": conv_rp,
+            "frac_rocprof": round(conv_flop / (conv_rp * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 5) if conv_rp else None}
+        # the same kernel where it is not latency-bound: one application of
+        # the network at 128^3 (the solver applies it at 32^3, about a launch's
+        # latency of work)
+        x128 = np.zeros((4, N + 2, N + 2, N + 2), dtype=np.float32)
+        x128[:, 1:-1, 1:-1, 1:-1] = np.random.default_rng(7).standard_normal((4, N, N, N)).astype(np.float32)
+        net.forward(x128)
+        with M.ktimer("unet_conv16") as k128:
+            for _ in range(3):
+                net.forward(x128)
+        M.synchronize()
+        fl128 = 2 * 27 * 16 * 16 * N ** 3
+        tf128 = fl128 / (k128.avg_ms * 1e-3) / 1e12 if k128.launches else 0.0
+        learned_out["roofline"]["at_128"] = {"achieved": round(tf128, 1), "frac": round(tf128 / BF16_PEAK_TFLOPS, 4),
+                                             "in_situ_us": round(1e3 * k128.avg_ms, 2), "flops": fl128}
+        del x128
 
     # every solution against the FP64 Jacobi-PCG converged to 1e-11 (constant removed)
     xr = M.SF(m, "CC").fill(0.0).set_BCs(types, None, all_Neumann=True)

@@ -132,6 +132,7 @@ int moons_init(int device);
 #define MOONS_KTIMER_PCG_MATVEC 2   /* FP64 PCG kernels of the PPE */
 #define MOONS_KTIMER_PCG_UPDATE 3
 #define MOONS_KTIMER_PCG_PUPDATE 4
+#define MOONS_KTIMER_UNET_CONV16 5  /* the U-net's forward 16 -> 16 convolutions of its finest level */
 int moons_ktimer_start(int which, int max_launches);
 int moons_ktimer_stop(int *launches, double *avg_ms);
 int moons_stream_create(void **out);
@@ -656,6 +657,10 @@ int moons_unet_train(moons_unet_t net, int n0, int n1, int n2, int npair, const 
 /* D = A B for 16x16 A and B rounded to BF16, through one MFMA with the lane
  * layout the U-net kernels use (a self-check of that layout). */
 int moons_mfma_check(const float *A, const float *B, float *D);
+/* D = A B for a 16x32 A and a 32x16 B rounded to BF16, through one
+ * v_mfma_f32_16x16x32_bf16 with the lane layout of the finest-level
+ * convolutions (a self-check of that layout).  No reference counterpart. */
+int moons_mfma_check32(const float *A, const float *B, float *D);
 /* Learned + mixed: inside a MOONS_PREC_MG_FP32 solver's V-cycle, the network
  * gives the initial guess of multigrid level `level` (0 = finest; a level
  * launched over the grid, with cell counts that are multiples of 4): before

@@ -163,6 +163,7 @@ SIGNATURES = {
     "moons_unet_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_float_p, c_float_p, C.c_int,
                                    C.c_int, C.c_double, C.c_ulonglong, c_double_p]),
     "moons_mfma_check": (C.c_int, [c_float_p, c_float_p, c_float_p]),
+    "moons_mfma_check32": (C.c_int, [c_float_p, c_float_p, c_float_p]),
     "moons_pcg_set_unet": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_pcg_set_mg_unet": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "moons_pcg_set_mg_unet_cycles": (C.c_int, [C.c_void_p, C.c_int]),

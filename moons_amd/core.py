@@ -35,7 +35,7 @@ def synchronize() -> None:
     check(load().moons_synchronize())
 
 
-KTIMERS = {"mg_line_fine": 1, "pcg_matvec": 2, "pcg_update": 3, "pcg_pupdate": 4}
+KTIMERS = {"mg_line_fine": 1, "pcg_matvec": 2, "pcg_update": 3, "pcg_pupdate": 4, "unet_conv16": 5}
 
 
 class ktimer:
@@ -808,6 +808,19 @@ def mfma_check(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     b = np.ascontiguousarray(B, dtype=np.float32)
     d = np.zeros((16, 16), dtype=np.float32)
     check(load().moons_mfma_check(_fp(a), _fp(b), _fp(d)))
+    return d
+
+
+def mfma_check32(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """D = A @ B (A 16x32, B 32x16, operands rounded to BF16) through one
+    v_mfma_f32_16x16x32_bf16 with the finest-level convolutions' lane layout."""
+    if not _initialised:
+        init(0)
+    a = np.ascontiguousarray(A, dtype=np.float32)
+    b = np.ascontiguousarray(B, dtype=np.float32)
+    assert a.shape == (16, 32) and b.shape == (32, 16)
+    d = np.zeros((16, 16), dtype=np.float32)
+    check(load().moons_mfma_check32(_fp(a), _fp(b), _fp(d)))
     return d
 
 

@@ -29,6 +29,8 @@ namespace moons {
 typedef unsigned short bf16;
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((unsigned)h) << 16); }
 __device__ __forceinline__ bf16 f2bf(float f) {
@@ -119,6 +121,92 @@ __global__ void __launch_bounds__(kUT) k_conv_mfma(const bf16 *__restrict__ X, i
         Y[(size_t)pv * Cout + co] = f2bf(o);
       }
     }
+}
+
+// ------------------------------------------------ LDS-tiled MFMA conv
+// The finest level's convolutions (n0 a multiple of 16) on gfx950's
+// v_mfma_f32_16x16x32_bf16.  A workgroup (4 waves) owns a 16 x 2 x 2 voxel
+// tile: it stages the tile's input halo (18 x 4 x 4 voxels, all Cin = 16 CG
+// channels) in LDS once with 16-byte coalesced loads, then each wave computes
+// one 16-voxel x-row for all Cout = 16 NB channels.  K runs over the
+// 27 CG 16-channel groups (tap-major), two groups per MFMA: lane l holds
+// A[voxel l % 16][channels 8 (l / 16 % 2) ..+8 of group 2c + l / 32] read
+// from LDS as one 16-byte word, and B pre-packed in the same k order
+// (WB32[(c NB + nb) 64 + lane]); an odd group count pads the last MFMA with
+// zeros.  Every input voxel is read from HBM once per tile (1.5x for the
+// halo) instead of 27 times through L2; the 16 -> 16 layer's weight
+// fragments are staged in LDS beside the halo, so its MFMA chain reads only
+// LDS, and a workgroup loops over several tiles along z on large grids so
+// that they are staged once for all of them.
+constexpr int kCTX = 16, kCTY = 2, kCTZ = 2; // output tile (one row of 16 voxels a wave)
+constexpr int kHX = kCTX + 2, kHY = kCTY + 2, kHZ = kCTZ + 2;
+
+template <int CG, int NB>
+__global__ void __launch_bounds__(256) k_conv_tile(const bf16 *__restrict__ X, const u16x8 *__restrict__ WB,
+                                                  const float *__restrict__ bias, bf16 *__restrict__ Y, int n0,
+                                                  int n1, int n2, int relu, int zt, const int *done) {
+  if (gated(done)) return;
+  constexpr int CIN = 16 * CG, COUT = 16 * NB;
+  constexpr int V8 = CIN / 8;                  // 16-byte words per voxel
+  constexpr int NH = kHX * kHY * kHZ;          // halo voxels
+  constexpr int G = 27 * CG, NC = (G + 1) / 2; // channel groups, MFMA steps
+  constexpr int NW = NC * NB * 64;             // weight fragments
+  // the weights join the halo in LDS when both fit in 32 KB (the 16 -> 16
+  // layer; the larger ones keep their occupancy and read the weights via L2)
+  constexpr bool WLDS = (NH * V8 + NW) * 16 <= 32 * 1024;
+  __shared__ u16x8 sx[NH * V8 + (WLDS ? NW : 0)];
+  const int x0 = blockIdx.x * kCTX, y0 = blockIdx.y * kCTY;
+  const int s0 = n0 + 2, s1 = n1 + 2;
+  if constexpr (WLDS)
+    for (int q = threadIdx.x; q < NW; q += 256) sx[NH * V8 + q] = WB[q];
+  const u16x8 *wb = WLDS ? sx + NH * V8 : WB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ry = wave & 1, rz = wave >> 1, m = lane & 15, kh = (lane >> 4) & 1, gs = lane >> 5;
+  // the workgroup's zt tiles along z, one after another (the weights staged once)
+  for (int t = 0; t < zt; ++t) {
+  const int z0 = (blockIdx.z * zt + t) * kCTZ;
+  if (t > 0) __syncthreads(); // the previous tile's halo reads are done
+  // stage the halo: padded voxel (x0 + hx, y0 + hy, z0 + hz) is interior
+  // voxel (x0 + hx - 1, ...), so the halo's first cell is the padded origin
+  for (int q = threadIdx.x; q < NH * V8; q += 256) {
+    const int v = q / V8, w = q - v * V8;
+    const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+    const size_t pv = ((size_t)(z0 + hz) * s1 + (y0 + hy)) * s0 + (x0 + hx);
+    sx[q] = *reinterpret_cast<const u16x8 *>(X + pv * CIN + w * 8);
+  }
+  __syncthreads();
+  f32x4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int g = 2 * c + gs;
+    u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < G) {
+      const int tap = g / CG, cg = g - tap * CG;
+      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+      a = sx[(((rz + dz) * kHY + (ry + dy)) * kHX + (m + dx)) * V8 + cg * 2 + kh];
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const u16x8 b = wb[(c * NB + nb) * 64 + lane];
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                        acc[nb], 0, 0, 0);
+    }
+  }
+  const size_t prow = ((size_t)(z0 + rz + 1) * s1 + (y0 + ry + 1)) * s0 + (x0 + 1);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const size_t pv = prow + (lane >> 4) * 4 + j;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = nb * 16 + m;
+      float o = acc[nb][j] + (bias ? bias[co] : 0.f);
+      if (relu) o = fmaxf(o, 0.f);
+      Y[pv * COUT + co] = f2bf(o);
+    }
+  }
+  }
 }
 
 // The same convolution with its 27 taps split into groups, for layers whose
@@ -272,52 +360,53 @@ __global__ void __launch_bounds__(kUT) k_dbias(const bf16 *dY, int C, int n0, in
 __global__ void __launch_bounds__(kUT) k_conv_in(const bf16 *__restrict__ X, const float *__restrict__ W,
                                                 const float *__restrict__ bias, bf16 *__restrict__ Y, int n0, int n1,
                                                 int n2, const int *done) {
-  // a thread per voxel and group of 4 output channels (the group fastest, so
-  // the 8-byte stores of a wave are contiguous)
+  // a thread per voxel and output channel (the channel fastest: the 16
+  // threads of a voxel share its input loads and store 32 contiguous bytes)
   if (gated(done)) return;
   const int t = blockIdx.x * kUT + threadIdx.x;
-  const int v = t >> 2, cg = (t & 3) * 4;
+  const int v = t >> 4, co = t & 15;
   if (v >= n0 * n1 * n2) return;
   const int s0 = n0 + 2, s1 = n1 + 2;
   const int pv = pvox(v, n0, n1);
-  float acc[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) acc[c] = bias[cg + c];
+  float acc = bias[co];
   for (int tap = 0; tap < 27; ++tap) {
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const bf16 *x = X + (size_t)(pv + (dz * s1 + dy) * s0 + dx) * 4;
+    // the voxel's 4 channels in one 8-byte load
+    const uint2 q = *reinterpret_cast<const uint2 *>(X + (size_t)(pv + (dz * s1 + dy) * s0 + dx) * 4);
+    const unsigned qw[2] = {q.x, q.y};
 #pragma unroll
     for (int ci = 0; ci < 4; ++ci) {
-      const float xv = bf2f(x[ci]);
-      const float *w = W + (tap * 4 + ci) * 16 + cg;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += xv * w[c];
+      const float xv = __uint_as_float((ci & 1) ? (qw[ci >> 1] & 0xFFFF0000u) : (qw[ci >> 1] << 16));
+      acc += xv * W[(tap * 4 + ci) * 16 + co];
     }
   }
-  bf16 o[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) o[c] = f2bf(fmaxf(acc[c], 0.f));
-  *reinterpret_cast<uint2 *>(Y + (size_t)pv * 16 + cg) = *reinterpret_cast<const uint2 *>(o);
+  Y[(size_t)pv * 16 + co] = f2bf(fmaxf(acc, 0.f));
 }
 
-// last layer: out[pv] = scale * (b + sum_tap,ci W[tap][ci] X[v+tap][ci]), Cin = 16, FP32 padded output
+// last layer: out[pv] = scale * (b + sum_ci sum_tap W[tap][ci] X[v+tap][ci]),
+// Cin = 16, FP32 padded output.  16 threads a voxel, one per input channel,
+// each summing its channel over the taps; the 16 sums are added in a fixed
+// butterfly (deterministic) and lane ci = 0 writes.
 __global__ void __launch_bounds__(kUT) k_conv_out(const bf16 *__restrict__ X, const float *__restrict__ W,
                                                  const float *__restrict__ bias, float *__restrict__ out, int n0, int n1,
                                                  int n2, const float *scale, const int *done) {
   if (gated(done)) return;
-  const int v = blockIdx.x * kUT + threadIdx.x;
-  if (v >= n0 * n1 * n2) return;
+  const int t = blockIdx.x * kUT + threadIdx.x;
+  const int v = t >> 4, ci = t & 15;
+  const bool live = v < n0 * n1 * n2;
   const int s0 = n0 + 2, s1 = n1 + 2;
-  const int pv = pvox(v, n0, n1);
-  float acc = bias[0];
+  const int pv = pvox(live ? v : 0, n0, n1);
+  float acc = 0.f;
   for (int tap = 0; tap < 27; ++tap) {
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const bf16 *x = X + (size_t)(pv + (dz * s1 + dy) * s0 + dx) * 16;
-    const float *w = W + tap * 16;
-#pragma unroll
-    for (int ci = 0; ci < 16; ++ci) acc += bf2f(x[ci]) * w[ci];
+    acc += bf2f(X[(size_t)(pv + (dz * s1 + dy) * s0 + dx) * 16 + ci]) * W[tap * 16 + ci];
   }
-  out[pv] = scale ? acc * (*scale) : acc;
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 16);
+  if (live && ci == 0) {
+    const float r = bias[0] + acc;
+    out[pv] = scale ? r * (*scale) : r;
+  }
 }
 
 // last layer backward: dX[v][ci] = sum_tap W[tap][ci] * dout[v - tap] (dout zero outside)
@@ -498,6 +587,30 @@ __global__ void __launch_bounds__(kUT) k_pack(const float *W, int Cin, int Cout,
   WB[t] = r;
 }
 
+// pack FP32 W into the K = 32 B fragments of k_conv_tile: MFMA step c holds
+// the 16-channel groups 2c and 2c + 1 (group g = tap Ci/16 + channel block),
+// lane l the k values 8 (l / 16) ..+8; flip as in k_pack.
+__global__ void __launch_bounds__(kUT) k_pack32(const float *W, int Cin, int Cout, int flip, u16x8 *WB) {
+  const int Ci = flip ? Cout : Cin, Co = flip ? Cin : Cout;
+  const int CG = Ci / 16, NB = Co / 16, G = 27 * CG, NC = (G + 1) / 2;
+  const size_t t = (size_t)blockIdx.x * kUT + threadIdx.x;
+  if (t >= (size_t)NC * NB * 64) return;
+  const int lane = (int)(t % 64), nb = (int)((t / 64) % NB), c = (int)(t / 64 / NB);
+  u16x8 r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = (lane >> 4) * 8 + j;
+    const int g = 2 * c + (k >> 4);
+    float w = 0.f;
+    if (g < G) {
+      const int tap = g / CG, ci = (g - tap * CG) * 16 + (k & 15), co = nb * 16 + (lane & 15);
+      w = flip ? W[((size_t)(26 - tap) * Cin + co) * Cout + ci] : W[((size_t)tap * Cin + ci) * Cout + co];
+    }
+    r[j] = f2bf(w);
+  }
+  WB[t] = r;
+}
+
 // BF16-rounded FP32 copy of the small layers' weights (the values the network uses)
 __global__ void __launch_bounds__(kUT) k_round_bf16(const float *W, float *Wr, size_t n) {
   const size_t t = (size_t)blockIdx.x * kUT + threadIdx.x;
@@ -631,6 +744,22 @@ __global__ void __launch_bounds__(64) k_mfma_check(const float *A, const float *
   for (int j = 0; j < 4; ++j) D[((l >> 4) * 4 + j) * 16 + (l & 15)] = c[j];
 }
 
+// the same for v_mfma_f32_16x16x32_bf16: A 16 x 32, B 32 x 16, lane l holding
+// A[l % 16][8 (l / 16) ..+8] and B[8 (l / 16) ..+8][l % 16]
+__global__ void __launch_bounds__(64) k_mfma_check32(const float *A, const float *B, float *D) {
+  const int l = threadIdx.x;
+  u16x8 a, b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    a[j] = f2bf(A[(l & 15) * 32 + (l >> 4) * 8 + j]);
+    b[j] = f2bf(B[((l >> 4) * 8 + j) * 16 + (l & 15)]);
+  }
+  f32x4 c = {0.f, 0.f, 0.f, 0.f};
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) D[((l >> 4) * 4 + j) * 16 + (l & 15)] = c[j];
+}
+
 // ------------------------------------------------------------------ host
 struct Layer {
   int Cin, Cout, lev; // level 0 = finest
@@ -655,6 +784,7 @@ struct moons_unet_s {
   size_t nparam = 0;
   float *W = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr, *Wr = nullptr; // master, grad, Adam m/v, rounded
   s16x4 *WB[11] = {}, *WT[11] = {};
+  u16x8 *WB32[11] = {}, *WT32[11] = {}; // k_conv_tile fragments (finest-level layers)
   int adam_t = 0;
   // activations for one grid size
   int n[3] = {0, 0, 0};
@@ -721,12 +851,24 @@ int ensure_acts(moons_unet_s *U, int n0, int n1, int n2) {
   return MOONS_OK;
 }
 
+// fragments of k_pack32 for a convolution of Ci input and Co output channels
+size_t pack32_n(int Ci, int Co) { return (size_t)((27 * (Ci / 16) + 1) / 2) * (Co / 16) * 64; }
+
+// the (Cin, Cout) pairs k_conv_tile is instantiated for
+bool tile_conv(int Ci, int Co) { return (Ci == 16 && Co == 16) || (Ci == 48 && Co == 16) || (Ci == 16 && Co == 48); }
+
 int pack_weights(moons_unet_s *U) {
   for (int l = 1; l <= 9; ++l) {
     const Layer &L = U->L[l];
     const size_t nf = (size_t)27 * (L.Cin / 16) * (L.Cout / 16) * 64;
     hipLaunchKernelGGL(k_pack, dim3(ublk(nf)), dim3(kUT), 0, stream(), U->W + L.w_off, L.Cin, L.Cout, 0, U->WB[l]);
     hipLaunchKernelGGL(k_pack, dim3(ublk(nf)), dim3(kUT), 0, stream(), U->W + L.w_off, L.Cin, L.Cout, 1, U->WT[l]);
+    if (U->WB32[l]) {
+      hipLaunchKernelGGL(k_pack32, dim3(ublk(pack32_n(L.Cin, L.Cout))), dim3(kUT), 0, stream(), U->W + L.w_off, L.Cin,
+                         L.Cout, 0, U->WB32[l]);
+      hipLaunchKernelGGL(k_pack32, dim3(ublk(pack32_n(L.Cout, L.Cin))), dim3(kUT), 0, stream(), U->W + L.w_off, L.Cin,
+                         L.Cout, 1, U->WT32[l]);
+    }
   }
   hipLaunchKernelGGL(k_round_bf16, dim3(ublk(U->nparam)), dim3(kUT), 0, stream(), U->W, U->Wr, U->nparam);
   MOONS_LAUNCH_CHECK();
@@ -748,8 +890,26 @@ static void conv_split(const moons_unet_s *U, const s16x4 *WB, const float *bias
 }
 
 void conv_mfma(const moons_unet_s *U, const s16x4 *WB, const float *bias, const Buf &X, Buf &Y, int n0, int n1, int n2,
-               int relu, const int *done) {
+               int relu, const int *done, const u16x8 *WB32 = nullptr) {
   const int nvox = n0 * n1 * n2;
+  if (WB32 && n0 % kCTX == 0 && n1 % kCTY == 0 && n2 % kCTZ == 0 && tile_conv(X.C, Y.C)) {
+    // tiles per workgroup along z: keep at least 2048 workgroups (8 a CU)
+    const int tz = n2 / kCTZ, wxy = (n0 / kCTX) * (n1 / kCTY);
+    int zt = 1;
+    while (tz % (2 * zt) == 0 && (size_t)wxy * (tz / (2 * zt)) >= 2048) zt *= 2;
+    const dim3 grid((unsigned)(n0 / kCTX), (unsigned)(n1 / kCTY), (unsigned)(tz / zt));
+    if (X.C == 16 && Y.C == 16) {
+      const int timer = bias ? MOONS_KTIMER_UNET_CONV16 : 0; // forward layers carry a bias
+      ktimer_mark(timer);
+      hipLaunchKernelGGL((k_conv_tile<1, 1>), grid, dim3(256), 0, stream(), X.d, WB32, bias, Y.d, n0, n1, n2, relu, zt, done);
+      ktimer_mark(timer);
+    }
+    else if (X.C == 48)
+      hipLaunchKernelGGL((k_conv_tile<3, 1>), grid, dim3(256), 0, stream(), X.d, WB32, bias, Y.d, n0, n1, n2, relu, zt, done);
+    else
+      hipLaunchKernelGGL((k_conv_tile<1, 3>), grid, dim3(256), 0, stream(), X.d, WB32, bias, Y.d, n0, n1, n2, relu, zt, done);
+    return;
+  }
   const unsigned grid = (unsigned)((nvox + 127) / 128);
   const int waves = (nvox + 31) / 32;
   // a wide input (48 channels or more: 81+ MFMA steps a wave) splits up to
@@ -787,9 +947,9 @@ int forward(moons_unet_s *U, float *out, const float *scale, const int *done) {
   const int n0 = U->n[0], n1 = U->n[1], n2 = U->n[2];
   const float *W = U->Wr;
   auto nv = [&](int lev) { return (size_t)(n0 >> lev) * (n1 >> lev) * (n2 >> lev); };
-  hipLaunchKernelGGL(k_conv_in, dim3(ublk(4 * nv(0))), dim3(kUT), 0, stream(), U->in.d, W + U->L[0].w_off,
+  hipLaunchKernelGGL(k_conv_in, dim3(ublk(16 * nv(0))), dim3(kUT), 0, stream(), U->in.d, W + U->L[0].w_off,
                      U->W + U->L[0].b_off, U->a0.d, n0, n1, n2, done);
-  conv_mfma(U, U->WB[1], U->W + U->L[1].b_off, U->a0, U->e1, n0, n1, n2, 1, done);
+  conv_mfma(U, U->WB[1], U->W + U->L[1].b_off, U->a0, U->e1, n0, n1, n2, 1, done, U->WB32[1]);
   hipLaunchKernelGGL(k_pool, dim3(ublk(nv(1) * 16)), dim3(kUT), 0, stream(), U->e1.d, U->p1.d, 16, n0 / 2, n1 / 2, n2 / 2, done);
   conv_mfma(U, U->WB[2], U->W + U->L[2].b_off, U->p1, U->a2, n0 / 2, n1 / 2, n2 / 2, 1, done);
   conv_mfma(U, U->WB[3], U->W + U->L[3].b_off, U->a2, U->e2, n0 / 2, n1 / 2, n2 / 2, 1, done);
@@ -800,9 +960,9 @@ int forward(moons_unet_s *U, float *out, const float *scale, const int *done) {
   conv_mfma(U, U->WB[6], U->W + U->L[6].b_off, U->c2, U->a6, n0 / 2, n1 / 2, n2 / 2, 1, done);
   conv_mfma(U, U->WB[7], U->W + U->L[7].b_off, U->a6, U->d2, n0 / 2, n1 / 2, n2 / 2, 1, done);
   hipLaunchKernelGGL(k_upcat, dim3(ublk(nv(0) * 48)), dim3(kUT), 0, stream(), U->d2.d, 32, U->e1.d, 16, U->c1.d, n0, n1, n2, done);
-  conv_mfma(U, U->WB[8], U->W + U->L[8].b_off, U->c1, U->a8, n0, n1, n2, 1, done);
-  conv_mfma(U, U->WB[9], U->W + U->L[9].b_off, U->a8, U->d1, n0, n1, n2, 1, done);
-  hipLaunchKernelGGL(k_conv_out, dim3(ublk(nv(0))), dim3(kUT), 0, stream(), U->d1.d, W + U->L[10].w_off,
+  conv_mfma(U, U->WB[8], U->W + U->L[8].b_off, U->c1, U->a8, n0, n1, n2, 1, done, U->WB32[8]);
+  conv_mfma(U, U->WB[9], U->W + U->L[9].b_off, U->a8, U->d1, n0, n1, n2, 1, done, U->WB32[9]);
+  hipLaunchKernelGGL(k_conv_out, dim3(ublk(16 * nv(0))), dim3(kUT), 0, stream(), U->d1.d, W + U->L[10].w_off,
                      U->W + U->L[10].b_off, out, n0, n1, n2, scale, done);
   MOONS_LAUNCH_CHECK();
   return MOONS_OK;
@@ -851,7 +1011,7 @@ int backward(moons_unet_s *U, int acc) {
   };
   auto dgrad = [&](int l, const Buf &dY, Buf &dX, int a0, int a1, int a2) {
     // dX = conv(dY; W flipped/transposed): Cin' = Cout, Cout' = Cin
-    conv_mfma(U, U->WT[l], nullptr, dY, dX, a0, a1, a2, 0, nullptr);
+    conv_mfma(U, U->WT[l], nullptr, dY, dX, a0, a1, a2, 0, nullptr, U->WT32[l]);
   };
   relu_bwd(U->g_d1, U->d1, 0);
   grad_mfma(U, 9, U->a8, U->g_d1, n0, n1, n2, acc);
@@ -1045,6 +1205,9 @@ int moons_unet_create(unsigned long long seed, moons_unet_t *out) {
   for (int l = 1; l <= 9 && ok; ++l) {
     const size_t nf = (size_t)27 * (U->L[l].Cin / 16) * (U->L[l].Cout / 16) * 64;
     ok = hipMalloc(&U->WB[l], nf * sizeof(s16x4)) == hipSuccess && hipMalloc(&U->WT[l], nf * sizeof(s16x4)) == hipSuccess;
+    if (ok && U->L[l].lev == 0 && tile_conv(U->L[l].Cin, U->L[l].Cout))
+      ok = hipMalloc(&U->WB32[l], pack32_n(U->L[l].Cin, U->L[l].Cout) * sizeof(u16x8)) == hipSuccess &&
+           hipMalloc(&U->WT32[l], pack32_n(U->L[l].Cout, U->L[l].Cin) * sizeof(u16x8)) == hipSuccess;
   }
   if (!ok) {
     moons_unet_destroy(U);
@@ -1073,6 +1236,8 @@ int moons_unet_destroy(moons_unet_t U) {
   for (int l = 0; l < 11; ++l) {
     if (U->WB[l]) (void)hipFree(U->WB[l]);
     if (U->WT[l]) (void)hipFree(U->WT[l]);
+    if (U->WB32[l]) (void)hipFree(U->WB32[l]);
+    if (U->WT32[l]) (void)hipFree(U->WT32[l]);
   }
   delete U;
   return MOONS_OK;
@@ -1198,6 +1363,19 @@ int moons_mfma_check(const float *A, const float *B, float *D) {
   MOONS_HIP(hipMemcpyAsync(d + 256, B, 256 * 4, hipMemcpyHostToDevice, stream()));
   hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, stream(), d, d + 256, d + 512);
   MOONS_HIP(hipMemcpyAsync(D, d + 512, 256 * 4, hipMemcpyDeviceToHost, stream()));
+  MOONS_HIP(hipStreamSynchronize(stream()));
+  (void)hipFree(d);
+  return MOONS_OK;
+}
+
+int moons_mfma_check32(const float *A, const float *B, float *D) {
+  if (!A || !B || !D) return fail(MOONS_ERR_ARG, "moons_mfma_check32: null argument");
+  float *d = nullptr;
+  MOONS_HIP(hipMalloc(&d, (512 + 512 + 256) * sizeof(float)));
+  MOONS_HIP(hipMemcpyAsync(d, A, 512 * 4, hipMemcpyHostToDevice, stream()));
+  MOONS_HIP(hipMemcpyAsync(d + 512, B, 512 * 4, hipMemcpyHostToDevice, stream()));
+  hipLaunchKernelGGL(k_mfma_check32, dim3(1), dim3(64), 0, stream(), d, d + 512, d + 1024);
+  MOONS_HIP(hipMemcpyAsync(D, d + 1024, 256 * 4, hipMemcpyDeviceToHost, stream()));
   MOONS_HIP(hipStreamSynchronize(stream()));
   (void)hipFree(d);
   return MOONS_OK;

@@ -22,6 +22,16 @@ def test_mfma_lane_layout(gpu):
     np.testing.assert_array_equal(D, A @ B)
 
 
+def test_mfma32_lane_layout(gpu):
+    """v_mfma_f32_16x16x32_bf16 with the finest-level convolutions' lane layout
+    (small integers: exact in BF16 and in the FP32 sums)."""
+    rng = np.random.default_rng(1)
+    A = rng.integers(-8, 8, (16, 32)).astype(np.float32)
+    B = rng.integers(-8, 8, (32, 16)).astype(np.float32)
+    D = gpu.mfma_check32(A, B)
+    np.testing.assert_array_equal(D, A @ B)
+
+
 def sample_input(shape, seed):
     rng = np.random.default_rng(seed)
     x = np.zeros((4,) + tuple(s + 2 for s in shape), dtype=np.float32)
