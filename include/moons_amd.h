@@ -216,6 +216,11 @@ int moons_field_set_bcs(moons_field_t f, const int type[6], const double val[6],
  * the wall conductance ratio c_w of each face, which Robin faces use, and the
  * Robin coefficients kept beside it (robin_coeff may be NULL). */
 int moons_field_set_bc_props(moons_field_t f, const double c_w[6], const double robin_coeff[6]);
+/* dst takes src's BCs (types, values, all_Neumann, c_w, Robin coefficients),
+ * as the reference's init(SF, SF) copies them for a solver's fields
+ * (init(PCG%p, x) and init(PCG%x_BC, x_BC), code/handwritten/PCG/PCG_solver_extend.f90:72-73);
+ * the two fields share a data location. */
+int moons_field_copy_bcs(moons_field_t dst, moons_field_t src);
 /* Host <-> device copies of the whole ghost-padded array (n = element count). */
 int moons_field_upload(moons_field_t f, const double *host, size_t n);
 int moons_field_download(moons_field_t f, double *host, size_t n);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "moons_field_shape": (C.c_int, [C.c_void_p, c_int_p]),
     "moons_field_set_bcs": (C.c_int, [C.c_void_p, c_int_p, c_double_p, C.c_int]),
     "moons_field_set_bc_props": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "moons_field_copy_bcs": (C.c_int, [C.c_void_p, C.c_void_p]),
     "moons_field_upload": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
     "moons_field_download": (C.c_int, [C.c_void_p, c_double_p, C.c_size_t]),
     "moons_field_fill": (C.c_int, [C.c_void_p, C.c_double]),

@@ -986,6 +986,13 @@ int moons_field_set_bc_props(moons_field_t f, const double c_w[6], const double 
   return MOONS_OK;
 }
 
+int moons_field_copy_bcs(moons_field_t dst, moons_field_t src) {
+  if (!dst || !src) return fail(MOONS_ERR_ARG, "moons_field_copy_bcs: null argument");
+  if (dst->dl != src->dl) return fail(MOONS_ERR_ARG, "moons_field_copy_bcs: fields of different data locations");
+  copy_bcs_of(dst, src);
+  return MOONS_OK;
+}
+
 int moons_field_upload(moons_field_t f, const double *host, size_t n) {
   if (!f || !host || n != f->n) return fail(MOONS_ERR_ARG, "moons_field_upload: size mismatch");
   MOONS_HIP(hipMemcpyAsync(f->d, host, n * sizeof(double), hipMemcpyHostToDevice, g_stream));

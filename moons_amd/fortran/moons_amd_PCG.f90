@@ -30,8 +30,9 @@ module moons_amd_PCG_mod
   private
 
   public :: mesh_gpu, SF_gpu, PCG_solver_SF_gpu, PCG_solver_VF_gpu, FFT_solver_SF_gpu, energy_gpu
+  public :: exit_criteria_gpu, iter_solver_params_gpu, matrix_free_params_gpu
   public :: init, solve, delete, clean_div, update_MFP, set_joule
-  public :: init_CC, init_Face, set_BCs
+  public :: init_CC, init_Face, init_Edge, set_BCs
   public :: export, import, export_raw, import_raw
 
   type mesh_gpu
@@ -43,6 +44,28 @@ module moons_amd_PCG_mod
     type(c_ptr) :: h = c_null_ptr
     integer :: s(3) = 0
     integer(c_int) :: dl = 0
+  end type
+
+  ! The reference's solver parameters with its field names:
+  ! iter_solver_params (code/generated_code/var_set/iter_solver_params.f90:44-55)
+  ! with its exit_criteria (code/generated_code/var_set/exit_criteria.f90:44-50),
+  ! and matrix_free_params (code/generated_code/PCG/matrix_free_params.f90:39-47).
+  type exit_criteria_gpu
+    integer :: iter_max = 0
+    real(c_double) :: tol_abs = 0.0d0, tol_rel = 0.0d0
+  end type
+
+  type iter_solver_params_gpu
+    type(exit_criteria_gpu) :: EC
+    logical :: exit_loop(3) = .false.
+    integer :: iter_total = 0, iter_per_call = 0, n_skip_check_res = 0
+    logical :: export_convergence = .false., export_heavy = .false.
+  end type
+
+  type matrix_free_params_gpu
+    logical :: suppress_warning = .false.
+    real(c_double) :: alpha = 0.0d0, beta = 0.0d0, coeff_natural = 0.0d0, coeff_explicit = 0.0d0
+    real(c_double) :: coeff_implicit = 0.0d0, coeff_implicit_time_split = 0.0d0
   end type
 
   type PCG_solver_SF_gpu
@@ -79,9 +102,11 @@ module moons_amd_PCG_mod
 
   interface init;   module procedure init_mesh_gpu; module procedure init_PCG_SF_gpu
                     module procedure init_PCG_VF_gpu; module procedure init_FFT_gpu
-                    module procedure init_energy_gpu; end interface
+                    module procedure init_energy_gpu; module procedure init_ISP_gpu
+                    module procedure init_PCG_SF_ref; module procedure init_PCG_VF_ref; end interface
   interface solve;  module procedure solve_PCG_SF_gpu; module procedure solve_PCG_VF_gpu
-                    module procedure solve_FFT_gpu; module procedure solve_energy_gpu; end interface
+                    module procedure solve_FFT_gpu; module procedure solve_energy_gpu
+                    module procedure solve_PCG_SF_ref; end interface
   interface set_joule; module procedure set_joule_energy_gpu; end interface
   interface clean_div; module procedure clean_div_PCG_gpu; module procedure clean_div_FFT_gpu; end interface
   interface update_MFP; module procedure update_MFP_PCG_VF_gpu; end interface
@@ -132,6 +157,16 @@ contains
     call init_field(f, m, int(ishft(1, dir - 1), c_int))
   end subroutine
 
+  ! edge field along dir (1, 2, 3): node-centred off dir
+  subroutine init_Edge(f, m, dir)
+    type(SF_gpu), intent(inout) :: f
+    type(mesh_gpu), intent(in) :: m
+    integer, intent(in) :: dir
+    integer(c_int) :: edl(3)
+    edl = [MOONS_DL_EDGE_X, MOONS_DL_EDGE_Y, MOONS_DL_EDGE_Z]
+    call init_field(f, m, edl(dir))
+  end subroutine
+
   ! BC type per face (MOONS_BC_*), uniform value per face, all_Neumann flag.
   subroutine set_BCs(f, types, vals, all_Neumann)
     type(SF_gpu), intent(inout) :: f
@@ -142,6 +177,86 @@ contains
     an = 0
     if (all_Neumann) an = 1
     call moons_check(moons_field_set_bcs(f%h, types, vals, an), 'set_BCs')
+  end subroutine
+
+  ! init_ISP (code/handwritten/var_set/iter_solver_params_extend.f90:44-59)
+  subroutine init_ISP_gpu(ISP, iter_max, tol_rel, tol_abs, n_skip_check_res, export_convergence, export_heavy)
+    type(iter_solver_params_gpu), intent(inout) :: ISP
+    integer, intent(in) :: iter_max, n_skip_check_res
+    real(c_double), intent(in) :: tol_rel, tol_abs
+    logical, intent(in) :: export_convergence, export_heavy
+    ISP%iter_total = 0
+    ISP%iter_per_call = 0
+    ISP%EC%iter_max = iter_max
+    ISP%EC%tol_rel = tol_rel
+    ISP%EC%tol_abs = tol_abs
+    ISP%export_convergence = export_convergence
+    ISP%export_heavy = export_heavy
+    ISP%n_skip_check_res = n_skip_check_res
+    ISP%exit_loop = .false.
+  end subroutine
+
+  subroutine refuse_debug_options(testSymmetry, exportOperator, where)
+    logical, intent(in) :: testSymmetry, exportOperator
+    character(len=*), intent(in) :: where
+    if (testSymmetry .or. exportOperator) then
+      write(*, '(a)') 'Error: '//where//': testSymmetry and exportOperator have no device path'
+      error stop 'moons_amd: unsupported solver option'
+    endif
+  end subroutine
+
+  ! init_PCG_SF in the reference's argument order (code/handwritten/PCG/PCG_solver_extend.f90:51-107).
+  ! operator (with operator_explicit) and prec are the codes MOONS_OP_* and
+  ! MOONS_PREC_*, since a Fortran procedure cannot run on the device; x_BC
+  ! carries the BCs (a device field, set with set_BCs); k: the three face
+  ! fields of nrg_diffusion's coefficient (their handles may be null for the
+  ! Laplacian, whose k the reference does not read); MFP%coeff_implicit_time_split
+  ! is the operator's coefficient (nrg_diffusion); var_dir and var_name name the
+  ! solve in its messages.  testSymmetry and exportOperator must be .false.
+  subroutine init_PCG_SF_ref(PCG, operator, prec, m, ISP, MFP, x, x_BC, k, var_dir, var_name, testSymmetry, &
+                             exportOperator)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    integer(c_int), intent(in) :: operator, prec
+    type(mesh_gpu), intent(in) :: m
+    type(iter_solver_params_gpu), intent(in) :: ISP
+    type(matrix_free_params_gpu), intent(in) :: MFP
+    type(SF_gpu), intent(in) :: x, x_BC
+    type(SF_gpu), intent(in) :: k(3)
+    character(len=*), intent(in) :: var_dir, var_name
+    logical, intent(in) :: testSymmetry, exportOperator
+    type(c_ptr) :: hk(3)
+    integer :: d
+    call refuse_debug_options(testSymmetry, exportOperator, 'init_PCG_SF '//var_dir//var_name)
+    if (x%dl .ne. MOONS_DL_CC) error stop 'Error: init_PCG_SF_gpu: x must be cell-centred'
+    call init_CC(PCG%x, m)
+    call init_CC(PCG%b, m)
+    call moons_check(moons_field_copy_bcs(PCG%x%h, x_BC%h), 'init_PCG_SF '//var_name)
+    call moons_check(moons_pcg_sf_create(m%h, operator, prec, PCG%x%h, PCG%h), 'init_PCG_SF '//var_name)
+    PCG%iter_max = ISP%EC%iter_max
+    PCG%tol_rel = ISP%EC%tol_rel
+    PCG%tol_abs = ISP%EC%tol_abs
+    PCG%n_skip_check_res = max(ISP%n_skip_check_res, 1)
+    call moons_check(moons_pcg_set_isp(PCG%h, int(PCG%iter_max, c_int), PCG%tol_rel, PCG%tol_abs, &
+                                       int(PCG%n_skip_check_res, c_int)), 'ISP '//var_name)
+    if (operator .eq. MOONS_OP_NRG_DIFFUSION) then
+      do d = 1, 3
+        hk(d) = k(d)%h
+      enddo
+      call moons_check(moons_pcg_sf_set_k(PCG%h, hk), 'init_PCG_SF k '//var_name)
+      call moons_check(moons_pcg_sf_update_mfp(PCG%h, MFP%coeff_implicit_time_split, 1_c_int), 'MFP '//var_name)
+    endif
+  end subroutine
+
+  ! solve_PCG_SF in the reference's argument order (PCG_solver_extend.f90:182-193):
+  ! x, b host arrays (ghosts included); m names the mesh as there.
+  subroutine solve_PCG_SF_ref(PCG, x, b, m, compute_norms)
+    type(PCG_solver_SF_gpu), intent(inout) :: PCG
+    real(c_double), intent(inout), contiguous, target :: x(:, :, :)
+    real(c_double), intent(in), contiguous, target :: b(:, :, :)
+    type(mesh_gpu), intent(in) :: m
+    logical, intent(in) :: compute_norms
+    if (.not. c_associated(m%h)) error stop 'Error: solve_PCG_SF_gpu: mesh not initialised'
+    call solve_PCG_SF_gpu(PCG, x, b, compute_norms)
   end subroutine
 
   ! init_PCG_SF: operator Lap_uniform_SF, prec prec_Lap_SF (prec = MOONS_PREC_LAP_SF)
@@ -334,6 +449,38 @@ contains
     call moons_check(moons_pcg_vf_create(m%h, op, hx, hk, PCG%h), 'init_PCG_VF')
     call moons_check(moons_pcg_vf_set_isp(PCG%h, int(iter_max, c_int), tol_rel, tol_abs, &
                                           int(n_skip_check_res, c_int)), 'ISP')
+  end subroutine
+
+  ! init_PCG_VF in the reference's argument order (PCG_solver_extend.f90:117-180):
+  ! operator MOONS_OP_MOM_DIFFUSION (prec_mom_VF) or MOONS_OP_IND_DIFFUSION
+  ! (prec_ind_VF); x_BC(3) carries the BCs of the three face components; k(3):
+  ! sigmaInv on the x, y, z edges (ind_diffusion; null handles for momentum).
+  subroutine init_PCG_VF_ref(PCG, operator, m, ISP, MFP, x, x_BC, k, var_dir, var_name, testSymmetry, exportOperator)
+    type(PCG_solver_VF_gpu), intent(inout) :: PCG
+    integer(c_int), intent(in) :: operator
+    type(mesh_gpu), intent(in) :: m
+    type(iter_solver_params_gpu), intent(in) :: ISP
+    type(matrix_free_params_gpu), intent(in) :: MFP
+    type(SF_gpu), intent(in) :: x(3), x_BC(3), k(3)
+    character(len=*), intent(in) :: var_dir, var_name
+    logical, intent(in) :: testSymmetry, exportOperator
+    type(c_ptr) :: hx(3), hk(3)
+    integer :: d
+    call refuse_debug_options(testSymmetry, exportOperator, 'init_PCG_VF '//var_dir//var_name)
+    do d = 1, 3
+      call init_Face(PCG%x(d), m, d)
+      call init_Face(PCG%b(d), m, d)
+      call moons_check(moons_field_copy_bcs(PCG%x(d)%h, x_BC(d)%h), 'init_PCG_VF '//var_name)
+      hx(d) = PCG%x(d)%h
+      hk(d) = c_null_ptr
+      if (operator .eq. MOONS_OP_IND_DIFFUSION) hk(d) = k(d)%h
+    enddo
+    if (x(1)%dl .ne. MOONS_DL_FACE_X) error stop 'Error: init_PCG_VF_gpu: x must be the three face components'
+    call moons_check(moons_pcg_vf_create(m%h, operator, hx, hk, PCG%h), 'init_PCG_VF '//var_name)
+    call moons_check(moons_pcg_vf_set_isp(PCG%h, int(ISP%EC%iter_max, c_int), ISP%EC%tol_rel, ISP%EC%tol_abs, &
+                                          int(max(ISP%n_skip_check_res, 1), c_int)), 'ISP '//var_name)
+    if (MFP%coeff_implicit_time_split .ne. 0.0d0) &
+      call moons_check(moons_pcg_vf_update_mfp(PCG%h, MFP%coeff_implicit_time_split, 1_c_int), 'MFP '//var_name)
   end subroutine
 
   subroutine update_MFP_PCG_VF_gpu(PCG, coeff_implicit_time_split, update_preconditioner)

@@ -13,7 +13,7 @@ module moons_amd_bindings
   integer(c_int), parameter, public :: MOONS_BC_NONE = 0, MOONS_BC_DIRICHLET = 1, MOONS_BC_NEUMANN = 2
   integer(c_int), parameter, public :: MOONS_BC_PERIODIC = 3, MOONS_BC_SYMMETRIC = 4, MOONS_BC_ANTISYMMETRIC = 5
   integer(c_int), parameter, public :: MOONS_BC_ROBIN = 6
-  integer(c_int), parameter, public :: MOONS_OP_LAP_UNIFORM_SF = 1
+  integer(c_int), parameter, public :: MOONS_OP_LAP_UNIFORM_SF = 1, MOONS_OP_NRG_DIFFUSION = 3
   integer(c_int), parameter, public :: MOONS_PREC_LAP_SF = 1, MOONS_PREC_MG_FP32 = 2
   integer(c_int), parameter, public :: MOONS_OP_MOM_DIFFUSION = 1, MOONS_OP_IND_DIFFUSION = 2
   integer(c_int), parameter, public :: MOONS_DL_EDGE_X = 6, MOONS_DL_EDGE_Y = 5, MOONS_DL_EDGE_Z = 3
@@ -90,6 +90,7 @@ module moons_amd_bindings
   public :: moons_nodes_uniform, moons_nodes_roberts_both, moons_nodes_ext
   public :: moons_mesh_create, moons_mesh_destroy
   public :: moons_field_create, moons_field_destroy, moons_field_set_bcs, moons_field_set_bc_props
+  public :: moons_field_copy_bcs, moons_pcg_sf_set_k, moons_pcg_sf_update_mfp
   public :: moons_field_upload, moons_field_download, moons_field_fill
   public :: moons_apply_bcs, moons_grad, moons_div
   public :: moons_pcg_sf_create, moons_pcg_destroy, moons_pcg_set_isp, moons_pcg_sf_solve
@@ -168,6 +169,21 @@ module moons_amd_bindings
       import c_int, c_double, c_ptr
       type(c_ptr), value :: f
       real(c_double), intent(in) :: c_w(6), robin_coeff(6)
+    end function
+    integer(c_int) function moons_field_copy_bcs(dst, src) bind(C, name="moons_field_copy_bcs")
+      import c_int, c_ptr
+      type(c_ptr), value :: dst, src
+    end function
+    integer(c_int) function moons_pcg_sf_set_k(pcg, k) bind(C, name="moons_pcg_sf_set_k")
+      import c_int, c_ptr
+      type(c_ptr), value :: pcg
+      type(c_ptr), intent(in) :: k(3)
+    end function
+    integer(c_int) function moons_pcg_sf_update_mfp(pcg, coeff, update_prec) bind(C, name="moons_pcg_sf_update_mfp")
+      import c_int, c_double, c_ptr
+      type(c_ptr), value :: pcg
+      real(c_double), value :: coeff
+      integer(c_int), value :: update_prec
     end function
     integer(c_int) function moons_field_upload(f, host, n) bind(C, name="moons_field_upload")
       import c_int, c_double, c_ptr, c_size_t

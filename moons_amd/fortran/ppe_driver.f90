@@ -1,5 +1,9 @@
 ! ppe_driver.f90 -- Fortran host driving the GPU pressure-Poisson solve
-! through moons_amd_PCG_mod, as a MOONS host would at clean_div/solve.
+! through moons_amd_PCG_mod, as a MOONS host would at clean_div/solve: the
+! solver is built with the reference's init(PCG, operator, prec, m, ISP, MFP,
+! x, x_BC, k, var_dir, var_name, testSymmetry, exportOperator) argument order
+! (code/handwritten/governing_equations/momentum_extend.f90:166-167) and solved
+! with its solve(PCG, x, b, m, compute_norms).
 !
 ! usage: ppe_driver N beta prec iter_max tol b.bin x.bin [rawdir]
 !   prec: MOONS_PREC_* of the PCG, or 0 for the cosine-transform direct
@@ -24,6 +28,9 @@ program ppe_driver
   type(mesh_gpu) :: m
   type(PCG_solver_SF_gpu) :: PCG
   type(FFT_solver_SF_gpu) :: FFT
+  type(iter_solver_params_gpu) :: ISP
+  type(matrix_free_params_gpu) :: MFP
+  type(SF_gpu) :: p_gpu, k_none(3)
   integer(c_int) :: types(6)
   real(c_double) :: vals(6)
 
@@ -60,14 +67,20 @@ program ppe_driver
     write(*, '(a)') ' FFT solve done'
     call delete(FFT)
   else
-    call init(PCG, m, types, vals, .true., iter_max, tol, 1.0d-300, 1, int(prec_i, c_int))
-    call solve(PCG, x, b, .true.)
+    ! p with its BCs, as mom%p carries them into init
+    call init_CC(p_gpu, m)
+    call set_BCs(p_gpu, types, vals, .true.)
+    call init(ISP, iter_max, tol, 1.0d-300, 1, .false., .false.)
+    call init(PCG, MOONS_OP_LAP_UNIFORM_SF, int(prec_i, c_int), m, ISP, MFP, p_gpu, p_gpu, k_none, &
+              'out/', 'p', .false., .false.)
+    call solve(PCG, x, b, m, .true.)
     if (len_trim(rawdir) .gt. 0) then
       ! the device copy of the solution as a raw Tecplot file and as restart primitives
       call export_raw(m, PCG%x, trim(rawdir), 'x', 0)
       call export(PCG%x, trim(rawdir)//'x_restart.txt')
     endif
     call delete(PCG)
+    call delete(p_gpu)
   endif
 
   open(newunit=u, file=trim(xpath), access='stream', form='unformatted', status='replace')

@@ -1,6 +1,7 @@
 ! vf_driver.f90 -- Fortran host driving the GPU face-centred implicit
 ! diffusion solve through moons_amd_PCG_mod, as solve_momentum /
-! solve_induction do (init_PCG_VF, update_MFP, solve).
+! solve_induction do (init_PCG_VF, update_MFP, solve): the momentum solve with
+! the BC-array init, the induction solve with the reference's argument order.
 !
 ! usage: vf_driver N beta op c iter_max tol x0.bin b.bin x.bin [sig.bin]
 !   mesh: x uniform on [0,1], y and z Roberts beta (uniform when beta <= 1)
@@ -21,6 +22,9 @@ program vf_driver
   real(c_double), allocatable :: s1(:, :, :), s2(:, :, :), s3(:, :, :)
   type(mesh_gpu) :: m
   type(PCG_solver_VF_gpu) :: PCG
+  type(iter_solver_params_gpu) :: ISP
+  type(matrix_free_params_gpu) :: MFP
+  type(SF_gpu) :: xbc(3), sig(3)
   integer(c_int) :: types(6, 3)
   real(c_double) :: vals(6, 3)
 
@@ -70,14 +74,31 @@ program vf_driver
     open(newunit=u, file=trim(spath), access='stream', form='unformatted', status='old')
     read(u) s1, s2, s3
     close(u)
-    call init(PCG, m, MOONS_OP_IND_DIFFUSION, types, vals, iter_max, tol, 1.0d-13, 1, s1, s2, s3)
+    ! the reference's init(PCG_B, ind_diffusion, ..., ISP, MFP, Bstar, B,
+    ! sigmaInv_edge, ...) order (induction_extend.f90:206-207), with B's BCs
+    ! on device fields and sigmaInv on the edges
+    do d = 1, 3
+      call init_Face(xbc(d), m, d)
+      call set_BCs(xbc(d), types(:, d), vals(:, d), .false.)
+      call init_Edge(sig(d), m, d)
+    enddo
+    call moons_check(moons_field_upload(sig(1)%h, s1, int(size(s1), c_size_t)), 'upload sigmaInv')
+    call moons_check(moons_field_upload(sig(2)%h, s2, int(size(s2), c_size_t)), 'upload sigmaInv')
+    call moons_check(moons_field_upload(sig(3)%h, s3, int(size(s3), c_size_t)), 'upload sigmaInv')
+    call init(ISP, iter_max, tol, 1.0d-13, 1, .false., .false.)
+    MFP%coeff_implicit_time_split = c
+    call init(PCG, MOONS_OP_IND_DIFFUSION, m, ISP, MFP, xbc, xbc, sig, 'out/', 'B', .false., .false.)
   endif
-  call update_MFP(PCG, c, .true.)
+  if (op .eq. 1) call update_MFP(PCG, c, .true.)
   call solve(PCG, x1, x2, x3, b1, b2, b3, .true.)
 
   open(newunit=u, file=trim(xpath), access='stream', form='unformatted', status='replace')
   write(u) x1, x2, x3
   close(u)
   call delete(PCG)
+  do d = 1, 3
+    call delete(xbc(d))
+    call delete(sig(d))
+  enddo
   call delete(m)
 end program
