@@ -88,6 +88,76 @@ def test_gradients_match_restatement(gpu, shape):
     assert cos > 0.999, cos
 
 
+def exact_weights(seed, pcenter):
+    """A network whose arithmetic is exact in BF16 and FP32: every output
+    channel copies one input channel (distinct ones where there are enough)
+    through one tap, the centre tap with probability pcenter, with weight 1
+    and no bias; the output layer adds every channel at the centre tap and at
+    one more tap each.  With small positive integer inputs every activation
+    is a small dyadic number (pooling divides by 8), so BF16 stores it exactly."""
+    rng = np.random.default_rng(seed)
+    w = []
+    for ci, co in UO.LAYERS:
+        W = np.zeros((27, ci, co))
+        if co == 1:
+            W[13, :, 0] = 1.0
+            W[rng.integers(27, size=ci), np.arange(ci), 0] = 1.0
+        else:
+            src = rng.permutation(ci)[:co] if ci >= co else rng.integers(ci, size=co)
+            for c in range(co):
+                tap = 13 if rng.random() < pcenter else rng.integers(27)
+                W[tap, src[c], c] = 1.0
+        w.append(np.concatenate([W.ravel(), np.zeros(co)]))
+    return np.concatenate(w)
+
+
+# (shape, seed, pcenter) whose restatement is exact (checked in the test):
+# 16 x 8 x 8 runs the finest level through the LDS-tiled 16x16x32 kernels and
+# the coarser ones through the tap-split kernels; 8^3 every level split
+EXACT_CASES = [((8, 8, 8), 2, 0.5), ((16, 8, 8), 1, 0.8), ((16, 8, 8), 2, 0.5)]
+
+
+@pytest.mark.parametrize("shape,seed,pc", EXACT_CASES)
+def test_forward_and_gradients_exact(gpu, shape, seed, pc):
+    """Forward output, loss and every weight and bias gradient equal the
+    float64 restatement bit for bit on a network and data whose arithmetic is
+    exact (unit weights, small positive integer inputs, targets a quarter-step
+    multiple away from the output, a power-of-two voxel count): no BF16 or
+    FP32 rounding occurs on either side, so any difference is a mistake in a
+    kernel's indexing, tap order, transposition, pooling, upsampling, skip
+    concatenation, ReLU mask or reduction, not rounding."""
+    print(f"unet exact {shape} seed {seed}", flush=True)
+    w = exact_weights(seed, pc)
+    rng = np.random.default_rng(seed + 100)
+    n0, n1, n2 = shape
+    xi = rng.integers(1, 3, (n2, n1, n0, 4)).astype(np.float64)
+    ref = UO.Net(w, bf=False)
+    out0 = ref.forward(xi)
+    ti = out0 - rng.integers(-2, 3, (n2, n1, n0)).astype(np.float64) * 0.25
+    rloss, rg, rout = UO.Net(w, bf=False).loss_grad(xi, ti)
+    bloss, bg, bout = UO.Net(w, bf=True).loss_grad(xi, ti)
+    # the case is exact: BF16 storage changes nothing, FP32 holds every value
+    assert bloss == rloss and np.array_equal(bg, rg) and np.array_equal(bout, rout)
+    assert np.array_equal(rg.astype(np.float32).astype(np.float64), rg)
+    net = gpu.UNet(seed=0)
+    net.set_weights(w.astype(np.float32))
+    x = np.zeros((4, n0 + 2, n1 + 2, n2 + 2), dtype=np.float32)
+    x[:, 1:-1, 1:-1, 1:-1] = np.transpose(xi, (3, 2, 1, 0))
+    t = np.zeros((n0 + 2, n1 + 2, n2 + 2), dtype=np.float32)
+    t[1:-1, 1:-1, 1:-1] = np.transpose(ti, (2, 1, 0))
+    out = net.forward(x)[1:-1, 1:-1, 1:-1]
+    np.testing.assert_array_equal(out.astype(np.float64), UO.from_internal(rout))
+    loss, g = net.grad(x, t)
+    assert loss == rloss, (loss, rloss)
+    np.testing.assert_array_equal(g.astype(np.float64), rg)
+    # every layer's gradient is exercised
+    off = 0
+    for ci, co in UO.LAYERS:
+        n = 27 * ci * co + co
+        assert np.count_nonzero(rg[off:off + n]) > 0
+        off += n
+
+
 def test_training_lowers_the_loss(gpu):
     print("unet training", flush=True)
     from moons_amd import learned
