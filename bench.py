@@ -324,21 +324,37 @@ def run_duct(M, args):
     from moons_amd.duct import Duct
 
     N = args.duct
-    D = Duct(n=(N, N, N), prec_p="MG_FP32")
+    # phi cleaning by the mixed multigrid too (the reference's Jacobi beside it)
+    D = Duct(n=(N, N, N), prec_p="MG_FP32", prec_phi="MG_FP32")
     D.step(args.warmup)
     M.synchronize()
     t0 = time.perf_counter()
     info = D.step(args.steps)
     M.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    DJ = Duct(n=(N, N, N), prec_p="MG_FP32", prec_phi="Lap_SF")
+    DJ.step(args.warmup)
+    M.synchronize()
+    t0 = time.perf_counter()
+    info_j = DJ.step(args.steps)
+    M.synchronize()
+    ms_j = 1e3 * (time.perf_counter() - t0) / args.steps
+    # the two steps' fields agree (both solves to their tolerances)
+    fd, fj = D.fields(), DJ.fields()
+    agree = {f"{k}{d}": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+             for k in ("U", "B", "J") for d, (a, b) in enumerate(zip(fd[k], fj[k]))}
     out = {"metric": "duct MHD time steps per second", "value": round(1e3 / ms, 2), "unit": "steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"duct_mhd_{N}", "cells": f"{N}^3", "Re": 1000, "Ha": 100, "c_w": 0.01,
                       "mesh": "x periodic uniform [-1,1]; y,z Roberts beta=1.05 on [-1,1]", "dt": 1e-3,
-                      "ppe": "mixed FP32 V-cycle in FP64 FPCG to 1e-8", "phi": "Jacobi-PCG to 1e-12",
-                      "U, B": "5 iterations", "execution": "eager, one stream"},
-           "last_step_iterations": info["iterations"]}
+                      "ppe": "mixed FP32 V-cycle in FP64 FPCG to 1e-8",
+                      "phi": "mixed FP32 V-cycle in FP64 FPCG to 1e-12", "U, B": "5 iterations",
+                      "execution": "eager, one stream"},
+           "last_step_iterations": info["iterations"],
+           "phi_jacobi": {"ms_per_step": round(ms_j, 3), "last_step_iterations": info_j["iterations"],
+                          "phi": "the reference's Jacobi-PCG to 1e-12"},
+           "fields_vs_phi_jacobi_rel_l2": agree}
     print(json.dumps(out), flush=True)
 
 
@@ -422,6 +438,26 @@ def run_energy(M, args):
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                         "frac": round(gbs / 8000.0, 4), "traffic": None, "kernel": "nrg_matvec",
                         "bytes_per_cell": 40}}
+    # the same step with every source of add_all_energy_sources on (advection,
+    # diffusion into L, KE diffusion, viscous dissipation, Joule heating of a
+    # seeded J with sigmaInv 1, volumetric heating) and the 3-stage RK with
+    # method 5: three sources-and-solve stages a step
+    J = [M.SF(m, dl).upload(np.asfortranarray(splitmix(int(np.prod(m.shape(dl))), 40 + q, 0.1)
+                                              .reshape(m.shape(dl), order="F")))
+         for q, dl in enumerate((6, 5, 3))]
+    sig = M.SF(m, "CC").fill(1.0)
+    T2 = M.SF(m, "CC").upload(T0).set_BCs(*T_bcs)
+    st2 = M.Energy_step(m, T2, U, k, dt=5e-4, method=5, Pe=100.0, advection=(True, -1.0), diffusion=(True, 0.01),
+                        isp_T=(1000, TOL, 1e-300, 1), KE_diffusion=(True, 0.01), viscous_dissipation=(True, 0.01),
+                        joule_heating=(True, 0.01), volumetric_heating=(True, 0.01), rk_stages=3, J=J, sigma_inv=sig)
+    st2.step(args.warmup)
+    M.synchronize()
+    t0 = time.perf_counter()
+    info2 = st2.step(args.steps)
+    M.synchronize()
+    out["all_sources_rk3"] = {"ms_per_step": round(1e3 * (time.perf_counter() - t0) / args.steps, 3),
+                              "last_stage_iterations": info2["iterations"],
+                              "config": "method 5 with the 3-stage RK (three stages a step); every source on"}
     print(json.dumps(out), flush=True)
 
 

@@ -494,6 +494,9 @@ typedef struct {
   int mpg_dir;
   double mpg_scale;
   int finite_Rem;
+  /* the phi-cleaning preconditioner: 0 or MOONS_PREC_LAP_SF (the reference's
+   * Jacobi), or MOONS_PREC_MG_FP32 (the mixed-precision multigrid, as for the PPE) */
+  int prec_phi;
 } moons_mhd_params;
 
 typedef struct {

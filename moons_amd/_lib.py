@@ -32,7 +32,7 @@ class MhdParams(C.Structure):
                 ("alpha_U", C.c_double), ("induction", C.c_int), ("prec_p", C.c_int), ("B0", C.c_double * 3),
                 ("iter_max", C.c_int * 4), ("tol_rel", C.c_double * 4), ("tol_abs", C.c_double * 4),
                 ("n_skip", C.c_int * 4), ("mpg_dir", C.c_int), ("mpg_scale", C.c_double),
-                ("finite_Rem", C.c_int)]
+                ("finite_Rem", C.c_int), ("prec_phi", C.c_int)]
 
 
 class MhdInfo(C.Structure):

@@ -599,7 +599,9 @@ class MHD_step:
     implicit diffusion, phi cleaning, J and J x B.  U, p, B, phi are the
     caller's fields (with their BCs) and are advanced in place.
     ISPs are (iter_max, tol_rel, tol_abs, n_skip_check_res); mpg_dir 1..3
-    drives the flow by a mean pressure gradient of mpg_scale."""
+    drives the flow by a mean pressure gradient of mpg_scale; prec_p and
+    prec_phi pick the PPE's and the phi cleaning's preconditioner ("Lap_SF",
+    the reference's Jacobi, or "MG_FP32")."""
 
     FIELDS = {"F": 1, "Fnm1": 2, "Ustar": 3, "U_E": 4, "J": 5, "jCrossB": 6, "FB": 7, "ppe_rhs": 8}
 
@@ -607,7 +609,7 @@ class MHD_step:
                  alpha_U=1.0, B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
                  isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), prec_p="Lap_SF", m_ind: mesh | None = None,
                  domain: "mesh_domain | None" = None, sigma_inv=None, mpg_dir: int = 0, mpg_scale: float = 1.0,
-                 finite_Rem: bool = False):
+                 finite_Rem: bool = False, prec_phi: str = "Lap_SF"):
         P = MhdParams()
         P.Re, P.Ha, P.Rem, P.dt, P.alpha_U = Re, Ha, Rem, dt, alpha_U
         # mean pressure gradient forcing along mpg_dir (1..3, 0 off): compute_add_MPG
@@ -615,6 +617,8 @@ class MHD_step:
         P.finite_Rem = 1 if finite_Rem else 0  # Btot = B0 + B (else B0)
         P.induction = 1 if B is not None else 0
         P.prec_p = PCG_solver_SF.PRECS[prec_p]
+        # the phi cleaning's preconditioner: the reference's Jacobi, or the mixed multigrid
+        P.prec_phi = PCG_solver_SF.PRECS[prec_phi]
         for d in range(3):
             P.B0[d] = B0[d]
         for q, isp in enumerate((isp_U, isp_p, isp_B, isp_phi)):

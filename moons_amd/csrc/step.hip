@@ -387,6 +387,8 @@ int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, con
   if (!(P->dt > 0.0) || !(P->Re > 0.0) || (P->alpha_U != 0.0 && P->alpha_U != 1.0))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: need dt > 0, Re > 0, alpha_U 0 (method 4) or 1 (method 6)");
   if (P->mpg_dir < 0 || P->mpg_dir > 3) return fail(MOONS_ERR_ARG, "moons_mhd_create: mpg_dir must be 0..3");
+  if (P->prec_phi != 0 && P->prec_phi != MOONS_PREC_LAP_SF && P->prec_phi != MOONS_PREC_MG_FP32)
+    return fail(MOONS_ERR_ARG, "moons_mhd_create: prec_phi must be MOONS_PREC_LAP_SF or MOONS_PREC_MG_FP32");
   if (P->induction && (!B || !phi || !(P->Rem > 0.0)))
     return fail(MOONS_ERR_ARG, "moons_mhd_create: induction needs B, phi and Rem > 0");
   if (mi != m && (!dom || dom->mi != m || dom->mt != mi || !P->induction))
@@ -475,7 +477,8 @@ int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, con
     }
     if (!st) st = moons_pcg_vf_create(mi, MOONS_OP_IND_DIFFUSION, S->Bstar, S->sig, &S->pcg_B);
     if (!st) st = moons_pcg_vf_set_isp(S->pcg_B, P->iter_max[2], P->tol_rel[2], P->tol_abs[2], P->n_skip[2]);
-    if (!st) st = moons_pcg_sf_create(mi, MOONS_OP_LAP_UNIFORM_SF, MOONS_PREC_LAP_SF, phi, &S->pcg_phi);
+    if (!st) st = moons_pcg_sf_create(mi, MOONS_OP_LAP_UNIFORM_SF, P->prec_phi ? P->prec_phi : MOONS_PREC_LAP_SF, phi,
+                                      &S->pcg_phi);
     if (!st) st = moons_pcg_set_isp(S->pcg_phi, P->iter_max[3], P->tol_rel[3], P->tol_abs[3], P->n_skip[3]);
     if (!st) st = post_induction(S);
   }

@@ -20,8 +20,9 @@ periodic flow direction, Roberts-clustered walls and thin conducting walls:
   solver settings for U and B, the PPE to 1e-8 and phi cleaning to 1e-12.
 
 Velocity noise (seeded SplitMix64, amplitude 1e-3) on the interior faces makes
-the first steps non-trivial.  `prec_p` picks the PPE solver: "Lap_SF" (the
-FP64 Jacobi-PCG drop-in) or "MG_FP32" (the mixed-precision multigrid).
+the first steps non-trivial.  `prec_p` picks the PPE solver and `prec_phi`
+the phi cleaning's: "Lap_SF" (the FP64 Jacobi-PCG drop-in) or "MG_FP32" (the
+mixed-precision multigrid).
 """
 from __future__ import annotations
 
@@ -82,7 +83,8 @@ class Duct:
     """Device state of the duct flow: mesh, fields and the MHD step."""
 
     def __init__(self, n=(128, 128, 128), beta=1.05, Re=1000.0, Ha=100.0, Rem=1.0, c_w=0.01, dt=1e-3,
-                 prec_p="MG_FP32", seed=3, isp_p=PPE_ISP, isp_phi=PHI_ISP, isp_U=PRESET_ISP, isp_B=PRESET_ISP):
+                 prec_p="MG_FP32", seed=3, isp_p=PPE_ISP, isp_phi=PHI_ISP, isp_U=PRESET_ISP, isp_B=PRESET_ISP,
+                 prec_phi="Lap_SF"):
         self.n, self.beta, self.Re, self.Ha, self.Rem, self.c_w, self.dt = tuple(n), beta, Re, Ha, Rem, c_w, dt
         self.isp = {"U": isp_U, "p": isp_p, "B": isp_B, "phi": isp_phi}
         self.hs = duct_nodes(self.n, beta)
@@ -99,7 +101,8 @@ class Duct:
         self.phi = M.SF(self.m, "CC").fill(0.0).set_BCs(phib[0])
         self.st = M.MHD_step(self.m, self.U, self.p, B=self.B, phi=self.phi, Re=Re, Ha=Ha, Rem=Rem, dt=dt,
                              alpha_U=1.0, B0=(0.0, 0.0, 1.0), isp_U=isp_U, isp_p=isp_p, isp_B=isp_B,
-                             isp_phi=isp_phi, prec_p=prec_p, mpg_dir=1, mpg_scale=1.0, finite_Rem=True)
+                             isp_phi=isp_phi, prec_p=prec_p, mpg_dir=1, mpg_scale=1.0, finite_Rem=True,
+                             prec_phi=prec_phi)
 
     def step(self, k: int = 1):
         return self.st.step(k)

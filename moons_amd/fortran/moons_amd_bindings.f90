@@ -40,6 +40,7 @@ module moons_amd_bindings
     integer(c_int) :: mpg_dir
     real(c_double) :: mpg_scale
     integer(c_int) :: finite_Rem
+    integer(c_int) :: prec_phi
   end type
 
   type, bind(C), public :: moons_mhd_info
