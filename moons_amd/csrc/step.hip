@@ -150,6 +150,10 @@ const int kNoDiag[6][2] = {{1, 0}, {2, 0}, {0, 1}, {2, 1}, {0, 2}, {1, 2}};
 int fcreate(moons_mesh_s *m, int dl, moons_field_s **f) { return moons_field_create(m, dl, f); }
 
 void mhd_free(moons_mhd_s *S) {
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  S->exec = nullptr;
+  S->graph = nullptr;
   if (S->P.finite_Rem)
     for (int d = 0; d < 3; ++d)
       if (S->Btot[d]) moons_field_destroy(S->Btot[d]);
@@ -180,8 +184,6 @@ void mhd_free(moons_mhd_s *S) {
   if (S->pcg_B) moons_pcg_vf_destroy(S->pcg_B);
   if (S->ev0) (void)hipEventDestroy(S->ev0);
   if (S->ev1) (void)hipEventDestroy(S->ev1);
-  if (S->exec) (void)hipGraphExecDestroy(S->exec);
-  if (S->graph) (void)hipGraphDestroy(S->graph);
   delete S;
 }
 
@@ -495,7 +497,9 @@ int moons_mhd_create_md(moons_mesh_t m, moons_mesh_t mi, moons_domain_t dom, con
 
 int moons_mhd_destroy(moons_mhd_t S) {
   if (!S) return MOONS_OK;
-  if (stream()) (void)hipStreamSynchronize(stream());
+  // a captured step may have been replayed on a caller's stream: wait for the device
+  if (S->exec) (void)hipDeviceSynchronize();
+  else if (stream()) (void)hipStreamSynchronize(stream());
   mhd_free(S);
   return MOONS_OK;
 }

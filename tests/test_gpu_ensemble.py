@@ -97,3 +97,24 @@ def test_ensemble_uneven_sizes_match_oracle(gpu):
         print(f"  step {n}: worst rel L2 per member " +
               ", ".join(f"{mb.N}^3 (Re {mb.Re:.0f}, Ha {mb.Ha:.1f}) {e:.1e}" for mb, e in zip(ens.members, worst)),
               flush=True)
+
+
+def test_64_graphs_on_four_streams_equal_eager(gpu):
+    """The ensemble's launch pattern at its member count: 64 captured steps
+    (8^3, 12^3 and 16^3 members) replayed over 4 streams for 3 steps equal the
+    same members stepped eagerly, bit for bit."""
+    from moons_amd.ensemble import Ensemble
+
+    print("64 graphs on 4 streams vs eager", flush=True)
+    kw = dict(n=64, sizes=(8, 12, 16), seed=11, dt=5e-4, isp=ISP, n_streams=4)
+    ens = Ensemble(**kw)
+    ref = Ensemble(capture=False, **kw)
+    assert [mb.N for mb in ens.members] == [mb.N for mb in ref.members]
+    for n in range(3):
+        ens.step(1)
+        for mb in ref.members:
+            mb.st.step(1)
+        gpu.synchronize()
+        for q, (a, b) in enumerate(zip(ens.members, ref.members)):
+            for x, y in zip(state_arrays(a), state_arrays(b)):
+                assert np.array_equal(x, y), (n, q, a.N)
