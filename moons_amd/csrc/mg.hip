@@ -232,29 +232,35 @@ static long long line_min_cells() {
 }
 
 // Row a x[t-1] + x[t] + c x[t+1] = d of cell t (0-based) of a line (normalised).
+// A row past the line's end (t >= nl) or of a line past the level's edge
+// (!ok) is the identity row (0, 0, 0); its loads go to an in-bounds cell and
+// are discarded, so that no branch separates the rows' loads and all of a
+// thread's loads are in flight together.
 template <int D>
 __device__ __forceinline__ float3 line_row(const LevelDev &L, int i, int m, int q, int t, int nl, bool ok,
-                                           bool zero_guess, size_t &at) {
-  const int j = D == 1 ? t + 1 : q, k = D == 1 ? q : t + 1;
+                                           bool zero_guess, int &at) {
+  const bool live = ok && t < nl;
+  const int ic = ok ? i : 1, mc = ok ? m : 0, tc = t < nl ? t : nl - 1;
+  const int j = D == 1 ? tc + 1 : q, k = D == 1 ? q : tc + 1;
   const int h0 = L.h0, n1 = L.n1;
-  const int col = (i + j + k) & 1;
+  const int col = (ic + j + k) & 1;
   const int row = h0 * ((j - 1) + n1 * (k - 1));
-  at = (size_t)col * L.half + row + m;
-  if (!ok || t >= nl) return make_float3(0.f, 0.f, 0.f);
-  const float *xo = L.x + (size_t)(1 - col) * L.half;
-  const int im = (i == 1) ? (L.per0 ? L.n0 : 1) : i - 1;
-  const int ip = (i == L.n0) ? (L.per0 ? 1 : L.n0) : i + 1;
-  float s = L.wm[0][i] * xo[row + ((im - 1) >> 1)] + L.wp[0][i] * xo[row + ((ip - 1) >> 1)];
+  at = col * L.half + row + mc;
+  const float *xo = L.x + (1 - col) * L.half;
+  const int im = (ic == 1) ? (L.per0 ? L.n0 : 1) : ic - 1;
+  const int ip = (ic == L.n0) ? (L.per0 ? 1 : L.n0) : ic + 1;
+  float s = L.wm[0][ic] * xo[row + ((im - 1) >> 1)] + L.wp[0][ic] * xo[row + ((ip - 1) >> 1)];
   if (D == 1) { // off-line neighbours along z
     const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1, kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
-    s += L.wm[2][k] * xo[h0 * ((j - 1) + n1 * (km - 1)) + m] + L.wp[2][k] * xo[h0 * ((j - 1) + n1 * (kp - 1)) + m];
+    s += L.wm[2][k] * xo[h0 * ((j - 1) + n1 * (km - 1)) + mc] + L.wp[2][k] * xo[h0 * ((j - 1) + n1 * (kp - 1)) + mc];
   } else {      // off-line neighbours along y
     const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1, jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
-    s += L.wm[1][j] * xo[h0 * ((jm - 1) + n1 * (k - 1)) + m] + L.wp[1][j] * xo[h0 * ((jp - 1) + n1 * (k - 1)) + m];
+    s += L.wm[1][j] * xo[h0 * ((jm - 1) + n1 * (k - 1)) + mc] + L.wp[1][j] * xo[h0 * ((jp - 1) + n1 * (k - 1)) + mc];
   }
   const float d = L.f[at] - (zero_guess ? 0.f : s);
-  const float r = __builtin_amdgcn_rcpf(L.dg[0][i] + L.dg[1][j] + L.dg[2][k]);
-  return make_float3(L.wm[D][t + 1] * r, L.wp[D][t + 1] * r, d * r);
+  const float r = __builtin_amdgcn_rcpf(L.dg[0][ic] + L.dg[1][j] + L.dg[2][k]);
+  const float3 row3 = make_float3(L.wm[D][tc + 1] * r, L.wp[D][tc + 1] * r, d * r);
+  return live ? row3 : make_float3(0.f, 0.f, 0.f);
 }
 
 template <int D, int NP, int TB>
@@ -276,7 +282,7 @@ __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_gues
   const int i = 2 * m + 1 + p;
   const bool ok = i <= L.n0;
   const int me = threadIdx.x;
-  size_t at[4];
+  int at[4];
   float3 r[4];
 #pragma unroll
   for (int s4 = 0; s4 < 4; ++s4) r[s4] = line_row<D>(L, i, m, q, 4 * u + s4, nl, ok, zero_guess, at[s4]);
