@@ -231,6 +231,14 @@ static long long line_min_cells() {
   return e ? atoll(e) : kLineMinCells;
 }
 
+// Element e of a level array by a 32-bit byte offset from its base (the
+// scalar-base + 32-bit vector-offset load form, without 64-bit address
+// arithmetic per load).  Valid while 4 e < 2^32: line smoothing is used only
+// on levels below 2^30 cells.
+__device__ __forceinline__ float ld32(const float *__restrict__ base, unsigned e) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (e << 2));
+}
+
 // Row a x[t-1] + x[t] + c x[t+1] = d of cell t (0-based) of a line (normalised).
 // A row past the line's end (t >= nl) or of a line past the level's edge
 // (!ok) is the identity row (0, 0, 0); its loads go to an in-bounds cell and
@@ -242,24 +250,30 @@ __device__ __forceinline__ float3 line_row(const LevelDev &L, int i, int m, int 
   const bool live = ok && t < nl;
   const int ic = ok ? i : 1, mc = ok ? m : 0, tc = t < nl ? t : nl - 1;
   const int j = D == 1 ? tc + 1 : q, k = D == 1 ? q : tc + 1;
-  const int h0 = L.h0, n1 = L.n1;
+  // plane offsets: h0 (j - 1) and h0 n1 (k - 1), one of them block-uniform;
+  // every factor is below 2^24 (lines of at most 128 cells, levels below 2^30)
+  const int h0 = L.h0, hn = L.h0 * L.n1;
+  const int oj = (int)__umul24((unsigned)h0, (unsigned)(j - 1)), ok_ = (int)__umul24((unsigned)hn, (unsigned)(k - 1));
   const int col = (ic + j + k) & 1;
-  const int row = h0 * ((j - 1) + n1 * (k - 1));
-  at = col * L.half + row + mc;
-  const float *xo = L.x + (1 - col) * L.half;
+  const int row = oj + ok_;
+  at = (col ? L.half : 0) + row + mc;
+  const unsigned xo = (unsigned)(col ? 0 : L.half); // the other colour's half of x
   const int im = (ic == 1) ? (L.per0 ? L.n0 : 1) : ic - 1;
   const int ip = (ic == L.n0) ? (L.per0 ? 1 : L.n0) : ic + 1;
-  float s = L.wm[0][ic] * xo[row + ((im - 1) >> 1)] + L.wp[0][ic] * xo[row + ((ip - 1) >> 1)];
-  if (D == 1) { // off-line neighbours along z
+  float s = ld32(L.wm[0], ic) * ld32(L.x, xo + row + ((im - 1) >> 1)) +
+            ld32(L.wp[0], ic) * ld32(L.x, xo + row + ((ip - 1) >> 1));
+  if (D == 1) { // off-line neighbours along z (k is block-uniform)
     const int km = (k == 1) ? (L.per2 ? L.n2 : 1) : k - 1, kp = (k == L.n2) ? (L.per2 ? 1 : L.n2) : k + 1;
-    s += L.wm[2][k] * xo[h0 * ((j - 1) + n1 * (km - 1)) + mc] + L.wp[2][k] * xo[h0 * ((j - 1) + n1 * (kp - 1)) + mc];
-  } else {      // off-line neighbours along y
+    s += ld32(L.wm[2], k) * ld32(L.x, xo + oj + hn * (km - 1) + mc) +
+         ld32(L.wp[2], k) * ld32(L.x, xo + oj + hn * (kp - 1) + mc);
+  } else {      // off-line neighbours along y (j is block-uniform)
     const int jm = (j == 1) ? (L.per1 ? L.n1 : 1) : j - 1, jp = (j == L.n1) ? (L.per1 ? 1 : L.n1) : j + 1;
-    s += L.wm[1][j] * xo[h0 * ((jm - 1) + n1 * (k - 1)) + mc] + L.wp[1][j] * xo[h0 * ((jp - 1) + n1 * (k - 1)) + mc];
+    s += ld32(L.wm[1], j) * ld32(L.x, xo + h0 * (jm - 1) + ok_ + mc) +
+         ld32(L.wp[1], j) * ld32(L.x, xo + h0 * (jp - 1) + ok_ + mc);
   }
-  const float d = L.f[at] - (zero_guess ? 0.f : s);
-  const float r = __builtin_amdgcn_rcpf(L.dg[0][ic] + L.dg[1][j] + L.dg[2][k]);
-  const float3 row3 = make_float3(L.wm[D][tc + 1] * r, L.wp[D][tc + 1] * r, d * r);
+  const float d = ld32(L.f, at) - (zero_guess ? 0.f : s);
+  const float r = __builtin_amdgcn_rcpf(ld32(L.dg[0], ic) + ld32(L.dg[1], j) + ld32(L.dg[2], k));
+  const float3 row3 = make_float3(ld32(L.wm[D], tc + 1) * r, ld32(L.wp[D], tc + 1) * r, d * r);
   return live ? row3 : make_float3(0.f, 0.f, 0.f);
 }
 
@@ -983,7 +997,9 @@ MGHier *mg_create(const moons_field_s *x, int max_levels, int nu1, int nu2, int 
         }
         const long long cells = (long long)hl.n[0] * hl.n[1] * hl.n[2];
         const bool per = bct[2 * d] == MOONS_BC_PERIODIC;
-        const int ln = !per && n >= 4 && n <= kLineMax && cells > line_min_cells() && dmax > 1.5 * dmin;
+        // (the line kernels address the level by 32-bit byte offsets: below 2^30 cells)
+        const int ln = !per && n >= 4 && n <= kLineMax && cells > line_min_cells() && cells < (1LL << 30) &&
+                       dmax > 1.5 * dmin;
         (d == 1 ? L.line1 : L.line2) = ln;
       }
       std::vector<float> wm(s, 0.f), wp(s, 0.f), dg(s, 0.f), rf(s, 0.f), pa(s, 0.f), pb(s, 0.f);
