@@ -206,19 +206,19 @@ __global__ void __launch_bounds__(256) k_mg_smooth(LevelDev L, int color, int ze
 // colour.  Each line's tridiagonal system (the weights of D along the line,
 // diagonal dg_x + dg_y + dg_z, the off-line neighbours moved to the
 // right-hand side; with zero_guess they are zero) is solved exactly.  A
-// thread owns the cells 4u .. 4u + 3 of a line (NP >= the line length, a
-// power of two; NP / 4 threads per line) with their rows normalised to a unit
-// diagonal.  Two odd-even reduction levels leave one row per thread, coupling
-// x[4u] to x[4u - 4] and x[4u + 4]: row 4u + 2 drops x[4u + 1] and x[4u + 3]
-// (its own cells), then row 4u drops x[4u - 1] and x[4u - 2] with the
-// predecessor's row 4u - 1 and reduced row 4u - 2 (one LDS exchange).  Those
-// rows are solved by log2(NP / 4) parallel cyclic reduction steps (rows
-// (a, c, d) in two struct-of-arrays LDS buffers, one barrier per step), and
-// x[4u + 2], x[4u + 1], x[4u + 3] follow from their rows with the
-// successor's x[4u + 4].  Cells past the line's end hold the identity
+// thread owns the C cells Cu .. Cu + C - 1 of a line (NP >= the line length,
+// a power of two; NP / C threads per line) with their rows normalised to a
+// unit diagonal.  Cyclic reduction inside the thread (log2 C strides) leaves
+// one row per thread, coupling x[Cu] to x[C(u - 1)] and x[C(u + 1)]: rows
+// Cu + t, t > 0, need only the thread's own rows, and row Cu the
+// predecessor's rows C(u + 1) - h as eliminated at each stride h (one LDS
+// exchange).  Those rows are solved by log2(NP / C) parallel cyclic reduction
+// steps (rows (a, c, d) in two struct-of-arrays LDS buffers, one barrier per
+// step); the eliminated rows then follow, widest stride first, with the
+// successor's x[C(u + 1)].  Cells past the line's end hold the identity
 // equation.  Threads of a block take consecutive lines fastest, contiguous in
 // the colour-split layout, so the gather and the store are coalesced across
-// lines.  A block of TB threads holds 4 TB / NP lines; blocks are numbered
+// lines.  A block of TB threads holds C TB / NP lines; blocks are numbered
 // so that each XCD (the hardware deals consecutive block ids round-robin over
 // the 8 XCDs) sweeps a contiguous slab of planes q, whose neighbouring blocks
 // share its L2: the other colour's x read by the planes q - 1, q, q + 1 and
@@ -277,11 +277,13 @@ __device__ __forceinline__ float3 line_row(const LevelDev &L, int i, int m, int 
   return live ? row3 : make_float3(0.f, 0.f, 0.f);
 }
 
-template <int D, int NP, int TB>
+template <int D, int NP, int TB, int C>
 __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_guess, const int *done, int gx) {
-  constexpr int NQ = NP / 4;             // threads per line
+  constexpr int NQ = NP / C;             // threads per line
   constexpr int LPB = TB / NQ;           // lines per block
-  __shared__ float E[6][TB];             // rows 4u + 3 and reduced 4u + 2, read by the successor
+  constexpr int LV = C == 2 ? 1 : C == 4 ? 2 : C == 8 ? 3 : 4; // log2(C) local reduction levels
+  static_assert(C == 2 || C == 4 || C == 8 || C == 16, "cells per thread");
+  __shared__ float E[3 * LV][TB];        // rows C - h (h = 1, 2, .., C / 2) as eliminated, read by the successor
   __shared__ float S[2][3][TB];          // PCR rows (a, c, d), two buffers
   const int nl = D == 1 ? L.n1 : L.n2;   // line length, <= NP
   // XCD-contiguous numbering: the first 8 * (nb / 8) ids are dealt as slabs
@@ -291,60 +293,69 @@ __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_gues
   const int bx = lb % gx;
   const int q = lb / gx + 1;             // the other line index (k for y-lines, j for z-lines)
   const int p = (c + 1 + q) & 1;         // i-parity of the lines of colour c at this q
-  const int l = threadIdx.x % LPB, u = threadIdx.x / LPB; // line in the block, cell quad
+  const int l = threadIdx.x % LPB, u = threadIdx.x / LPB; // line in the block, cell group
   const int m = bx * LPB + l;
   const int i = 2 * m + 1 + p;
   const bool ok = i <= L.n0;
   const int me = threadIdx.x;
-  int at[4];
-  float3 r[4];
+  int at[C];
+  float a[C], cc[C], d[C];
 #pragma unroll
-  for (int s4 = 0; s4 < 4; ++s4) r[s4] = line_row<D>(L, i, m, q, 4 * u + s4, nl, ok, zero_guess, at[s4]);
-  // row 4u + 2 without x[4u + 1], x[4u + 3]: A2 x[4u] + x[4u + 2] + C2 x[4u + 4] = D2
-  float A2, C2, D2;
-  {
-    const float rr = __builtin_amdgcn_rcpf(1.f - r[2].x * r[1].y - r[2].y * r[3].x);
-    A2 = -r[2].x * r[1].x * rr;
-    C2 = -r[2].y * r[3].y * rr;
-    D2 = (r[2].z - r[2].x * r[1].z - r[2].y * r[3].z) * rr;
+  for (int s4 = 0; s4 < C; ++s4) {
+    const float3 r = line_row<D>(L, i, m, q, C * u + s4, nl, ok, zero_guess, at[s4]);
+    a[s4] = r.x;
+    cc[s4] = r.y;
+    d[s4] = r.z;
   }
-  E[0][me] = r[3].x;
-  E[1][me] = r[3].y;
-  E[2][me] = r[3].z;
-  E[3][me] = A2;
-  E[4][me] = C2;
-  E[5][me] = D2;
+  // Cyclic reduction inside the thread: at stride h every row t (a multiple
+  // of 2h, t > 0) drops x[t - h] and x[t + h] with those rows (as left by
+  // stride h / 2), coupling to x[t - 2h] and x[t + 2h].  Rows are updated in
+  // place; a row keeps the values it had when it was eliminated.
+#pragma unroll
+  for (int h = 1; h < C; h <<= 1)
+#pragma unroll
+    for (int t = 2 * h; t < C; t += 2 * h) {
+      const float rr = __builtin_amdgcn_rcpf(1.f - a[t] * cc[t - h] - cc[t] * a[t + h]);
+      const float an = -a[t] * a[t - h] * rr, cn = -cc[t] * cc[t + h] * rr;
+      d[t] = (d[t] - a[t] * d[t - h] - cc[t] * d[t + h]) * rr;
+      a[t] = an;
+      cc[t] = cn;
+    }
+  // row 0 needs the predecessor's row C - h at each stride h (one exchange)
+#pragma unroll
+  for (int v = 0; v < LV; ++v) {
+    E[3 * v][me] = a[C - (1 << v)];
+    E[3 * v + 1][me] = cc[C - (1 << v)];
+    E[3 * v + 2][me] = d[C - (1 << v)];
+  }
   __syncthreads();
-  float a3p = 0.f, c3p = 0.f, d3p = 0.f, a2p = 0.f, c2p = 0.f, d2p = 0.f; // the predecessor's
-  if (u > 0) {
-    const int o = me - LPB;
-    a3p = E[0][o];
-    c3p = E[1][o];
-    d3p = E[2][o];
-    a2p = E[3][o];
-    c2p = E[4][o];
-    d2p = E[5][o];
+  float a0 = a[0], c0 = cc[0], d0 = d[0];
+#pragma unroll
+  for (int v = 0; v < LV; ++v) {
+    const int h = 1 << v;
+    float am = 0.f, cm = 0.f, dm = 0.f; // the predecessor's row C - h
+    if (u > 0) {
+      const int o = me - LPB;
+      am = E[3 * v][o];
+      cm = E[3 * v + 1][o];
+      dm = E[3 * v + 2][o];
+    }
+    const float rr = __builtin_amdgcn_rcpf(1.f - a0 * cm - c0 * a[h]);
+    const float an = -a0 * am * rr, cn = -c0 * cc[h] * rr;
+    d0 = (d0 - a0 * dm - c0 * d[h]) * rr;
+    a0 = an;
+    c0 = cn;
   }
-  float a, cc, d;
-  {
-    // row 4u without x[4u - 1], x[4u + 1]: couples to x[4u - 2], x[4u + 2]
-    const float rr = __builtin_amdgcn_rcpf(1.f - r[0].x * c3p - r[0].y * r[1].x);
-    const float A0 = -r[0].x * a3p * rr, C0 = -r[0].y * r[1].y * rr;
-    const float D0 = (r[0].z - r[0].x * d3p - r[0].y * r[1].z) * rr;
-    // then without x[4u - 2], x[4u + 2]: couples to x[4u - 4], x[4u + 4]
-    const float r2 = __builtin_amdgcn_rcpf(1.f - A0 * c2p - C0 * A2);
-    a = -A0 * a2p * r2;
-    cc = -C0 * C2 * r2;
-    d = (D0 - A0 * d2p - C0 * D2) * r2;
-  }
+  // rows C u couple to x[C (u - 1)] and x[C (u + 1)]: parallel cyclic reduction
+  float aa = a0, ac = c0, dd = d0;
   int buf = 0;
 #pragma unroll
   for (int st = 1; st < NQ; st <<= 1) {
     // ping-pong buffers: a buffer is rewritten two steps later, after the
     // barrier that follows every read of it
-    S[buf][0][me] = a;
-    S[buf][1][me] = cc;
-    S[buf][2][me] = d;
+    S[buf][0][me] = aa;
+    S[buf][1][me] = ac;
+    S[buf][2][me] = dd;
     __syncthreads();
     float am = 0.f, cm = 0.f, dm = 0.f, ap = 0.f, cp = 0.f, dp = 0.f;
     if (u - st >= 0) {
@@ -359,27 +370,29 @@ __global__ void __launch_bounds__(TB) k_mg_line(LevelDev L, int c, int zero_gues
       cp = S[buf][1][o];
       dp = S[buf][2][o];
     }
-    const float rr = __builtin_amdgcn_rcpf(1.f - a * cm - cc * ap);
-    const float an = -a * am * rr, cn = -cc * cp * rr;
-    d = (d - a * dm - cc * dp) * rr;
-    a = an;
-    cc = cn;
+    const float rr = __builtin_amdgcn_rcpf(1.f - aa * cm - ac * ap);
+    const float an = -aa * am * rr, cn = -ac * cp * rr;
+    dd = (dd - aa * dm - ac * dp) * rr;
+    aa = an;
+    ac = cn;
     buf ^= 1;
   }
-  // d is x[4u]; x[4u + 4] from the successor, then the eliminated cells
-  S[buf][0][me] = d;
+  // dd is x[C u]; x[C (u + 1)] from the successor, then the eliminated rows
+  // from the widest stride down
+  S[buf][0][me] = dd;
   __syncthreads();
-  const float x4 = u + 1 < NQ ? S[buf][0][me + LPB] : 0.f;
-  const float x2 = D2 - A2 * d - C2 * x4;
-  const float x1 = r[1].z - r[1].x * d - r[1].y * x2;
-  const float x3 = r[3].z - r[3].x * x2 - r[3].y * x4;
+  float x[C + 1];
+  x[0] = dd;
+  x[C] = u + 1 < NQ ? S[buf][0][me + LPB] : 0.f;
+#pragma unroll
+  for (int h = C / 2; h >= 1; h >>= 1)
+#pragma unroll
+    for (int t = h; t < C; t += 2 * h) x[t] = d[t] - a[t] * x[t - h] - cc[t] * x[t + h];
   const bool skip = done && *done; // stores skipped once the solver has exited
   if (ok && !skip) {
-    const int t0 = 4 * u;
-    if (t0 < nl) L.x[at[0]] = d;
-    if (t0 + 1 < nl) L.x[at[1]] = x1;
-    if (t0 + 2 < nl) L.x[at[2]] = x2;
-    if (t0 + 3 < nl) L.x[at[3]] = x3;
+#pragma unroll
+    for (int s4 = 0; s4 < C; ++s4)
+      if (C * u + s4 < nl) L.x[at[s4]] = x[s4];
   }
 }
 
@@ -1272,22 +1285,24 @@ static void sweep2(LevelDev &L, int c_first, bool zero_first, const int *done) {
   std::swap(L.x, L.xalt);
 }
 
-// Zebra line half-sweep of colour c along direction D of level L.
-template <int D, int NP, int TB>
+// Zebra line half-sweep of colour c along direction D of level L: C cells
+// a thread, NP / C threads a line, TB threads a block.
+template <int D, int NP, int TB, int C>
 static void line_launch(const LevelDev &L, int c, int zg, const int *done, int o) {
-  const int lpb = TB / (NP / 4);
+  const int lpb = TB / (NP / C);
   const int gx = (L.h0 + lpb - 1) / lpb;
-  hipLaunchKernelGGL((k_mg_line<D, NP, TB>), dim3((unsigned)(gx * o)), dim3(TB), 0, stream(), L, c, zg, done, gx);
+  hipLaunchKernelGGL((k_mg_line<D, NP, TB, C>), dim3((unsigned)(gx * o)), dim3(TB), 0, stream(), L, c, zg, done,
+                     gx);
 }
 
 template <int D>
 static void line_half_d(const LevelDev &L, int c, bool zero_guess, const int *done) {
   const int nl = D == 1 ? L.n1 : L.n2, o = D == 1 ? L.n2 : L.n1;
   const int zg = zero_guess ? 1 : 0;
-  if (nl <= 16) line_launch<D, 16, 64>(L, c, zg, done, o);
-  else if (nl <= 32) line_launch<D, 32, 128>(L, c, zg, done, o);
-  else if (nl <= 64) line_launch<D, 64, 128>(L, c, zg, done, o);
-  else line_launch<D, kLineMax, 512>(L, c, zg, done, o);
+  if (nl <= 16) line_launch<D, 16, 64, 4>(L, c, zg, done, o);
+  else if (nl <= 32) line_launch<D, 32, 128, 4>(L, c, zg, done, o);
+  else if (nl <= 64) line_launch<D, 64, 128, 4>(L, c, zg, done, o);
+  else line_launch<D, kLineMax, 512, 8>(L, c, zg, done, o);
 }
 
 static void line_half(const LevelDev &L, int D, int c, bool zero_guess, const int *done) {
