@@ -34,19 +34,36 @@ using CCGeom = CCGeomT<double>;
 // gx(node) = f(i-1)*D + f(i)*U, div = gx(i)*D' + gx(i+1)*U', then the y and z
 // terms added left to right (code/handwritten/stencils/stencils.f90:47-59,108-120,
 // composed as in lap_centered_SF_new, code/handwritten/ops/ops_discrete.f90:124-132).
+// The six stencil coefficients of one direction at index i: CC->N at i - 1
+// and i, N->CC at i (loaded once where a thread reuses them).
+template <typename T>
+struct St6 {
+  T cDm, cUm, cD, cU, nD, nU;
+};
+template <typename T>
+__device__ __forceinline__ St6<T> st6(const Coef1T<T> &X, int i) {
+  return St6<T>{X.cD[i - 1], X.cU[i - 1], X.cD[i], X.cU[i], X.nD[i], X.nU[i]};
+}
+
+template <typename T>
+__device__ __forceinline__ T lap7s(T c, T xm, T xp, T ym, T yp, T zm, T zp, const St6<T> &X, const St6<T> &Y,
+                                   const St6<T> &Z) {
+  const T gxm = xm * X.cDm + c * X.cUm;
+  const T gxp = c * X.cD + xp * X.cU;
+  T v = gxm * X.nD + gxp * X.nU;
+  const T gym = ym * Y.cDm + c * Y.cUm;
+  const T gyp = c * Y.cD + yp * Y.cU;
+  v = v + gym * Y.nD + gyp * Y.nU;
+  const T gzm = zm * Z.cDm + c * Z.cUm;
+  const T gzp = c * Z.cD + zp * Z.cU;
+  v = v + gzm * Z.nD + gzp * Z.nU;
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T lap7(T c, T xm, T xp, T ym, T yp, T zm, T zp, const Coef1T<T> &X, int i,
                                   const Coef1T<T> &Y, int j, const Coef1T<T> &Z, int k) {
-  const T gxm = xm * X.cD[i - 1] + c * X.cU[i - 1];
-  const T gxp = c * X.cD[i] + xp * X.cU[i];
-  T v = gxm * X.nD[i] + gxp * X.nU[i];
-  const T gym = ym * Y.cD[j - 1] + c * Y.cU[j - 1];
-  const T gyp = c * Y.cD[j] + yp * Y.cU[j];
-  v = v + gym * Y.nD[j] + gyp * Y.nU[j];
-  const T gzm = zm * Z.cD[k - 1] + c * Z.cU[k - 1];
-  const T gzp = c * Z.cD[k] + zp * Z.cU[k];
-  v = v + gzm * Z.nD[k] + gzp * Z.nU[k];
-  return v;
+  return lap7s(c, xm, xp, ym, yp, zm, zp, st6(X, i), st6(Y, j), st6(Z, k));
 }
 
 // Face-ghost value of a cell-centred field (never an edge or corner ghost).

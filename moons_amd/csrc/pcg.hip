@@ -208,19 +208,33 @@ __global__ void __launch_bounds__(kNT) k_pcg_matvec(const double *__restrict__ p
     const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
     const double dxy = G.c[0].dh[i] * G.c[1].dh[j];
     const double *pc = p + i + (size_t)j * sx;
-    double pm = pc[(size_t)(t.k0 - 1) * sxy];
-    double p0 = pc[(size_t)t.k0 * sxy];
-    for (int k = t.k0; k < t.k1; ++k) {
-      const size_t ok = (size_t)k * sxy;
-      const double pp = pc[ok + sxy];
-      const double v = NRG ? nrg7(K, G, p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, i, j, k)
-                           : lap7(p0, pc[ok - 1], pc[ok + 1], pc[ok - sx], pc[ok + sx], pm, pp, G.c[0], i, G.c[1],
-                                  j, G.c[2], k);
-      const double qv = v * (dxy * G.c[2].dh[k]);
-      q[i + (size_t)j * sx + ok] = qv;
-      acc += p0 * qv;
-      pm = p0;
-      p0 = pp;
+    const int nk = t.k1 - t.k0; // 1 .. KZ planes, the same in the whole block
+    // every load of the thread's planes first (planes past the slab read the
+    // last one's and are not used), so that they are in flight together
+    double pz[KZ + 2], nb[KZ][4];
+    const St6<double> SX = st6(G.c[0], i), SY = st6(G.c[1], j);
+#pragma unroll
+    for (int u = 0; u < KZ + 2; ++u) pz[u] = pc[(size_t)min(t.k0 - 1 + u, t.k1) * sxy];
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      const size_t ok = (size_t)min(t.k0 + u, t.k1 - 1) * sxy;
+      nb[u][0] = pc[ok - 1];
+      nb[u][1] = pc[ok + 1];
+      nb[u][2] = pc[ok - sx];
+      nb[u][3] = pc[ok + sx];
+    }
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      if (u < nk) {
+        const int k = t.k0 + u;
+        const double p0 = pz[u + 1];
+        const double v = NRG ? nrg7(K, G, p0, nb[u][0], nb[u][1], nb[u][2], nb[u][3], pz[u], pz[u + 2], i, j, k)
+                             : lap7s(p0, nb[u][0], nb[u][1], nb[u][2], nb[u][3], pz[u], pz[u + 2], SX, SY,
+                                     st6(G.c[2], k));
+        const double qv = v * (dxy * G.c[2].dh[k]);
+        q[i + (size_t)j * sx + (size_t)k * sxy] = qv;
+        acc += p0 * qv;
+      }
     }
   }
   const double s = block_sum<kNT>(acc, sh);
