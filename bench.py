@@ -163,7 +163,8 @@ def load_rocprof_avg(name, kernel_prefix, grid=None):
 
 def rocprof_summary(d, top=60):
     """Per (kernel, grid size) dispatch count and duration statistics of the
-    rocprofv3 kernel-trace CSVs under directory d, longest total first."""
+    rocprofv3 kernel-trace CSVs under directory d, longest total first, and
+    per (kernel, grid size, counter) statistics of its counter-collection CSVs."""
     import csv
     import glob
 
@@ -185,6 +186,19 @@ def rocprof_summary(d, top=60):
     for (kname, grid), v in rows[:top]:
         print(f"kernel: {kname[:120]} grid: {grid} calls: {len(v)} avg_us: {sum(v) / len(v):.3f} "
               f"min_us: {min(v):.3f} max_us: {max(v):.3f} total_ms: {sum(v) / 1e3:.3f}")
+    # PMC runs (--pmc): per (kernel, grid, counter) mean, min and max over dispatches
+    cfiles = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+    counters = {}
+    for fpath in cfiles:
+        with open(fpath) as fh:
+            for r in csv.DictReader(fh):
+                key = (r["Kernel_Name"], int(r.get("Grid_Size", 0)), r["Counter_Name"])
+                counters.setdefault(key, []).append(float(r["Counter_Value"]))
+    if cfiles:
+        print(f"counter files: {len(cfiles)}")
+    for (kname, grid, cname), v in sorted(counters.items()):
+        print(f"kernel: {kname[:120]} grid: {grid} counter: {cname} dispatches: {len(v)} "
+              f"mean: {sum(v) / len(v):.1f} min: {min(v):.1f} max: {max(v):.1f}")
 
 
 def run_ensemble(M, args):

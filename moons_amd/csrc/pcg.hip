@@ -1174,7 +1174,10 @@ int moons_pcg_time_kernels(moons_pcg_t S, int reps, double ms[3]) {
   int st;
   if ((st = pcg_launch(S, xs, 1, 1 | 2 | 4))) return st;
   for (int kk = 0; kk < 3; ++kk) {
-    if ((st = pcg_launch(S, xs, 2, 1 << kk))) return st; // warm-up
+    // warm-up: as many launches as are timed (the first measurement after
+    // other work otherwise catches the device's clock and cache transients)
+    for (int rep = 0; rep < reps; ++rep)
+      if ((st = pcg_launch(S, xs, 2, 1 << kk))) return st;
     MOONS_HIP(hipEventRecord(e[0], stream()));
     for (int rep = 0; rep < reps; ++rep)
       if ((st = pcg_launch(S, xs, 2, 1 << kk))) return st;

@@ -49,3 +49,21 @@ def test_summary_and_weighted_average(bench, tmp_path, capsys, monkeypatch):
     assert bench.load_rocprof_avg("s.txt", "k_pcg_update") == pytest.approx(10.5)
     assert bench.load_rocprof_avg("s.txt", "k_absent") is None
     assert bench.load_rocprof_avg("missing.txt", "k_mg_line") is None
+
+
+def test_counter_summary(bench, tmp_path, capsys):
+    d = tmp_path / "pmc"
+    d.mkdir()
+    cols = ["Dispatch_Id", "Grid_Size", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    rows = [(1, 131072, "k_mg_line<1, 128, 512, 8>", "FETCH_SIZE", 4380.0),
+            (2, 131072, "k_mg_line<1, 128, 512, 8>", "FETCH_SIZE", 4390.0),
+            (3, 32768, "k_mg_line<1, 64, 128, 4>", "FETCH_SIZE", 614.0)]
+    with open(d / "f_counter_collection.csv", "w") as fh:
+        fh.write(",".join(f'"{c}"' for c in cols) + "\n")
+        for r in rows:
+            fh.write(f'{r[0]},{r[1]},"{r[2]}","{r[3]}",{r[4]}\n')
+    bench.rocprof_summary(str(d))
+    out = capsys.readouterr().out
+    assert ("kernel: k_mg_line<1, 128, 512, 8> grid: 131072 counter: FETCH_SIZE dispatches: 2 "
+            "mean: 4385.0 min: 4380.0 max: 4390.0") in out
+    assert "grid: 32768 counter: FETCH_SIZE dispatches: 1 mean: 614.0" in out
