@@ -393,11 +393,16 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
   const int total = nd * nlo * nhi;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total || nlo <= 0 || nhi <= 0) return;
-  const int o = t % nd;
-  const int rest = t / nd;
-  const int plo = pad + rest % nlo, phi = pad + rest / nlo;
-  int io[3];
-  io[dir] = o; io[lo] = plo; io[hi] = phi;
+  // threads take the output elements in memory order (x fastest), so that a
+  // wave's loads and stores are contiguous whatever the direction
+  int ext[3], io[3];
+  ext[dir] = nd; ext[lo] = nlo; ext[hi] = nhi;
+  io[0] = t % ext[0];
+  io[1] = (t / ext[0]) % ext[1];
+  io[2] = t / (ext[0] * ext[1]);
+  io[lo] += pad;
+  io[hi] += pad;
+  const int o = io[dir], plo = io[lo], phi = io[hi];
   const size_t oo = idx3(So.s, io[0], io[1], io[2]);
   const size_t fs = (dir == 0) ? 1 : (dir == 1 ? (size_t)Sf.s[0] : (size_t)Sf.s[0] * Sf.s[1]);
   int iff[3];
@@ -474,11 +479,12 @@ __global__ void __launch_bounds__(256) k_interp(double *__restrict__ out, const 
   const int total = nd * So.s[a] * So.s[b];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  const int o = t % nd;
-  const int rest = t / nd;
-  const int pa = rest % So.s[a], pb = rest / So.s[a];
+  // threads take the output elements in memory order (x fastest)
   int io[3], ii[3];
-  io[dir] = o; io[a] = pa; io[b] = pb;
+  io[0] = t % So.s[0];
+  io[1] = (t / So.s[0]) % So.s[1];
+  io[2] = t / (So.s[0] * So.s[1]);
+  const int o = io[dir], pa = io[a], pb = io[b];
   ii[dir] = 0; ii[a] = pa; ii[b] = pb;
   const double *C = in + idx3(Si.s, ii[0], ii[1], ii[2]);
   const size_t fs = (dir == 0) ? 1 : (dir == 1 ? (size_t)Si.s[0] : (size_t)Si.s[0] * Si.s[1]);
