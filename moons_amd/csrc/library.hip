@@ -380,11 +380,12 @@ __global__ void __launch_bounds__(256) k_scale_plane(double *u, Shape3 S, int d,
 // output element.  kind: 0 stag CC->N (gt=1), 1 stag N->CC (gt=0), 2 col CC, 3 col N.
 // Follows code/handwritten/stencils/stencils.f90 element by element, including
 // the end entries the staggered kernels zero before accumulating.
+template <int DIR, int KIND>
 __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const double *__restrict__ f, Shape3 So,
-                                             Shape3 Sf, int dir, int kind, const double *__restrict__ L,
-                                             const double *__restrict__ D, const double *__restrict__ U,
-                                             int gen, int pad, const int *gate) {
+                                             Shape3 Sf, const double *__restrict__ L, const double *__restrict__ D,
+                                             const double *__restrict__ U, int gen, int pad, const int *gate) {
   if (gate && *gate) return;
+  constexpr int dir = DIR, kind = KIND;
   const int a = (dir + 1) % 3, b = (dir + 2) % 3;
   const int lo = a < b ? a : b, hi = a < b ? b : a;
   const int nd = So.s[dir];
@@ -409,14 +410,14 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
   iff[dir] = 0; iff[lo] = plo; iff[hi] = phi;
   const double *fp = f + idx3(Sf.s, iff[0], iff[1], iff[2]);
   const int s = Sf.s[dir];
-  double cur = out[oo];
+  double cur = 0.0; // the output's old value, read only where the result uses it
   double res;
   if (kind == 0 || kind == 1) {
     const int gt = (kind == 0) ? 1 : 0;
     const bool zero_end = (o == 0 || o == nd - 1);
     const int i = o - gt;
     const bool written = (i >= 0 && i <= s - 2);
-    if (zero_end) cur = 0.0;
+    if (!zero_end && (gen != 1 || !written)) cur = out[oo];
     if (!written) {
       res = cur;
     } else {
@@ -428,6 +429,7 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
     }
   } else {
     // collocated, pad1 = pad2 = 0
+    if (gen != 1) cur = out[oo];
 #define FF(q) fp[(size_t)(q) * fs]
     if (o == 0 || o == s - 1) {
       res = 0.0;
@@ -470,9 +472,11 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
 // interpO2_SF along dir, one thread per output element: C->N with theta
 // weights and extrapolated end nodes (GF_interp.f90:19-47, GF_extrap.f90:17-46)
 // or N->C half sums (GF_interp.f90:49-74).
+template <int DIR, int C2N>
 __global__ void __launch_bounds__(256) k_interp(double *__restrict__ out, const double *__restrict__ in, Shape3 So,
-                                                Shape3 Si, int dir, int c2n, const double *__restrict__ tD,
+                                                Shape3 Si, const double *__restrict__ tD,
                                                 const double *__restrict__ tU) {
+  constexpr int dir = DIR, c2n = C2N;
   const int a = (dir + 1) % 3, b = (dir + 2) % 3;
   const int nd = So.s[dir];
   // 32-bit: the launcher rejects fields of 2^31 or more elements
@@ -683,8 +687,14 @@ int launch_del(moons_field_s *out, const moons_field_s *f, int n, int dir, int p
   size_t total = (size_t)out->s[dir] * (size_t)(f->s[a] - 2 * pad) * (size_t)(f->s[b] - 2 * pad);
   if (total >= ((size_t)1 << 31) || out->n >= ((size_t)1 << 31) || f->n >= ((size_t)1 << 31))
     return fail(MOONS_ERR_ARG, "del: fields of 2^31 or more elements are not supported");
-  hipLaunchKernelGGL(k_del, dim3(nblk(total)), dim3(256), 0, g_stream, out->d, f->d, So, Sf, dir, kind, L, D, U,
-                     gen, pad, g_gate);
+  // direction and kind as template arguments: no run-time branch on them per element
+  using KDel = void (*)(double *, const double *, Shape3, Shape3, const double *, const double *, const double *, int,
+                        int, const int *);
+  static const KDel kdel[3][4] = {{k_del<0, 0>, k_del<0, 1>, k_del<0, 2>, k_del<0, 3>},
+                                  {k_del<1, 0>, k_del<1, 1>, k_del<1, 2>, k_del<1, 3>},
+                                  {k_del<2, 0>, k_del<2, 1>, k_del<2, 2>, k_del<2, 3>}};
+  hipLaunchKernelGGL(kdel[dir][kind], dim3(nblk(total)), dim3(256), 0, g_stream, out->d, f->d, So, Sf, L, D, U, gen,
+                     pad, g_gate);
   MOONS_LAUNCH_CHECK();
   if (gen == 1 && pad > 0) {
     // diff(): assign with padding zeroes the ghost planes of the other directions
@@ -703,7 +713,10 @@ int launch_interp(moons_field_s *out, const moons_field_s *in, int dir) {
   Shape3 So{{out->s[0], out->s[1], out->s[2]}}, Si{{in->s[0], in->s[1], in->s[2]}};
   if (out->n >= ((size_t)1 << 31) || in->n >= ((size_t)1 << 31))
     return fail(MOONS_ERR_ARG, "interp: fields of 2^31 or more elements are not supported");
-  hipLaunchKernelGGL(k_interp, dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, in->d, So, Si, dir, nout, c.theta_D,
+  using KInterp = void (*)(double *, const double *, Shape3, Shape3, const double *, const double *);
+  static const KInterp kint[3][2] = {{k_interp<0, 0>, k_interp<0, 1>}, {k_interp<1, 0>, k_interp<1, 1>},
+                                     {k_interp<2, 0>, k_interp<2, 1>}};
+  hipLaunchKernelGGL(kint[dir][nout], dim3(nblk(out->n)), dim3(256), 0, g_stream, out->d, in->d, So, Si, c.theta_D,
                      c.theta_U);
   MOONS_LAUNCH_CHECK();
   return MOONS_OK;

@@ -67,7 +67,9 @@ struct WallVol {
   int per[3];       // periodic pair: plane s-2 set to 0
 };
 
-__device__ __forceinline__ double wall_vol(const WallVol &W, int d, int pd, int sd, int t, double v) {
+// (volv: the cell volume W.vol[d][t], loaded by the caller ahead of the
+// arithmetic so that its latency overlaps the operator's loads)
+__device__ __forceinline__ double wall_vol(const WallVol &W, int d, int pd, int sd, double volv, double v) {
   if (pd == 1) {
     if (W.lo[d] == 1) v = 0.0;
     else if (W.lo[d] == 2) v = v * 0.5;
@@ -77,19 +79,22 @@ __device__ __forceinline__ double wall_vol(const WallVol &W, int d, int pd, int 
     else if (W.hi[d] == 2) v = v * 0.5;
     if (W.per[d]) v = 0.0;
   }
-  return v * W.vol[d][t];
+  return v * volv;
 }
 
 // The value k_del (staggered kinds, gen 1, pad 0) stores at position q along
 // a pencil of the first pass: kind 0 CC -> N reads input q-1, q; kind 1 N -> CC
 // reads q, q+1; entries without both inputs are the zeroed ends.
+// Entries without both inputs load from position 0 and are discarded, so no
+// branch separates the loads of the calls.
 __device__ __forceinline__ double stag_first(const double *__restrict__ fp, int fs, int q, int s_in, int kind,
                                              const double *__restrict__ D, const double *__restrict__ U) {
   const int i = q - (kind == 0 ? 1 : 0);
-  if (i < 0 || i > s_in - 2) return 0.0;
-  const double x0 = fp[i * fs] * D[i];
-  const double x1 = fp[(i + 1) * fs] * U[i];
-  return x0 + x1;
+  const bool ok = i >= 0 && i <= s_in - 2;
+  const int ic = ok ? i : 0;
+  const double x0 = fp[ic * fs] * D[ic];
+  const double x1 = fp[(ic + 1) * fs] * U[ic];
+  return ok ? x0 + x1 : 0.0;
 }
 
 // Ax = x + c lap(x) for the three face components: mom_diffusion
@@ -114,6 +119,7 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
   const int io[3] = {t % s0, (t / s0) % s1, t / (s0 * s1)};
   const int str[3] = {1, s0, s0 * s1};
   const double *__restrict__ xp = x.p[d];
+  const double volv = W.on ? W.vol[d][t] : 0.0;
   double acc = 0.0;
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
@@ -130,12 +136,15 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
     const int i2 = o - (kind2 == 0 ? 1 : 0);
     const bool written = (i2 >= 0 && i2 <= nt - 2);
     const bool zero_end = (o == 0 || o == se - 1);
-    double v = 0.0;
+    // both passes' loads without a branch (positions clamped in range where
+    // the entry is not written), then the entry selected
+    const int i2c = written ? i2 : 0;
+    const double t0 = stag_first(fp, str[e], i2c, se, kind1, D1, U1);
+    const double t1 = stag_first(fp, str[e], i2c + 1, se, kind1, D1, U1);
+    const double x0 = t0 * D2[i2c];
+    const double x1 = t1 * U2[i2c];
+    double v;
     if (written) {
-      const double t0 = stag_first(fp, str[e], i2, se, kind1, D1, U1);
-      const double t1 = stag_first(fp, str[e], i2 + 1, se, kind1, D1, U1);
-      const double x0 = t0 * D2[i2];
-      const double x1 = t1 * U2[i2];
       if (e == 0) {
         v = x0 + x1;
       } else {
@@ -149,7 +158,7 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
   }
   const double w = acc * c;
   double res = w + xp[t];
-  if (W.on) res = wall_vol(W, d, io[d], s[d], t, res);
+  if (W.on) res = wall_vol(W, d, io[d], s[d], volv, res);
   Ax.p[d][t] = res;
 }
 
@@ -171,6 +180,8 @@ __global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C
   const int s0 = out.s[d][0], s1 = out.s[d][1];
   const int io[3] = {t % s0, (t / s0) % s1, t / (s0 * s1)};
   const int kind = mode == 0 ? 0 : 1;
+  const double volv = (mode != 0 && W.on) ? W.vol[d][t] : 0.0;
+  const double auxv = aux.p[d][t];
   double v = 0.0;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -190,25 +201,23 @@ __global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C
     const int i = o - (kind == 0 ? 1 : 0);
     const bool written = (i >= 0 && i <= s_in - 2);
     const bool zero_end = (o == 0 || o == nd - 1);
+    // loads at a clamped position where the entry is not written (discarded)
+    const int ic = written ? i : 0;
+    const double x0 = fp[ic * stri[e]] * D[ic];
+    const double x1 = fp[(ic + 1) * stri[e]] * U[ic];
     if (q == 0) {
-      v = written ? fp[i * stri[e]] * D[i] + fp[(i + 1) * stri[e]] * U[i] : 0.0;
+      v = written ? x0 + x1 : 0.0;
     } else {
       const double cur = zero_end ? 0.0 : v;
-      if (written) {
-        const double x0 = fp[i * stri[e]] * D[i];
-        const double x1 = fp[(i + 1) * stri[e]] * U[i];
-        v = cur - (x0 + x1);
-      } else {
-        v = cur;
-      }
+      v = written ? cur - (x0 + x1) : cur;
     }
   }
   if (mode == 0) {
-    out.p[d][t] = v * aux.p[d][t];
+    out.p[d][t] = v * auxv;
   } else {
     const double w = v * c;
-    double res = w + aux.p[d][t];
-    if (W.on) res = wall_vol(W, d, io[d], out.s[d][d], t, res);
+    double res = w + auxv;
+    if (W.on) res = wall_vol(W, d, io[d], out.s[d][d], volv, res);
     out.p[d][t] = res;
   }
 }
