@@ -550,6 +550,23 @@ int moons_mhd_graph_capture(moons_mhd_t s);
 int moons_mhd_graph_launch(moons_mhd_t s, int n_steps, void *hip_stream);
 /* Handles of internal fields (MOONS_MHD_*), 3 or 6 entries, for inspection. */
 int moons_mhd_field(moons_mhd_t s, int which, moons_field_t out[6]);
+/* Buoyancy and gravity in the momentum sources, after J x B
+ * (add_all_momentum_sources.f90:70-78):
+ *   buoyancy (compute_add_buoyancy, momentum_sources.f90:133-150):
+ *     F_d += cellCenter2Face_d(T * g_d) * buoyancy_scale   (Gr/Re^2)
+ *   gravity (compute_add_gravity, momentum_sources.f90:152-167):
+ *     F_d += cellCenter2Face_d(g_d) * gravity_scale        (1/Fr^2)
+ * T is the energy's temperature (cell-centred, on the fluid mesh) and g the
+ * three cell-centred components of the gravity field (nrg%gravity, e.g. the
+ * reference's uniform_gravity_field: 1 in one component everywhere, ghosts
+ * included, 0 in the others, init_gravity_field.f90:41-50).  The faces are
+ * taken onto the fluid mesh as J x B is (extractFace with the energy on the
+ * fluid mesh: every face but the non-periodic ghost planes, which are 0).
+ * The fields stay the caller's and are read at every step (T as the energy
+ * step left it).  add_buoyancy / add_gravity 0 switch a term off; T may be
+ * NULL when buoyancy is off; both off unbinds. */
+int moons_mhd_set_buoyancy(moons_mhd_t s, moons_field_t T, moons_field_t gravity[3], int add_buoyancy,
+                           double buoyancy_scale, int add_gravity, double gravity_scale);
 
 /* ---------------------------------------------------- energy equation */
 /* One energy step (solve_energy after add_all_energy_sources,

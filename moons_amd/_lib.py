@@ -148,6 +148,8 @@ SIGNATURES = {
     "moons_mhd_destroy": (C.c_int, [C.c_void_p]),
     "moons_mhd_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(MhdInfo)]),
     "moons_mhd_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "moons_mhd_set_buoyancy": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_double,
+                                         C.c_int, C.c_double]),
     "moons_mhd_graph_capture": (C.c_int, [C.c_void_p]),
     "moons_mhd_graph_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "moons_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),

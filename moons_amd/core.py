@@ -591,6 +591,15 @@ class PCG_solver_VF:
             pass
 
 
+def uniform_gravity_field(m: mesh, direction: int):
+    """The reference's uniform_gravity_field (code/handwritten/solvers/energy/
+    init_gravity_field.f90:41-50): three cell-centred fields, component
+    direction (1..3) equal to 1 everywhere, ghosts included, the others 0."""
+    if direction not in (1, 2, 3):
+        raise ValueError("uniform_gravity_field: direction must be 1, 2 or 3")
+    return [SF(m, "CC").fill(1.0 if d == direction - 1 else 0.0) for d in range(3)]
+
+
 class MHD_step:
     """One MOONS time step on the device (solve_MOONS,
     code/handwritten/MOONS/MOONS_solver.f90:47-144): momentum sources, AB2 time
@@ -646,6 +655,22 @@ class MHD_step:
     def launch(self, n: int = 1, stream=None) -> None:
         """Replay the captured step n times on `stream` (a Stream, or the library stream)."""
         check(load().moons_mhd_graph_launch(self.handle, n, stream.handle if stream is not None else None))
+
+    def set_buoyancy(self, T: "SF | None", gravity, buoyancy_scale: float | None = None,
+                     gravity_scale: float | None = None) -> None:
+        """Buoyancy F += cellCenter2Face(T g) * buoyancy_scale and gravity
+        F += cellCenter2Face(g) * gravity_scale in the momentum sources, after
+        J x B (compute_add_buoyancy / compute_add_gravity,
+        code/handwritten/solvers/momentum/momentum_sources.f90:133-167).  g is
+        three cell-centred fields; a scale of None switches its term off; T and
+        g stay the caller's and are read at every step."""
+        self._buoy = (T, list(gravity) if gravity is not None else None)  # keep the fields alive
+        gv = (C.c_void_p * 3)(*[f.handle for f in gravity]) if gravity is not None else None
+        check(load().moons_mhd_set_buoyancy(self.handle, T.handle if T is not None else None, gv,
+                                            0 if buoyancy_scale is None else 1,
+                                            0.0 if buoyancy_scale is None else float(buoyancy_scale),
+                                            0 if gravity_scale is None else 1,
+                                            0.0 if gravity_scale is None else float(gravity_scale)))
 
     def field(self, which: str):
         out = (C.c_void_p * 6)()

@@ -134,6 +134,10 @@ struct moons_mhd_s {
   bool own_i = false; // UEi, T1i, T2i, cci are fields of their own
   moons_pcg_t pcg_p = nullptr, pcg_phi = nullptr;
   moons_pcgvf_t pcg_U = nullptr, pcg_B = nullptr;
+  // buoyancy and gravity sources (caller's fields: T cell-centred, g three cell-centred components)
+  const moons_field_s *Tb = nullptr, *grav[3] = {};
+  int add_buoy = 0, add_grav = 0;
+  double buoy_scale = 0.0, grav_scale = 0.0;
   int iters[4] = {0, 0, 0, 0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraph_t graph = nullptr;
@@ -321,6 +325,22 @@ int momentum(moons_mhd_s *S) {
         return st;
       if ((st = launch_assign_ghost_xperiodic(S->T1[d], 0.0))) return st;
       if ((st = axpy(S->F[d], S->T1[d], 1.0))) return st;
+    }
+  if (S->add_buoy)
+    for (int d = 0; d < 3; ++d) {
+      // compute_add_buoyancy: temp_CC = T, times gravity, cellCenter2Face,
+      // extractFace (as J x B), F += buoyancy * scale
+      if ((st = mul3(S->tcc, S->Tb, S->grav[d]))) return st;
+      if ((st = launch_interp(S->T1[d], S->tcc, d))) return st;
+      if ((st = launch_assign_ghost_xperiodic(S->T1[d], 0.0))) return st;
+      if ((st = axpy(S->F[d], S->T1[d], S->buoy_scale))) return st;
+    }
+  if (S->add_grav)
+    for (int d = 0; d < 3; ++d) {
+      // compute_add_gravity: cellCenter2Face(g), extractFace, F += gravity * scale
+      if ((st = launch_interp(S->T1[d], S->grav[d], d))) return st;
+      if ((st = launch_assign_ghost_xperiodic(S->T1[d], 0.0))) return st;
+      if ((st = axpy(S->F[d], S->T1[d], S->grav_scale))) return st;
     }
   double scale;
   if (S->P.alpha_U == 0.0) {
@@ -558,6 +578,31 @@ int moons_mhd_graph_launch(moons_mhd_t S, int n_steps, void *hip_stream) {
   if (!S->exec) return fail(MOONS_ERR_STATE, "moons_mhd_graph_launch: capture the step first");
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : stream();
   for (int n = 0; n < n_steps; ++n) MOONS_HIP(hipGraphLaunch(S->exec, st));
+  return MOONS_OK;
+}
+
+int moons_mhd_set_buoyancy(moons_mhd_t S, moons_field_t T, moons_field_t gravity[3], int add_buoyancy,
+                           double buoyancy_scale, int add_gravity, double gravity_scale) {
+  if (!S) return fail(MOONS_ERR_ARG, "moons_mhd_set_buoyancy: null state");
+  if (S->exec) return fail(MOONS_ERR_STATE, "moons_mhd_set_buoyancy: set the sources before capturing the step");
+  if (!add_buoyancy && !add_gravity) {
+    S->add_buoy = S->add_grav = 0;
+    S->Tb = nullptr;
+    for (int d = 0; d < 3; ++d) S->grav[d] = nullptr;
+    return MOONS_OK;
+  }
+  if (!gravity) return fail(MOONS_ERR_ARG, "moons_mhd_set_buoyancy: the gravity field is needed");
+  for (int d = 0; d < 3; ++d)
+    if (!gravity[d] || gravity[d]->m != S->m || gravity[d]->dl != MOONS_DL_CC)
+      return fail(MOONS_ERR_ARG, "moons_mhd_set_buoyancy: gravity must be three cell-centred fields on the fluid mesh");
+  if (add_buoyancy && (!T || T->m != S->m || T->dl != MOONS_DL_CC))
+    return fail(MOONS_ERR_ARG, "moons_mhd_set_buoyancy: T must be a cell-centred field on the fluid mesh");
+  S->Tb = add_buoyancy ? T : nullptr;
+  for (int d = 0; d < 3; ++d) S->grav[d] = gravity[d];
+  S->add_buoy = add_buoyancy ? 1 : 0;
+  S->add_grav = add_gravity ? 1 : 0;
+  S->buoy_scale = buoyancy_scale;
+  S->grav_scale = gravity_scale;
   return MOONS_OK;
 }
 

@@ -101,6 +101,7 @@ module moons_amd_bindings
   public :: moons_pcg_vf_create, moons_pcg_vf_destroy, moons_pcg_vf_set_isp, moons_pcg_vf_update_mfp
   public :: moons_pcg_vf_solve
   public :: moons_mhd_create, moons_mhd_destroy, moons_mhd_step, moons_mhd_graph_capture, moons_mhd_graph_launch
+  public :: moons_mhd_set_buoyancy
   public :: moons_field_export, moons_field_import, moons_export_raw, moons_import_raw
   public :: moons_nrg_create, moons_nrg_destroy, moons_nrg_step, moons_nrg_set_joule, moons_nrg_field
 
@@ -369,6 +370,16 @@ module moons_amd_bindings
       import c_int, c_ptr
       type(c_ptr), value :: s, stream
       integer(c_int), value :: n
+    end function
+    ! buoyancy F += cellCenter2Face(T g) * buoyancy_scale, gravity F += cellCenter2Face(g) * gravity_scale
+    ! (compute_add_buoyancy / compute_add_gravity, momentum_sources.f90:133-167); gravity(3) CC fields
+    integer(c_int) function moons_mhd_set_buoyancy(s, T, gravity, add_buoyancy, buoyancy_scale, add_gravity, &
+                                                   gravity_scale) bind(C, name="moons_mhd_set_buoyancy")
+      import c_int, c_ptr, c_double
+      type(c_ptr), value :: s, T
+      type(c_ptr), intent(in) :: gravity(3)
+      integer(c_int), value :: add_buoyancy, add_gravity
+      real(c_double), value :: buoyancy_scale, gravity_scale
     end function
     integer(c_int) function moons_nrg_create(m, params, T, U, k, s) bind(C, name="moons_nrg_create")
       import c_int, c_ptr, moons_nrg_params

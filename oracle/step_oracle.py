@@ -10,6 +10,9 @@ Sequence (code/handwritten/MOONS/MOONS_solver.f90:47-144, one RK stage):
                F += coeff_explicit lap(U) compute_add_diffusion (momentum_sources.f90:87-95),
                                           lapUniformCoeff_VF (ops/ops_discrete.f90:265-273)
                F += 1 * jCrossB           extract_add_JCrossB (momentum_sources.f90:111-121)
+               F += buoyancy * scale      compute_add_buoyancy (momentum_sources.f90:133-150):
+                                          cellCenter2Face(T g), extractFace
+               F += gravity * scale       compute_add_gravity (momentum_sources.f90:152-167)
              solve_momentum (governing_equations/momentum_extend.f90:326-374)
                method 4: Euler_time_no_diff_AB2_sources (solvers/time_marching_methods.f90:266-296)
                method 6: Euler_time_AB2_sources (time_marching_methods.f90:117-150)
@@ -130,8 +133,14 @@ class MHDState:
     def __init__(self, g, U, Ubcs, pbcs, Re, dt, alpha_U=1.0, B=None, Bbcs=None, phibcs=None, Rem=1.0, Ha=0.0,
                  B0=(0.0, 1.0, 0.0), isp_U=(5, 1e-6, 1e-13, 1), isp_p=(5, 1e-6, 1e-13, 1),
                  isp_B=(5, 1e-6, 1e-13, 1), isp_phi=(5, 1e-6, 1e-13, 1), gi=None, dom=None, sigInv=None,
-                 mpg_dir=0, mpg_scale=1.0, finite_Rem=False):
+                 mpg_dir=0, mpg_scale=1.0, finite_Rem=False, T=None, gravity=None, buoyancy_scale=None,
+                 gravity_scale=None):
         self.g = g
+        # buoyancy (Gr/Re^2) and gravity (1/Fr^2) sources: the energy's T (CC) and
+        # the CC gravity field (SP%MT%Buoyancy, SP%MT%Gravity; None = off)
+        self.T = T
+        self.gravity = gravity
+        self.buoyancy_scale, self.gravity_scale = buoyancy_scale, gravity_scale
         # mean pressure gradient forcing (SP%MT%mean_pressure_grad, SP%SCP%mpg_dir; 0 = off)
         self.mpg_dir, self.mpg_scale = mpg_dir, mpg_scale
         self.finite_Rem = finite_Rem  # Btot = B0 + B (SP%SCP%finite_Rem), else B0
@@ -192,6 +201,19 @@ class MHDState:
                 jcb = [zero_ghosts(g, D.extract(g.zeros(1 << d), self.jCrossB[d], 1 << d, self.dom), 1 << d)
                        for d in range(3)]
             F = [F[d] + jcb[d] * 1.0 for d in range(3)]
+        if self.buoyancy_scale is not None:
+            # compute_add_buoyancy (momentum_sources.f90:133-150): temp_CC = T, times
+            # gravity, cellCenter2Face (interp along each component's direction),
+            # extractFace onto the fluid faces (energy on the fluid mesh), F += b*scale
+            for d in range(3):
+                tcc = np.asfortranarray(self.T * self.gravity[d])
+                b = zero_ghosts(g, O.interp(g, tcc, 0, 1 << d, d), 1 << d)
+                F[d] = F[d] + b * self.buoyancy_scale
+        if self.gravity_scale is not None:
+            # compute_add_gravity (momentum_sources.f90:152-167)
+            for d in range(3):
+                gf = zero_ghosts(g, O.interp(g, np.asfortranarray(self.gravity[d]), 0, 1 << d, d), 1 << d)
+                F[d] = F[d] + gf * self.gravity_scale
         self.F = [np.asfortranarray(f) for f in F]
 
     def _ab2_rhs(self, X, Xbcs, F, Fnm1, L, g=None):

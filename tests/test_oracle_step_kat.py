@@ -354,3 +354,40 @@ def test_robin_rules_satisfy_u_plus_cw_dudx():
                 # the outward-normal reading u + c_w du/dn = 0 fails at the min faces
                 if not is_max:
                     assert np.abs((uw - cw[face] * dudx - target)[core]).max() > 1e-6
+
+
+def test_buoyancy_and_gravity_closed_form():
+    """compute_add_buoyancy / compute_add_gravity (momentum_sources.f90:133-167)
+    with U = 0 (no other source), uniform gravity along y (the reference's
+    uniform_gravity_field, 1 in g_y everywhere) and T linear in y on a uniform
+    grid: cellCenter2Face is exact on linear data, so on every y-face but the
+    ghost planes F_y = Gr/Re^2 * T(y_face) + 1/Fr^2, F_x = F_z = 0 there, and the
+    ghost planes are 0 (extractFace)."""
+    N = (6, 8, 5)
+    hs = [O.uniform_nodes(0.0, 1.0, N[0]), O.uniform_nodes(-1.0, 1.0, N[1]), O.uniform_nodes(0.0, 2.0, N[2])]
+    g = O.Grid(*hs)
+    U0 = [g.zeros(1 << d) for d in range(3)]
+    bcs = [O.BCs.make(["Dirichlet"] * 6) for _ in range(3)]
+    yc = g.coord(1, 1)
+    T = np.asfortranarray(np.broadcast_to((0.75 + 2.5 * yc)[None, :, None], g.shape(0)).copy())
+    grav = [g.zeros(0), np.asfortranarray(np.ones(g.shape(0))), g.zeros(0)]
+    sb, sg = 0.37, -1.9
+    st = S.MHDState(g, U0, bcs, O.BCs.make(["Neumann"] * 6), Re=100.0, dt=1e-3, alpha_U=1.0, T=T, gravity=grav,
+                    buoyancy_scale=sb, gravity_scale=sg)
+    st.momentum_sources()
+    Fy = st.F[1]
+    yf = g.coord(1, 0)
+    expect = sb * (0.75 + 2.5 * yf) + sg
+    inner = Fy[1:-1, 1:-1, 1:-1]
+    np.testing.assert_allclose(inner, np.broadcast_to(expect[None, 1:-1, None], inner.shape), rtol=0, atol=1e-13)
+    for F in (Fy, st.F[0], st.F[2]):
+        for ax in range(3):
+            for idx in (0, -1):
+                assert not np.take(F, idx, axis=ax).any()
+    assert not st.F[0].any() and not st.F[2].any()
+    # buoyancy alone: the gravity term off leaves the T term only
+    st2 = S.MHDState(g, U0, bcs, O.BCs.make(["Neumann"] * 6), Re=100.0, dt=1e-3, alpha_U=1.0, T=T, gravity=grav,
+                     buoyancy_scale=sb)
+    st2.momentum_sources()
+    np.testing.assert_allclose(st2.F[1][1:-1, 1:-1, 1:-1],
+                               np.broadcast_to((sb * (0.75 + 2.5 * yf))[None, 1:-1, None], inner.shape), atol=1e-13)
