@@ -392,7 +392,9 @@ __global__ void __launch_bounds__(256) k_del(double *__restrict__ out, const dou
   const int nlo = Sf.s[lo] - 2 * pad, nhi = Sf.s[hi] - 2 * pad;
   // 32-bit: the launcher rejects fields of 2^31 or more elements
   const int total = nd * nlo * nhi;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous block numbering: each XCD takes one contiguous range of
+  // planes, so the neighbouring input planes a derivative reads share its L2
+  const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
   if (t >= total || nlo <= 0 || nhi <= 0) return;
   // threads take the output elements in memory order (x fastest), so that a
   // wave's loads and stores are contiguous whatever the direction
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(256) k_interp(double *__restrict__ out, const 
   const int nd = So.s[dir];
   // 32-bit: the launcher rejects fields of 2^31 or more elements
   const int total = nd * So.s[a] * So.s[b];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x; // as in k_del
   if (t >= total) return;
   // threads take the output elements in memory order (x fastest)
   int io[3], ii[3];

@@ -112,7 +112,9 @@ __global__ void __launch_bounds__(kVT) k_vf_mom_op(VF3 Ax, VF3 x, StagCoefs C, d
                                                    const int *gate) {
   if (gate && *gate) return;
   const int d = blockIdx.y;
-  const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
+  // fields stay below 2^31 elements; XCD-contiguous block numbering keeps the
+  // neighbouring planes a derivative reads in one L2
+  const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x) * kVT + (int)threadIdx.x;
   if (t >= (int)Ax.n[d]) return;
   const int s0 = x.s[d][0], s1 = x.s[d][1];
   const int s[3] = {s0, s1, x.s[d][2]};
@@ -175,7 +177,9 @@ __global__ void __launch_bounds__(kVT) k_vf_curl_op(VF3 out, VF3 in, StagCoefs C
                                                     WallVol W, const int *gate) {
   if (gate && *gate) return;
   const int d = blockIdx.y;
-  const int t = blockIdx.x * kVT + threadIdx.x; // fields stay below 2^31 elements
+  // fields stay below 2^31 elements; XCD-contiguous block numbering keeps the
+  // neighbouring planes a derivative reads in one L2
+  const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x) * kVT + (int)threadIdx.x;
   if (t >= (int)out.n[d]) return;
   const int s0 = out.s[d][0], s1 = out.s[d][1];
   const int io[3] = {t % s0, (t / s0) % s1, t / (s0 * s1)};
