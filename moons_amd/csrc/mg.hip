@@ -1363,6 +1363,14 @@ int mg_vcycle(MGHier *h, const int *done) {
   // gamma-cycle from level c; K = 1, c = g: the plain V-cycle)
   const int c = h->gamma_level >= 0 && h->gamma_level < g ? h->gamma_level : g;
   const int K = c < g ? std::max(1, h->gamma_cycles) : 1;
+  // sweeps per level: nu1/nu2 on the finest; MOONS_MG_COARSE_NU="a,b" sets
+  // the levels below it (tuning runs)
+  static const int cnu[2] = {
+      getenv("MOONS_MG_COARSE_NU") ? atoi(getenv("MOONS_MG_COARSE_NU")) : -1,
+      getenv("MOONS_MG_COARSE_NU") && strchr(getenv("MOONS_MG_COARSE_NU"), ',')
+          ? atoi(strchr(getenv("MOONS_MG_COARSE_NU"), ',') + 1) : -1};
+  auto nu1_at = [&](int l) { return l > 0 && cnu[0] >= 0 ? cnu[0] : h->nu1; };
+  auto nu2_at = [&](int l) { return l > 0 && cnu[1] >= 0 ? cnu[1] : h->nu2; };
   auto descend = [&](int l, bool zero_first) -> int {
     LevelDev &L = h->L[l];
     const bool learned = zero_first && h->learn && l == h->learn_level &&
@@ -1378,7 +1386,7 @@ int mg_vcycle(MGHier *h, const int *done) {
       hipLaunchKernelGGL(k_mg_galerkin_part, dim3(nb), dim3(256), 0, stream(), L, h->gpart, done);
       hipLaunchKernelGGL(k_mg_galerkin_scale, dim3(nb), dim3(256), 0, stream(), L, h->gpart, (int)nb, done);
     }
-    for (int s = 0; s < h->nu1; ++s)
+    for (int s = 0; s < nu1_at(l); ++s)
       sweep(L, 0, s == 0 && zero_first && !learned, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
     restrict_to(L, h->L[l + 1], done);
     return MOONS_OK;
@@ -1386,7 +1394,7 @@ int mg_vcycle(MGHier *h, const int *done) {
   auto ascend = [&](int l) {
     LevelDev &L = h->L[l];
     prolong_from(L, h->L[l + 1], done);
-    for (int s = 0; s < h->nu2; ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
+    for (int s = 0; s < nu2_at(l); ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
   };
   int st;
   for (int l = 0; l < c; ++l)
@@ -1395,8 +1403,8 @@ int mg_vcycle(MGHier *h, const int *done) {
     for (int l = c; l < g; ++l)
       if ((st = descend(l, l > c || k == 0))) return st;
     if (tail)
-      hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail, h->nu1,
-                         h->nu2, h->coarse_sweeps, done);
+      hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), h->tail_lds, stream(), h->tailimg, h->tail,
+                         nu1_at(g), nu2_at(g), h->coarse_sweeps, done);
     else
       hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, stream(), h->L[nl - 1], h->coarse_sweeps, done);
     for (int l = g - 1; l >= c; --l) ascend(l);
