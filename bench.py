@@ -453,7 +453,8 @@ def run_energy(M, args):
                       "sources": "advection (scale -1) by a seeded velocity"},
            "last_step_iterations": info["iterations"], "kernels": kern,
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                        "frac": round(gbs / 8000.0, 4), "traffic": None, "kernel": "nrg_matvec",
+                        "frac": round(gbs / 8000.0, 4), "traffic": load_traffic("energy_step_128.md") if N == 128 else None,
+                        "kernel": "nrg_matvec",
                         "bytes_per_cell": 40}}
     # the same step with every source of add_all_energy_sources on (advection,
     # diffusion into L, KE diffusion, viscous dissipation, Joule heating of a
