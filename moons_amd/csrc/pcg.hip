@@ -743,14 +743,22 @@ static int pcg_launch(moons_pcg_s *S, double *x, int it, int which) {
 }
 
 // One multigrid-preconditioned iteration: matvec, update (+ f0 = r/vol),
-// exit test, V-cycle, (z,r),(z,q), p update.
-static int pcg_launch_iteration_mg(moons_pcg_s *S, double *x, int it) {
+// exit test (the head), then V-cycle, (z,r),(z,q), p update (the tail).
+static int pcg_launch_iteration_mg_head(moons_pcg_s *S, double *x, int it) {
   int st = pcg_launch(S, x, it, 1 | 2);
   if (st) return st;
   hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kNT), 0, stream(), S->ctrl, S->partB, S->G.nblocks, it);
   MOONS_LAUNCH_CHECK();
-  if ((st = flex_apply(S))) return st;
+  return MOONS_OK;
+}
+static int pcg_launch_iteration_mg_tail(moons_pcg_s *S, double *x, int it) {
+  int st = flex_apply(S);
+  if (st) return st;
   return pcg_launch(S, x, it, 8 | 16);
+}
+static int pcg_launch_iteration_mg(moons_pcg_s *S, double *x, int it) {
+  int st = pcg_launch_iteration_mg_head(S, x, it);
+  return st ? st : pcg_launch_iteration_mg_tail(S, x, it);
 }
 
 static int pcg_launch_iteration(moons_pcg_s *S, double *x, int it) { return pcg_launch(S, x, it, 7); }
@@ -965,17 +973,33 @@ static int pcg_solve_impl(moons_pcg_s *S, moons_field_t x, moons_field_t b) {
     }
     return launch_apply_bcs(x, false);
   }
-  // Chunks of iterations; every kernel returns at once after the exit decision.
+  // Chunks of iterations; every kernel returns at once after the exit decision
+  // (the multigrid's kernels only skip their stores, so a V-cycle after the
+  // exit costs nearly a full one).  The multigrid iteration that ends a chunk
+  // launches only its head (matvec, update, exit test); its tail (V-cycle,
+  // (z,r),(z,q), p update) waits for the exit flag and is launched before the
+  // next chunk only if the solve goes on.
   int launched = 0;
+  bool tail_pending = false;
   for (;;) {
     MOONS_HIP(hipMemcpyAsync(S->host_ctrl, S->ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, stream()));
     MOONS_HIP(hipStreamSynchronize(stream()));
     if (S->host_ctrl->done || launched >= S->iter_max) break;
+    if (tail_pending) {
+      if ((st = pcg_launch_iteration_mg_tail(S, x->d, launched))) return st;
+      tail_pending = false;
+    }
     int chunk = launched == 0 ? (mg ? 3 : 16)
                               : next_chunk(*S->host_ctrl, S->tol_rel, mg ? 1 : 4, mg ? 32 : 512);
     chunk = std::min(chunk, S->iter_max - launched);
     for (int c = 0; c < chunk; ++c) {
-      st = mg ? pcg_launch_iteration_mg(S, x->d, launched + c + 1) : pcg_launch_iteration(S, x->d, launched + c + 1);
+      const int it = launched + c + 1;
+      if (mg && c == chunk - 1) {
+        st = pcg_launch_iteration_mg_head(S, x->d, it);
+        tail_pending = true;
+      } else {
+        st = mg ? pcg_launch_iteration_mg(S, x->d, it) : pcg_launch_iteration(S, x->d, it);
+      }
       if (st) return st;
     }
     launched += chunk;
