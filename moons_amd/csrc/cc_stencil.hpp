@@ -159,6 +159,45 @@ __device__ __forceinline__ double prologue_sum(const double *part, int nb, int s
   return block_sum_all<NT>(v, sh);
 }
 
+// Two prologue sums at once, r0 over pa[q stride + oa] and r1 over
+// pb[q stride + ob]: each has prologue_sum's arithmetic and order (so the
+// same result); their partial loads are in flight together and the block
+// reduction's barriers are shared.  sh holds 2 NT / kWave + 2 doubles.
+template <int NT>
+__device__ __forceinline__ void prologue_sum2(const double *pa, int oa, const double *pb, int ob, int nb, int stride,
+                                              double *sh, double &r0, double &r1) {
+  constexpr int NW = NT / kWave;
+  double v0 = 0.0, v1 = 0.0;
+  for (int q = threadIdx.x; q < nb; q += NT) {
+    v0 += pa[(size_t)q * stride + oa];
+    v1 += pb[(size_t)q * stride + ob];
+  }
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    v0 += __shfl_down(v0, off, kWave);
+    v1 += __shfl_down(v1, off, kWave);
+  }
+  if ((tid & (kWave - 1)) == 0) {
+    sh[tid / kWave] = v0;
+    sh[NW + tid / kWave] = v1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s0 += sh[w];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s1 += sh[NW + w];
+    sh[2 * NW] = s0;
+    sh[2 * NW + 1] = s1;
+  }
+  __syncthreads();
+  r0 = sh[2 * NW];
+  r1 = sh[2 * NW + 1];
+  __syncthreads();
+}
+
 CCGeom make_geom(const moons_field_s *x, int kz);
 int pick_kz(int n2);
 

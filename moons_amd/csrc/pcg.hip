@@ -177,9 +177,9 @@ __global__ void __launch_bounds__(kNT) k_pcg_rhs(const double *__restrict__ b, c
 // Set-up exit test (PCG_solver_algorithm.f90:60-68), one block:
 // res0 = |r0|, rho = (r,z), res = sqrt|rho|; exit_loop(2) skips the loop.
 __global__ void __launch_bounds__(kNT) k_pcg_init(PcgCtrl *ctrl, const double *partB, int nb) {
-  __shared__ double sh[kNT / kWave];
-  const double rr = prologue_sum<kNT>(partB, nb, 2, 0, sh);
-  const double rz = prologue_sum<kNT>(partB, nb, 2, 1, sh);
+  __shared__ double sh[2 * (kNT / kWave) + 2];
+  double rr, rz;
+  prologue_sum2<kNT>(partB, 0, partB, 1, nb, 2, sh, rr, rz);
   if (threadIdx.x == 0) {
     const double res0 = sqrt(rr);
     const double res = sqrt(fabs(rz));
@@ -345,7 +345,7 @@ template <int KZ>
 __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, const double *__restrict__ r, CCGeom G,
                                                     PcgCtrl *ctrl, const double *partB, int it) {
   if (ctrl->done) return;
-  __shared__ double sh[kNT / kWave];
+  __shared__ double sh[2 * (kNT / kWave) + 2];
   const TileIdx t = tile_of(G);
   const int i = t.i, j = t.j;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
@@ -359,8 +359,8 @@ __global__ void __launch_bounds__(kNT) k_pcg_pupdate(double *__restrict__ p, con
       pv[u] = p[base + (size_t)u * sxy];
     }
   }
-  const double rr = prologue_sum<kNT>(partB, gridDim.x, 2, 0, sh);
-  const double rz = prologue_sum<kNT>(partB, gridDim.x, 2, 1, sh);
+  double rr, rz;
+  prologue_sum2<kNT>(partB, 0, partB, 1, gridDim.x, 2, sh, rr, rz);
   const double rho_old = ctrl->rho[(it - 1) & 1];
   const ExitDecision e = exit_decision(ctrl, it, rr);
   const double beta = rz / rho_old;
@@ -434,9 +434,9 @@ __global__ void __launch_bounds__(kNT) k_mg_zdots(const float *__restrict__ z32,
 
 // Set-up exit test with z = V(r0): res0 = |r0|, rho0 = (z,r0), res = sqrt|rho0|.
 __global__ void __launch_bounds__(kNT) k_mg_init(PcgCtrl *ctrl, const double *partB, const double *partC, int nb) {
-  __shared__ double sh[kNT / kWave];
-  const double rr = prologue_sum<kNT>(partB, nb, 2, 0, sh);
-  const double zr = prologue_sum<kNT>(partC, nb, 2, 0, sh);
+  __shared__ double sh[2 * (kNT / kWave) + 2];
+  double rr, zr;
+  prologue_sum2<kNT>(partB, 0, partC, 0, nb, 2, sh, rr, zr);
   if (threadIdx.x == 0) {
     const double res0 = sqrt(rr);
     const double res = sqrt(fabs(zr));
@@ -456,7 +456,7 @@ template <int KZ>
 __global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, const float *__restrict__ z32, CCGeom G,
                                                    PcgCtrl *ctrl, const double *partC, int it, F32Map M) {
   if (ctrl->done) return;
-  __shared__ double sh[kNT / kWave];
+  __shared__ double sh[2 * (kNT / kWave) + 2];
   const TileIdx t = tile_of(G);
   const int i = t.i, j = t.j;
   const size_t sx = G.s0, sxy = (size_t)G.s0 * G.s1;
@@ -473,8 +473,8 @@ __global__ void __launch_bounds__(kNT) k_mg_pupdate(double *__restrict__ p, cons
   }
   double beta = 0.0;
   if (it > 0) {
-    const double zr = prologue_sum<kNT>(partC, gridDim.x, 2, 0, sh);
-    const double zq = prologue_sum<kNT>(partC, gridDim.x, 2, 1, sh);
+    double zr, zq;
+    prologue_sum2<kNT>(partC, 0, partC, 1, gridDim.x, 2, sh, zr, zq);
     const double rho_old = ctrl->rho[(it - 1) & 1];
     beta = -ctrl->alpha * zq / rho_old;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
