@@ -555,6 +555,19 @@ __global__ void __launch_bounds__(256) k_mg_restrict(LevelDev F, LevelDev C, int
 // read-modify-writes are coalesced, and the 27 coarse values the children
 // interpolate from are loaded once.  Each child's arithmetic is the
 // per-fine-cell formula (source I and pI2, weights pa and pb per direction).
+// MODE selects the children corrected: 0 all; otherwise only the cells that
+// the fine level's first post-smoothing half-sweep reads, since that
+// half-sweep overwrites its own colour without reading it (a wall's clamped
+// neighbour is read with weight zero): 1 the z-line colour 0, (i + j) even
+// (cx + cy even); 2 the y-line colour 0, (i + k) even (cx + cz even); 3 the
+// red-black colour 0, (i + j + k) even (cx + cy + cz odd).
+template <int MODE>
+__device__ __forceinline__ constexpr bool prolong_child(int cx, int cy, int cz) {
+  return MODE == 0 || (MODE == 1 && ((cx + cy) & 1) == 0) || (MODE == 2 && ((cx + cz) & 1) == 0) ||
+         (MODE == 3 && ((cx + cy + cz) & 1) == 1);
+}
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, int kz, const int *done) {
   const bool skip = done && *done; // tested at the stores (overlaps the loads)
   int I, J, K0;
@@ -586,24 +599,37 @@ __global__ void __launch_bounds__(256) k_mg_prolong(LevelDev F, LevelDev C, int 
     }
     zs[1] = K;
     float v[3][3][3], xf[2][2][2];
+    // the coarse values some corrected child interpolates from
+    auto used = [](int x, int y, int z) {
+      for (int cz = 0; cz < 2; ++cz)
+        for (int cy = 0; cy < 2; ++cy)
+          for (int cx = 0; cx < 2; ++cx)
+            if (prolong_child<MODE>(cx, cy, cz) && (x == 1 || x == 2 * cx) && (y == 1 || y == 2 * cy) &&
+                (z == 1 || z == 2 * cz))
+              return true;
+      return false;
+    };
 #pragma unroll
     for (int z = 0; z < 3; ++z)
 #pragma unroll
       for (int y = 0; y < 3; ++y)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) v[z][y][x] = c[lidx(C, xs[x], ys[y], zs[z])];
+        for (int x = 0; x < 3; ++x) v[z][y][x] = used(x, y, z) ? c[lidx(C, xs[x], ys[y], zs[z])] : 0.f;
 #pragma unroll
     for (int cz = 0; cz < 2; ++cz)
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int cx = 0; cx < 2; ++cx) xf[cz][cy][cx] = F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)];
+        for (int cx = 0; cx < 2; ++cx)
+          xf[cz][cy][cx] =
+              prolong_child<MODE>(cx, cy, cz) ? F.x[lidx(F, 2 * I - 1 + cx, 2 * J - 1 + cy, 2 * K - 1 + cz)] : 0.f;
 #pragma unroll
     for (int cz = 0; cz < 2; ++cz)
 #pragma unroll
       for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int cx = 0; cx < 2; ++cx) {
+          if (!prolong_child<MODE>(cx, cy, cz)) continue;
           const int z2 = 2 * cz, y2 = 2 * cy, x2 = 2 * cx; // the pI2 source of this child
           const float e = ax[cx] * (ay[cy] * (az[cz] * v[1][1][1] + bz[cz] * v[z2][1][1]) +
                                     by[cy] * (az[cz] * v[1][y2][1] + bz[cz] * v[z2][y2][1])) +
@@ -1350,9 +1376,19 @@ static void restrict_to(const LevelDev &F, const LevelDev &C, const int *done) {
   hipLaunchKernelGGL(k_mg_restrict, t.grid, t.block, 0, stream(), F, C, t.kz, done);
 }
 
-static void prolong_from(const LevelDev &F, const LevelDev &C, const int *done) {
+// post: the fine level's post-smoothing sweeps that follow (with none, every
+// cell is corrected)
+static void prolong_from(const LevelDev &F, const LevelDev &C, const int *done, int post) {
   const TileLaunch t = tiles(C.n0, C.n1, C.n2);
-  hipLaunchKernelGGL(k_mg_prolong, t.grid, t.block, 0, stream(), F, C, t.kz, done);
+  // the first post-smoothing half-sweep: z-lines (or y-lines) of colour 1,
+  // or the red-black colour 1
+  const int mode = post < 1 ? 0 : F.line2 ? 1 : F.line1 ? 2 : 3;
+  switch (mode) {
+    case 1: hipLaunchKernelGGL(k_mg_prolong<1>, t.grid, t.block, 0, stream(), F, C, t.kz, done); break;
+    case 2: hipLaunchKernelGGL(k_mg_prolong<2>, t.grid, t.block, 0, stream(), F, C, t.kz, done); break;
+    case 3: hipLaunchKernelGGL(k_mg_prolong<3>, t.grid, t.block, 0, stream(), F, C, t.kz, done); break;
+    default: hipLaunchKernelGGL(k_mg_prolong<0>, t.grid, t.block, 0, stream(), F, C, t.kz, done); break;
+  }
 }
 
 int mg_vcycle(MGHier *h, const int *done) {
@@ -1393,7 +1429,7 @@ int mg_vcycle(MGHier *h, const int *done) {
   };
   auto ascend = [&](int l) {
     LevelDev &L = h->L[l];
-    prolong_from(L, h->L[l + 1], done);
+    prolong_from(L, h->L[l + 1], done, nu2_at(l));
     for (int s = 0; s < nu2_at(l); ++s) sweep(L, 1, false, done, l == 0 ? MOONS_KTIMER_MG_LINE_FINE : 0);
   };
   int st;
@@ -1452,7 +1488,7 @@ int mg_time_kernels(MGHier *h, int reps, double ms[10]) {
           else smooth(h->L[0], 0, false, nullptr);
           return MOONS_OK;
         case 1: restrict_to(h->L[0], h->L[1], nullptr); return MOONS_OK;
-        case 2: prolong_from(h->L[0], h->L[1], nullptr); return MOONS_OK;
+        case 2: prolong_from(h->L[0], h->L[1], nullptr, h->nu2); return MOONS_OK;
         case 5: case 7: case 9: sweep2(h->L[lw], 0, false, nullptr); return MOONS_OK;
         case 6: case 8:
           if (lines(lw)) line_half(h->L[lw], ldir(lw), 0, false, nullptr);
