@@ -874,9 +874,14 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(const float *__restric
     const int n0 = T.n[l][0], n1 = T.n[l][1], n2 = T.n[l][2];
     float *x = sm + T.xo[l];
     const float *c = sm + T.xo[l + 1];
-    const int nf = n0 * n1 * n2;
+    // with post-sweeps to follow, only the red-black colour 0 is corrected:
+    // the first post half-sweep rewrites colour 1 without reading it (a
+    // wall's clamped neighbour has weight zero)
+    const bool colour0 = nu2 > 0;
+    const int nf = colour0 ? ((n0 + 1) / 2) * n1 * n2 : n0 * n1 * n2;
     for (int t = threadIdx.x; t < nf; t += kTailThreads) {
-      const int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      int i = 1 + t % n0, j = 1 + (t / n0) % n1, k = 1 + t / (n0 * n1);
+      if (colour0 && !tail_color_cell(n0, n1, n2, t, 0, i, j, k)) continue;
       const int I = TCOEFI(l, 6, 0, i), I2 = TCOEFI(l, 7, 0, i);
       const int J = TCOEFI(l, 6, 1, j), J2 = TCOEFI(l, 7, 1, j);
       const int K = TCOEFI(l, 6, 2, k), K2 = TCOEFI(l, 7, 2, k);
