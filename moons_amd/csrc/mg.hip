@@ -1396,6 +1396,25 @@ static void prolong_from(const LevelDev &F, const LevelDev &C, const int *done, 
   }
 }
 
+// MOONS_MG_COARSE_NU="a,b[,l]": a pre- and b post-smoothing sweeps on level
+// l (1 when not given) and every level below it, the finest level's counts
+// elsewhere (tuning runs); -1: not set.
+struct CoarseNu {
+  int nu1 = -1, nu2 = -1, from = 1;
+};
+
+static CoarseNu coarse_nu() {
+  CoarseNu c;
+  if (const char *e = getenv("MOONS_MG_COARSE_NU")) {
+    c.nu1 = atoi(e);
+    if (const char *p = strchr(e, ',')) {
+      c.nu2 = atoi(p + 1);
+      if (const char *q = strchr(p + 1, ',')) c.from = std::max(1, atoi(q + 1));
+    }
+  }
+  return c;
+}
+
 int mg_vcycle(MGHier *h, const int *done) {
   const int nl = (int)h->L.size();
   const bool tail = h->tail0 < nl;
@@ -1404,14 +1423,10 @@ int mg_vcycle(MGHier *h, const int *done) {
   // gamma-cycle from level c; K = 1, c = g: the plain V-cycle)
   const int c = h->gamma_level >= 0 && h->gamma_level < g ? h->gamma_level : g;
   const int K = c < g ? std::max(1, h->gamma_cycles) : 1;
-  // sweeps per level: nu1/nu2 on the finest; MOONS_MG_COARSE_NU="a,b" sets
-  // the levels below it (tuning runs)
-  static const int cnu[2] = {
-      getenv("MOONS_MG_COARSE_NU") ? atoi(getenv("MOONS_MG_COARSE_NU")) : -1,
-      getenv("MOONS_MG_COARSE_NU") && strchr(getenv("MOONS_MG_COARSE_NU"), ',')
-          ? atoi(strchr(getenv("MOONS_MG_COARSE_NU"), ',') + 1) : -1};
-  auto nu1_at = [&](int l) { return l > 0 && cnu[0] >= 0 ? cnu[0] : h->nu1; };
-  auto nu2_at = [&](int l) { return l > 0 && cnu[1] >= 0 ? cnu[1] : h->nu2; };
+  // sweeps per level: nu1/nu2, or MOONS_MG_COARSE_NU's on its levels
+  static const CoarseNu cnu = coarse_nu();
+  auto nu1_at = [&](int l) { return l >= cnu.from && cnu.nu1 >= 0 ? cnu.nu1 : h->nu1; };
+  auto nu2_at = [&](int l) { return l >= cnu.from && cnu.nu2 >= 0 ? cnu.nu2 : h->nu2; };
   auto descend = [&](int l, bool zero_first) -> int {
     LevelDev &L = h->L[l];
     const bool learned = zero_first && h->learn && l == h->learn_level &&
