@@ -60,6 +60,38 @@ def test_vcycle_matches_restatement(gpu, case):
     assert P.mg_info()[0] == len(levels)
 
 
+@pytest.mark.parametrize("nu", [(1, 0, 16), (2, 1, 16), (3, 2, 8), (1, 3, 4)])
+@pytest.mark.parametrize("case", ["duct48_lines", "ylines48_zperiodic", "dirichlet_mix20"])
+def test_vcycle_sweep_counts_match_restatement(gpu, case, nu):
+    """Sweep counts other than 2 + 2 (moons_pcg_set_mg) against the
+    restatement, whose prolongation corrects every fine cell.  With no
+    post-smoothing the device does too; otherwise it corrects only the cells
+    that the first post-smoothing half-sweep reads (z-line, y-line or
+    red-black colour 0, on the grid levels and in the LDS tail: the 48^3 duct,
+    the y-lines with z periodic, and the 20^3 hierarchy held in the tail)."""
+    nu1, nu2, cs = nu
+    N, beta, bct = CASES[case]
+    print(f"sweep counts {case} {nu}", flush=True)
+    hs, g, m = meshes(gpu, N, beta)
+    x = gpu.SF(m, "CC").set_BCs([NAMES[t] for t in bct], None, all_Neumann=(1 not in bct))
+    P = mg_solver(gpu, m, x)
+    P.set_MG(16, nu1, nu2, cs)
+    rng = np.random.default_rng(31)
+    r = np.zeros(g.shape(0), order="F")
+    r[1:-1, 1:-1, 1:-1] = rng.standard_normal(tuple(n - 2 for n in g.shape(0)))
+    Z = gpu.SF(m, "CC")
+    P.apply_prec(gpu.SF(m, "CC").upload(r), Z)
+    z = Z.download()
+    levels = MGO.build(hs, bct)
+    vol = O.volume(g, 0)
+    f = np.zeros(g.shape(0))
+    f[1:-1, 1:-1, 1:-1] = r[1:-1, 1:-1, 1:-1] / vol[1:-1, 1:-1, 1:-1]
+    zr = MGO.vcycle(levels, f, bct, nu1=nu1, nu2=nu2, coarse_sweeps=cs)
+    zi, zri = z[1:-1, 1:-1, 1:-1], zr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm(zi - zri) / np.linalg.norm(zri)
+    assert err < 2e-5, err
+
+
 @pytest.mark.parametrize("level,cycles", [(0, 2), (1, 3)])
 def test_gamma_cycle_matches_restatement(gpu, level, cycles):
     """The V-cycle with the part from `level` down cycled `cycles` times per
