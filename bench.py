@@ -338,29 +338,33 @@ def run_duct(M, args):
     from moons_amd.duct import Duct
 
     N = args.duct
+    # The fields of this set-up grow without bound after a few steps (max|B|
+    # 3-8x a step from the fourth, NaN by the 18th at 128^3), so the duct is
+    # timed over the first 7 steps: 2 untimed, then at most 5.
+    warmup, steps = min(args.warmup, 2), min(args.steps, 5)
     # phi cleaning by the mixed multigrid too (the reference's Jacobi beside it)
     D = Duct(n=(N, N, N), prec_p="MG_FP32", prec_phi="MG_FP32")
-    D.step(args.warmup)
+    D.step(warmup)
     M.synchronize()
     t0 = time.perf_counter()
-    info = D.step(args.steps)
+    info = D.step(steps)
     M.synchronize()
-    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    ms = 1e3 * (time.perf_counter() - t0) / steps
     ms_j, info_j, agree = None, {"iterations": None}, None
     if not args.duct_no_jacobi:
         DJ = Duct(n=(N, N, N), prec_p="MG_FP32", prec_phi="Lap_SF")
-        DJ.step(args.warmup)
+        DJ.step(warmup)
         M.synchronize()
         t0 = time.perf_counter()
-        info_j = DJ.step(args.steps)
+        info_j = DJ.step(steps)
         M.synchronize()
-        ms_j = 1e3 * (time.perf_counter() - t0) / args.steps
+        ms_j = 1e3 * (time.perf_counter() - t0) / steps
         # the two steps' fields agree (both solves to their tolerances)
         fd, fj = D.fields(), DJ.fields()
         agree = {f"{k}{d}": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
                  for k in ("U", "B", "J") for d, (a, b) in enumerate(zip(fd[k], fj[k]))}
     out = {"metric": "duct MHD time steps per second", "value": round(1e3 / ms, 2), "unit": "steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"duct_mhd_{N}", "cells": f"{N}^3", "Re": 1000, "Ha": 100, "c_w": 0.01,
                       "mesh": "x periodic uniform [-1,1]; y,z Roberts beta=1.05 on [-1,1]", "dt": 1e-3,
