@@ -339,8 +339,11 @@ def run_duct(M, args):
 
     N = args.duct
     # The fields of this set-up grow without bound after a few steps (max|B|
-    # 3-8x a step from the fourth, NaN by the 18th at 128^3), so the duct is
-    # timed over the first 7 steps: 2 untimed, then at most 5.
+    # 3-8x a step from the fourth, NaN by the 18th at 128^3): the reference's
+    # thin-wall ghost rule for cell-located components, reproduced here, is
+    # (2c+1)/(2c-1) times the interior value on min walls (c = c_w/dh, 1.34 at
+    # c_w 0.01 on this mesh), an anti-diffusive wall.  So the duct is timed
+    # over the first 7 steps: 2 untimed, then at most 5.
     warmup, steps = min(args.warmup, 2), min(args.steps, 5)
     # phi cleaning by the mixed multigrid too (the reference's Jacobi beside it)
     D = Duct(n=(N, N, N), prec_p="MG_FP32", prec_phi="MG_FP32")
