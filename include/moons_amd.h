@@ -63,7 +63,11 @@ extern "C" {
  * thin-wall conducting BC, apply_Robin_C/N
  * (code/handwritten/BCs/direct/apply_BCs_faces_raw.f90:236-282) and
  * apply_Robin_C/N_implicit
- * (code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90:213-257). */
+ * (code/handwritten/BCs/implicit/apply_BCs_faces_raw_implicit.f90:213-257).
+ * As there, the cell-located rule's ghost factor is
+ * (2 c_w nhat/dh - 1)/(2 c_w nhat/dh + 1) with nhat = -1 on min faces and
+ * dh > 0, i.e. (2c + 1)/(2c - 1), c = c_w/dh, on min faces: the condition
+ * u - c_w du/dn = 0 there (singular at c = 1/2).  Kept for parity. */
 #define MOONS_BC_ROBIN 6
 
 /* ----------------------------------------------------------- operators */
